@@ -1,0 +1,206 @@
+/*
+ * kronhip.h — C ABI of the MI355X solve path for kronspice circuits.
+ *
+ * The library replaces the linear-solver seam of kronspice's analysis package
+ * (analysis.Options.Solver / SolverFactory, analysis/op.go:50-70, and the
+ * solver.Matrix interface, solver/matrix.go:23-46) and adds batched entry
+ * points that run analysis.OP / analysis.Tran semantics (analysis/op.go:331,
+ * analysis/tran.go:70) for many circuits that share one topology.
+ *
+ * All signatures use plain C types: int32/int64, double, pointers, sizes.
+ * Device pointers are hipMalloc'd memory on the current device; host pointers
+ * are ordinary process memory. Every call returns a kh_status; the library
+ * never aborts the process.
+ */
+#ifndef KRONHIP_H
+#define KRONHIP_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (per call and per circuit) ------------------------------
+ * Mirror the sentinel errors of solver/matrix.go:16-20 (ErrSingularMatrix,
+ * ErrBadSize, ErrNonFinite), analysis/newton.go:20 (ErrNoConvergence),
+ * analysis/tran.go:18-22 (ErrTimestepTooSmall, ErrTooManySteps,
+ * ErrBadTranSpec) and em/energy.go:18 (ErrLedgerResidual). */
+typedef int32_t kh_status;
+enum {
+  KH_OK = 0,
+  KH_SINGULAR = 1,          /* solver.ErrSingularMatrix */
+  KH_NONFINITE = 2,         /* solver.ErrNonFinite */
+  KH_BADSIZE = 3,           /* solver.ErrBadSize */
+  KH_NOCONV = 4,            /* analysis.ErrNoConvergence */
+  KH_TSTEP_TOO_SMALL = 5,   /* analysis.ErrTimestepTooSmall */
+  KH_TOO_MANY_STEPS = 6,    /* analysis.ErrTooManySteps */
+  KH_INTERRUPTED = 7,       /* context deadline (analysis/op.go:81-89) */
+  KH_LEDGER = 8,            /* em.ErrLedgerResidual */
+  KH_BAD_TRAN_SPEC = 9,     /* analysis.ErrBadTranSpec */
+  KH_UNSUPPORTED = 10,      /* device.ErrUnsupported */
+  KH_BAD_ARGUMENT = 11,     /* invalid handle, pointer or shape */
+  KH_HIP_ERROR = 12,        /* a HIP runtime call failed */
+  KH_NO_DEVICE = 13,        /* no usable gfx950 device */
+  KH_CAPACITY = 14,         /* an L/U capacity bound was exceeded */
+  KH_RUNNING = 15           /* internal: circuit still in progress */
+};
+
+/* ---- circuit description ---------------------------------------------------
+ * A circuit is a node count (ground = node 0 included, as
+ * circuit.Circuit.NumNodes) and an ordered element list. Node numbering is
+ * the reference builder's: order of first appearance (circuit/builder.go:38-48).
+ * Element order is the device order of device.FromCircuit
+ * (device/device.go:347-451), which fixes the unknown numbering and the
+ * stamp order (analysis/op.go:295-299).
+ *
+ * node[] per kind (as circuit.Element Nodes + Extra, circuit/builder.go:101-118):
+ *   R C L V I D : node[0] = n+, node[1] = n-
+ *   M (MOSFET)  : node[0] = drain, node[1] = source, node[2] = gate, node[3] = bulk
+ *   Q (BJT)     : node[0] = collector, node[1] = emitter, node[2] = base, node[3] = substrate (-1: none)
+ *
+ * Each element reads its parameters at params[poff ...]; the per-kind layout
+ * is below (already resolved by the host setup step: defaults, area factors,
+ * process-parameter derivation, waveform defaults for the analysis). */
+enum {
+  KH_ELEM_R = 1, KH_ELEM_C = 2, KH_ELEM_L = 3, KH_ELEM_V = 4,
+  KH_ELEM_I = 5, KH_ELEM_D = 6, KH_ELEM_M = 7, KH_ELEM_Q = 8
+};
+
+typedef struct kh_element {
+  int32_t kind;
+  int32_t node[4];
+  int32_t poff;
+} kh_element;
+
+/* Parameter layouts (doubles at params[poff + k]).
+ * R: 0 R, 1 g = 1/R                                       (device/passive.go:65-69)
+ * C: 0 C, 1 IC, 2 hasIC                                   (device/reactive.go:43-47)
+ * L: 0 L, 1 IC, 2 hasIC                                   (device/reactive.go:214-218)
+ * V, I: 0 dc, 1 hasDC, 2 wave kind (0 none, 1 PULSE, 2 SIN, 3 EXP),
+ *       3..9 resolved wave args a[0..6] (device/waveform.go:25-66),
+ *       10 AC magnitude, 11 AC phase [deg]                (device/source.go:21-35)
+ * D: 0 IS, 1 N, 2 RS, 3 CJO, 4 VJ, 5 M, 6 TT, 7 BV, 8 IBV, 9 FC, 10 hasBV,
+ *    11 vte, 12 vcrit, 13 xbv, 14 grs, 15 hasCharge       (device/diode.go:85-128)
+ * M (level 1): 0 typ, 1 beta, 2 vbi, 3 vto (type-normalized), 4 GAMMA,
+ *    5 PHI, 6 LAMBDA, 7 gdpr, 8 gspr, 9 isd, 10 iss, 11 vcritD, 12 vcritS,
+ *    13 czbd, 14 czbdsw, 15 czbs, 16 czbssw, 17 PB, 18 MJ, 19 MJSW, 20 FC,
+ *    21..23 fbt[3], 24..26 fsw[3], 27 cgso, 28 cgdo, 29 cgbo, 30 oxideCap,
+ *    31 vt, 32 hasJcap, 33 hasGcap                        (device/mosfet.go:218-323)
+ */
+#define KH_NPAR_R 2
+#define KH_NPAR_C 3
+#define KH_NPAR_L 3
+#define KH_NPAR_SRC 12
+#define KH_NPAR_D 16
+#define KH_NPAR_M 34
+
+/* Analysis options: analysis.Options (analysis/op.go:22-62) with the same
+ * zero-value defaults (analysis/op.go:91-110). */
+typedef struct kh_options {
+  double reltol;            /* 0 -> 1e-3 */
+  double abstol;            /* 0 -> 1e-12 */
+  double vntol;             /* 0 -> 1e-6 */
+  double gmin;              /* 0 -> 1e-12 */
+  int32_t itl1;             /* 0 -> 100 */
+  int32_t itl4;             /* 0 -> 10 */
+  int32_t no_gmin_stepping;
+  int32_t no_source_stepping;
+  int32_t no_pseudo_transient;
+  int32_t skip_energy_check;
+  double ledger_tol;        /* 0 -> 1e-9 (em.DefaultTolerance) */
+} kh_options;
+
+/* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
+typedef struct kh_tran_spec {
+  double step, stop, start, max_step;
+  int32_t uic;
+  int32_t method;           /* 0 or 2 -> trapezoidal, 1 -> backward Euler */
+  double chgtol;            /* 0 -> 1e-14 */
+  double trtol;             /* 0 -> 7 */
+  double min_step;          /* 0 -> 1e-12 * stop */
+  int64_t max_steps;        /* 0 -> 1,000,000 */
+} kh_tran_spec;
+
+/* Per-batch counters filled by the batched entry points. */
+typedef struct kh_stats {
+  int64_t solves;            /* linear solves executed (factor + solve) */
+  int64_t newton_iterations; /* sum over circuits */
+  int64_t accepted_steps;    /* transient: sum over circuits */
+  int64_t rejected_steps;
+  int64_t analyses;          /* symbolic analyses (orderings) computed on the host */
+  int64_t patterns;          /* distinct sparsity patterns seen */
+  int64_t ticks;             /* device iterations launched */
+  int64_t bytes_algorithmic; /* sum over executed solves of 8*(nnzA + 2n) */
+  double  seconds;           /* wall time of the device loop */
+} kh_stats;
+
+/* ---- library ---------------------------------------------------------------*/
+kh_status kh_init(void);                 /* selects the current HIP device */
+const char* kh_status_string(kh_status s);
+const char* kh_last_error(void);         /* detail of the last failing call (thread-local) */
+
+/* ---- per-system seam: solver.Matrix (solver/matrix.go:23-36) --------------
+ * A system is stamped with Add (accumulating, ground = negative index ignored,
+ * solver/sparse.go:114-143), solved with Gilbert–Peierls LU after minimum-degree
+ * ordering (solver/sparse.go:154-401) on the GPU, and exposes AbsProduct
+ * (solver/sparse.go:481-490). kh_sys_add_batch is the batched form of Add: the
+ * Go shim stages Add calls host-side and crosses cgo once per Solve. */
+typedef struct kh_sys_s* kh_sys;
+kh_status kh_sys_create(int32_t n, kh_sys* out);                     /* solver.NewSparse */
+kh_status kh_sys_clear(kh_sys s);                                    /* Matrix.Clear */
+kh_status kh_sys_add_batch(kh_sys s, const int32_t* i, const int32_t* j,
+                           const double* v, int64_t count);          /* Matrix.Add x count */
+kh_status kh_sys_solve(kh_sys s, const double* b, double* x);        /* Matrix.Solve */
+kh_status kh_sys_absproduct(kh_sys s, const double* x, double* y);   /* Matrix.AbsProduct */
+kh_status kh_sys_nnz(kh_sys s, int64_t* a, int64_t* l, int64_t* u);  /* Sparse.NNZ */
+kh_status kh_sys_pivots(kh_sys s, int32_t* q, int32_t* prow);        /* ordering q and pivot rows of the held factorization */
+void kh_sys_destroy(kh_sys s);
+
+/* ---- batched numeric LU on one pattern ---------------------------------------
+ * B systems share the entry list (ri[k], cj[k]), k < nnz, in first-Add order;
+ * values are SoA on the device: vals[k*B + c]; b and x are [n*B] SoA
+ * (x[i*B + c]). Each system is factored and solved exactly as
+ * sparseLU.factor + solve (solver/sparse.go:199-401), pivoting per system.
+ * status[c] receives KH_OK / KH_SINGULAR / KH_NONFINITE; prow (optional,
+ * [n*B]) the pivot row of each step. */
+typedef struct kh_lu_batch_s* kh_lu_batch;
+kh_status kh_lu_batch_create(int32_t n, int64_t nnz, const int32_t* ri, const int32_t* cj,
+                             int32_t B, kh_lu_batch* out);
+kh_status kh_lu_batch_solve(kh_lu_batch h, const double* d_vals, const double* d_b,
+                            double* d_x, int32_t* d_status, int32_t* d_prow, void* stream);
+kh_status kh_lu_batch_info(kh_lu_batch h, int32_t* q, int64_t* nnzL, int64_t* nnzU,
+                           int64_t* capL, int64_t* capU);
+void kh_lu_batch_destroy(kh_lu_batch h);
+
+/* ---- batched analyses of circuits sharing one topology ---------------------
+ * A topology is the element list (kinds, nodes, parameter offsets) with nparams
+ * doubles per circuit. Parameters for B circuits are SoA on the device:
+ * params[p*B + c]. All parameters that decide the unknown layout (RS > 0 of a
+ * diode, RD/RS > 0 of a MOSFET) must agree across the batch. */
+typedef struct kh_topology_s* kh_topology;
+kh_status kh_topology_create(int32_t num_nodes, const kh_element* elems, int32_t n_elems,
+                             int32_t nparams, const double* params_c0, kh_topology* out);
+int32_t kh_topology_size(kh_topology t);        /* number of MNA unknowns n */
+void kh_topology_destroy(kh_topology t);
+
+/* analysis.OP for B circuits (analysis/op.go:331-357): dcSolve with the
+ * gmin / source / pseudo-transient fallbacks (analysis/newton.go:95-311),
+ * Accept, ledger check. x_out is [n*B] SoA on the device. iters (optional) the
+ * Newton iterations per circuit. */
+kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                      double* d_x, int32_t* d_status, int32_t* d_iters, kh_stats* stats);
+
+/* analysis.Tran for B circuits (analysis/tran.go:70-231). Records the final
+ * accepted solution per circuit (d_x, [n*B]) and the accepted / rejected step
+ * counts; d_ledger (optional, [B]) the run's MaxRelativeResidual. */
+kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                        const kh_tran_spec* spec, double* d_x, int32_t* d_status,
+                        int32_t* d_accepted, int32_t* d_rejected, double* d_ledger,
+                        kh_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KRONHIP_H */

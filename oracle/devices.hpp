@@ -1,0 +1,1114 @@
+// ORACLE — test infrastructure only (see sparse.hpp).
+//
+// CPU restatement of kronspice's device models for the elements the solve path
+// uses: R (device/passive.go), C and L (device/reactive.go), V and I
+// (device/source.go, device/waveform.go), the SPICE3 diode (device/diode.go),
+// the level-1 MOSFET (device/mosfet.go, device/mosfet_levels.go) and the LTE
+// estimate (device/lte.go). Parameters arrive already resolved by the host
+// setup step in the layout of include/kronhip.h.
+#pragma once
+#include <cmath>
+#include <complex>
+#include <limits>
+#include <vector>
+#include "../include/kronhip.h"
+
+namespace oracle {
+
+using cplx = std::complex<double>;
+constexpr double kInf = std::numeric_limits<double>::infinity();
+constexpr double kE = 2.71828182845904523536028747135266249775724709369995957496696763;  // math.E
+constexpr double kPi = 3.14159265358979323846264338327950288419716939937510582097494459;  // math.Pi
+
+enum Mode { ModeOP = 0, ModeTranOP = 1, ModeUICInit = 2, ModeTran = 3 };  // device/device.go:105-116
+enum Method { BackwardEuler = 1, Trapezoidal = 2 };                       // device/device.go:121-124
+
+struct Stamper {
+  virtual void add(int i, int j, double v) = 0;
+  virtual ~Stamper() {}
+};
+struct CStamper {
+  virtual void add(int i, int j, cplx v) = 0;
+  virtual ~CStamper() {}
+};
+
+// LoadContext, device/device.go:127-148
+struct LoadContext {
+  Stamper* M = nullptr;
+  std::vector<double>* RHS = nullptr;
+  int mode = ModeOP;
+  double time = 0, H = 0;
+  int method = Trapezoidal;
+  const double* X = nullptr;
+  int nx = 0;
+  bool initJct = false;
+  double gmin = 1e-12;
+  bool sourceStepping = false;
+  double srcFact = 0;
+  void addRHS(int i, double v) {  // :151-155
+    if (i >= 0 && i < (int)RHS->size()) (*RHS)[i] += v;
+  }
+};
+
+struct ACContext {  // device/device.go:162-174
+  CStamper* M = nullptr;
+  std::vector<cplx>* RHS = nullptr;
+  double omega = 0;
+  void addRHS(int i, cplx v) {
+    if (i >= 0 && i < (int)RHS->size()) (*RHS)[i] += v;
+  }
+};
+
+struct TruncContext {  // device/lte.go:10-19
+  double time = 0, H = 0;
+  int method = Trapezoidal;
+  double relTol = 1e-3, absTol = 1e-12, vnTol = 1e-6, chgTol = 1e-14, trTol = 7;
+};
+
+// history, device/lte.go:23-37
+struct History {
+  double t[3] = {0, 0, 0}, q[3] = {0, 0, 0};
+  int n = 0;
+  void push(double tt, double qq) {
+    t[2] = t[1]; t[1] = t[0]; t[0] = tt;
+    q[2] = q[1]; q[1] = q[0]; q[0] = qq;
+    if (n < 3) n++;
+  }
+  void reset() { n = 0; }
+};
+
+// lteStep, device/lte.go:50-85
+inline double lteStep(const History& hist, double qn, double in, double inPrev, double absTol, const TruncContext& tc) {
+  int k = tc.method;
+  if (k < 1 || k > 2 || hist.n < k + 1) return kInf;
+  double ts[4] = {tc.time, hist.t[0], hist.t[1], hist.t[2]};
+  double qs[4] = {qn, hist.q[0], hist.q[1], hist.q[2]};
+  double dd[4] = {0, 0, 0, 0};
+  for (int i = 0; i < k + 2; i++) dd[i] = qs[i];
+  for (int order = 1; order <= k + 1; order++) {
+    for (int i = 0; i + order <= k + 1; i++) {
+      double den = ts[i] - ts[i + order];
+      if (den == 0) return kInf;
+      dd[i] = (dd[i] - dd[i + 1]) / den;
+    }
+  }
+  double deriv = dd[0];
+  double fact = 2.0, c = 0.5;
+  if (k == 2) { fact = 6; c = 1.0 / 12; }
+  double h = tc.H;
+  double errI = c * std::pow(h, (double)k) * std::fabs(deriv * fact);
+  if (errI == 0 || std::isnan(errI)) return kInf;
+  double qmax = std::max(std::fabs(qn), std::fabs(hist.q[0]));
+  double allowed = tc.trTol * std::max(tc.relTol * std::max(std::fabs(in), std::fabs(inPrev)) + absTol,
+                                       tc.relTol * std::max(qmax, tc.chgTol) / h);
+  return h * std::pow(allowed / errI, 1 / (double)k);
+}
+
+inline double at(const double* x, int nx, int i) { return (i < 0 || i >= nx) ? 0.0 : x[i]; }  // device/diode.go:140-145
+
+struct OperatingPoint {  // device/device.go:186-197
+  double voltage = 0, current = 0, power = 0, storage = 0;
+};
+
+// Ledger accumulator, em/energy.go:25-79
+struct Accumulator {
+  double inflow = 0, outflow = 0, dissipation = 0, storageRate = 0, modelSupplied = 0, scale = 0;
+  // channel: 0 electrical, 2 thermal (exchange/channel.go:28-31)
+  void addPort(int ch, double pw) {
+    if (pw == 0) return;
+    scale += std::fabs(pw);
+    if (ch == 2 && pw < 0) dissipation += -pw;
+    else if (pw > 0) inflow += pw;
+    else outflow += -pw;
+  }
+  void addStorage(double p) { storageRate += p; scale += std::fabs(p); }
+  void addModelSupplied(double p) {
+    if (p == 0) return;
+    modelSupplied += p;
+    scale += std::fabs(p);
+  }
+  double residual() const { return inflow + modelSupplied - outflow - dissipation - storageRate; }
+};
+
+struct SetupContext {  // device/device.go:36-100
+  int numNodes = 1, branches = 0;
+  std::vector<int> internal;
+  int unknown(int node) const { return node - 1; }
+  int allocBranch() { return numNodes - 1 + branches++; }
+  int allocUnknown() { int i = allocBranch(); internal.push_back(i); return i; }
+  int size() const { return numNodes - 1 + branches; }
+  std::vector<char> voltageUnknowns() const {
+    std::vector<char> v(size(), 0);
+    for (int i = 0; i < numNodes - 1; i++) v[i] = 1;
+    for (int i : internal) v[i] = 1;
+    return v;
+  }
+};
+
+// Resolved source waveform, device/waveform.go:16-239 (PULSE, SIN, EXP).
+struct Wave {
+  int kind = 0;  // 0 none, 1 pulse, 2 sin, 3 exp
+  double a[7] = {0, 0, 0, 0, 0, 0, 0};
+  double value(double t) const {  // :69-126
+    switch (kind) {
+      case 1: {
+        double v1 = a[0], v2 = a[1], td = a[2], tr = a[3], tf = a[4], pw = a[5], per = a[6];
+        if (t < td) return v1;
+        double tt = t - td;
+        if (per > 0) tt = std::fmod(tt, per);
+        if (tt < tr) return v1 + (v2 - v1) * tt / tr;
+        if (tt < tr + pw) return v2;
+        if (tt < tr + pw + tf) return v2 + (v1 - v2) * (tt - tr - pw) / tf;
+        return v1;
+      }
+      case 2: {
+        double vo = a[0], va = a[1], f = a[2], td = a[3], theta = a[4], ph = a[5] * kPi / 180;
+        if (t < td) return vo + va * std::sin(ph);
+        double tt = t - td;
+        return vo + va * std::exp(-tt * theta) * std::sin(2 * kPi * f * tt + ph);
+      }
+      case 3: {
+        double v1 = a[0], v2 = a[1], td1 = a[2], tau1 = a[3], td2 = a[4], tau2 = a[5];
+        if (t < td1) return v1;
+        if (t < td2) return v1 + (v2 - v1) * (1 - std::exp(-(t - td1) / tau1));
+        return v1 + (v2 - v1) * (1 - std::exp(-(t - td1) / tau1)) + (v1 - v2) * (1 - std::exp(-(t - td2) / tau2));
+      }
+    }
+    return 0;
+  }
+  double nextBreakpoint(double t) const {  // :190-239
+    const double rel = 1e-12;
+    auto after = [&](double b) { return b > t + rel * std::max(std::fabs(t), 1e-15); };
+    switch (kind) {
+      case 1: {
+        double td = a[2], tr = a[3], tf = a[4], pw = a[5], per = a[6];
+        double edges[4] = {0, tr, tr + pw, tr + pw + tf};
+        if (per <= 0) {
+          for (double e : edges)
+            if (after(td + e)) return td + e;
+          return kInf;
+        }
+        double k = std::floor((t - td) / per);
+        if (k < 0) k = 0;
+        for (double kk = k; kk <= k + 1; kk++) {
+          double base = td + kk * per;
+          for (double e : edges)
+            if (after(base + e)) return base + e;
+        }
+        return td + (k + 2) * per;
+      }
+      case 2:
+        if (after(a[3])) return a[3];
+        break;
+      case 3:
+        if (after(a[2])) return a[2];
+        if (after(a[4])) return a[4];
+        break;
+    }
+    return kInf;
+  }
+};
+
+struct Device {
+  int kind = 0;
+  int nodes[4] = {0, 0, 0, 0};
+  int u[2] = {-1, -1};
+  virtual ~Device() {}
+  void setupNodes(SetupContext& c) { u[0] = c.unknown(nodes[0]); u[1] = c.unknown(nodes[1]); }
+  double voltage(const double* x) const {  // device/device.go:310-319
+    double v = 0;
+    if (u[0] >= 0) v += x[u[0]];
+    if (u[1] >= 0) v -= x[u[1]];
+    return v;
+  }
+  cplx voltageAC(const cplx* x) const {
+    cplx v = 0;
+    if (u[0] >= 0) v += x[u[0]];
+    if (u[1] >= 0) v -= x[u[1]];
+    return v;
+  }
+  virtual void setup(SetupContext& c) { setupNodes(c); }
+  virtual void load(LoadContext& lc) = 0;
+  virtual OperatingPoint operating(const double* x, int nx) = 0;
+  virtual void accept(const double*, int, double, int) {}
+  virtual bool reactive() const { return false; }
+  virtual double truncate(const double*, int, const TruncContext&) { return kInf; }
+  virtual bool nonlinear() const { return false; }
+  virtual bool limited() const { return false; }
+  virtual bool convTest(const double*, int, double, double) { return true; }
+  virtual double nextBreakpoint(double) const { return kInf; }
+  virtual bool breakpointer() const { return false; }
+  // Ledger rows in channel order; returns thermal/electrical powers.
+  virtual void ports(const OperatingPoint& op, Accumulator& acc) = 0;
+  virtual bool modelSupplier() const { return false; }
+  virtual double modelSupplied(const OperatingPoint&) const { return 0; }
+  virtual void loadAC(ACContext& ac) = 0;
+  virtual void initialCondition() {}
+};
+
+inline void stampY(CStamper* m, int a, int b, cplx y) {  // device/device.go:333-338
+  m->add(a, a, y);
+  m->add(b, b, y);
+  m->add(a, b, -y);
+  m->add(b, a, -y);
+}
+
+// Resistor, device/passive.go:32-134 (no ECE terms).
+struct Resistor : Device {
+  double R = 0, g = 0;
+  void load(LoadContext& lc) override {
+    int a = u[0], b = u[1];
+    lc.M->add(a, a, g);
+    lc.M->add(b, b, g);
+    lc.M->add(a, b, -g);
+    lc.M->add(b, a, -g);
+  }
+  OperatingPoint operating(const double* x, int) override {
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    op.current = op.voltage * g;
+    op.power = op.voltage * op.current;
+    return op;
+  }
+  void ports(const OperatingPoint& op, Accumulator& acc) override {
+    if (R > 0) acc.addPort(2, -(op.power));  // ExteriorPower, :99-104
+  }
+  bool modelSupplier() const override { return true; }
+  double modelSupplied(const OperatingPoint& op) const override { return R < 0 ? -op.power : 0; }
+  void loadAC(ACContext& ac) override { stampY(ac.M, u[0], u[1], cplx(g, 0)); }
+};
+
+// Capacitor, device/reactive.go:25-184, :355-374
+struct Capacitor : Device {
+  double C = 0, ic = 0;
+  bool hasIC = false;
+  double geq = 0, ihist = 0;
+  bool tran = false;
+  double omega = 0;
+  double vPrev = 0, iPrev = 0;
+  History hist;
+  void load(LoadContext& lc) override {
+    int a = u[0], b = u[1];
+    if (lc.mode == ModeOP || lc.mode == ModeTranOP) {
+      tran = false; geq = 0; ihist = 0;
+      return;
+    }
+    if (lc.mode == ModeUICInit) {
+      tran = false;
+      double g = 1e9;  // uicConductance, :57
+      lc.M->add(a, a, g); lc.M->add(b, b, g); lc.M->add(a, b, -g); lc.M->add(b, a, -g);
+      lc.addRHS(a, g * vPrev);
+      lc.addRHS(b, -g * vPrev);
+      geq = 0; ihist = 0;
+      return;
+    }
+    tran = true;
+    if (lc.method == Trapezoidal) {
+      geq = 2 * C / lc.H;
+      ihist = -(geq * vPrev + iPrev);
+    } else {
+      geq = C / lc.H;
+      ihist = -geq * vPrev;
+    }
+    double g = geq;
+    lc.M->add(a, a, g); lc.M->add(b, b, g); lc.M->add(a, b, -g); lc.M->add(b, a, -g);
+    lc.addRHS(a, -ihist);
+    lc.addRHS(b, ihist);
+  }
+  OperatingPoint operating(const double* x, int) override {
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    double i = 0;
+    if (tran) i = geq * op.voltage + ihist;
+    op.current = i;
+    op.power = op.voltage * i;
+    op.storage = op.voltage * i;
+    return op;
+  }
+  void accept(const double* x, int nx, double t, int mode) override {
+    if (mode == ModeUICInit) {
+      iPrev = 0;
+      hist.reset();
+    } else if (mode == ModeOP || mode == ModeTranOP) {
+      vPrev = voltage(x);
+      iPrev = 0;
+      hist.reset();
+    } else {
+      OperatingPoint op = operating(x, nx);
+      vPrev = op.voltage;
+      iPrev = op.current;
+    }
+    hist.push(t, C * vPrev);
+  }
+  void initialCondition() override { vPrev = hasIC ? ic : 0; iPrev = 0; }
+  bool reactive() const override { return true; }
+  double truncate(const double* x, int nx, const TruncContext& tc) override {
+    OperatingPoint op = operating(x, nx);
+    return lteStep(hist, C * op.voltage, op.current, iPrev, tc.absTol, tc);
+  }
+  void ports(const OperatingPoint&, Accumulator&) override {}
+  void loadAC(ACContext& ac) override {
+    omega = ac.omega;
+    stampY(ac.M, u[0], u[1], cplx(0, ac.omega * C));
+  }
+};
+
+// Inductor, device/reactive.go:199-352, :376-401
+struct Inductor : Device {
+  double L = 0, ic = 0;
+  bool hasIC = false;
+  int branch = -1;
+  double vPrev = 0, iPrev = 0;
+  History hist;
+  void setup(SetupContext& c) override {
+    setupNodes(c);
+    branch = c.allocBranch();
+  }
+  void load(LoadContext& lc) override {
+    int a = u[0], b = u[1], k = branch;
+    lc.M->add(a, k, 1);
+    lc.M->add(b, k, -1);
+    if (lc.mode == ModeOP || lc.mode == ModeTranOP) {
+      lc.M->add(k, a, 1);
+      lc.M->add(k, b, -1);
+      return;
+    }
+    if (lc.mode == ModeUICInit) {
+      lc.M->add(k, k, 1);
+      lc.addRHS(k, iPrev);
+      return;
+    }
+    double req = L / lc.H;
+    double rhs = -req * iPrev;
+    if (lc.method == Trapezoidal) {
+      req = 2 * L / lc.H;
+      rhs = -req * iPrev - vPrev;
+    }
+    lc.M->add(k, a, 1);
+    lc.M->add(k, b, -1);
+    lc.M->add(k, k, -req);
+    lc.addRHS(k, rhs);
+  }
+  OperatingPoint operating(const double* x, int) override {
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    op.current = x[branch];
+    op.power = op.voltage * op.current;
+    op.storage = op.power;
+    return op;
+  }
+  void accept(const double* x, int nx, double t, int mode) override {
+    if (mode == ModeUICInit) {
+      vPrev = 0;
+      hist.reset();
+    } else if (mode == ModeOP || mode == ModeTranOP) {
+      vPrev = 0;
+      iPrev = x[branch];
+      hist.reset();
+    } else {
+      OperatingPoint op = operating(x, nx);
+      vPrev = op.voltage;
+      iPrev = op.current;
+    }
+    hist.push(t, L * iPrev);
+  }
+  void initialCondition() override { vPrev = 0; iPrev = hasIC ? ic : 0; }
+  bool reactive() const override { return true; }
+  double truncate(const double* x, int nx, const TruncContext& tc) override {
+    OperatingPoint op = operating(x, nx);
+    return lteStep(hist, L * op.current, op.voltage, vPrev, tc.vnTol, tc);
+  }
+  void ports(const OperatingPoint&, Accumulator&) override {}
+  void loadAC(ACContext& ac) override {
+    int a = u[0], b = u[1], k = branch;
+    ac.M->add(a, k, 1);
+    ac.M->add(b, k, -1);
+    ac.M->add(k, a, 1);
+    ac.M->add(k, b, -1);
+    ac.M->add(k, k, cplx(0, -ac.omega * L));
+  }
+};
+
+// sourceValue, device/source.go:21-73
+struct SourceValue {
+  double dc = 0;
+  bool hasDC = false;
+  Wave w;
+  cplx ac = 0;
+  double at(const LoadContext& lc) const {
+    double v = raw(lc);
+    if (lc.sourceStepping && lc.mode != ModeTran) v *= lc.srcFact;
+    return v;
+  }
+  double raw(const LoadContext& lc) const {
+    if (w.kind == 0) return dc;
+    if (lc.mode == ModeOP && hasDC) return dc;
+    double t = lc.time;
+    if (lc.mode != ModeTran) t = 0;
+    return w.value(t);
+  }
+  double nextBreakpoint(double t) const { return w.kind == 0 ? kInf : w.nextBreakpoint(t); }
+};
+
+inline void loadSource(SourceValue& s, const double* p) {
+  s.dc = p[0];
+  s.hasDC = p[1] != 0;
+  s.w.kind = (int)p[2];
+  for (int k = 0; k < 7; k++) s.w.a[k] = p[3 + k];
+  double mag = p[10], ph = p[11] * kPi / 180;
+  s.ac = cplx(mag * std::cos(ph), mag * std::sin(ph));  // cmplx.Rect
+}
+
+// VoltageSource, device/source.go:77-129, :186-199
+struct VoltageSource : Device {
+  SourceValue src;
+  int branch = -1;
+  double value = 0;
+  void setup(SetupContext& c) override {
+    setupNodes(c);
+    branch = c.allocBranch();
+  }
+  void load(LoadContext& lc) override {
+    int a = u[0], b = u[1], k = branch;
+    value = src.at(lc);
+    lc.M->add(a, k, 1);
+    lc.M->add(b, k, -1);
+    lc.M->add(k, a, 1);
+    lc.M->add(k, b, -1);
+    lc.addRHS(k, value);
+  }
+  OperatingPoint operating(const double* x, int) override {
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    op.current = x[branch];
+    op.power = op.voltage * op.current;
+    return op;
+  }
+  bool breakpointer() const override { return true; }
+  double nextBreakpoint(double t) const override { return src.nextBreakpoint(t); }
+  void ports(const OperatingPoint& op, Accumulator& acc) override { acc.addPort(0, -op.power); }
+  void loadAC(ACContext& ac) override {
+    int a = u[0], b = u[1], k = branch;
+    ac.M->add(a, k, 1);
+    ac.M->add(b, k, -1);
+    ac.M->add(k, a, 1);
+    ac.M->add(k, b, -1);
+    ac.addRHS(k, src.ac);
+  }
+};
+
+// CurrentSource, device/source.go:133-177, :207-217
+struct CurrentSource : Device {
+  SourceValue src;
+  double value = 0;
+  void load(LoadContext& lc) override {
+    value = src.at(lc);
+    lc.addRHS(u[0], -value);
+    lc.addRHS(u[1], value);
+  }
+  OperatingPoint operating(const double* x, int) override {
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    op.current = value;
+    op.power = op.voltage * value;
+    return op;
+  }
+  bool breakpointer() const override { return true; }
+  double nextBreakpoint(double t) const override { return src.nextBreakpoint(t); }
+  void ports(const OperatingPoint& op, Accumulator& acc) override { acc.addPort(0, -op.power); }
+  void loadAC(ACContext& ac) override {
+    ac.addRHS(u[0], -src.ac);
+    ac.addRHS(u[1], src.ac);
+  }
+};
+
+// pnjlim, device/diode.go:214-226
+inline double pnjlim(double vnew, double vold, double vt, double vcrit, bool* lim) {
+  if (vnew > vcrit && std::fabs(vnew - vold) > 2 * vt) {
+    *lim = true;
+    if (vold > 0) {
+      double arg = 1 + (vnew - vold) / vt;
+      if (arg > 0) return vold + vt * std::log(arg);
+      return vcrit;
+    }
+    return vt * std::log(vnew / vt);
+  }
+  *lim = false;
+  return vnew;
+}
+
+// Diode, device/diode.go:61-435
+struct Diode : Device {
+  double IS = 0, N = 1, RS = 0, CJO = 0, VJ = 1, M = 0.5, TT = 0, BV = 0, IBV = 1e-3, FC = 0.5;
+  bool hasBV = false;
+  double vte = 0, vcrit = 0, xbv = 0, grs = 0;
+  bool hasCharge = false;
+  int ai = -1;
+  double vd = 0, cd = 0, gd = 0, iq = 0, geq = 0;
+  bool tran = false, lim = false;
+  double gmin = 1e-12, qPrev = 0, iqPrev = 0, vdAcc = 0, omega = 0;
+  History hist;
+
+  void setup(SetupContext& c) override {
+    setupNodes(c);
+    ai = u[0];
+    if (grs > 0) ai = c.allocUnknown();
+  }
+  void current(double v, double* id, double* g) const {  // :148-162
+    if (v >= -3 * vte) {
+      double e = std::exp(std::min(v / vte, 700.0));
+      *id = IS * (e - 1);
+      *g = IS * e / vte;
+    } else if (!hasBV || v >= -xbv) {
+      double a = 3 * vte / (v * kE);
+      a = a * a * a;
+      *id = -IS * (1 + a);
+      *g = IS * 3 * a / v;
+    } else {
+      double e = std::exp(std::min(-(xbv + v) / vte, 700.0));
+      *id = -IS * e;
+      *g = IS * e / vte;
+    }
+  }
+  void charge(double v, double id, double g, double* q, double* c) const {  // :191-211
+    *q = TT * id;
+    *c = TT * g;
+    if (CJO == 0) return;
+    double fcv = FC * VJ;
+    if (v < fcv) {
+      double arg = 1 - v / VJ;
+      double sarg = std::pow(arg, -M);
+      *q += CJO * VJ * (1 - arg * sarg) / (1 - M);
+      *c += CJO * sarg;
+      return;
+    }
+    double f1 = VJ * (1 - std::pow(1 - FC, 1 - M)) / (1 - M);
+    double f2 = std::pow(1 - FC, 1 + M);
+    double f3 = 1 - FC * (1 + M);
+    *q += CJO * f1 + CJO / f2 * (f3 * (v - fcv) + M / (2 * VJ) * (v * v - fcv * fcv));
+    *c += CJO / f2 * (f3 + M * v / VJ);
+  }
+  void load(LoadContext& lc) override {  // :229-275
+    int a = u[0], k = u[1];
+    gmin = lc.gmin;
+    double vraw = at(lc.X, lc.nx, ai) - at(lc.X, lc.nx, k);
+    if (lc.initJct) {
+      vd = vcrit;
+      lim = true;
+    } else {
+      vd = pnjlim(vraw, vdAcc, vte, vcrit, &lim);
+      if (hasBV && vraw < std::min(0.0, -xbv + 10 * vte)) {
+        bool l2;
+        double vt = pnjlim(-(vraw + xbv), -(vdAcc + xbv), vte, vcrit, &l2);
+        vd = -(vt + xbv);
+        lim = l2;
+      }
+    }
+    vdAcc = vd;
+    double id, g0;
+    current(vd, &id, &g0);
+    cd = id + gmin * vd;
+    gd = g0 + gmin;
+    tran = false; iq = 0; geq = 0;
+    if (lc.mode == ModeTran && hasCharge) {
+      double q, c;
+      charge(vd, id, g0, &q, &c);
+      tran = true;
+      if (lc.method == Trapezoidal) {
+        iq = 2 * (q - qPrev) / lc.H - iqPrev;
+        geq = 2 * c / lc.H;
+      } else {
+        iq = (q - qPrev) / lc.H;
+        geq = c / lc.H;
+      }
+    }
+    double g = gd + geq;
+    double ieq = cd + iq - g * vd;
+    lc.M->add(ai, ai, g);
+    lc.M->add(k, k, g);
+    lc.M->add(ai, k, -g);
+    lc.M->add(k, ai, -g);
+    lc.addRHS(ai, -ieq);
+    lc.addRHS(k, ieq);
+    if (grs > 0) {
+      lc.M->add(a, a, grs);
+      lc.M->add(ai, ai, grs);
+      lc.M->add(a, ai, -grs);
+      lc.M->add(ai, a, -grs);
+    }
+  }
+  bool nonlinear() const override { return true; }
+  bool limited() const override { return lim; }
+  bool convTest(const double* x, int nx, double reltol, double abstol) override {  // :282-289
+    double v = at(x, nx, ai) - at(x, nx, u[1]);
+    double cdhat = cd + gd * (v - vd);
+    double id, g0;
+    current(v, &id, &g0);
+    double cdn = id + gmin * v;
+    double tol = reltol * std::max(std::fabs(cdhat), std::fabs(cdn)) + abstol;
+    return std::fabs(cdhat - cdn) <= tol;
+  }
+  OperatingPoint operating(const double* x, int nx) override {  // :293-301
+    double v = at(x, nx, ai) - at(x, nx, u[1]);
+    double ir = cd + gd * (v - vd);
+    double ic = iq + geq * (v - vd);
+    OperatingPoint op;
+    op.current = ir + ic;
+    op.voltage = voltage(x);
+    op.power = op.voltage * op.current;
+    op.storage = v * ic;
+    return op;
+  }
+  void accept(const double* x, int nx, double t, int mode) override {  // :304-322
+    double v = at(x, nx, ai) - at(x, nx, u[1]);
+    vdAcc = v;
+    double id, g0, q, c;
+    current(v, &id, &g0);
+    charge(v, id, g0, &q, &c);
+    if (mode == ModeTran) {
+      iqPrev = tran ? iq + geq * (v - vd) : 0;
+    } else {
+      iqPrev = 0;
+      hist.reset();
+    }
+    qPrev = q;
+    hist.push(t, q);
+  }
+  bool reactive() const override { return true; }
+  double truncate(const double* x, int nx, const TruncContext& tc) override {  // :388-397
+    if (!hasCharge) return kInf;
+    double v = at(x, nx, ai) - at(x, nx, u[1]);
+    double id, g0, q, c;
+    current(v, &id, &g0);
+    charge(v, id, g0, &q, &c);
+    double ic = iq + geq * (v - vd);
+    return lteStep(hist, q, ic, iqPrev, tc.absTol, tc);
+  }
+  void ports(const OperatingPoint& op, Accumulator& acc) override { acc.addPort(2, -(op.power - op.storage)); }
+  void loadAC(ACContext& ac) override {  // :409-420
+    int a = u[0], k = u[1];
+    double id, g0, q, c;
+    current(vdAcc, &id, &g0);
+    charge(vdAcc, id, g0, &q, &c);
+    stampY(ac.M, ai, k, cplx(g0 + gmin, ac.omega * c));
+    if (grs > 0) stampY(ac.M, a, ai, cplx(grs, 0));
+    omega = ac.omega;
+  }
+};
+
+// fetlim, device/mosfet.go:347-386
+inline double fetlim(double vnew, double vold, double vto) {
+  double vtsthi = std::fabs(2 * (vold - vto)) + 2;
+  double vtstlo = vtsthi / 2 + 2;
+  double vtox = vto + 3.5;
+  double delv = vnew - vold;
+  if (vold >= vto) {
+    if (vold >= vtox) {
+      if (delv <= 0) {
+        if (vnew >= vtox) {
+          if (-delv > vtstlo) vnew = vold - vtstlo;
+        } else {
+          vnew = std::max(vnew, vto + 2);
+        }
+      } else if (delv >= vtsthi) {
+        vnew = vold + vtsthi;
+      }
+    } else if (delv <= 0) {
+      vnew = std::max(vnew, vto - 0.5);
+    } else {
+      vnew = std::min(vnew, vto + 4);
+    }
+  } else if (delv <= 0) {
+    if (-delv > vtsthi) vnew = vold - vtsthi;
+  } else {
+    double vtemp = vto + 0.5;
+    if (vnew <= vtemp) {
+      if (delv > vtstlo) vnew = vold + vtstlo;
+    } else {
+      vnew = vtemp;
+    }
+  }
+  return vnew;
+}
+
+// limvds, device/mosfet.go:388-404
+inline double limvds(double vnew, double vold) {
+  if (vold >= 3.5) {
+    if (vnew > vold) return std::min(vnew, 3 * vold + 2);
+    if (vnew < 3.5) return std::max(vnew, 2.0);
+    return vnew;
+  }
+  if (vnew > vold) return std::min(vnew, 4.0);
+  return std::max(vnew, -0.5);
+}
+
+// depletion, device/bjt.go:358-370
+inline void depletion(double v, double cz, double vj, double m, double fc, double f1, double f2, double f3,
+                      double* qv, double* c) {
+  if (cz == 0) { *qv = 0; *c = 0; return; }
+  double fcv = fc * vj;
+  if (v < fcv) {
+    double arg = 1 - v / vj;
+    double sarg = std::exp(-m * std::log(arg));
+    *qv = vj * cz * (1 - arg * sarg) / (1 - m);
+    *c = cz * sarg;
+    return;
+  }
+  double czf2 = cz / f2;
+  *qv = cz * f1 + czf2 * (f3 * (v - fcv) + (m / (vj + vj)) * (v * v - fcv * fcv));
+  *c = czf2 * (f3 + m * v / vj);
+}
+
+// meyer, device/mosfet.go:419-438
+inline void meyer(double vgs, double vgd, double von, double vdsat, double phi, double cox, double* cgs, double* cgd,
+                  double* cgb) {
+  double vgst = vgs - von;
+  vdsat = std::max(vdsat, 0.0);
+  if (vgst <= -phi) { *cgs = 0; *cgd = 0; *cgb = cox; return; }
+  if (vgst <= -phi / 2) { *cgs = 0; *cgd = 0; *cgb = -vgst * cox / phi; return; }
+  if (vgst <= 0) { *cgs = 2 * (vgst * cox / (1.5 * phi) + cox / 3); *cgd = 0; *cgb = -vgst * cox / phi; return; }
+  double vds = vgs - vgd;
+  if (vdsat <= vds) { *cgs = 2 * cox / 3; *cgd = 0; *cgb = 0; return; }
+  double vddif = 2 * vdsat - vds;
+  double vddif1 = vdsat - vds;
+  double vddif2 = vddif * vddif;
+  *cgs = 2 * cox * (1 - vddif1 * vddif1 / vddif2) / 3;
+  *cgd = 2 * cox * (1 - vdsat * vdsat / vddif2) / 3;
+  *cgb = 0;
+}
+
+// MOSFET level 1, device/mosfet.go:43-846 and device/mosfet_levels.go:79-113
+struct Mosfet : Device {
+  double typ = 1, beta = 0, vbi = 0, vto = 0, GAMMA = 0, PHI = 0.6, LAMBDA = 0, gdpr = 0, gspr = 0;
+  double isd = 0, iss = 0, vcritD = 0, vcritS = 0, czbd = 0, czbdsw = 0, czbs = 0, czbssw = 0;
+  double PB = 0.8, MJ = 0.5, MJSW = 0.5, FC = 0.5, fbt[3] = {0, 0, 0}, fsw[3] = {0, 0, 0};
+  double cgso = 0, cgdo = 0, cgbo = 0, oxideCap = 0, vt = 0;
+  bool hasJcap = false, hasGcap = false;
+  int g = -1, b = -1, dp = -1, sp = -1;
+  double vgs = 0, vds = 0, vbs = 0, vbd = 0, vgd = 0, vgb = 0;
+  double mode = 1, cdrain = 0, gm = 0, gds = 0, gmbs = 0, von = 0, vdsat = 0;
+  double cbs = 0, gbs = 0, cbd = 0, gbd = 0;
+  bool lim = false, tran = false;
+  double ag = 0, gmin = 1e-12;
+  double capG[3] = {0, 0, 0}, capB[2] = {0, 0}, iq[5] = {0, 0, 0, 0, 0}, dv[5] = {0, 0, 0, 0, 0};
+  struct { double vgs = 0, vds = 0, vbs = 0, vgd = 0, vgb = 0, vbd = 0, von = 0; } acc;
+  double meyerAcc[3] = {0, 0, 0}, qPrev[5] = {0, 0, 0, 0, 0}, iqPrev[5] = {0, 0, 0, 0, 0}, vPrev[5] = {0, 0, 0, 0, 0};
+  History hist[5];
+  struct { double gch[3] = {0, 0, 0}, gbs = 0, gbd = 0, cap[5] = {0, 0, 0, 0, 0}, omega = 0; } ac;
+
+  void setup(SetupContext& c) override {  // :326-337
+    setupNodes(c);
+    g = c.unknown(nodes[2]);
+    b = c.unknown(nodes[3]);
+    dp = u[0];
+    sp = u[1];
+    if (gdpr > 0) dp = c.allocUnknown();
+    if (gspr > 0) sp = c.allocUnknown();
+  }
+  void voltages(const double* x, int nx, double* ovgs, double* ovds, double* ovbs, double* ovgb) const {
+    double t = typ;
+    double vg = at(x, nx, g), vb = at(x, nx, b), vdp = at(x, nx, dp), vsp = at(x, nx, sp);
+    *ovgs = t * (vg - vsp);
+    *ovds = t * (vdp - vsp);
+    *ovbs = t * (vb - vsp);
+    *ovgb = t * (vg - vb);
+  }
+  void bulkDiode(double v, double is, double* i, double* gg) const {  // :408-415
+    if (v <= -3 * vt) { *i = gmin * v - is; *gg = gmin; return; }
+    double e = std::exp(std::min(v / vt, 700.0));
+    *i = is * (e - 1) + gmin * v;
+    *gg = is * e / vt + gmin;
+  }
+  void meyerCaps(double a_vgs, double a_vgd, double a_von, double a_vdsat, double md, double out[3]) const {  // :441-451
+    if (oxideCap == 0) { out[0] = out[1] = out[2] = 0; return; }
+    double a, bb, c;
+    if (md > 0) {
+      meyer(a_vgs, a_vgd, a_von, a_vdsat, PHI, oxideCap, &a, &bb, &c);
+      out[0] = a; out[1] = bb; out[2] = c;
+      return;
+    }
+    meyer(a_vgd, a_vgs, a_von, a_vdsat, PHI, oxideCap, &a, &bb, &c);
+    out[0] = bb; out[1] = a; out[2] = c;
+  }
+  void bulkCharge(double v, double cz, double czsw, double* qv, double* c) const {  // :455-460
+    double q1, c1, q2, c2;
+    depletion(v, cz, PB, MJ, FC, fbt[0], fbt[1], fbt[2], &q1, &c1);
+    depletion(v, czsw, PB, MJSW, FC, fsw[0], fsw[1], fsw[2], &q2, &c2);
+    *qv = q1 + q2;
+    *c = c1 + c2;
+  }
+  // idsLevel1, device/mosfet_levels.go:79-113
+  void ids(double a_vgs, double a_vds, double a_vbs, double* cd, double* ogm, double* ogds, double* ogmbs, double* ovon,
+           double* ovdsat) const {
+    double sarg;
+    if (a_vbs <= 0) {
+      sarg = std::sqrt(PHI - a_vbs);
+    } else {
+      sarg = std::sqrt(PHI);
+      sarg = std::max(0.0, sarg - a_vbs / (sarg + sarg));
+    }
+    double v_on = vbi + GAMMA * sarg;
+    double vgst = a_vgs - v_on;
+    *ovon = v_on;
+    *ovdsat = std::max(vgst, 0.0);
+    double arg = 0;
+    if (sarg > 0) arg = GAMMA / (sarg + sarg);
+    if (vgst <= 0) { *cd = 0; *ogm = 0; *ogds = 0; *ogmbs = 0; return; }
+    double betap = beta * (1 + LAMBDA * a_vds);
+    if (vgst <= a_vds) {
+      *cd = betap * vgst * vgst * 0.5;
+      *ogm = betap * vgst;
+      *ogds = LAMBDA * beta * vgst * vgst * 0.5;
+    } else {
+      *cd = betap * a_vds * (vgst - 0.5 * a_vds);
+      *ogm = betap * a_vds;
+      *ogds = betap * (vgst - a_vds) + LAMBDA * beta * a_vds * (vgst - 0.5 * a_vds);
+    }
+    *ogmbs = *ogm * arg;
+  }
+  // channel, device/mosfet.go:464-473
+  void channel(double a_vgs, double a_vds, double a_vbs, double* md, double* cd, double* ogm, double* ogds, double* ogmbs,
+               double* ovon, double* ovdsat) const {
+    *md = 1;
+    double vg = a_vgs, vd = a_vds, vb = a_vbs;
+    if (a_vds < 0) {
+      *md = -1;
+      vg = a_vgs - a_vds;
+      vd = -a_vds;
+      vb = a_vbs - a_vds;
+    }
+    ids(vg, vd, vb, cd, ogm, ogds, ogmbs, ovon, ovdsat);
+  }
+  double capk(int k) const { return k < 3 ? capG[k] : capB[k - 3]; }
+  // branch, device/mosfet.go:563-578
+  void branch(LoadContext& lc, int a, int bb, double i0, int n, const double gg[3], const int p[3], const int m[3],
+              const double v0[3]) {
+    double ieq = i0;
+    for (int k = 0; k < n; k++) {
+      if (gg[k] == 0) continue;
+      lc.M->add(a, p[k], gg[k]);
+      lc.M->add(a, m[k], -gg[k]);
+      lc.M->add(bb, p[k], -gg[k]);
+      lc.M->add(bb, m[k], gg[k]);
+      ieq -= gg[k] * v0[k];
+    }
+    ieq *= typ;
+    lc.addRHS(a, -ieq);
+    lc.addRHS(bb, ieq);
+  }
+  void channelBranch(double* i0, double gc[3]) const {  // :582-589
+    if (mode > 0) { *i0 = cdrain; gc[0] = gm; gc[1] = gds; gc[2] = gmbs; return; }
+    *i0 = -cdrain;
+    gc[0] = -gm;
+    gc[1] = gm + gds + gmbs;
+    gc[2] = -gmbs;
+  }
+  static void stampG(Stamper* m, int a, int bb, double gg) {  // device/bjt.go:482-490
+    if (gg == 0) return;
+    m->add(a, a, gg);
+    m->add(bb, bb, gg);
+    m->add(a, bb, -gg);
+    m->add(bb, a, -gg);
+  }
+  void stamp(LoadContext& lc) {  // :591-611
+    double i0, gc[3];
+    channelBranch(&i0, gc);
+    { int p[3] = {g, dp, b}, m[3] = {sp, sp, sp}; double v0[3] = {vgs, vds, vbs}; branch(lc, dp, sp, i0, 3, gc, p, m, v0); }
+    { double gg[3] = {gbs, 0, 0}; int p[3] = {b, 0, 0}, m[3] = {sp, 0, 0}; double v0[3] = {vbs, 0, 0}; branch(lc, b, sp, cbs, 1, gg, p, m, v0); }
+    { double gg[3] = {gbd, 0, 0}; int p[3] = {b, 0, 0}, m[3] = {dp, 0, 0}; double v0[3] = {vbd, 0, 0}; branch(lc, b, dp, cbd, 1, gg, p, m, v0); }
+    if (tran) {
+      int ends[5][2] = {{g, sp}, {g, dp}, {g, b}, {b, sp}, {b, dp}};
+      for (int k = 0; k < 5; k++) {
+        double c = ag * capk(k);
+        double gg[3] = {c, 0, 0};
+        int p[3] = {ends[k][0], 0, 0}, m[3] = {ends[k][1], 0, 0};
+        double v0[3] = {dv[k], 0, 0};
+        branch(lc, ends[k][0], ends[k][1], iq[k], 1, gg, p, m, v0);
+      }
+    }
+    if (dp != u[0]) stampG(lc.M, u[0], dp, gdpr);
+    if (sp != u[1]) stampG(lc.M, u[1], sp, gspr);
+  }
+  void load(LoadContext& lc) override {  // :476-551
+    gmin = lc.gmin;
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(lc.X, lc.nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    lim = false;
+    if (lc.initJct) {
+      a_vbs = -1; a_vgs = vto; a_vds = 0;
+      lim = true;
+    } else {
+      double a_vgd = a_vgs - a_vds;
+      double vgdOld = acc.vgs - acc.vds;
+      double v_on = acc.von;
+      if (acc.vds >= 0) {
+        a_vgs = fetlim(a_vgs, acc.vgs, v_on);
+        a_vds = a_vgs - a_vgd;
+        a_vds = limvds(a_vds, acc.vds);
+        a_vgd = a_vgs - a_vds;
+      } else {
+        a_vgd = fetlim(a_vgd, vgdOld, v_on);
+        a_vds = a_vgs - a_vgd;
+        a_vds = -limvds(-a_vds, -acc.vds);
+        a_vgs = a_vgd + a_vds;
+      }
+      bool l;
+      if (a_vds >= 0) {
+        a_vbs = pnjlim(a_vbs, acc.vbs, vt, vcritS, &l);
+      } else {
+        bool l2;
+        double a_vbd = pnjlim(a_vbs - a_vds, acc.vbs - acc.vds, vt, vcritD, &l2);
+        a_vbs = a_vbd + a_vds;
+        l = l2;
+      }
+      lim = l;
+    }
+    double a_vbd = a_vbs - a_vds, a_vgd = a_vgs - a_vds;
+    double a_vgb2 = a_vgs - a_vbs;
+    vgs = a_vgs; vds = a_vds; vbs = a_vbs; vbd = a_vbd; vgd = a_vgd; vgb = a_vgb2;
+    acc.vgs = a_vgs; acc.vds = a_vds; acc.vbs = a_vbs;
+    bulkDiode(vbs, iss, &cbs, &gbs);
+    bulkDiode(vbd, isd, &cbd, &gbd);
+    channel(vgs, vds, vbs, &mode, &cdrain, &gm, &gds, &gmbs, &von, &vdsat);
+    acc.von = von;
+    tran = false;
+    ag = 0;
+    for (int k = 0; k < 5; k++) iq[k] = 0;
+    dv[0] = vgs; dv[1] = vgd; dv[2] = vgb; dv[3] = vbs; dv[4] = vbd;
+    if (lc.mode == ModeTran && (hasGcap || hasJcap)) {
+      tran = true;
+      ag = lc.method == Trapezoidal ? 2 / lc.H : 1 / lc.H;
+      double mc[3];
+      meyerCaps(vgs, vgd, von, vdsat, mode, mc);
+      double ov[3] = {cgso, cgdo, cgbo};
+      for (int k = 0; k < 3; k++) capG[k] = (mc[k] + meyerAcc[k]) / 2 + ov[k];
+      double qbs, cbsc, qbd, cbdc;
+      bulkCharge(vbs, czbs, czbssw, &qbs, &cbsc);
+      bulkCharge(vbd, czbd, czbdsw, &qbd, &cbdc);
+      capB[0] = cbsc; capB[1] = cbdc;
+      double qs[5] = {qPrev[0] + capG[0] * (vgs - vPrev[0]), qPrev[1] + capG[1] * (vgd - vPrev[1]),
+                      qPrev[2] + capG[2] * (vgb - vPrev[2]), qbs, qbd};
+      for (int k = 0; k < 5; k++) {
+        iq[k] = ag * (qs[k] - qPrev[k]);
+        if (lc.method == Trapezoidal) iq[k] -= iqPrev[k];
+      }
+    }
+    stamp(lc);
+  }
+  bool nonlinear() const override { return true; }
+  bool limited() const override { return lim; }
+  bool convTest(const double* x, int nx, double reltol, double abstol) override {  // :618-631
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(x, nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    double a_vbd = a_vbs - a_vds;
+    double i0, gc[3];
+    channelBranch(&i0, gc);
+    double ich = i0 + gc[0] * (a_vgs - vgs) + gc[1] * (a_vds - vds) + gc[2] * (a_vbs - vbs);
+    double ibs = cbs + gbs * (a_vbs - vbs);
+    double ibd = cbd + gbd * (a_vbd - vbd);
+    double md, cd, t1, t2, t3, t4, t5;
+    channel(a_vgs, a_vds, a_vbs, &md, &cd, &t1, &t2, &t3, &t4, &t5);
+    double ichN = md * cd;
+    double ibsN, ibdN, gg;
+    bulkDiode(a_vbs, iss, &ibsN, &gg);
+    bulkDiode(a_vbd, isd, &ibdN, &gg);
+    auto ok = [&](double a, double bb) { return std::fabs(a - bb) <= reltol * std::max(std::fabs(a), std::fabs(bb)) + abstol; };
+    return ok(ich - ibd, ichN - ibdN) && ok(ibs + ibd, ibsN + ibdN);
+  }
+  void terminalCurrents(const double* x, int nx, double* id, double* ig, double* is, double* ib, double* ps) {  // :635-657
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(x, nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    double a_vbd = a_vbs - a_vds, a_vgd = a_vgs - a_vds;
+    double t = typ;
+    double i0, gc[3];
+    channelBranch(&i0, gc);
+    double ich = t * (i0 + gc[0] * (a_vgs - vgs) + gc[1] * (a_vds - vds) + gc[2] * (a_vbs - vbs));
+    double ibs = t * (cbs + gbs * (a_vbs - vbs));
+    double ibd = t * (cbd + gbd * (a_vbd - vbd));
+    double iqv[5] = {0, 0, 0, 0, 0};
+    *ps = 0;
+    if (tran) {
+      double v[5] = {a_vgs, a_vgd, a_vgb, a_vbs, a_vbd};
+      for (int k = 0; k < 5; k++) {
+        iqv[k] = t * (iq[k] + ag * capk(k) * (v[k] - dv[k]));
+        *ps += v[k] * t * iqv[k];
+      }
+    }
+    *id = ich - ibd - iqv[1] - iqv[4];
+    *is = -ich - ibs - iqv[0] - iqv[3];
+    *ig = iqv[0] + iqv[1] + iqv[2];
+    *ib = ibs + ibd - iqv[2] + iqv[3] + iqv[4];
+  }
+  OperatingPoint operating(const double* x, int nx) override {  // :660-665
+    double id, ig, is, ib, ps;
+    terminalCurrents(x, nx, &id, &ig, &is, &ib, &ps);
+    double vd = at(x, nx, u[0]), vs = at(x, nx, u[1]);
+    double p = vd * id + at(x, nx, g) * ig + vs * is + at(x, nx, b) * ib;
+    OperatingPoint op;
+    op.voltage = vd - vs;
+    op.current = id;
+    op.power = p;
+    op.storage = ps;
+    return op;
+  }
+  void accept(const double* x, int nx, double t, int md_mode) override {  // :668-705
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(x, nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    double a_vgd = a_vgs - a_vds, a_vbd = a_vbs - a_vds;
+    double md, cd, t1, t2, t3, v_on, v_dsat;
+    channel(a_vgs, a_vds, a_vbs, &md, &cd, &t1, &t2, &t3, &v_on, &v_dsat);
+    acc.vgs = a_vgs; acc.vds = a_vds; acc.vbs = a_vbs; acc.von = v_on;
+    acc.vgd = a_vgd; acc.vgb = a_vgb; acc.vbd = a_vbd;
+    double mc[3];
+    meyerCaps(a_vgs, a_vgd, v_on, v_dsat, md, mc);
+    double v[5] = {a_vgs, a_vgd, a_vgb, a_vbs, a_vbd};
+    double qbs, qbd, cc;
+    bulkCharge(a_vbs, czbs, czbssw, &qbs, &cc);
+    bulkCharge(a_vbd, czbd, czbdsw, &qbd, &cc);
+    if (md_mode == ModeTran && tran) {
+      double qs[5];
+      for (int k = 0; k < 3; k++) qs[k] = qPrev[k] + capG[k] * (v[k] - vPrev[k]);
+      qs[3] = qbs;
+      qs[4] = qbd;
+      for (int k = 0; k < 5; k++) {
+        iqPrev[k] = iq[k] + ag * capk(k) * (v[k] - dv[k]);
+        qPrev[k] = qs[k];
+        hist[k].push(t, qs[k]);
+      }
+    } else {
+      double ov[3] = {cgso, cgdo, cgbo};
+      for (int k = 0; k < 3; k++) qPrev[k] = (mc[k] + ov[k]) * v[k];
+      qPrev[3] = qbs;
+      qPrev[4] = qbd;
+      for (int k = 0; k < 5; k++) {
+        iqPrev[k] = 0;
+        hist[k].reset();
+        hist[k].push(t, qPrev[k]);
+      }
+    }
+    for (int k = 0; k < 5; k++) vPrev[k] = v[k];
+    for (int k = 0; k < 3; k++) meyerAcc[k] = mc[k];
+  }
+  bool reactive() const override { return true; }
+  double truncate(const double* x, int nx, const TruncContext& tc) override {  // :734-757
+    if (!tran) return kInf;
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(x, nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    double v[5] = {a_vgs, a_vgs - a_vds, a_vgb, a_vbs, a_vbs - a_vds};
+    double h = kInf;
+    for (int k = 0; k < 5; k++) {
+      double qk, cc;
+      if (k < 3) qk = qPrev[k] + capG[k] * (v[k] - vPrev[k]);
+      else if (k == 3) bulkCharge(v[k], czbs, czbssw, &qk, &cc);
+      else bulkCharge(v[k], czbd, czbdsw, &qk, &cc);
+      if (qk == 0 && qPrev[k] == 0) continue;
+      double i = iq[k] + ag * capk(k) * (v[k] - dv[k]);
+      h = std::min(h, lteStep(hist[k], qk, i, iqPrev[k], tc.absTol, tc));
+    }
+    return h;
+  }
+  void ports(const OperatingPoint& op, Accumulator& accu) override { accu.addPort(2, -(op.power - op.storage)); }
+  void loadAC(ACContext& ctx) override;
+};
+
+}  // namespace oracle

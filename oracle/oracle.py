@@ -1,0 +1,250 @@
+"""ctypes front of the FP64 oracle (liboracle.so).
+
+ORACLE — test infrastructure only.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module, and only as the checker:
+nothing in the product path (kronspice_amd/) uses it.
+
+The library is the CPU restatement of kronspice's Go solve path (see
+sparse.hpp, devices.hpp and sim.cpp for the file:line citations).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ERR_NAMES = {0: "ok", 1: "singular", 2: "nonfinite", 3: "badsize", 4: "noconv", 5: "tstep", 6: "maxsteps",
+             8: "ledger", 9: "badspec", 10: "unsupported"}
+
+
+class KhOptions(C.Structure):
+    _fields_ = [("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double), ("gmin", C.c_double),
+                ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
+                ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
+                ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double)]
+
+
+class KhTranSpec(C.Structure):
+    _fields_ = [("step", C.c_double), ("stop", C.c_double), ("start", C.c_double), ("max_step", C.c_double),
+                ("uic", C.c_int32), ("method", C.c_int32), ("chgtol", C.c_double), ("trtol", C.c_double),
+                ("min_step", C.c_double), ("max_steps", C.c_int64)]
+
+
+class OpInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("iters", C.c_int32), ("solves", C.c_int64), ("nnz_a", C.c_int64),
+                ("nnz_l", C.c_int64), ("nnz_u", C.c_int64), ("analyses", C.c_int32), ("err_step", C.c_int32),
+                ("ledger_rel", C.c_double), ("inflow", C.c_double), ("outflow", C.c_double),
+                ("dissipation", C.c_double), ("storage", C.c_double)]
+
+
+class TranInfo(C.Structure):
+    _fields_ = [("n", C.c_int32), ("accepted", C.c_int32), ("rejected", C.c_int32), ("iters", C.c_int32),
+                ("count", C.c_int32), ("min_step", C.c_double), ("max_step", C.c_double), ("max_rel", C.c_double),
+                ("t_end", C.c_double)]
+
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        _lib.oracle_sparse_new.restype = P
+        _lib.oracle_sparse_new.argtypes = [C.c_int32, C.c_int32]
+        _lib.oracle_sparse_free.argtypes = [P, C.c_int32]
+        _lib.oracle_sparse_clear.argtypes = [P, C.c_int32]
+        _lib.oracle_sparse_add.argtypes = [P, C.c_int32, C.c_int32, C.c_double, C.c_double, C.c_int32]
+        _lib.oracle_sparse_solve.argtypes = [P, P, P, C.c_int32]
+        _lib.oracle_sparse_info.argtypes = [P, C.c_int32, P, P, P, P, P, P]
+        _lib.oracle_sparse_absproduct.argtypes = [P, P, P]
+        _lib.oracle_min_degree.argtypes = [C.c_int32, P, P, C.c_int64, P]
+        _lib.oracle_dense_solve.argtypes = [C.c_int32, P, P, P, C.c_int64, P, P]
+        _lib.oracle_size.argtypes = [C.c_int32, P, C.c_int32, P]
+        _lib.oracle_op.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
+        _lib.oracle_op_polished.argtypes = [C.c_int32, P, C.c_int32, P, P, P, P, P]
+        _lib.oracle_tran.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
+        _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
+        _lib.oracle_batch_op.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int32,
+                                         C.c_int32, P, P, P]
+        _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_int32,
+                                           P, P, P, P, P]
+    return _lib
+
+
+def _p(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def options(**kw) -> KhOptions:
+    o = KhOptions()
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Sparse:
+    """solver.Sparse / CSparse restatement (solver/sparse.go)."""
+
+    def __init__(self, n: int, complex_: bool = False):
+        self.n = n
+        self.cplx = 1 if complex_ else 0
+        self.h = lib().oracle_sparse_new(n, self.cplx)
+        if not self.h:
+            raise ValueError("bad size")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_sparse_free(self.h, self.cplx)
+            self.h = None
+
+    def clear(self):
+        lib().oracle_sparse_clear(self.h, self.cplx)
+
+    def add(self, i: int, j: int, v):
+        v = complex(v)
+        lib().oracle_sparse_add(self.h, i, j, v.real, v.imag, self.cplx)
+
+    def solve(self, b) -> Tuple[int, np.ndarray]:
+        dt = np.complex128 if self.cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=dt)
+        x = np.zeros(self.n, dtype=dt)
+        e = lib().oracle_sparse_solve(self.h, _p(b), _p(x), self.cplx)
+        return e, x
+
+    def info(self):
+        a, l, u = C.c_int64(), C.c_int64(), C.c_int64()
+        q = np.zeros(self.n, np.int32)
+        prow = np.zeros(self.n, np.int32)
+        es = C.c_int32()
+        lib().oracle_sparse_info(self.h, self.cplx, C.byref(a), C.byref(l), C.byref(u), _p(q), _p(prow), C.byref(es))
+        return dict(nnz_a=a.value, nnz_l=l.value, nnz_u=u.value, q=q, prow=prow, err_step=es.value)
+
+    def absproduct(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        lib().oracle_sparse_absproduct(self.h, _p(x), _p(y))
+        return y
+
+
+def min_degree(n: int, ri, ci) -> np.ndarray:
+    ri = np.ascontiguousarray(ri, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    out = np.zeros(n, np.int32)
+    lib().oracle_min_degree(n, _p(ri), _p(ci), len(ri), _p(out))
+    return out
+
+
+def dense_solve(n: int, ri, ci, v, b):
+    ri = np.ascontiguousarray(ri, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    v = np.ascontiguousarray(v, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    x = np.zeros(n)
+    e = lib().oracle_dense_solve(n, _p(ri), _p(ci), _p(v), len(v), _p(b), _p(x))
+    return e, x
+
+
+def size(num_nodes: int, els: np.ndarray, params: np.ndarray) -> int:
+    return lib().oracle_size(num_nodes, _p(els), len(els), _p(params))
+
+
+def op(num_nodes: int, els: np.ndarray, params: np.ndarray, opts: Optional[KhOptions] = None, backend: int = 2,
+       guess: Optional[np.ndarray] = None):
+    """analysis.OP restatement. Returns (status, x, info dict with q/prow/currents)."""
+    n = size(num_nodes, els, params)
+    if n <= 0:
+        return -n, None, {}
+    x = np.zeros(n)
+    q = np.zeros(n, np.int32)
+    prow = np.zeros(n, np.int32)
+    cur = np.zeros(len(els))
+    info = OpInfo()
+    o = opts if opts is not None else KhOptions()
+    g = None if guess is None else np.ascontiguousarray(guess, np.float64)
+    e = lib().oracle_op(num_nodes, _p(els), len(els), _p(params), C.byref(o), backend, _p(g), _p(x), _p(q),
+                        _p(prow), _p(cur), C.byref(info))
+    d = {f[0]: getattr(info, f[0]) for f in OpInfo._fields_}
+    d.update(q=q, prow=prow, current=cur)
+    return e, x, d
+
+
+def op_polished(num_nodes, els, params, opts=None, guess=None):
+    n = size(num_nodes, els, params)
+    x = np.zeros(n)
+    it = C.c_int32()
+    o = opts if opts is not None else KhOptions()
+    g = None if guess is None else np.ascontiguousarray(guess, np.float64)
+    e = lib().oracle_op_polished(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(g), _p(x), C.byref(it))
+    return e, x, it.value
+
+
+def tran(num_nodes: int, els: np.ndarray, params: np.ndarray, spec: KhTranSpec, opts: Optional[KhOptions] = None,
+         backend: int = 2, cap: int = 0):
+    n = size(num_nodes, els, params)
+    x = np.zeros(n)
+    times = np.zeros(max(cap, 1))
+    xs = np.zeros((max(cap, 1), n))
+    cur = np.zeros((max(cap, 1), len(els)))
+    info = TranInfo()
+    o = opts if opts is not None else KhOptions()
+    e = lib().oracle_tran(num_nodes, _p(els), len(els), _p(params), C.byref(o), C.byref(spec), backend, _p(x),
+                          _p(times), _p(xs), _p(cur), cap, C.byref(info))
+    d = {f[0]: getattr(info, f[0]) for f in TranInfo._fields_}
+    c = info.count
+    d.update(times=times[:c], xs=xs[:c], current=cur[:c])
+    return e, x, d
+
+
+def ac(num_nodes: int, els: np.ndarray, params: np.ndarray, freqs, opts: Optional[KhOptions] = None):
+    n = size(num_nodes, els, params)
+    freqs = np.ascontiguousarray(freqs, np.float64)
+    out = np.zeros((len(freqs), n), np.complex128)
+    mr = C.c_double()
+    o = opts if opts is not None else KhOptions()
+    e = lib().oracle_ac(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(freqs), len(freqs), _p(out),
+                        C.byref(mr))
+    return e, out, mr.value
+
+
+def batch_op(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, opts=None, threads: int = 1,
+             polish: bool = False):
+    """OP over B circuits of one topology; params [P, B]; returns x [n, B], status, iters."""
+    P, B = params_soa.shape
+    params_soa = np.ascontiguousarray(params_soa, np.float64)
+    x = np.zeros((n, B))
+    st = np.zeros(B, np.int32)
+    it = np.zeros(B, np.int32)
+    o = opts if opts is not None else KhOptions()
+    lib().oracle_batch_op(num_nodes, _p(els), len(els), P, B, _p(params_soa), C.byref(o), threads,
+                          1 if polish else 0, _p(x), _p(st), _p(it))
+    return x, st, it
+
+
+def batch_tran(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, spec: KhTranSpec, opts=None,
+               threads: int = 1):
+    P, B = params_soa.shape
+    params_soa = np.ascontiguousarray(params_soa, np.float64)
+    x = np.zeros((n, B))
+    st = np.zeros(B, np.int32)
+    acc = np.zeros(B, np.int32)
+    rej = np.zeros(B, np.int32)
+    led = np.zeros(B)
+    o = opts if opts is not None else KhOptions()
+    lib().oracle_batch_tran(num_nodes, _p(els), len(els), P, B, _p(params_soa), C.byref(o), C.byref(spec), threads,
+                            _p(x), _p(st), _p(acc), _p(rej), _p(led))
+    return x, st, acc, rej, led

@@ -1,0 +1,262 @@
+"""Pins the FP64 oracle (oracle/) against the reference's own fixtures and
+known answers, before it is trusted as the parity checker of the GPU path.
+
+Sources of each check:
+  * solver/sparse_test.go: zero-diagonal exact solve (:149-161), singular and
+    non-finite classes (:163-194), ground ignored (:197-206), pattern growth
+    (:255-275), fill of a 40x40 grid with 4 sources (:278-290; counts
+    nnz A 7,848, L 19,880, U 21,493 from doc/M6.md:77-78), sparse vs dense on
+    MNA grids (:84-104), backward error <= 1e-13 on random systems (:108-146).
+  * analysis/op_test.go:24-52 divider (5 V, -5 mA, ledger 0.05 W) and the
+    reference's golden netlists (testdata/, copied under tests/golden/reference)
+    with the tolerances of internal/golden/golden.go.
+  * closed forms: minimum degree on a path graph eliminates from the ends;
+    pnjlim (device/diode_test.go:79-100) is checked through the diode OP goldens.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle
+from kronspice_amd import circuit as kc
+from tests import golden_runner as gr
+
+
+# ----------------------------------------------------------------------------- sparse LU
+def grid_mna(s, w, h, nv, rng, cond=None):
+    """gridMNA of solver/sparse_test.go:26-56 with a numpy RNG for the values."""
+    nn = w * h
+    n = nn + nv
+    s.clear()
+
+    def g(a, b, v):
+        s.add(a, a, v); s.add(b, b, v); s.add(a, b, -v); s.add(b, a, -v)
+
+    for y in range(h):
+        for x in range(w):
+            i = y * w + x
+            if x + 1 < w:
+                g(i, i + 1, 1 / (10 + 990 * rng.random()))
+            if y + 1 < h:
+                g(i, i + w, 1 / (10 + 990 * rng.random()))
+            s.add(i, i, 1e-9)
+    b = np.zeros(n)
+    for k in range(nv):
+        node, br = (k * 7919) % nn, nn + k
+        s.add(node, br, 1)
+        s.add(br, node, 1)
+        b[br] = 1 + rng.random()
+    return b
+
+
+class DenseRec:
+    def __init__(self, n):
+        self.n = n
+        self.a = np.zeros((n, n))
+        self.entries = []
+
+    def clear(self):
+        self.a[:] = 0
+        self.entries = []
+
+    def add(self, i, j, v):
+        if 0 <= i < self.n and 0 <= j < self.n:
+            self.a[i, j] += v
+            self.entries.append((i, j, v))
+
+
+def test_zero_diagonal_exact():
+    s = oracle.Sparse(2)
+    s.add(0, 0, 1e-3); s.add(0, 1, 1); s.add(1, 0, 1)
+    e, x = s.solve([0, 5])
+    assert e == 0
+    assert abs(x[0] - 5) <= 1e-12 and abs(x[1] + 5e-3) <= 1e-15
+
+
+@pytest.mark.parametrize("case", ["empty", "zero column", "dependent rows"])
+def test_singular_classes(case):
+    s = oracle.Sparse(3)
+    if case == "zero column":
+        s.add(0, 0, 1); s.add(1, 0, 1); s.add(2, 2, 1)
+    elif case == "dependent rows":
+        for j in range(3):
+            s.add(0, j, 1); s.add(1, j, 2)
+        s.add(2, 2, 1)
+    e, _ = s.solve(np.zeros(3))
+    assert e == 1  # ErrSingularMatrix
+
+
+def test_nonfinite_and_ground():
+    s = oracle.Sparse(2)
+    s.add(0, 0, float("nan")); s.add(1, 1, 1)
+    assert s.solve(np.zeros(2))[0] == 2  # ErrNonFinite
+    s = oracle.Sparse(1)
+    s.add(-1, 0, 5); s.add(0, -1, 5); s.add(0, 0, 2)
+    e, x = s.solve([4])
+    assert e == 0 and x[0] == 2
+    with pytest.raises(ValueError):
+        oracle.Sparse(0)
+
+
+def test_pattern_growth():
+    s = oracle.Sparse(2)
+    s.add(0, 0, 2); s.add(1, 1, 3)
+    e, x = s.solve([2, 3])
+    assert e == 0 and x[0] == 1 and x[1] == 1
+    s.add(0, 1, 1); s.add(1, 0, -1)
+    e, x = s.solve([2, 3])
+    a = np.array([[2, 1], [-1, 3]], float)
+    xd = np.linalg.solve(a, [2, 3])
+    assert e == 0 and np.max(np.abs(x - xd)) / np.max(np.abs(xd)) <= 1e-14
+
+
+def test_fill_40x40_grid_matches_published_counts():
+    # doc/M6.md:77-78: n 1604, nnz A 7,848, L 19,880, U 21,493. The grid is
+    # diagonally dominant, so pivots (and fill) do not depend on the values.
+    rng = np.random.default_rng(5)
+    s = oracle.Sparse(40 * 40 + 4)
+    b = grid_mna(s, 40, 40, 4, rng)
+    e, _ = s.solve(b)
+    assert e == 0
+    info = s.info()
+    assert (info["nnz_a"], info["nnz_l"], info["nnz_u"]) == (7848, 19880, 21493)
+
+
+@pytest.mark.parametrize("w,h,nv", [(1, 1, 1), (2, 2, 1), (5, 1, 2), (6, 5, 3), (12, 10, 6), (25, 20, 10)])
+def test_sparse_matches_dense_on_grids(w, h, nv):
+    n = w * h + nv
+    rng = np.random.default_rng(w * 100 + h)
+    d = DenseRec(n)
+    b = grid_mna(d, w, h, nv, np.random.default_rng(7))
+    s = oracle.Sparse(n)
+    grid_mna(s, w, h, nv, np.random.default_rng(7))
+    e, xs = s.solve(b)
+    assert e == 0
+    ri, ci, v = zip(*d.entries)
+    e2, xd = oracle.dense_solve(n, ri, ci, v, b)
+    assert e2 == 0
+    xr = np.linalg.solve(d.a, b)
+    assert np.max(np.abs(xs - xd)) / np.max(np.abs(xd)) <= 1e-9
+    assert np.max(np.abs(xs - xr)) / np.max(np.abs(xr)) <= 1e-9
+
+
+def backward_error(a, x, b):
+    r = b - a @ x
+    n = len(b)
+    return np.max(np.abs(r)) / (np.max(np.abs(a)) * np.max(np.abs(x)) * n + np.max(np.abs(b)))
+
+
+def test_random_unsymmetric_backward_error():
+    rng = np.random.default_rng(7)
+    for trial in range(50):
+        n = 1 + int(rng.integers(60))
+        s = oracle.Sparse(n)
+        d = DenseRec(n)
+        for i in range(n):
+            j = int(rng.integers(n))
+            s.add(i, (i + j) % n, 1); d.add(i, (i + j) % n, 1)
+        for _ in range(3 * n):
+            i, j, v = int(rng.integers(n)), int(rng.integers(n)), rng.normal() * 0.1
+            s.add(i, j, v); d.add(i, j, v)
+        b = rng.normal(size=n)
+        e, xs = s.solve(b)
+        ri, ci, vv = zip(*d.entries)
+        ed, xd = oracle.dense_solve(n, ri, ci, vv, b)
+        assert (e == 0) == (ed == 0), (trial, e, ed)
+        if e == 0:
+            assert backward_error(d.a, xs, b) <= 1e-13
+
+
+def test_min_degree_path_closed_form():
+    # A path graph 0-1-...-(n-1): both ends have degree 1; the lowest index
+    # wins ties, so vertex 0 goes first, then the new end 1, and so on.
+    n = 9
+    ri = [i for i in range(n - 1)] + [i + 1 for i in range(n - 1)]
+    ci = [i + 1 for i in range(n - 1)] + [i for i in range(n - 1)]
+    assert list(oracle.min_degree(n, ri, ci)) == list(range(n))
+    # A star: the leaves (degree 1) go first in index order; once four are
+    # gone the hub and the last leaf both have degree 1 and the hub (index 0)
+    # wins the tie.
+    ri = [0] * 5 + [1, 2, 3, 4, 5]
+    ci = [1, 2, 3, 4, 5] + [0] * 5
+    assert list(oracle.min_degree(6, ri, ci)) == [1, 2, 3, 4, 0, 5]
+
+
+def test_pivot_rule_diagonal_preference_and_ties():
+    # 2x2 with A = [[eps, 1], [1, 0]]: column 0's best is row 1 (|1| > eps);
+    # diagonal kept only if |a00| >= 1e-3 * best.
+    for eps, want in ((2e-3, 0), (5e-4, 1)):
+        s = oracle.Sparse(2)
+        s.add(0, 0, eps); s.add(0, 1, 1); s.add(1, 0, 1)
+        e, x = s.solve([1, 2])
+        assert e == 0
+        info = s.info()
+        # ordering: both vertices have degree 1 -> q = [0, 1]
+        assert list(info["q"]) == [0, 1]
+        assert info["prow"][0] == want
+        assert np.allclose([[eps, 1], [1, 0]] @ x, [1, 2], rtol=1e-15, atol=1e-15)
+
+
+# ----------------------------------------------------------------------------- analyses
+class OracleBackend:
+    def op(self, c):
+        els, p = c.flatten()
+        st, x, info = oracle.op(c.num_nodes, els, p)
+        return st, x, info.get("current")
+
+    def tran(self, c, a):
+        els, p = c.flatten(a["step"], a["stop"])
+        spec = oracle.KhTranSpec(step=a["step"], stop=a["stop"], start=a["start"], max_step=a["max_step"],
+                                 uic=1 if a["uic"] else 0)
+        st, x, info = oracle.tran(c.num_nodes, els, p, spec, cap=200000)
+        return st, info["times"], info["xs"], info["current"], info["max_rel"]
+
+    def ac(self, c, freqs):
+        els, p = c.flatten()
+        st, X, mr = oracle.ac(c.num_nodes, els, p, freqs)
+        return st, X
+
+
+@pytest.mark.parametrize("name", gr.names())
+def test_golden_independent(name):
+    msgs = gr.check(name, OracleBackend())
+    assert not msgs, msgs[:5]
+
+
+NGSPICE = [n for n in gr.names() if __import__("os").path.exists(__import__("os").path.join(gr.REF, "ngspice", n + ".csv"))]
+
+
+@pytest.mark.parametrize("name", NGSPICE)
+def test_golden_ngspice_twin(name):
+    msgs = gr.check(name, OracleBackend(), ngspice=True)
+    assert not msgs, msgs[:5]
+
+
+def test_divider_ledger_closed_form():
+    b = kc.Builder("div").vsource("V1", "in", "0", 10).resistor("R1", "in", "out", 1e3).resistor("R2", "out", "0", 1e3)
+    c = b.build()
+    els, p = c.flatten()
+    st, x, info = oracle.op(c.num_nodes, els, p)
+    assert st == 0
+    assert abs(x[1] - 5) <= 1e-12
+    assert abs(x[2] + 5e-3) <= 1e-15  # i(V1)
+    assert abs(info["inflow"] - 0.05) <= 1e-15 and abs(info["dissipation"] - 0.05) <= 1e-15
+    assert info["ledger_rel"] <= 1e-12
+
+
+def test_voltage_loop_is_singular():
+    c = kc.Builder("loop").vsource("V1", "a", "0", 1).vsource("V2", "a", "0", 2).resistor("R1", "a", "0", 1).build()
+    els, p = c.flatten()
+    st, _, _ = oracle.op(c.num_nodes, els, p)
+    assert st == 1
+
+
+def test_no_convergence_without_fallbacks():
+    # analysis/newton_test.go:64-72: ITL1 = 2 and no fallbacks -> ErrNoConvergence.
+    c = (kc.Builder("d").vsource("V1", "a", "0", 5).resistor("R1", "a", "b", 1e3)
+         .diode("D1", "b", "0", "dm").model("dm", "d", {"is": 1e-14}).build())
+    els, p = c.flatten()
+    o = oracle.options(itl1=2, no_gmin_stepping=1, no_source_stepping=1, no_pseudo_transient=1)
+    st, _, _ = oracle.op(c.num_nodes, els, p, opts=o)
+    assert st == 4
