@@ -1,0 +1,143 @@
+"""Benchmark of the MI355X solve path: circuits solved per second.
+
+Workload (one "step" = one batched analysis over the whole batch, inputs
+already resident in HBM):
+  rectifier_rc_ladder_tran — 4096 half-wave rectifier + 32-section RC ladder
+  circuits of one topology (sine amplitude U(1,20) V, f log-U(50 Hz, 1 MHz);
+  diode IS log-U(1e-15,1e-12) A, N U(1,2); R log-U(10,10k) ohm, C log-U(1n,10u) F),
+  each run for 4 periods of its own source with max step = stop/2000:
+  trapezoidal integration with LTE step control, Newton to reltol 1e-3,
+  abstol 1e-12, the reference's DC operating point first (analysis.Tran).
+
+Prints one JSON line. `roofline` is for the dominant kernel (the factor+solve
+kernel k_lu): algorithmic bytes 8*(nnzA + 2n) per executed solve divided by
+the summed HIP-event duration of its launches. `cpu_baseline` times the FP64
+CPU restatement of the reference (oracle/, one worker thread per core, one
+task per circuit) on a bounded sample of the same batch.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "circuits solved/sec at ≤1e-9 rel err vs FP64 oracle: chosen form vs plain GPU"
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--parity-sample", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); circuits shard across GPUs with no "
+              "exchange, so N GPUs would each run this workload on their own share.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from kronspice_amd import api, workloads as W
+
+    b = W.rectifier_rc_ladders(args.batch)
+    topo = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    d_params = api.to_device(b.params)
+    d_lane = api.to_device(b.lane_spec)
+    torch.cuda.synchronize()
+
+    res = None
+    for _ in range(args.warmup):
+        res = api.tran(topo, d_params, spec, lane_spec=d_lane)
+    torch.cuda.synchronize()
+    lu_s = ctl_s = 0.0
+    lu_n = 0
+    bytes_alg = 0
+    solves = 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = api.tran(topo, d_params, spec, lane_spec=d_lane)
+        s = res.stats
+        lu_s += s["lu_seconds"]
+        ctl_s += s["control_seconds"]
+        lu_n += s["lu_launches"]
+        bytes_alg += s["bytes_algorithmic"]
+        solves += s["solves"]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = res.status.cpu().numpy()
+    acc = res.accepted.cpu().numpy()
+    x_gpu = res.x.cpu().numpy()
+    ms_per_step = dt / args.steps * 1e3
+
+    # Parity on a sample: circuits whose status is OK and whose final state is
+    # within 1e-9 of the oracle count as solved.
+    from oracle import oracle
+    ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    k = min(args.parity_sample, b.B)
+    idx = np.linspace(0, b.B - 1, k).astype(int)
+    threads = os.cpu_count() or 1
+    xo, so, ao, ro, lo = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, idx], b.n, ospec,
+                                           threads=threads, lane_spec=b.lane_spec[:, idx])
+    rel = np.max(np.abs(x_gpu[:, idx] - xo), axis=0) / np.maximum(np.max(np.abs(xo), axis=0), 1e-300)
+    ok = (st[idx] == 0) & (so == 0) & (rel <= 1e-9)
+    solved_frac = float(ok.mean())
+    value = b.B * solved_frac / (ms_per_step / 1e3)
+
+    cpu = None
+    if not args.no_cpu:
+        m = min(args.cpu_sample, b.B)
+        sidx = np.arange(m)
+        tc = time.perf_counter()
+        oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, sidx], b.n, ospec, threads=threads,
+                          lane_spec=b.lane_spec[:, sidx])
+        tcpu = time.perf_counter() - tc
+        cpu = {"value": m / tcpu, "unit": "circuits/s", "cores": threads, "kind": "port",
+               "sample": f"first {m} circuits of the batch, FP64 C++ restatement of the reference's Go path, "
+                         f"{threads} worker threads, one task per circuit ({tcpu:.2f} s)"}
+
+    achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "circuits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded counter-based generator)",
+        "config": {"workload": "rectifier_rc_ladder_tran: half-wave rectifier + 32-section RC ladder, "
+                               "4 source periods, max step stop/2000, trapezoidal + LTE, FP64 GP LU",
+                   "circuits": b.B, "unknowns": b.n, "form": "plain FP64 GPU path",
+                   "parallelism": "one lane per circuit, 1 GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_lu", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
+                     "bytes_per_launch": bytes_alg / max(lu_n, 1)},
+        "cpu_baseline": cpu,
+        "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
+                   "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),
+                   "same_step_count": float((acc[idx] == ao).mean())},
+        "detail": {"ticks": res.stats["ticks"], "solves_per_step": solves / args.steps,
+                   "control_ms_per_step": ctl_s / args.steps * 1e3, "lu_ms_per_step": lu_s / args.steps * 1e3},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

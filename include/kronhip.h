@@ -110,6 +110,9 @@ typedef struct kh_options {
   int32_t no_pseudo_transient;
   int32_t skip_energy_check;
   double ledger_tol;        /* 0 -> 1e-9 (em.DefaultTolerance) */
+  int32_t polish;           /* kh_batch_op: polish the operating point to round-off
+                               after convergence (Sim.polish, analysis/op.go:243-271) */
+  int32_t reserved;
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -134,6 +137,10 @@ typedef struct kh_stats {
   int64_t ticks;             /* device iterations launched */
   int64_t bytes_algorithmic; /* sum over executed solves of 8*(nnzA + 2n) */
   double  seconds;           /* wall time of the device loop */
+  double  lu_seconds;        /* summed duration of the factor+solve launches (HIP events) */
+  double  control_seconds;   /* summed duration of the stamp/control launches (HIP events) */
+  int64_t lu_launches;
+  int64_t control_launches;
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/
@@ -194,11 +201,15 @@ kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh
 
 /* analysis.Tran for B circuits (analysis/tran.go:70-231). Records the final
  * accepted solution per circuit (d_x, [n*B]) and the accepted / rejected step
- * counts; d_ledger (optional, [B]) the run's MaxRelativeResidual. */
+ * counts; d_ledger (optional, [B]) the run's MaxRelativeResidual.
+ * d_lane_spec (optional, device [4*B]: step, stop, max_step, min_step per
+ * circuit) gives each circuit its own time scale — a batch of candidates
+ * whose source frequencies differ by decades — as if each ran its own
+ * analysis.Tran with that TranSpec; the other fields come from spec. */
 kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
-                        const kh_tran_spec* spec, double* d_x, int32_t* d_status,
-                        int32_t* d_accepted, int32_t* d_rejected, double* d_ledger,
-                        kh_stats* stats);
+                        const kh_tran_spec* spec, const double* d_lane_spec, double* d_x,
+                        int32_t* d_status, int32_t* d_accepted, int32_t* d_rejected,
+                        double* d_ledger, kh_stats* stats);
 
 #ifdef __cplusplus
 }

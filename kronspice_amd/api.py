@@ -1,0 +1,286 @@
+"""Python front of libkronhip.so (include/kronhip.h).
+
+The host-side mirror of kronspice's solve-path interfaces:
+  * `Sparse` — solver.Matrix (solver/matrix.go:23-36) backed by the GPU:
+    Clear / Add / Solve / AbsProduct, pivots and fill as solver.Sparse.
+  * `LUBatch` — numeric Gilbert–Peierls LU of B systems sharing one pattern.
+  * `op` / `tran` — analysis.OP / analysis.Tran (analysis/op.go:331,
+    analysis/tran.go:70) for a batch of circuits of one topology, with the
+    reference's Options / TranOptions fields and its sentinel errors as
+    per-circuit status codes.
+
+Device memory and streams come from PyTorch (ROCm); every computation runs
+in the HIP library. There is no CPU fallback: when the library or the GPU is
+missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkronhip.so")
+
+# Status codes (include/kronhip.h), named as the reference's sentinel errors.
+OK, SINGULAR, NONFINITE, BADSIZE, NOCONV, TSTEP_TOO_SMALL, TOO_MANY_STEPS = 0, 1, 2, 3, 4, 5, 6
+INTERRUPTED, LEDGER, BAD_TRAN_SPEC, UNSUPPORTED, BAD_ARGUMENT, HIP_ERROR, NO_DEVICE, CAPACITY = 7, 8, 9, 10, 11, 12, 13, 14
+
+
+class KhError(RuntimeError):
+    def __init__(self, status: int, detail: str):
+        super().__init__(f"kronhip status {status}: {detail}")
+        self.status = status
+
+
+class Options(C.Structure):
+    """analysis.Options (analysis/op.go:22-62); zero values take the SPICE defaults."""
+    _fields_ = [("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double), ("gmin", C.c_double),
+                ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
+                ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
+                ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
+                ("polish", C.c_int32), ("reserved", C.c_int32)]
+
+
+class TranSpec(C.Structure):
+    """TranSpec + TranOptions (analysis/tran.go:26-59)."""
+    _fields_ = [("step", C.c_double), ("stop", C.c_double), ("start", C.c_double), ("max_step", C.c_double),
+                ("uic", C.c_int32), ("method", C.c_int32), ("chgtol", C.c_double), ("trtol", C.c_double),
+                ("min_step", C.c_double), ("max_steps", C.c_int64)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("solves", C.c_int64), ("newton_iterations", C.c_int64), ("accepted_steps", C.c_int64),
+                ("rejected_steps", C.c_int64), ("analyses", C.c_int64), ("patterns", C.c_int64),
+                ("ticks", C.c_int64), ("bytes_algorithmic", C.c_int64), ("seconds", C.c_double),
+                ("lu_seconds", C.c_double), ("control_seconds", C.c_double), ("lu_launches", C.c_int64),
+                ("control_launches", C.c_int64)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load libkronhip.so; raises if it was not built (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        P, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.kh_init.restype = i32
+        L.kh_last_error.restype = C.c_char_p
+        L.kh_status_string.restype = C.c_char_p
+        L.kh_status_string.argtypes = [i32]
+        L.kh_sys_create.argtypes = [i32, P]
+        L.kh_sys_clear.argtypes = [P]
+        L.kh_sys_add_batch.argtypes = [P, P, P, P, i64]
+        L.kh_sys_solve.argtypes = [P, P, P]
+        L.kh_sys_absproduct.argtypes = [P, P, P]
+        L.kh_sys_nnz.argtypes = [P, P, P, P]
+        L.kh_sys_pivots.argtypes = [P, P, P]
+        L.kh_sys_destroy.argtypes = [P]
+        L.kh_sys_destroy.restype = None
+        L.kh_lu_batch_create.argtypes = [i32, i64, P, P, i32, P]
+        L.kh_lu_batch_solve.argtypes = [P, P, P, P, P, P, P]
+        L.kh_lu_batch_info.argtypes = [P, P, P, P, P, P]
+        L.kh_lu_batch_destroy.argtypes = [P]
+        L.kh_lu_batch_destroy.restype = None
+        L.kh_topology_create.argtypes = [i32, P, i32, i32, P, P]
+        L.kh_topology_size.argtypes = [P]
+        L.kh_topology_destroy.argtypes = [P]
+        L.kh_topology_destroy.restype = None
+        L.kh_batch_op.argtypes = [P, i32, P, P, P, P, P, P]
+        L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
+        _lib = L
+        st = L.kh_init()
+        if st != OK:
+            raise KhError(st, L.kh_last_error().decode())
+    return _lib
+
+
+def _check(st: int):
+    if st != OK:
+        raise KhError(st, lib().kh_last_error().decode())
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(a.data_ptr())  # torch tensor on the device
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("no GPU visible to PyTorch: the solve path runs only on the GPU")
+    return torch
+
+
+def status_name(s: int) -> str:
+    return lib().kh_status_string(int(s)).decode()
+
+
+# ------------------------------------------------------------------------------ solver seam
+class Sparse:
+    """solver.Matrix on the GPU: stamps are staged on the host and cross into
+    the library once per Solve (kh_sys_add_batch)."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.h = C.c_void_p()
+        _check(lib().kh_sys_create(n, C.byref(self.h)))
+        self._i, self._j, self._v = [], [], []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_sys_destroy(self.h)
+            self.h = None
+
+    def clear(self):
+        self._flush()
+        _check(lib().kh_sys_clear(self.h))
+
+    def add(self, i: int, j: int, v: float):
+        self._i.append(i); self._j.append(j); self._v.append(v)
+
+    def _flush(self):
+        if self._i:
+            i = np.asarray(self._i, np.int32); j = np.asarray(self._j, np.int32); v = np.asarray(self._v, np.float64)
+            _check(lib().kh_sys_add_batch(self.h, _p(i), _p(j), _p(v), len(i)))
+            self._i, self._j, self._v = [], [], []
+
+    def solve(self, b) -> np.ndarray:
+        self._flush()
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n)
+        _check(lib().kh_sys_solve(self.h, _p(b), _p(x)))
+        return x
+
+    def abs_product(self, x) -> np.ndarray:
+        self._flush()
+        x = np.ascontiguousarray(x, np.float64)
+        y = np.zeros(self.n)
+        _check(lib().kh_sys_absproduct(self.h, _p(x), _p(y)))
+        return y
+
+    def nnz(self):
+        a, l, u = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().kh_sys_nnz(self.h, C.byref(a), C.byref(l), C.byref(u)))
+        return a.value, l.value, u.value
+
+    def pivots(self):
+        q = np.zeros(self.n, np.int32); prow = np.zeros(self.n, np.int32)
+        _check(lib().kh_sys_pivots(self.h, _p(q), _p(prow)))
+        return q, prow
+
+
+# ------------------------------------------------------------------------------ batched LU
+class LUBatch:
+    """B systems with one entry list (ri, ci); values SoA [nnz, B] on the device."""
+
+    def __init__(self, n: int, ri, ci, B: int):
+        self.n, self.B = n, B
+        ri = np.ascontiguousarray(ri, np.int32); ci = np.ascontiguousarray(ci, np.int32)
+        self.nnz = len(ri)
+        self.h = C.c_void_p()
+        _check(lib().kh_lu_batch_create(n, self.nnz, _p(ri), _p(ci), B, C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_lu_batch_destroy(self.h)
+            self.h = None
+
+    def solve(self, vals, b, x, status=None, prow=None, stream=None):
+        """vals [nnz, B], b and x [n, B] float64 device tensors; status [B] int32."""
+        _check(lib().kh_lu_batch_solve(self.h, _p(vals), _p(b), _p(x), _p(status), _p(prow),
+                                       C.c_void_p(stream) if stream else None))
+
+    def info(self):
+        q = np.zeros(self.n, np.int32)
+        l, u, cl, cu = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().kh_lu_batch_info(self.h, _p(q), C.byref(l), C.byref(u), C.byref(cl), C.byref(cu)))
+        return dict(q=q, nnz_l=l.value, nnz_u=u.value, cap_l=cl.value, cap_u=cu.value)
+
+
+# ------------------------------------------------------------------------------ analyses
+class Topology:
+    """One circuit topology (element table + parameter layout) registered with the library."""
+
+    def __init__(self, num_nodes: int, elements: np.ndarray, params_c0: np.ndarray):
+        self.num_nodes = num_nodes
+        self.elements = np.ascontiguousarray(elements)
+        p0 = np.ascontiguousarray(params_c0, np.float64)
+        self.nparams = len(p0)
+        self.h = C.c_void_p()
+        _check(lib().kh_topology_create(num_nodes, _p(self.elements), len(self.elements), self.nparams, _p(p0),
+                                        C.byref(self.h)))
+        self.n = lib().kh_topology_size(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_topology_destroy(self.h)
+            self.h = None
+
+
+@dataclass
+class Result:
+    x: object          # [n, B] device tensor
+    status: object     # [B] int32 device tensor
+    iters: object = None
+    accepted: object = None
+    rejected: object = None
+    ledger: object = None
+    stats: dict = field(default_factory=dict)
+
+
+def to_device(a: np.ndarray, dtype=None):
+    torch = _torch()
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    return t.to(device="cuda", dtype=dtype) if dtype is not None else t.to("cuda")
+
+
+def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = False) -> Result:
+    """analysis.OP for every circuit; params [P, B] (device tensor or numpy)."""
+    torch = _torch()
+    if isinstance(params, np.ndarray):
+        params = to_device(params)
+    B = params.shape[1]
+    x = torch.zeros((topo.n, B), dtype=torch.float64, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    it = torch.zeros(B, dtype=torch.int32, device="cuda")
+    o = Options() if opts is None else opts
+    o.polish = 1 if polish else o.polish
+    s = Stats()
+    torch.cuda.synchronize()
+    _check(lib().kh_batch_op(topo.h, B, _p(params), C.byref(o), _p(x), _p(st), _p(it), C.byref(s)))
+    return Result(x, st, iters=it, stats=s.as_dict())
+
+
+def tran(topo: Topology, params, spec: TranSpec, opts: Optional[Options] = None, lane_spec=None) -> Result:
+    """analysis.Tran for every circuit; lane_spec [4, B] gives per-circuit step, stop, max step, min step."""
+    torch = _torch()
+    if isinstance(params, np.ndarray):
+        params = to_device(params)
+    if lane_spec is not None and isinstance(lane_spec, np.ndarray):
+        lane_spec = to_device(lane_spec)
+    B = params.shape[1]
+    x = torch.zeros((topo.n, B), dtype=torch.float64, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    acc = torch.zeros(B, dtype=torch.int32, device="cuda")
+    rej = torch.zeros(B, dtype=torch.int32, device="cuda")
+    led = torch.zeros(B, dtype=torch.float64, device="cuda")
+    o = Options() if opts is None else opts
+    s = Stats()
+    torch.cuda.synchronize()
+    _check(lib().kh_batch_tran(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(lane_spec), _p(x), _p(st),
+                               _p(acc), _p(rej), _p(led), C.byref(s)))
+    return Result(x, st, accepted=acc, rejected=rej, ledger=led, stats=s.as_dict())

@@ -1,0 +1,741 @@
+// kh_control.h — the per-circuit analysis loop as a resumable state machine.
+//
+// kronspice runs Newton (analysis/newton.go:36-82) inside the DC fallback
+// chain (dcSolve, gminStepping, sourceStepping, pseudoTransient,
+// analysis/newton.go:95-311), inside OP (analysis/op.go:340-357) or the
+// transient loop (analysis/tran.go:95-231). Every step of that control flow
+// that is not a linear solve is restated here for one lane; the lane stops
+// whenever it needs a solve (after stamping the system, Sim.solve,
+// analysis/op.go:289-322) and resumes on the next launch with the solve's
+// outcome. Lanes therefore advance independently — per-circuit Newton
+// iteration counts, fallbacks, step rejections and time steps — while the
+// host only relaunches.
+#pragma once
+#include "kh_devices.h"
+
+namespace kh {
+
+// Integer control words (SoA [NCI][B]).
+enum {
+  CI_PHASE = 0, CI_CALLER, CI_NT_IT, CI_NT_MAX, CI_INITJCT, CI_WARM, CI_SRCSTEP, CI_STATUS, CI_ITERS,
+  CI_FIRST_ERR, CI_GM_OK, CI_GM_K, CI_SS_K, CI_PT_K, CI_PT_QUIET, CI_TR_ORDER, CI_TR_HIT, CI_ACCEPTED,
+  CI_REJECTED, CI_WAIT, CI_NEED_SOLVE, CI_MODE, CI_METHOD, CI_LU, CI_DC_RET, CI_PL_K, CI_PID, CI_SOLVES, NCI
+};
+// Double control words (SoA [NCD][B]).
+enum {
+  CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
+  CD_SS_F, CD_PT_H, CD_TR_T, CD_TR_H, CD_TR_TN, CD_PEAK, CD_WORST, CD_MAXREL, CD_LEDGER, CD_MINSTEP_SEEN,
+  CD_MAXSTEP_SEEN, NCD
+};
+
+enum { PH_INIT = 0, PH_RUN = 1, PH_DONE = 2 };
+enum { AN_OP = 0, AN_TRAN = 1, AN_OP_POLISH = 2 };
+enum {
+  CALLER_DC_PLAIN = 1, CALLER_GMIN, CALLER_SRC0, CALLER_SRC, CALLER_PT, CALLER_PT_FINAL, CALLER_UIC, CALLER_TRAN,
+  CALLER_POLISH
+};
+enum { DCRET_OP = 0, DCRET_TRAN = 1 };
+enum {
+  ST_OK = 0, ST_SINGULAR = 1, ST_NONFINITE = 2, ST_BADSIZE = 3, ST_NOCONV = 4, ST_TSTEP = 5, ST_MAXSTEPS = 6,
+  ST_LEDGER = 8, ST_CAPACITY = 14, ST_RUNNING = 15
+};
+
+// Analysis options and transient spec, resolved on the host (defaults of
+// analysis/op.go:91-110 and analysis/tran.go:42-59).
+struct CtlParams {
+  double relTol, absTol, vnTol, gmin, ledgerTol;
+  int32_t itl1, itl4, noGmin, noSource, noPT, skipEnergy, analysis;
+  // transient
+  double step, stop, start, maxStep, chgTol, trTol, minStep;
+  int64_t maxSteps;
+  int32_t uic, method;
+};
+
+struct Circ {
+  int32_t n, nel, nslots, numNodes;
+  const int32_t* et;        // [nel][ET_W]
+  const int32_t* isV;       // [n]
+  const int32_t* slot_i;    // [nslots] row of each slot
+  const int32_t* slot_j;    // [nslots] column
+  const int32_t* diag_slot; // [n] slot of (i, i)
+  int32_t nonlin;           // any nonlinear device
+};
+
+struct CtlArgs {
+  Circ ckt;
+  CtlParams p;
+  int64_t B;
+  const double* par;
+  double* st;
+  double* val;
+  uint8_t* present;
+  double* rhs;
+  double* x;
+  double* xold;
+  double* xsave;    // gmin / source stepping save vector
+  double* ptprev;   // pseudo-transient previous state
+  double* xacc;     // transient: last accepted solution
+  double* absbuf;   // AbsProduct scratch
+  int32_t* ci;      // [NCI][B]
+  double* cd;       // [NCD][B]
+  const double* lane_spec;   // optional [4][B]: per-circuit step, stop, max step, min step
+  const int32_t* lu_status;  // [B] result of the last factor+solve
+  int32_t* counters;         // [0] lanes still running, [1] lanes waiting for a new pattern
+};
+
+KH_HD inline bool retryable(int e) { return e == ST_NOCONV || e == ST_NONFINITE || e == ST_SINGULAR; }
+
+struct Ctl {
+  const CtlArgs& A;
+  int64_t c, B;
+  int n;
+  int32_t ci[NCI];
+  double cd[NCD];
+  Lane L;
+  CtlParams P;  // options and transient spec as seen by this circuit
+
+  KH_HD Ctl(const CtlArgs& a, int64_t lane) : A(a), c(lane), B(a.B), n(a.ckt.n) {
+    for (int k = 0; k < NCI; k++) ci[k] = a.ci[(int64_t)k * B + c];
+    for (int k = 0; k < NCD; k++) cd[k] = a.cd[(int64_t)k * B + c];
+    L.c = c; L.B = B; L.n = n;
+    L.et = a.ckt.et; L.par = a.par; L.st = a.st; L.val = a.val; L.present = a.present; L.rhs = a.rhs;
+    L.X = a.xold;
+    L.grew = false;
+    P = a.p;
+    if (a.lane_spec) {  // per-circuit time scale: step, stop, max step, min step
+      P.step = a.lane_spec[c];
+      P.stop = a.lane_spec[B + c];
+      P.maxStep = a.lane_spec[2 * B + c];
+      P.minStep = a.lane_spec[3 * B + c];
+    }
+  }
+  KH_HD void store() {
+    for (int k = 0; k < NCI; k++) A.ci[(int64_t)k * B + c] = ci[k];
+    for (int k = 0; k < NCD; k++) A.cd[(int64_t)k * B + c] = cd[k];
+  }
+  KH_HD double& X(double* v, int i) { return v[(int64_t)i * B + c]; }
+  KH_HD const int32_t* el(int k) const { return A.ckt.et + (int64_t)k * ET_W; }
+
+  KH_HD void finish(int status) {
+    ci[CI_PHASE] = PH_DONE;
+    ci[CI_STATUS] = status;
+    ci[CI_NEED_SOLVE] = 0;
+  }
+
+  // ---- device dispatch -----------------------------------------------------
+  KH_HD void load_all() {
+    L.mode = ci[CI_MODE];
+    L.method = ci[CI_METHOD];
+    L.time = cd[CD_TIME];
+    L.H = cd[CD_H];
+    L.gmin = cd[CD_GMIN];
+    L.srcFact = cd[CD_SRCFACT];
+    L.initJct = ci[CI_INITJCT] != 0;
+    L.srcStepping = ci[CI_SRCSTEP] != 0;
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      switch (e[ET_KIND]) {
+        case K_R: load_R(L, e); break;
+        case K_C: load_C(L, e); break;
+        case K_L: load_L(L, e); break;
+        case K_V: load_V(L, e); break;
+        case K_I: load_I(L, e); break;
+        case K_D: load_D(L, e); break;
+        case K_M: load_M(L, e); break;
+      }
+    }
+  }
+  KH_HD OpPt operating(const int32_t* e, const double* x) {
+    switch (e[ET_KIND]) {
+      case K_R: {
+        OpPt o;
+        o.voltage = L.volt(e, x);
+        o.current = o.voltage * L.P(e, 1);
+        o.power = o.voltage * o.current;
+        o.storage = 0;
+        return o;
+      }
+      case K_C: return op_C(L, e, x);
+      case K_L: return op_L(L, e, x);
+      case K_V: return op_V(L, e, x);
+      case K_I: return op_I(L, e, x);
+      case K_D: return op_D(L, e, x);
+      case K_M: return op_M(L, e, x);
+    }
+    OpPt o = {0, 0, 0, 0};
+    return o;
+  }
+  KH_HD void accept(double t, int mode) {  // Sim.accept, analysis/op.go:324-328
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      switch (e[ET_KIND]) {
+        case K_C: accept_C(L, e, A.x, t, mode); break;
+        case K_L: accept_L(L, e, A.x, t, mode); break;
+        case K_D: accept_D(L, e, A.x, t, mode); break;
+        case K_M: accept_M(L, e, A.x, t, mode); break;
+      }
+    }
+  }
+  KH_HD bool converged() {  // analysis/newton.go:59-82
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      if (e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
+      if (e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
+    }
+    for (int i = 0; i < n; i++) {
+      double xv = X(A.x, i), old = X(A.xold, i);
+      double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
+      tol += P.relTol * fmax(fabs(xv), fabs(old));
+      if (fabs(xv - old) > tol) return false;
+    }
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      if (e[ET_KIND] == K_D && !conv_D(L, e, A.x, P.relTol, P.absTol)) return false;
+      if (e[ET_KIND] == K_M && !conv_M(L, e, A.x, P.relTol, P.absTol)) return false;
+    }
+    return true;
+  }
+  KH_HD double next_breakpoint(double t) {  // analysis/tran.go:233-243
+    double nb = kInf;
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      if (e[ET_KIND] == K_V || e[ET_KIND] == K_I) {
+        if ((int)L.P(e, 2) == 0) continue;
+        double b = wave_next_breakpoint(L, e, t);
+        if (b < nb) nb = b;
+      }
+    }
+    return nb;
+  }
+  KH_HD double truncate(double tn, double h, int order) {
+    TruncCtx tc;
+    tc.time = tn; tc.H = h; tc.method = order;
+    tc.relTol = P.relTol; tc.absTol = P.absTol; tc.vnTol = P.vnTol; tc.chgTol = P.chgTol; tc.trTol = P.trTol;
+    double prop = kInf;
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      double pk = kInf;
+      switch (e[ET_KIND]) {
+        case K_C: pk = trunc_C(L, e, A.x, tc); break;
+        case K_L: pk = trunc_L(L, e, A.x, tc); break;
+        case K_D: pk = trunc_D(L, e, A.x, tc); break;
+        case K_M: pk = trunc_M(L, e, A.x, tc); break;
+        default: continue;
+      }
+      if (pk < prop) prop = pk;
+    }
+    return prop;
+  }
+  // roundoff, analysis/op.go:275-282: AbsProduct (solver/sparse.go:481-490)
+  // accumulated over the stamped slots in slot order, then the weighted sum.
+  KH_HD double roundoff() {
+    for (int i = 0; i < n; i++) X(A.absbuf, i) = 0;
+    for (int s = 0; s < A.ckt.nslots; s++) {
+      int64_t ix = (int64_t)s * B + c;
+      if (!A.present[ix]) continue;
+      X(A.absbuf, A.ckt.slot_i[s]) += fabs(A.val[ix]) * fabs(X(A.x, A.ckt.slot_j[s]));
+    }
+    double sum = 0;
+    for (int i = 0; i < n; i++) sum += fabs(X(A.x, i)) * (X(A.absbuf, i) + fabs(X(A.rhs, i)));
+    return 64 * 0x1p-52 * sum;
+  }
+  // Ledger of an operating point: BuildLedger + RelativeResidual
+  // (em/energy.go:90-119, output/result.go:100-109).
+  KH_HD double ledger_op() {
+    double inflow = 0, outflow = 0, diss = 0, stor = 0, sup = 0, scale = 0;
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      OpPt o = operating(e, A.x);
+      double pw = 0;
+      int ch = -1;
+      int kind = e[ET_KIND];
+      if (kind == K_R) {
+        if (L.P(e, 0) > 0) { pw = -o.power; ch = 2; }
+      } else if (kind == K_V || kind == K_I) {
+        pw = -o.power; ch = 0;
+      } else if (kind == K_D || kind == K_M) {
+        pw = -(o.power - o.storage); ch = 2;
+      }
+      if (ch >= 0 && pw != 0) {
+        scale += fabs(pw);
+        if (ch == 2 && pw < 0) diss += -pw;
+        else if (pw > 0) inflow += pw;
+        else outflow += -pw;
+      }
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M) {
+        stor += o.storage;
+        scale += fabs(o.storage);
+      }
+      if (kind == K_R && L.P(e, 0) < 0) {
+        double ms = -o.power;
+        if (ms != 0) { sup += ms; scale += fabs(ms); }
+      }
+    }
+    double res = inflow + sup - outflow - diss - stor;
+    double ro = roundoff();
+    if (scale == 0 || fabs(res) <= fmax(1e-21, ro)) return 0;
+    return fabs(res / scale);
+  }
+  // recorder.record's per-point check (analysis/tran.go:291-347).
+  KH_HD int record(double t, bool imposed) {
+    double inflow = 0, outflow = 0, diss = 0, stor = 0, sup = 0, scale = 0;
+    // ports of every element, in element order
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      int kind = e[ET_KIND];
+      double pw = 0;
+      int ch = -1;
+      if (kind == K_R || kind == K_V || kind == K_I || kind == K_D || kind == K_M) {
+        OpPt o = operating(e, A.x);
+        if (kind == K_R) { if (L.P(e, 0) > 0) { pw = -o.power; ch = 2; } }
+        else if (kind == K_V || kind == K_I) { pw = -o.power; ch = 0; }
+        else { pw = -(o.power - o.storage); ch = 2; }
+      }
+      if (ch >= 0 && pw != 0) {
+        scale += fabs(pw);
+        if (ch == 2 && pw < 0) diss += -pw;
+        else if (pw > 0) inflow += pw;
+        else outflow += -pw;
+      }
+    }
+    for (int k = 0; k < A.ckt.nel; k++) {  // model-supplied (negative resistors)
+      const int32_t* e = el(k);
+      if (e[ET_KIND] == K_R && L.P(e, 0) < 0) {
+        OpPt o = operating(e, A.x);
+        double ms = -o.power;
+        if (ms != 0) { sup += ms; scale += fabs(ms); }
+      }
+    }
+    for (int k = 0; k < A.ckt.nel; k++) {  // storage of reactive devices
+      const int32_t* e = el(k);
+      int kind = e[ET_KIND];
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M) {
+        OpPt o = operating(e, A.x);
+        stor += o.storage;
+        scale += fabs(o.storage);
+      }
+    }
+    int err = ST_OK;
+    if (scale > cd[CD_PEAK]) cd[CD_PEAK] = scale;
+    double res = fabs(inflow + sup - outflow - diss - stor);
+    if (!imposed && res > fmax(1e-21, roundoff())) {
+      if (res > cd[CD_WORST]) cd[CD_WORST] = res;
+      if (res / cd[CD_PEAK] > 1e-3) err = ST_LEDGER;
+    }
+    (void)t;
+    return err;
+  }
+
+  // ---- solve request (Sim.solve, analysis/op.go:289-322): stamp now, factor
+  // in the next LU launch.
+  KH_HD void request_solve() {
+    for (int i = 0; i < n; i++) {
+      X(A.xold, i) = X(A.x, i);
+      X(A.rhs, i) = 0;
+    }
+    for (int s = 0; s < A.ckt.nslots; s++) A.val[(int64_t)s * B + c] = 0;
+    load_all();
+    double ptG = cd[CD_PTG];
+    if (ptG > 0) {
+      for (int i = 0; i < n; i++) {
+        int s = diag_slot(i);
+        if (A.ckt.isV[i]) { add_diag(s, ptG); X(A.rhs, i) += ptG * X(A.ptprev, i); }
+        else { add_diag(s, -ptG); X(A.rhs, i) -= ptG * X(A.ptprev, i); }
+      }
+    }
+    double g = cd[CD_DIAG_GMIN];
+    if (g > 0) {
+      for (int i = 0; i < n; i++)
+        if (A.ckt.isV[i]) add_diag(diag_slot(i), g);
+    }
+    ci[CI_NEED_SOLVE] = 1;
+    if (L.grew) ci[CI_WAIT] = 1;
+  }
+  // Slot of the diagonal entry (i, i): the device slot when a device stamps
+  // it, else a slot reserved after the device slots.
+  KH_HD int diag_slot(int i) const { return A.ckt.diag_slot[i]; }
+  KH_HD void add_diag(int s, double v) {
+    int64_t ix = (int64_t)s * B + c;
+    A.val[ix] = A.val[ix] + v;
+    if (!A.present[ix]) {
+      A.present[ix] = 1;
+      L.grew = true;
+    }
+  }
+
+  KH_HD void start_newton(int maxIter, int caller) {
+    ci[CI_NT_IT] = 0;
+    ci[CI_NT_MAX] = maxIter;
+    ci[CI_CALLER] = caller;
+    cd[CD_GMIN] = P.gmin;  // s.lc.Gmin = o.Gmin, newton.go:37
+    request_solve();
+  }
+
+  // After a factor+solve: the body of the Newton loop (analysis/newton.go:38-56).
+  KH_HD void after_solve(int e) {
+    ci[CI_ITERS]++;
+    ci[CI_SOLVES]++;
+    if (ci[CI_CALLER] == CALLER_POLISH) { polish_step(e); return; }
+    if (e != ST_OK) { newton_done(e); return; }
+    if (!A.ckt.nonlin) { newton_done(ST_OK); return; }
+    ci[CI_NT_IT]++;
+    if (ci[CI_INITJCT]) {
+      ci[CI_INITJCT] = 0;
+      if (ci[CI_NT_IT] >= ci[CI_NT_MAX]) newton_done(ST_NOCONV);
+      else request_solve();
+      return;
+    }
+    if (converged()) { newton_done(ST_OK); return; }
+    if (ci[CI_NT_IT] >= ci[CI_NT_MAX]) newton_done(ST_NOCONV);
+    else request_solve();
+  }
+
+  KH_HD void newton_done(int r) {
+    switch (ci[CI_CALLER]) {
+      case CALLER_DC_PLAIN: dc_plain_done(r); break;
+      case CALLER_GMIN: gmin_done(r); break;
+      case CALLER_SRC0: src0_done(r); break;
+      case CALLER_SRC: src_done(r); break;
+      case CALLER_PT: pt_done(r); break;
+      case CALLER_PT_FINAL: pt_end(r); break;
+      case CALLER_UIC: uic_done(r); break;
+      case CALLER_TRAN: tran_newton_done(r); break;
+    }
+  }
+
+  // ---- dcSolve, analysis/newton.go:95-128 -------------------------------------
+  KH_HD void dc_begin(int ret) {
+    ci[CI_DC_RET] = ret;
+    ci[CI_INITJCT] = ci[CI_WARM] ? 0 : 1;
+    start_newton(P.itl1, CALLER_DC_PLAIN);
+  }
+  KH_HD void dc_return(int r) {
+    if (ci[CI_DC_RET] == DCRET_OP) op_after_dc(r);
+    else tran_after_dc(r);
+  }
+  KH_HD void dc_plain_done(int r) {
+    if (r == ST_OK || !A.ckt.nonlin || !retryable(r)) {
+      if (r == ST_OK) ci[CI_WARM] = 1;
+      dc_return(r);
+      return;
+    }
+    ci[CI_FIRST_ERR] = r;
+    dc_try_gmin();
+  }
+  KH_HD void dc_try_gmin() {
+    if (!P.noGmin) gmin_begin();
+    else dc_try_source();
+  }
+  KH_HD void dc_try_source() {
+    if (!P.noSource) src_begin();
+    else dc_try_pt();
+  }
+  KH_HD void dc_try_pt() {
+    if (!P.noPT) pt_begin();
+    else dc_return(ci[CI_FIRST_ERR]);
+  }
+
+  // gminStepping, analysis/newton.go:134-176
+  KH_HD void gmin_begin() {
+    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.xsave, i) = 0; }
+    ci[CI_INITJCT] = 1;
+    cd[CD_GM_G] = 1e-2;
+    cd[CD_GM_FACTOR] = 10.0;
+    ci[CI_GM_OK] = 0;
+    ci[CI_GM_K] = 0;
+    cd[CD_DIAG_GMIN] = cd[CD_GM_G];
+    start_newton(P.itl1, CALLER_GMIN);
+  }
+  KH_HD void gmin_done(int r) {
+    double g = cd[CD_GM_G], factor = cd[CD_GM_FACTOR];
+    if (r == ST_OK) {
+      for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+      if (g == 0) { gmin_end(ST_OK); return; }
+      ci[CI_GM_OK] = 1;
+      factor = fmin(factor * sqrt(factor), 10.0);
+      if (g / factor < P.gmin) g = 0;
+      else g /= factor;
+      cd[CD_GM_G] = g; cd[CD_GM_FACTOR] = factor;
+      if (++ci[CI_GM_K] >= 200) { gmin_end(ST_NOCONV); return; }
+      cd[CD_DIAG_GMIN] = g;
+      start_newton(P.itl1, CALLER_GMIN);
+      return;
+    }
+    if (!retryable(r)) { gmin_end(r); return; }
+    if (!ci[CI_GM_OK]) { gmin_end(ST_NOCONV); return; }
+    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xsave, i);
+    double prev = g * factor;
+    if (g == 0) prev = P.gmin;
+    factor = sqrt(factor);
+    if (factor < 1.00005) { gmin_end(ST_NOCONV); return; }
+    g = prev / factor;
+    cd[CD_GM_G] = g; cd[CD_GM_FACTOR] = factor;
+    if (++ci[CI_GM_K] >= 200) { gmin_end(ST_NOCONV); return; }
+    cd[CD_DIAG_GMIN] = g;
+    start_newton(P.itl1, CALLER_GMIN);
+  }
+  KH_HD void gmin_end(int r) {
+    cd[CD_DIAG_GMIN] = 0;
+    if (r == ST_OK) { ci[CI_WARM] = 1; dc_return(ST_OK); return; }
+    dc_try_source();
+  }
+
+  // sourceStepping, analysis/newton.go:180-214
+  KH_HD void src_begin() {
+    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.xsave, i) = 0; }
+    ci[CI_SRCSTEP] = 1;
+    cd[CD_SS_DONE] = 0;
+    cd[CD_SS_STEP] = 0.1;
+    cd[CD_SRCFACT] = 0;
+    start_newton(P.itl1, CALLER_SRC0);
+  }
+  KH_HD void src0_done(int r) {
+    if (r != ST_OK) { src_end(r); return; }
+    for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+    ci[CI_SS_K] = 0;
+    src_next();
+  }
+  KH_HD void src_next() {
+    double f = fmin(cd[CD_SS_DONE] + cd[CD_SS_STEP], 1.0);
+    cd[CD_SS_F] = f;
+    cd[CD_SRCFACT] = f;
+    start_newton(P.itl1, CALLER_SRC);
+  }
+  KH_HD void src_done(int r) {
+    double f = cd[CD_SS_F];
+    if (r == ST_OK) {
+      if (f == 1) { src_end(ST_OK); return; }
+      for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+      cd[CD_SS_DONE] = f;
+      cd[CD_SS_STEP] = fmin(cd[CD_SS_STEP] * 1.5, 0.5);
+      if (++ci[CI_SS_K] >= 1000) { src_end(ST_NOCONV); return; }
+      src_next();
+      return;
+    }
+    if (!retryable(r)) { src_end(r); return; }
+    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xsave, i);
+    cd[CD_SS_STEP] /= 2;
+    if (cd[CD_SS_STEP] < 1e-3) { src_end(ST_NOCONV); return; }
+    if (++ci[CI_SS_K] >= 1000) { src_end(ST_NOCONV); return; }
+    src_next();
+  }
+  KH_HD void src_end(int r) {
+    ci[CI_SRCSTEP] = 0;
+    cd[CD_SRCFACT] = 0;
+    if (r == ST_OK) { ci[CI_WARM] = 1; dc_return(ST_OK); return; }
+    dc_try_pt();
+  }
+
+  // pseudoTransient, analysis/newton.go:260-311
+  KH_HD void pt_begin() {
+    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.ptprev, i) = 0; }
+    ci[CI_INITJCT] = 1;
+    cd[CD_PT_H] = 1e-6;
+    ci[CI_PT_QUIET] = 0;
+    ci[CI_PT_K] = 0;
+    cd[CD_PTG] = 1 / cd[CD_PT_H];
+    start_newton(P.itl1, CALLER_PT);
+  }
+  KH_HD void pt_done(int r) {
+    if (r != ST_OK) {
+      if (!retryable(r)) { pt_end(r); return; }
+      for (int i = 0; i < n; i++) X(A.x, i) = X(A.ptprev, i);
+      cd[CD_PT_H] /= 4;
+      if (cd[CD_PT_H] < 1e-15) { pt_end(ST_NOCONV); return; }
+      if (++ci[CI_PT_K] >= 2000) { pt_end(ST_NOCONV); return; }
+      cd[CD_PTG] = 1 / cd[CD_PT_H];
+      start_newton(P.itl1, CALLER_PT);
+      return;
+    }
+    bool moved = false;
+    for (int i = 0; i < n; i++) {
+      double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
+      double xv = X(A.x, i);
+      if (fabs(xv - X(A.ptprev, i)) > P.relTol * fabs(xv) + tol) { moved = true; break; }
+    }
+    for (int i = 0; i < n; i++) X(A.ptprev, i) = X(A.x, i);
+    if (!moved) ci[CI_PT_QUIET]++;
+    else ci[CI_PT_QUIET] = 0;
+    if (ci[CI_PT_QUIET] >= 3 && cd[CD_PT_H] >= 1e6) {
+      cd[CD_PTG] = 0;
+      start_newton(P.itl1, CALLER_PT_FINAL);
+      return;
+    }
+    cd[CD_PT_H] = fmin(cd[CD_PT_H] * 2, 1e12);
+    if (++ci[CI_PT_K] >= 2000) { pt_end(ST_NOCONV); return; }
+    cd[CD_PTG] = 1 / cd[CD_PT_H];
+    start_newton(P.itl1, CALLER_PT);
+  }
+  KH_HD void pt_end(int r) {
+    cd[CD_PTG] = 0;
+    if (r == ST_OK) { ci[CI_WARM] = 1; dc_return(ST_OK); return; }
+    dc_return(ci[CI_FIRST_ERR]);
+  }
+
+  // ---- OP, analysis/op.go:340-357 (+ polish, :251-271) -------------------------
+  KH_HD void op_begin() {
+    ci[CI_MODE] = MODE_OP;
+    cd[CD_TIME] = 0;
+    dc_begin(DCRET_OP);
+  }
+  KH_HD void op_after_dc(int r) {
+    if (r != ST_OK) { finish(r); return; }
+    accept(0, MODE_OP);
+    double rr = ledger_op();
+    cd[CD_LEDGER] = rr;
+    if (!P.skipEnergy && rr > P.ledgerTol) { finish(ST_LEDGER); return; }
+    if (P.analysis == AN_OP_POLISH && A.ckt.nonlin) {
+      ci[CI_PL_K] = 0;
+      ci[CI_CALLER] = CALLER_POLISH;
+      request_solve();
+      return;
+    }
+    finish(ST_OK);
+  }
+  KH_HD void polish_step(int e) {
+    if (e != ST_OK) { finish(e); return; }
+    bool done = true;
+    for (int i = 0; i < n; i++) {
+      double xv = X(A.x, i);
+      if (fabs(xv - X(A.xold, i)) > 1e-12 * fmax(1.0, fabs(xv))) done = false;
+    }
+    ci[CI_PL_K]++;
+    if (done || ci[CI_PL_K] >= 2) {
+      accept(0, MODE_OP);
+      finish(ST_OK);
+      return;
+    }
+    request_solve();
+  }
+
+  // ---- transient, analysis/tran.go:95-231 --------------------------------------
+  KH_HD void tran_begin() {
+    if (P.uic) {
+      for (int k = 0; k < A.ckt.nel; k++) {  // SetInitialCondition
+        const int32_t* e = el(k);
+        if (e[ET_KIND] == K_C) { L.S(e, 3) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; L.S(e, 4) = 0; }
+        if (e[ET_KIND] == K_L) { L.S(e, 0) = 0; L.S(e, 1) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; }
+      }
+      ci[CI_MODE] = MODE_UICINIT;
+      cd[CD_TIME] = 0;
+      ci[CI_INITJCT] = 1;
+      start_newton(P.itl1, CALLER_UIC);
+      return;
+    }
+    ci[CI_MODE] = MODE_TRANOP;
+    cd[CD_TIME] = 0;
+    dc_begin(DCRET_TRAN);
+  }
+  KH_HD void uic_done(int r) {
+    if (r != ST_OK) { finish(r); return; }
+    accept(0, MODE_UICINIT);
+    record(0, true);
+    tran_init_loop();
+  }
+  KH_HD void tran_after_dc(int r) {
+    if (r != ST_OK) { finish(r); return; }
+    accept(0, MODE_TRANOP);
+    int le = record(0, false);
+    if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
+    tran_init_loop();
+  }
+  KH_HD void tran_init_loop() {
+    cd[CD_TR_T] = 0;
+    cd[CD_TR_H] = fmin(P.maxStep, fmin(P.step, P.stop / 100)) / 10;
+    ci[CI_TR_ORDER] = BE;
+    for (int i = 0; i < n; i++) X(A.xacc, i) = X(A.x, i);
+    cd[CD_MINSTEP_SEEN] = kInf;
+    cd[CD_MAXSTEP_SEEN] = 0;
+    tran_loop_top();
+  }
+  KH_HD void tran_loop_top() {
+    double t = cd[CD_TR_T], h = cd[CD_TR_H];
+    double eps = 1e-12 * P.stop;
+    if (!(t < P.stop - eps)) { tran_finish(); return; }
+    if (ci[CI_ACCEPTED] >= P.maxSteps) { finish(ST_MAXSTEPS); return; }
+    double nb = next_breakpoint(t);
+    double tn = t + h;
+    int hit = 0;
+    if (tn >= nb - eps) { tn = nb; hit = 1; }
+    if (tn >= P.stop - eps) { tn = P.stop; hit = 0; }
+    h = tn - t;
+    cd[CD_TR_H] = h;
+    cd[CD_TR_TN] = tn;
+    ci[CI_TR_HIT] = hit;
+    ci[CI_MODE] = MODE_TRAN;
+    cd[CD_TIME] = tn;
+    cd[CD_H] = h;
+    ci[CI_METHOD] = ci[CI_TR_ORDER];
+    start_newton(P.itl4, CALLER_TRAN);
+  }
+  KH_HD bool reject(double hNew) {  // tran.go:136-143; true if the step became too small
+    ci[CI_REJECTED]++;
+    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xacc, i);
+    return hNew < P.minStep;
+  }
+  KH_HD void tran_newton_done(int r) {
+    double h = cd[CD_TR_H], tn = cd[CD_TR_TN];
+    int order = ci[CI_TR_ORDER];
+    if (r != ST_OK) {
+      if (!retryable(r) || !A.ckt.nonlin) { finish(r); return; }
+      h /= 8;
+      cd[CD_TR_H] = h;
+      if (reject(h)) { finish(ST_TSTEP); return; }
+      ci[CI_TR_ORDER] = BE;
+      tran_loop_top();
+      return;
+    }
+    double proposal = truncate(tn, h, order);
+    if (proposal < 0.9 * h) {
+      h = fmax(proposal, h / 8);
+      cd[CD_TR_H] = h;
+      if (reject(h)) { finish(ST_TSTEP); return; }
+      tran_loop_top();
+      return;
+    }
+    accept(tn, MODE_TRAN);
+    int le = record(tn, false);
+    if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
+    for (int i = 0; i < n; i++) X(A.xacc, i) = X(A.x, i);
+    ci[CI_ACCEPTED]++;
+    cd[CD_MINSTEP_SEEN] = fmin(cd[CD_MINSTEP_SEEN], h);
+    cd[CD_MAXSTEP_SEEN] = fmax(cd[CD_MAXSTEP_SEEN], h);
+    cd[CD_TR_T] = tn;
+    double next = fmin(proposal, 2 * h);
+    next = fmin(next, P.maxStep);
+    ci[CI_TR_ORDER] = P.method;
+    if (ci[CI_TR_HIT]) {
+      ci[CI_TR_ORDER] = BE;
+      double nb2 = next_breakpoint(tn);
+      next = fmin(next, 0.1 * fmin(P.maxStep, nb2 - tn));
+    }
+    cd[CD_TR_H] = fmax(next, P.minStep);
+    tran_loop_top();
+  }
+  KH_HD void tran_finish() {
+    if (cd[CD_PEAK] > 0) {
+      cd[CD_MAXREL] = cd[CD_WORST] / cd[CD_PEAK];
+      if (cd[CD_MAXREL] > 1e-9 && !P.skipEnergy) { finish(ST_LEDGER); return; }
+    }
+    finish(ST_OK);
+  }
+
+  // ---- one control step of the lane ------------------------------------------
+  KH_HD void step() {
+    if (ci[CI_PHASE] == PH_DONE) return;
+    if (ci[CI_WAIT]) return;  // waiting for the host to analyze a grown pattern
+    if (ci[CI_PHASE] == PH_INIT) {
+      ci[CI_PHASE] = PH_RUN;
+      if (P.analysis == AN_TRAN) tran_begin();
+      else op_begin();
+      return;
+    }
+    if (!ci[CI_NEED_SOLVE]) return;
+    int e = A.lu_status[c];
+    ci[CI_NEED_SOLVE] = 0;
+    if (e == ST_CAPACITY) { finish(ST_CAPACITY); return; }
+    after_solve(e);
+  }
+};
+
+}  // namespace kh

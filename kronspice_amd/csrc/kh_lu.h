@@ -1,0 +1,257 @@
+// kh_lu.h — per-lane Gilbert–Peierls LU for batches of MNA systems on gfx950.
+//
+// One lane (work-item) owns one circuit. All per-circuit arrays are stored
+// structure-of-arrays with the circuit index fastest: entry i of circuit c is
+// at base[i * B + c]. Lanes of a wavefront that share a sparsity pattern and
+// a pivot sequence therefore touch the same i at the same time and every
+// access is a coalesced 64-lane transaction; lanes whose pivots differ still
+// compute correctly, only with scattered accesses.
+//
+// The numeric factorization follows kronspice's sparseLU.factor / reach /
+// solve (solver/sparse.go:199-401) operation for operation — the same reach
+// order, the same `x -= l*x_j` updates (no FMA contraction: the library is
+// built with -ffp-contract=off), true divisions, the same pivot rule with
+// column-relative singularity threshold and KLU diagonal preference — so
+// pivot rows, L/U patterns and solutions are bit-identical to the reference
+// for the same stamped values.
+#pragma once
+#include <stdint.h>
+#include <math.h>
+
+#ifndef KH_HD
+#define KH_HD __host__ __device__
+#endif
+
+namespace kh {
+
+enum LuStatus { LU_OK = 0, LU_SINGULAR = 1, LU_NONFINITE = 2, LU_CAPACITY = 14 };
+
+// Permuted column structure of one pattern (solver/sparse.go:154-187):
+// column k holds rows arow[ap[k]..ap[k+1]) (permuted ids, ascending) with
+// values at slot aslot[...] of the lane's value array.
+struct PatDesc {
+  int32_t q;      // offset of q[n] in the pattern pool
+  int32_t qinv;   // offset of qinv[n]
+  int32_t ap;     // offset of ap[n+1]
+  int32_t arow;   // offset of arow[nnz]
+  int32_t aslot;  // offset of aslot[nnz]
+  int32_t nnz;
+  int32_t pad0, pad1;
+};
+
+// Everything a lane needs to factor and solve. Pointers are device memory.
+struct LuArgs {
+  int32_t n;          // unknowns
+  int32_t nslots;     // value slots per lane
+  int64_t B;          // lanes in the batch (SoA stride)
+  int64_t capL, capU; // L and U entry capacity per lane
+  const PatDesc* pats;
+  const int32_t* pool;
+  const int32_t* pid;       // [B] pattern id per lane
+  const double* val;        // [nslots][B]
+  const double* rhs;        // [n][B]
+  double* x;                // [n][B] solution (written on success)
+  // workspace
+  double* xw;               // [n][B] dense working column
+  double* w;                // [n][B] permuted right-hand side
+  double* udiag;            // [n][B]
+  double* lx;               // [capL][B]
+  double* ux;               // [capU][B]
+  int32_t* li;              // [capL][B]
+  int32_t* ui;              // [capU][B]
+  int32_t* lp;              // [n+1][B]
+  int32_t* up;              // [n+1][B]
+  int32_t* pinv;            // [n][B]
+  int32_t* prow;            // [n][B]
+  int32_t* xi;              // [n][B]
+  int32_t* pstack;          // [n][B]
+  int32_t* mark;            // [n][B]
+  int32_t* err_step;        // [B] pivot step of a singular failure
+};
+
+KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
+
+// reach, solver/sparse.go:301-352: non-recursive DFS from every row of
+// A(:,k) through the graph of L; returns top, the reach is xi[top..n).
+// mark[r] == k + 1 means "visited while processing column k" (the reference's
+// generation counter, restarted per factorization).
+KH_HD inline int lu_reach(const LuArgs& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
+  const int64_t B = a.B;
+  const int n = a.n;
+  const int32_t gen = k + 1;
+  int top = n;
+  for (int p = ap[k]; p < ap[k + 1]; p++) {
+    int32_t r = arow[p];
+    if (a.mark[(int64_t)r * B + c] == gen) continue;
+    int head = 0;
+    a.xi[c] = r;  // stack[0] = r (xi[0..) doubles as the DFS stack)
+    while (head >= 0) {
+      int32_t v = a.xi[(int64_t)head * B + c];
+      int32_t j = a.pinv[(int64_t)v * B + c];
+      if (a.mark[(int64_t)v * B + c] != gen) {
+        a.mark[(int64_t)v * B + c] = gen;
+        if (j >= 0) a.pstack[(int64_t)head * B + c] = a.lp[(int64_t)j * B + c];
+      }
+      bool done = true;
+      if (j >= 0) {
+        int32_t end = a.lp[(int64_t)(j + 1) * B + c];
+        for (int32_t q = a.pstack[(int64_t)head * B + c]; q < end; q++) {
+          int32_t wv = a.li[(int64_t)q * B + c];
+          if (a.mark[(int64_t)wv * B + c] == gen) continue;
+          a.pstack[(int64_t)head * B + c] = q + 1;
+          head++;
+          a.xi[(int64_t)head * B + c] = wv;
+          done = false;
+          break;
+        }
+      }
+      if (done) {
+        head--;
+        top--;
+        a.xi[(int64_t)top * B + c] = v;
+      }
+    }
+  }
+  return top;
+}
+
+// factor, solver/sparse.go:199-293.
+KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
+  const int64_t B = a.B;
+  const int n = a.n;
+  const PatDesc pd = a.pats[a.pid[c]];
+  const int32_t* ap = a.pool + pd.ap;
+  const int32_t* arow = a.pool + pd.arow;
+  const int32_t* aslot = a.pool + pd.aslot;
+  double scale = 0.0;
+  for (int s = 0; s < a.nslots; s++) {
+    double v = a.val[(int64_t)s * B + c];
+    if (!kh_finite(v)) return LU_NONFINITE;
+    double av = fabs(v);
+    if (av > scale) scale = av;
+  }
+  if (scale == 0) {
+    a.err_step[c] = -1;
+    return LU_SINGULAR;
+  }
+  for (int i = 0; i < n; i++) {
+    a.pinv[(int64_t)i * B + c] = -1;
+    a.mark[(int64_t)i * B + c] = 0;
+  }
+  int64_t nl = 0, nu = 0;
+  for (int k = 0; k < n; k++) {
+    a.lp[(int64_t)k * B + c] = (int32_t)nl;
+    a.up[(int64_t)k * B + c] = (int32_t)nu;
+    int top = lu_reach(a, c, k, ap, arow);
+    for (int p = top; p < n; p++) a.xw[(int64_t)a.xi[(int64_t)p * B + c] * B + c] = 0.0;
+    double colMax = 0.0;
+    for (int p = ap[k]; p < ap[k + 1]; p++) {
+      double v = a.val[(int64_t)aslot[p] * B + c];
+      a.xw[(int64_t)arow[p] * B + c] = v;
+      double av = fabs(v);
+      if (av > colMax) colMax = av;
+    }
+    double tiny = colMax * (double)n * 1e-14;
+    // x = L \ A(:,k) in reach order.
+    for (int p = top; p < n; p++) {
+      int32_t r = a.xi[(int64_t)p * B + c];
+      int32_t j = a.pinv[(int64_t)r * B + c];
+      if (j < 0) continue;
+      double xj = a.xw[(int64_t)r * B + c];
+      if (xj == 0) continue;
+      int32_t q1 = a.lp[(int64_t)(j + 1) * B + c];
+      for (int32_t q = a.lp[(int64_t)j * B + c]; q < q1; q++) {
+        int64_t ix = (int64_t)a.li[(int64_t)q * B + c] * B + c;
+        double prod = a.lx[(int64_t)q * B + c] * xj;
+        a.xw[ix] = a.xw[ix] - prod;
+      }
+    }
+    // Pivot among the rows not yet pivotal; U entries in reach order.
+    int32_t piv = -1;
+    double best = -1.0;
+    for (int p = top; p < n; p++) {
+      int32_t r = a.xi[(int64_t)p * B + c];
+      int32_t pr = a.pinv[(int64_t)r * B + c];
+      if (pr >= 0) {
+        if (nu >= a.capU) return LU_CAPACITY;
+        a.ui[nu * B + c] = pr;
+        a.ux[nu * B + c] = a.xw[(int64_t)r * B + c];
+        nu++;
+        continue;
+      }
+      double av = fabs(a.xw[(int64_t)r * B + c]);
+      if (av > best || (av == best && r < piv)) {
+        best = av;
+        piv = r;
+      }
+    }
+    if (piv < 0 || best <= tiny) {
+      a.err_step[c] = k;
+      return LU_SINGULAR;
+    }
+    // KLU diagonal preference (solver/sparse.go:274-277).
+    if (a.pinv[(int64_t)k * B + c] < 0 && a.mark[(int64_t)k * B + c] == k + 1) {
+      double ak = fabs(a.xw[(int64_t)k * B + c]);
+      if (ak >= 1e-3 * best && ak > tiny) piv = k;
+    }
+    double pv = a.xw[(int64_t)piv * B + c];
+    a.udiag[(int64_t)k * B + c] = pv;
+    a.pinv[(int64_t)piv * B + c] = k;
+    a.prow[(int64_t)k * B + c] = piv;
+    for (int p = top; p < n; p++) {
+      int32_t r = a.xi[(int64_t)p * B + c];
+      if (a.pinv[(int64_t)r * B + c] < 0) {
+        if (nl >= a.capL) return LU_CAPACITY;
+        a.li[nl * B + c] = r;
+        a.lx[nl * B + c] = a.xw[(int64_t)r * B + c] / pv;
+        nl++;
+      }
+    }
+  }
+  a.lp[(int64_t)n * B + c] = (int32_t)nl;
+  a.up[(int64_t)n * B + c] = (int32_t)nu;
+  return LU_OK;
+}
+
+// solve, solver/sparse.go:355-401 (the factorization is held in the lane's
+// workspace). x is written in original unknown order, stopping at the first
+// non-finite entry as the reference does.
+KH_HD inline int lu_solve(const LuArgs& a, int64_t c) {
+  const int64_t B = a.B;
+  const int n = a.n;
+  const PatDesc pd = a.pats[a.pid[c]];
+  const int32_t* q = a.pool + pd.q;
+  const int32_t* qinv = a.pool + pd.qinv;
+  for (int i = 0; i < n; i++) a.w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
+  double* y = a.xw;
+  for (int k = 0; k < n; k++) {
+    double yk = a.w[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
+    y[(int64_t)k * B + c] = yk;
+    if (yk == 0) continue;
+    int32_t p1 = a.lp[(int64_t)(k + 1) * B + c];
+    for (int32_t p = a.lp[(int64_t)k * B + c]; p < p1; p++) {
+      int64_t ix = (int64_t)a.li[(int64_t)p * B + c] * B + c;
+      double prod = a.lx[(int64_t)p * B + c] * yk;
+      a.w[ix] = a.w[ix] - prod;
+    }
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    double zk = y[(int64_t)k * B + c] / a.udiag[(int64_t)k * B + c];
+    y[(int64_t)k * B + c] = zk;
+    if (zk == 0) continue;
+    int32_t p1 = a.up[(int64_t)(k + 1) * B + c];
+    for (int32_t p = a.up[(int64_t)k * B + c]; p < p1; p++) {
+      int64_t ix = (int64_t)a.ui[(int64_t)p * B + c] * B + c;
+      double prod = a.ux[(int64_t)p * B + c] * zk;
+      y[ix] = y[ix] - prod;
+    }
+  }
+  for (int k = 0; k < n; k++) {
+    double v = y[(int64_t)k * B + c];
+    if (!kh_finite(v)) return LU_NONFINITE;
+    a.x[(int64_t)q[k] * B + c] = v;
+  }
+  return LU_OK;
+}
+
+}  // namespace kh

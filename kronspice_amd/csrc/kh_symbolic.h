@@ -1,0 +1,54 @@
+// kh_symbolic.h — host-side symbolic phase shared by every circuit of a
+// topology: value-slot layout of the stamps, minimum-degree ordering and the
+// permuted column structure of each sparsity pattern.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/kronhip.h"
+#include "kh_lu.h"
+
+namespace kh {
+
+// Exact minimum degree on the off-diagonal pattern of A + A^T, ties to the
+// lowest index, eliminated vertices' neighbours joined into a clique
+// (solver/sparse.go:412-451). Produces the same order as the reference's
+// O(n^2) scan with an ordered (degree, index) set.
+std::vector<int32_t> minimum_degree(int n, const std::vector<int32_t>& ri, const std::vector<int32_t>& ci);
+
+// Structure of one pattern: ordering and permuted CSC (solver/sparse.go:154-187).
+struct Pattern {
+  std::vector<int32_t> q, qinv, ap, arow, aslot;
+  int64_t fill_estimate = 0;  // structural |L| + |U| with diagonal pivots where possible
+};
+
+// Analyze the entries (ri[s], ci[s]) for the slots s listed in `slots`;
+// aslot holds slot ids (indices into the lane's value array).
+Pattern analyze_pattern(int n, const std::vector<int32_t>& slot_i, const std::vector<int32_t>& slot_j,
+                        const std::vector<int32_t>& slots);
+
+// Pool of analyzed patterns with their device descriptors.
+struct PatternPool {
+  std::vector<PatDesc> desc;
+  std::vector<int32_t> pool;
+  std::unordered_map<std::string, int32_t> by_key;
+  int64_t max_fill = 0;
+  int32_t add(const Pattern& p, const std::string& key);
+};
+
+// Topology: the element table of include/kronhip.h turned into the device
+// element table (kinds, terminal unknowns, parameter and state offsets, slot
+// tables), with the union of every value slot any Load can stamp.
+struct Topology {
+  int32_t num_nodes = 0, n = 0, nel = 0, nparams = 0, nstate = 0;
+  std::vector<int32_t> et;         // [nel][ET_W]
+  std::vector<int32_t> slot_i, slot_j, diag_slot, isV;
+  int32_t nslots = 0;
+  bool nonlin = false;
+  std::string error;
+  bool build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0);
+};
+
+}  // namespace kh

@@ -1,0 +1,165 @@
+"""Seeded synthetic batches for the solve path's benchmark configurations.
+
+Every value is drawn from a counter-based generator (SplitMix64 of
+seed, circuit index and field index), so any circuit regenerates in
+isolation.  Log-uniform draws are exp(ln a + u (ln b - ln a)).
+
+A batch is one topology (element table + parameter layout, include/kronhip.h)
+and a parameter matrix params[P, B] (SoA: parameter-major, circuit-minor).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import circuit as kc
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """SplitMix64 finalizer on uint64 arrays (Steele, Lea, Flood 2014)."""
+    x = (x + np.uint64(0x9E3779B97F4A7C15)) & np.uint64(MASK64)
+    z = x
+    z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & np.uint64(MASK64)
+    z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & np.uint64(MASK64)
+    return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, circuits: np.ndarray, field: int) -> np.ndarray:
+    """U[0,1) for (seed, circuit, field), 53-bit resolution."""
+    with np.errstate(over="ignore"):
+        key = (np.uint64(seed & MASK64) * np.uint64(0x100000001B3)) ^ (circuits.astype(np.uint64) << np.uint64(20)) \
+            ^ np.uint64(field & 0xFFFFF)
+        r = splitmix64(splitmix64(key))
+    return (r >> np.uint64(11)).astype(np.float64) * (1.0 / (1 << 53))
+
+
+def log_uniform(seed, circuits, field, a, b):
+    u = uniform(seed, circuits, field)
+    return np.exp(math.log(a) + u * (math.log(b) - math.log(a)))
+
+
+def lin_uniform(seed, circuits, field, a, b):
+    return a + uniform(seed, circuits, field) * (b - a)
+
+
+@dataclass
+class Batch:
+    """B circuits of one topology."""
+    name: str
+    circuit: kc.Circuit          # circuit 0 (template; values of circuit 0)
+    elements: np.ndarray         # ELEMENT_DTYPE
+    params: np.ndarray           # [P, B] float64
+    n: int                       # MNA unknowns
+    spec: Optional[dict] = None  # transient spec (uniform part)
+    lane_spec: Optional[np.ndarray] = None  # [4, B]: step, stop, max_step, min_step per circuit
+
+    @property
+    def B(self) -> int:
+        return self.params.shape[1]
+
+    def circuit_params(self, c: int) -> np.ndarray:
+        return np.ascontiguousarray(self.params[:, c])
+
+
+def _template(c: kc.Circuit, B: int, step=0.0, stop=0.0):
+    els, p0 = c.flatten(step, stop)
+    params = np.repeat(p0[:, None], B, axis=1)
+    return els, params
+
+
+def _set_resistors(els, params, idx: List[int], values: np.ndarray):
+    """values [len(idx), B]: R for each listed element (R and g = 1/R)."""
+    for row, k in enumerate(idx):
+        po = els[k]["poff"]
+        params[po] = values[row]
+        params[po + 1] = 1.0 / values[row]
+
+
+# ---------------------------------------------------------------------------- config 1
+def resistor_ladders(B: int = 1024, nodes: int = 64, seed: int = 1) -> Batch:
+    """Plain DC OP: B resistor ladders of `nodes` nodes; 10 V at node 1, a shunt
+    resistor at every node and a series resistor between consecutive nodes,
+    R log-uniform in [100 ohm, 100 kohm]. MNA size nodes + 1 (one branch)."""
+    b = kc.Builder("ladder").vsource("V1", "n1", "0", 10.0)
+    ridx = []
+    for i in range(1, nodes + 1):
+        b.resistor(f"Rp{i}", f"n{i}", "0", 1e3)
+        ridx.append(len(b.elems) - 1)
+        if i < nodes:
+            b.resistor(f"Rs{i}", f"n{i}", f"n{i + 1}", 1e3)
+            ridx.append(len(b.elems) - 1)
+    c = b.build()
+    els, params = _template(c, B)
+    cs = np.arange(B)
+    vals = np.stack([log_uniform(seed, cs, f, 100.0, 1e5) for f in range(len(ridx))])
+    _set_resistors(els, params, ridx, vals)
+    _refresh(c, els, params)
+    return Batch("resistor_ladder_dc_op", c, els, params, c.layout().n)
+
+
+def _refresh(c: kc.Circuit, els, params):
+    """Copy circuit 0's drawn values back into the template circuit."""
+    for k, e in enumerate(c.elements):
+        if e.kind == kc.R:
+            e.value = float(params[els[k]["poff"], 0])
+        elif e.kind in (kc.C, kc.L):
+            e.value = float(params[els[k]["poff"], 0])
+
+
+# ---------------------------------------------------------------------------- config 2
+def rectifier_rc_ladders(B: int = 4096, sections: int = 32, seed: int = 2, periods: float = 4.0,
+                         steps: int = 2000) -> Batch:
+    """Batched nonlinear transient: 1 V-source SIN (amplitude U(1,20) V, f
+    log-U(50 Hz, 1 MHz)) -> diode (IS log-U(1e-15,1e-12) A, N U(1,2)) -> an
+    RC ladder of `sections` series-R / shunt-C sections (R log-U(10,10k) ohm,
+    C log-U(1 nF, 10 uF)). Each circuit runs `periods` periods of its own
+    frequency with max step = stop / steps (trapezoidal, LTE control)."""
+    b = kc.Builder("rectifier_rc_ladder")
+    b.vsource("V1", "in", "0", 0.0, wave=kc.Wave(kc.WAVE_SIN, (0.0, 10.0, 1e3)))
+    b.diode("D1", "in", "o0", "dr")
+    b.model("dr", "d", {"is": 1e-14, "n": 1.0})
+    ridx, cidx = [], []
+    for k in range(1, sections + 1):
+        b.resistor(f"R{k}", f"o{k - 1}", f"o{k}", 1e3)
+        ridx.append(len(b.elems) - 1)
+        b.capacitor(f"C{k}", f"o{k}", "0", 1e-6)
+        cidx.append(len(b.elems) - 1)
+    c = b.build()
+    cs = np.arange(B)
+    amp = lin_uniform(seed, cs, 0, 1.0, 20.0)
+    freq = log_uniform(seed, cs, 1, 50.0, 1e6)
+    IS = log_uniform(seed, cs, 2, 1e-15, 1e-12)
+    N = lin_uniform(seed, cs, 3, 1.0, 2.0)
+    stop = periods / freq
+    step = stop / steps
+    els, params = _template(c, B, float(step[0]), float(stop[0]))
+    # source: params 3..9 resolved SIN args (vo, va, f, td, theta, phase)
+    pv = els[0]["poff"]
+    params[pv + 3] = 0.0
+    params[pv + 4] = amp
+    params[pv + 5] = freq
+    # diode: IS, N and the derived vte, vcrit (device/diode.go:116-117)
+    pd = els[1]["poff"]
+    vte = N * kc.BOLTZMANN_K * kc.TNOM_K / kc.CHARGE_Q
+    params[pd + 0] = IS
+    params[pd + 1] = N
+    params[pd + 11] = vte
+    params[pd + 12] = vte * np.log(vte / (math.sqrt(2) * IS))
+    _set_resistors(els, params, ridx, np.stack([log_uniform(seed, cs, 10 + k, 10.0, 1e4) for k in range(sections)]))
+    for k, ci in enumerate(cidx):
+        params[els[ci]["poff"]] = log_uniform(seed, cs, 100 + k, 1e-9, 1e-5)
+    _refresh(c, els, params)
+    lane = np.stack([step, stop, step, 1e-12 * stop])
+    return Batch("rectifier_rc_ladder_tran", c, els, params, c.layout().n,
+                 spec=dict(step=float(step[0]), stop=float(stop[0]), start=0.0, max_step=float(step[0])),
+                 lane_spec=lane)
+
+
+def circuit_of(batch: Batch, c: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Element table and parameter vector of circuit c of a batch."""
+    return batch.elements, batch.circuit_params(c)

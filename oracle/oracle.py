@@ -27,7 +27,8 @@ class KhOptions(C.Structure):
     _fields_ = [("reltol", C.c_double), ("abstol", C.c_double), ("vntol", C.c_double), ("gmin", C.c_double),
                 ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
-                ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double)]
+                ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
+                ("polish", C.c_int32), ("reserved", C.c_int32)]
 
 
 class KhTranSpec(C.Structure):
@@ -81,8 +82,8 @@ def lib():
         _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
         _lib.oracle_batch_op.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int32,
                                          C.c_int32, P, P, P]
-        _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, C.c_int32,
-                                           P, P, P, P, P]
+        _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
+                                           C.c_int32, P, P, P, P, P]
     return _lib
 
 
@@ -236,7 +237,7 @@ def batch_op(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, op
 
 
 def batch_tran(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, spec: KhTranSpec, opts=None,
-               threads: int = 1):
+               threads: int = 1, lane_spec: Optional[np.ndarray] = None):
     P, B = params_soa.shape
     params_soa = np.ascontiguousarray(params_soa, np.float64)
     x = np.zeros((n, B))
@@ -245,6 +246,7 @@ def batch_tran(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, 
     rej = np.zeros(B, np.int32)
     led = np.zeros(B)
     o = opts if opts is not None else KhOptions()
-    lib().oracle_batch_tran(num_nodes, _p(els), len(els), P, B, _p(params_soa), C.byref(o), C.byref(spec), threads,
-                            _p(x), _p(st), _p(acc), _p(rej), _p(led))
+    ls = None if lane_spec is None else np.ascontiguousarray(lane_spec, np.float64)
+    lib().oracle_batch_tran(num_nodes, _p(els), len(els), P, B, _p(params_soa), C.byref(o), C.byref(spec), _p(ls),
+                            threads, _p(x), _p(st), _p(acc), _p(rej), _p(led))
     return x, st, acc, rej, led

@@ -926,18 +926,26 @@ int oracle_batch_op(int32_t num_nodes, const kh_element* el, int32_t nel, int32_
 }
 
 int oracle_batch_tran(int32_t num_nodes, const kh_element* el, int32_t nel, int32_t nparams, int32_t B,
-                      const double* params, const kh_options* ko, const kh_tran_spec* spec, int32_t threads,
-                      double* x_out, int32_t* status, int32_t* accepted, int32_t* rejected, double* ledger) {
+                      const double* params, const kh_options* ko, const kh_tran_spec* spec, const double* lane_spec,
+                      int32_t threads, double* x_out, int32_t* status, int32_t* accepted, int32_t* rejected,
+                      double* ledger) {
   run_pool_impl(B, threads, [&](int c) {
     std::vector<double> p(nparams);
     for (int k = 0; k < nparams; k++) p[k] = params[(size_t)k * B + c];
     Sim s;
-    int e = s.build(num_nodes, el, nel, p.data(), 2, spec->step, spec->stop);
+    kh_tran_spec sp = *spec;
+    if (lane_spec) {  // per-circuit step, stop, max step, min step
+      sp.step = lane_spec[c];
+      sp.stop = lane_spec[(size_t)B + c];
+      sp.max_step = lane_spec[(size_t)2 * B + c];
+      sp.min_step = lane_spec[(size_t)3 * B + c];
+    }
+    int e = s.build(num_nodes, el, nel, p.data(), 2, sp.step, sp.stop);
     Sim::TranOut out;
     if (e == E_OK) {
       Options o;
       o.from(ko);
-      e = s.tran(*spec, o, out);
+      e = s.tran(sp, o, out);
     }
     if (status) status[c] = e;
     if (accepted) accepted[c] = out.accepted;

@@ -1,0 +1,257 @@
+"""Parity of the GPU solve path (libkronhip.so, through its C ABI) with the
+FP64 oracle and with the reference's own fixtures.
+
+Bars: bit-exact pivot rows, L/U fill and solutions wherever the computation
+is +,-,*,/ only (linear circuits, the LU itself); for circuits whose device
+models call exp/log/sin (the GPU's math library differs from glibc in the
+last ulp) the same status, the same Newton iteration and time-step counts on
+at least 99 % of circuits, and node voltages / branch currents within 1e-9
+relative to the oracle's (the reference's torn-vs-untorn bound,
+analysis/backend_test.go:59-117).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from kronspice_amd import api, workloads as W
+from oracle import oracle
+from tests import golden_runner as gr
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(a, dtype=None):
+    return api.to_device(a, dtype)
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+def relerr(a, b):
+    """max |a - b| / max |b| per column (circuit)."""
+    den = np.maximum(np.max(np.abs(b), axis=0), 1e-300)
+    return np.max(np.abs(a - b), axis=0) / den
+
+
+# ---------------------------------------------------------------------------- solver seam
+def test_seam_zero_diagonal_exact():
+    s = api.Sparse(2)
+    s.add(0, 0, 1e-3); s.add(0, 1, 1); s.add(1, 0, 1)
+    x = s.solve([0, 5])
+    assert abs(x[0] - 5) <= 1e-12 and abs(x[1] + 5e-3) <= 1e-15
+
+
+@pytest.mark.parametrize("case", ["empty", "zero column", "dependent rows", "nan"])
+def test_seam_error_classes(case):
+    s = api.Sparse(3)
+    want = api.SINGULAR
+    if case == "zero column":
+        s.add(0, 0, 1); s.add(1, 0, 1); s.add(2, 2, 1)
+    elif case == "dependent rows":
+        for j in range(3):
+            s.add(0, j, 1); s.add(1, j, 2)
+        s.add(2, 2, 1)
+    elif case == "nan":
+        s.add(0, 0, float("nan")); s.add(1, 1, 1); s.add(2, 2, 1)
+        want = api.NONFINITE
+    with pytest.raises(api.KhError) as e:
+        s.solve(np.zeros(3))
+    assert e.value.status == want
+
+
+def test_seam_ground_and_growth():
+    s = api.Sparse(1)
+    s.add(-1, 0, 5); s.add(0, -1, 5); s.add(0, 0, 2)
+    assert s.solve([4])[0] == 2
+    s = api.Sparse(2)
+    s.add(0, 0, 2); s.add(1, 1, 3)
+    x = s.solve([2, 3])
+    assert x[0] == 1 and x[1] == 1
+    s.add(0, 1, 1); s.add(1, 0, -1)
+    x = s.solve([2, 3])
+    o = oracle.Sparse(2)
+    for i, j, v in [(0, 0, 2), (1, 1, 3), (0, 1, 1), (1, 0, -1)]:
+        o.add(i, j, v)
+    assert np.array_equal(x, o.solve([2, 3])[1])
+
+
+def grid(s, w, h, nv, rng):
+    nn = w * h
+    def g(a, b, v):
+        s.add(a, a, v); s.add(b, b, v); s.add(a, b, -v); s.add(b, a, -v)
+    for y in range(h):
+        for x in range(w):
+            i = y * w + x
+            if x + 1 < w:
+                g(i, i + 1, 1 / (10 + 990 * rng.random()))
+            if y + 1 < h:
+                g(i, i + w, 1 / (10 + 990 * rng.random()))
+            s.add(i, i, 1e-9)
+    b = np.zeros(nn + nv)
+    for k in range(nv):
+        node, br = (k * 7919) % nn, nn + k
+        s.add(node, br, 1); s.add(br, node, 1)
+        b[br] = 1 + rng.random()
+    return b
+
+
+def test_seam_fill_40x40_matches_reference_counts():
+    # doc/M6.md:77-78: nnz A 7,848, L 19,880, U 21,493 for gridMNA(40, 40, 4).
+    s = api.Sparse(1604)
+    b = grid(s, 40, 40, 4, np.random.default_rng(5))
+    x = s.solve(b)
+    assert s.nnz() == (7848, 19880, 21493)
+    o = oracle.Sparse(1604)
+    grid(o, 40, 40, 4, np.random.default_rng(5))
+    e, xo = o.solve(b)
+    q, prow = s.pivots()
+    info = o.info()
+    assert np.array_equal(q, info["q"]) and np.array_equal(prow, info["prow"])
+    assert np.array_equal(x, xo)
+
+
+def test_seam_random_unsymmetric_bitexact():
+    rng = np.random.default_rng(7)
+    for trial in range(20):
+        n = 1 + int(rng.integers(60))
+        s, o = api.Sparse(n), oracle.Sparse(n)
+        ents = [(i, (i + int(rng.integers(n))) % n, 1.0) for i in range(n)]
+        ents += [(int(rng.integers(n)), int(rng.integers(n)), rng.normal() * 0.1) for _ in range(3 * n)]
+        for i, j, v in ents:
+            s.add(i, j, v); o.add(i, j, v)
+        b = rng.normal(size=n)
+        e, xo = o.solve(b)
+        if e != 0:
+            with pytest.raises(api.KhError):
+                s.solve(b)
+            continue
+        x = s.solve(b)
+        assert np.array_equal(x, xo), trial
+        assert np.array_equal(s.pivots()[1], o.info()["prow"]), trial
+
+
+# ---------------------------------------------------------------------------- batched LU
+def test_lu_batch_bitexact_pivots_and_solution():
+    rng = np.random.default_rng(11)
+    n, B = 66, 512
+    # A ladder-shaped MNA pattern with a branch row (zero diagonal): node 0
+    # driven by a source, series and shunt conductances with random values.
+    ri, ci = [], []
+    def ent(i, j):
+        ri.append(i); ci.append(j)
+    nn = n - 1
+    for i in range(nn):
+        ent(i, i)
+        if i + 1 < nn:
+            ent(i, i + 1); ent(i + 1, i)
+    ent(0, nn); ent(nn, 0)
+    nnz = len(ri)
+    vals = np.zeros((nnz, B))
+    g = np.exp(np.log(1e-5) + rng.random((2 * nn, B)) * np.log(1e3))
+    k = 0
+    for idx, (i, j) in enumerate(zip(ri, ci)):
+        if i == j:
+            v = g[i] + (g[nn + i - 1] if i > 0 else 0) + (g[nn + i] if i + 1 < nn else 0)
+        elif i == nn or j == nn:
+            v = np.ones(B)
+        else:
+            v = -g[nn + min(i, j)]
+        vals[idx] = v
+    rhs = np.zeros((n, B)); rhs[nn] = 10.0
+    lu = api.LUBatch(n, ri, ci, B)
+    dv, db = dev(vals), dev(rhs)
+    import torch
+    x = torch.zeros((n, B), dtype=torch.float64, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    prow = torch.zeros((n, B), dtype=torch.int32, device="cuda")
+    lu.solve(dv, db, x, st, prow)
+    torch.cuda.synchronize()
+    xg, sg, pg = host(x), host(st), host(prow)
+    assert (sg == 0).all()
+    for c in range(0, B, 7):
+        o = oracle.Sparse(n)
+        for idx, (i, j) in enumerate(zip(ri, ci)):
+            o.add(i, j, vals[idx, c])
+        e, xo = o.solve(rhs[:, c])
+        assert e == 0
+        assert np.array_equal(xg[:, c], xo), c
+        assert np.array_equal(pg[:, c], o.info()["prow"]), c
+
+
+# ---------------------------------------------------------------------------- analyses
+@pytest.mark.parametrize("name", [n for n in gr.names() if "tran" not in n and "_ac" not in n])
+def test_op_goldens_bitexact_with_oracle(name):
+    deck = gr.load_deck(name)
+    c = deck.circuit
+    kinds = [a.kind for a in deck.analyses]
+    points = [None]
+    if "dc" in kinds:
+        a = [a for a in deck.analyses if a.kind == "dc"][0].args
+        k = c.element_index(a["source"])
+        points = list(np.arange(a["start"], a["stop"] + a["step"] / 2, a["step"]))[::4]
+    for v in points:
+        if v is not None:
+            c.elements[k].value = float(v)
+        els, p = c.flatten()
+        t = api.Topology(c.num_nodes, els, p)
+        B = 64
+        r = api.op(t, np.repeat(p[:, None], B, axis=1))
+        x, st, it = host(r.x), host(r.status), host(r.iters)
+        e0, x0, info = oracle.op(c.num_nodes, els, p)
+        assert (st == e0).all(), (v, st[0], e0)
+        assert (it == info["iters"]).all(), (v, it[0], info["iters"])
+        rel = relerr(x, np.repeat(x0[:, None], B, axis=1))
+        linear = all(e.kind not in (6, 7) for e in c.elements)
+        if linear:
+            assert (x == x0[:, None]).all(), v
+        else:
+            assert rel.max() <= 1e-9, (v, rel.max())
+
+
+@pytest.mark.parametrize("name", [n for n in gr.names() if "tran" in n or n in ("rc_step", "rl_step", "rlc_step", "rectifier_rc")])
+def test_tran_goldens_same_steps_as_oracle(name):
+    deck = gr.load_deck(name)
+    c = deck.circuit
+    a = [a for a in deck.analyses if a.kind == "tran"][0].args
+    els, p = c.flatten(a["step"], a["stop"])
+    spec = api.TranSpec(step=a["step"], stop=a["stop"], start=a["start"], max_step=a["max_step"],
+                        uic=1 if a["uic"] else 0)
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.tran(t, p[:, None], spec)
+    ospec = oracle.KhTranSpec(step=a["step"], stop=a["stop"], start=a["start"], max_step=a["max_step"],
+                              uic=1 if a["uic"] else 0)
+    e0, x0, info = oracle.tran(c.num_nodes, els, p, ospec)
+    assert int(host(r.status)[0]) == e0
+    assert int(host(r.accepted)[0]) == info["accepted"]
+    assert int(host(r.rejected)[0]) == info["rejected"]
+    x = host(r.x)[:, 0]
+    assert np.max(np.abs(x - x0)) <= 1e-9 * np.max(np.abs(x0))
+
+
+def test_ladder_batch_op_bitexact():
+    b = W.resistor_ladders(1024)
+    t = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    r = api.op(t, b.params)
+    xo, so, io = oracle.batch_op(b.circuit.num_nodes, b.elements, b.params, b.n, threads=8)
+    assert (host(r.status) == 0).all() and (so == 0).all()
+    assert np.array_equal(host(r.x), xo)
+    assert r.stats["patterns"] == 1
+
+
+def test_rectifier_batch_tran_matches_oracle():
+    b = W.rectifier_rc_ladders(256)
+    spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    t = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    r = api.tran(t, b.params, spec, lane_spec=b.lane_spec)
+    ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    xo, so, ao, ro, lo = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params, b.n, ospec, threads=8,
+                                           lane_spec=b.lane_spec)
+    st, acc, rej = host(r.status), host(r.accepted), host(r.rejected)
+    assert (st == so).all()
+    same = (acc == ao) & (rej == ro)
+    assert same.mean() >= 0.99, same.mean()
+    rel = relerr(host(r.x), xo)
+    assert rel[same].max() <= 1e-9, rel[same].max()
+    assert host(r.ledger).max() <= 1e-9
