@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=256)
     ap.add_argument("--parity-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: library default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); circuits shard across GPUs with no "
@@ -53,13 +54,14 @@ def main():
     b = W.rectifier_rc_ladders(args.batch)
     topo = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
     spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    opts = api.Options(circuits_per_wave=args.cpw)
     d_params = api.to_device(b.params)
     d_lane = api.to_device(b.lane_spec)
     torch.cuda.synchronize()
 
     res = None
     for _ in range(args.warmup):
-        res = api.tran(topo, d_params, spec, lane_spec=d_lane)
+        res = api.tran(topo, d_params, spec, opts=opts, lane_spec=d_lane)
     torch.cuda.synchronize()
     lu_s = ctl_s = 0.0
     lu_n = 0
@@ -67,7 +69,7 @@ def main():
     solves = 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = api.tran(topo, d_params, spec, lane_spec=d_lane)
+        res = api.tran(topo, d_params, spec, opts=opts, lane_spec=d_lane)
         s = res.stats
         lu_s += s["lu_seconds"]
         ctl_s += s["control_seconds"]
@@ -124,7 +126,7 @@ def main():
         "config": {"workload": "rectifier_rc_ladder_tran: half-wave rectifier + 32-section RC ladder, "
                                "4 source periods, max step stop/2000, trapezoidal + LTE, FP64 GP LU",
                    "circuits": b.B, "unknowns": b.n, "form": "plain FP64 GPU path",
-                   "parallelism": "one lane per circuit, 1 GPU"},
+                   "parallelism": "one wavefront per circuit (persistent kernel), 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "k_lu", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
@@ -133,7 +135,8 @@ def main():
         "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
                    "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),
                    "same_step_count": float((acc[idx] == ao).mean())},
-        "detail": {"ticks": res.stats["ticks"], "solves_per_step": solves / args.steps,
+        "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
+                   "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "solves_per_step": solves / args.steps,
                    "control_ms_per_step": ctl_s / args.steps * 1e3, "lu_ms_per_step": lu_s / args.steps * 1e3},
     }
     print(json.dumps(out))

@@ -112,7 +112,8 @@ typedef struct kh_options {
   double ledger_tol;        /* 0 -> 1e-9 (em.DefaultTolerance) */
   int32_t polish;           /* kh_batch_op: polish the operating point to round-off
                                after convergence (Sim.polish, analysis/op.go:243-271) */
-  int32_t reserved;
+  int32_t circuits_per_wave; /* batched analyses: circuits sharing one wavefront
+                               (1..64; 0 -> the library's choice) */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -191,6 +192,28 @@ kh_status kh_topology_create(int32_t num_nodes, const kh_element* elems, int32_t
                              int32_t nparams, const double* params_c0, kh_topology* out);
 int32_t kh_topology_size(kh_topology t);        /* number of MNA unknowns n */
 void kh_topology_destroy(kh_topology t);
+
+/* An uneven batch: circuits of several topologies in one run (one group per
+ * topology). The run uses one persistent launch for all groups; every output
+ * pointer is optional device memory: d_x [n*B], the rest [B]. */
+typedef struct kh_group {
+  kh_topology topology;
+  int32_t B;
+  int32_t reserved;
+  const double* d_params;     /* [nparams*B] */
+  const double* d_lane_spec;  /* transient only, optional [4*B] (see kh_batch_tran) */
+  double* d_x;
+  int32_t* d_status;
+  int32_t* d_iters;           /* Newton iterations (solves) per circuit */
+  int32_t* d_accepted;
+  int32_t* d_rejected;
+  double* d_ledger;           /* transient: MaxRelativeResidual per circuit */
+} kh_group;
+
+/* Runs analysis.OP (spec == NULL) or analysis.Tran (spec != NULL) on every
+ * circuit of every group. */
+kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt,
+                       const kh_tran_spec* spec, kh_stats* stats);
 
 /* analysis.OP for B circuits (analysis/op.go:331-357): dcSolve with the
  * gmin / source / pseudo-transient fallbacks (analysis/newton.go:95-311),

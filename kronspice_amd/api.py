@@ -42,7 +42,7 @@ class Options(C.Structure):
                 ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
-                ("polish", C.c_int32), ("reserved", C.c_int32)]
+                ("polish", C.c_int32), ("circuits_per_wave", C.c_int32)]
 
 
 class TranSpec(C.Structure):
@@ -72,6 +72,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        # PyTorch ships its own HIP runtime with the same soname
+        # (libamdhip64.so.7). Loading torch first makes the library bind to that
+        # runtime, so both share one device context and one set of streams.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         P, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
         L.kh_init.restype = i32
@@ -96,6 +100,7 @@ def lib():
         L.kh_topology_size.argtypes = [P]
         L.kh_topology_destroy.argtypes = [P]
         L.kh_topology_destroy.restype = None
+        L.kh_batch_run.argtypes = [i32, P, P, P, P]
         L.kh_batch_op.argtypes = [P, i32, P, P, P, P, P, P]
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         _lib = L
@@ -263,6 +268,61 @@ def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = Fa
     torch.cuda.synchronize()
     _check(lib().kh_batch_op(topo.h, B, _p(params), C.byref(o), _p(x), _p(st), _p(it), C.byref(s)))
     return Result(x, st, iters=it, stats=s.as_dict())
+
+
+class Group(C.Structure):
+    _fields_ = [("topology", C.c_void_p), ("B", C.c_int32), ("reserved", C.c_int32), ("d_params", C.c_void_p),
+                ("d_lane_spec", C.c_void_p), ("d_x", C.c_void_p), ("d_status", C.c_void_p), ("d_iters", C.c_void_p),
+                ("d_accepted", C.c_void_p), ("d_rejected", C.c_void_p), ("d_ledger", C.c_void_p)]
+
+
+def run_groups(topos, params_list, spec: Optional[TranSpec] = None, opts: Optional[Options] = None,
+               lane_specs=None, polish: bool = False):
+    """An uneven batch in one run: group g = (topos[g], params_list[g] [P_g, B_g]).
+    spec None runs analysis.OP, else analysis.Tran. Returns one Result per group
+    and the batch counters."""
+    torch = _torch()
+    n = len(topos)
+    gs = (Group * n)()
+    results, keep = [], []
+    for g in range(n):
+        prm = params_list[g]
+        if isinstance(prm, np.ndarray):
+            prm = to_device(prm)
+        B = prm.shape[1]
+        ls = None
+        if spec is not None and lane_specs is not None:
+            ls = lane_specs[g]
+            if isinstance(ls, np.ndarray):
+                ls = to_device(ls)
+        r = Result(x=torch.zeros((topos[g].n, B), dtype=torch.float64, device="cuda"),
+                   status=torch.zeros(B, dtype=torch.int32, device="cuda"),
+                   iters=torch.zeros(B, dtype=torch.int32, device="cuda"),
+                   accepted=torch.zeros(B, dtype=torch.int32, device="cuda"),
+                   rejected=torch.zeros(B, dtype=torch.int32, device="cuda"),
+                   ledger=torch.zeros(B, dtype=torch.float64, device="cuda"))
+        keep += [prm, ls]
+        gs[g].topology = topos[g].h
+        gs[g].B = B
+        gs[g].d_params = prm.data_ptr()
+        gs[g].d_lane_spec = ls.data_ptr() if ls is not None else None
+        gs[g].d_x = r.x.data_ptr()
+        gs[g].d_status = r.status.data_ptr()
+        gs[g].d_iters = r.iters.data_ptr()
+        gs[g].d_accepted = r.accepted.data_ptr()
+        gs[g].d_rejected = r.rejected.data_ptr()
+        gs[g].d_ledger = r.ledger.data_ptr()
+        results.append(r)
+    o = Options() if opts is None else opts
+    if polish:
+        o.polish = 1
+    s = Stats()
+    torch.cuda.synchronize()
+    _check(lib().kh_batch_run(n, gs, C.byref(o), C.byref(spec) if spec is not None else None, C.byref(s)))
+    stats = s.as_dict()
+    for r in results:
+        r.stats = stats
+    return results, stats
 
 
 def tran(topo: Topology, params, spec: TranSpec, opts: Optional[Options] = None, lane_spec=None) -> Result:

@@ -1,13 +1,16 @@
 // kronhip.hip — gfx950 kernels and the C ABI of include/kronhip.h.
 //
-// Kernels (one lane = one circuit, SoA with the circuit index fastest):
-//   k_control  — advances every circuit's analysis state machine up to its
-//                next linear solve and stamps that system (kh_control.h)
-//   k_lu       — factors and solves every stamped system (kh_lu.h)
-//   k_hash     — fingerprints the sparsity pattern of lanes whose stamps
-//                created new entries, for the host's symbolic phase
-// The host loop relaunches k_control / k_lu until every circuit is done and
-// runs the symbolic phase (kh_symbolic.cpp) once per distinct pattern.
+// Kernels:
+//   k_persist — each circuit alternates factor+solve (kh_lu.h) with its
+//               analysis state machine (kh_control.h) until the analysis ends
+//               or its stamps need a new ordering; one launch covers every
+//               topology of an uneven batch (per-circuit storage, kh_group.h)
+//   k_lu      — factor + solve only, one lane per system with the system index
+//               fastest (SoA): the per-system seam and the batched LU
+//   k_hash    — fingerprints the sparsity pattern of lanes whose stamps
+//               created new entries, for the host's symbolic phase
+// The host relaunches k_persist until every circuit is done and runs the
+// symbolic phase (kh_symbolic.cpp) once per distinct pattern in between.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -20,7 +23,7 @@
 #include <vector>
 
 #include "../../include/kronhip.h"
-#include "kh_control.h"
+#include "kh_group.h"
 #include "kh_symbolic.h"
 
 using namespace kh;
@@ -44,14 +47,21 @@ kh_status fail(kh_status s, const std::string& msg) {
 constexpr int kBlock = 64;  // one wavefront per workgroup: lanes run long, divergent programs
 
 // ---- kernels ----------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) k_control(CtlArgs a) {
-  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= a.B) return;
-  Ctl ctl(a, c);
-  ctl.step();
-  ctl.store();
-  if (ctl.ci[CI_PHASE] != PH_DONE) atomicAdd(&a.counters[0], 1);
-  if (ctl.ci[CI_WAIT]) atomicAdd(&a.counters[1], 1);
+// A launch covers every group of an uneven batch: workgroup w runs circuits
+// of group bgroup[w] (kh_group.h); `cpb` circuits run in each one-wavefront
+// workgroup, one per lane. With cpb = 1 every circuit has its own wavefront
+// and the many resident wavefronts hide the latency of each circuit's
+// dependent accesses.
+__global__ void __launch_bounds__(kBlock) k_persist(const Group* G, const int32_t* bgroup, int32_t max_solves,
+                                                    int32_t* counters) {
+  const Group& g = G[bgroup[blockIdx.x]];
+  if ((int)threadIdx.x >= g.cpb) return;
+  int64_t cc = (int64_t)(blockIdx.x - g.block0) * g.cpb + threadIdx.x;
+  if (cc >= g.ctl.B) return;
+  bool running, waiting;
+  run_circuit(g, cc, max_solves, &running, &waiting);
+  if (running) atomicAdd(&counters[2 * g.gid], 1);
+  if (waiting) atomicAdd(&counters[2 * g.gid + 1], 1);
 }
 
 // Factor + solve the lanes that requested a solve and have a structure.
@@ -65,19 +75,46 @@ __global__ void __launch_bounds__(kBlock) k_lu(LuArgs a, const int32_t* need, co
   status[c] = st;
 }
 
+// FNV-1a over the present slot ids of each waiting circuit (contiguous
+// per-circuit storage).
 __global__ void __launch_bounds__(kBlock) k_hash(const uint8_t* present, int32_t nslots, int64_t B,
-                                                 const int32_t* wait, uint64_t* hash) {
+                                                 const int32_t* ci, uint64_t* hash) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= B) return;
-  if (!wait[c]) return;
+  if (!ci[c * NCI + CI_WAIT]) return;
+  const uint8_t* pr = present + c * nslots;
   uint64_t h = 1469598103934665603ull;
   for (int s = 0; s < nslots; s++) {
-    if (present[(int64_t)s * B + c]) {
+    if (pr[s]) {
       h ^= (uint64_t)(s + 1);
       h *= 1099511628211ull;
     }
   }
   hash[c] = h;
+}
+
+// dst[c*rows + r] = src[r*cols + c]: SoA (caller layout) <-> per-circuit layout.
+template <typename T>
+__global__ void __launch_bounds__(256) k_transpose(const T* src, T* dst, int64_t rows, int64_t cols) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= rows * cols) return;
+  int64_t r = k / cols, c = k % cols;
+  dst[c * rows + r] = src[k];
+}
+
+// Per-circuit results into the caller's SoA arrays.
+__global__ void __launch_bounds__(256) k_outputs(const int32_t* ci, const double* cd, int64_t B, int32_t* status,
+                                                 int32_t* iters, int32_t* acc, int32_t* rej, double* ledger,
+                                                 int32_t* solves) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B) return;
+  const int32_t* w = ci + c * NCI;
+  if (status) status[c] = w[CI_STATUS];
+  if (iters) iters[c] = w[CI_ITERS];
+  if (acc) acc[c] = w[CI_ACCEPTED];
+  if (rej) rej[c] = w[CI_REJECTED];
+  if (ledger) ledger[c] = cd[c * NCD + CD_MAXREL];
+  if (solves) solves[c] = w[CI_SOLVES];
 }
 
 // ---- device buffers -----------------------------------------------------------------
@@ -450,25 +487,23 @@ void kh_topology_destroy(kh_topology t) { delete t; }
 // ---- the batch driver -----------------------------------------------------------------------
 namespace {
 
-struct BatchRun {
-  kh_topology_s* topo;
-  int64_t B;
-  int n;
-  DBuf st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, counters, pid, hash;
+// Device state of the circuits of one topology, stored per circuit.
+struct GroupRun {
+  kh_topology_s* topo = nullptr;
+  int64_t B = 0;
+  int n = 0;
+  int32_t gid = 0;
+  DBuf par, lane, st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, pid, hash, solves;
   DevPatterns pats;
   LuWork work;
-  int32_t* hcount = nullptr;
-  const double* lane_spec = nullptr;
+  bool has_lane = false;
   kh_stats stats = {};
 
-  ~BatchRun() {
-    if (hcount) (void)hipHostFree(hcount);
-  }
-
-  kh_status setup() {
+  kh_status setup(const double* d_params, const double* d_lane_spec) {
     const Topology& T = topo->t;
     n = T.n;
     size_t nd = (size_t)n * B * 8;
+    KH_HIP(par.alloc((size_t)T.nparams * B * 8));
     KH_HIP(st.alloc((size_t)std::max(T.nstate, 1) * B * 8));
     KH_HIP(val.alloc((size_t)T.nslots * B * 8));
     KH_HIP(present.alloc((size_t)T.nslots * B));
@@ -477,14 +512,24 @@ struct BatchRun {
     KH_HIP(ci.alloc((size_t)NCI * B * 4));
     KH_HIP(cd.alloc((size_t)NCD * B * 8));
     KH_HIP(lustatus.alloc((size_t)B * 4));
-    KH_HIP(counters.alloc(16));
     KH_HIP(pid.alloc((size_t)B * 4));
     KH_HIP(hash.alloc((size_t)B * 8));
-    KH_HIP(hipHostMalloc((void**)&hcount, 16, hipHostMallocDefault));
+    KH_HIP(solves.alloc((size_t)B * 4));
+    int64_t tot = (int64_t)T.nparams * B;
+    hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, d_params,
+                       par.as<double>(), (int64_t)T.nparams, B);
+    KH_HIP(hipGetLastError());
+    has_lane = d_lane_spec != nullptr;
+    if (has_lane) {
+      KH_HIP(lane.alloc((size_t)4 * B * 8));
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((4 * B + 255) / 256)), dim3(256), 0, 0, d_lane_spec,
+                         lane.as<double>(), (int64_t)4, B);
+      KH_HIP(hipGetLastError());
+    }
     return KH_OK;
   }
 
-  CtlArgs ctl_args(const double* d_params, const CtlParams& p) const {
+  CtlArgs ctl_args(const CtlParams& p) const {
     const Topology& T = topo->t;
     CtlArgs a = {};
     a.ckt.n = T.n; a.ckt.nel = T.nel; a.ckt.nslots = T.nslots; a.ckt.numNodes = T.num_nodes;
@@ -494,15 +539,15 @@ struct BatchRun {
     a.ckt.nonlin = T.nonlin ? 1 : 0;
     a.p = p;
     a.B = B;
-    a.par = d_params;
+    a.par = par.as<double>();
     a.st = st.as<double>(); a.val = val.as<double>(); a.present = present.as<uint8_t>();
     a.rhs = rhs.as<double>(); a.x = x.as<double>(); a.xold = xold.as<double>();
     a.xsave = xsave.as<double>(); a.ptprev = ptprev.as<double>(); a.xacc = xacc.as<double>();
     a.absbuf = absbuf.as<double>();
     a.ci = ci.as<int32_t>(); a.cd = cd.as<double>();
     a.lu_status = lustatus.as<int32_t>();
-    a.counters = counters.as<int32_t>();
-    a.lane_spec = lane_spec;
+    a.counters = nullptr;
+    a.lane_spec = has_lane ? lane.as<double>() : nullptr;
     return a;
   }
 
@@ -522,16 +567,17 @@ struct BatchRun {
     return a;
   }
 
-  // Symbolic phase for lanes whose stamps grew their pattern.
+  // Symbolic phase for circuits whose stamps grew their pattern: one ordering
+  // per distinct pattern, shared by every circuit that has it.
   kh_status new_patterns() {
     const Topology& T = topo->t;
-    int32_t* wait_row = ci.as<int32_t>() + (size_t)CI_WAIT * B;
-    hipLaunchKernelGGL(k_hash, dim3(grid_for(B)), dim3(kBlock), 0, 0, present.as<uint8_t>(), T.nslots, B, wait_row,
-                       hash.as<uint64_t>());
+    hipLaunchKernelGGL(k_hash, dim3(grid_for(B)), dim3(kBlock), 0, 0, present.as<uint8_t>(), T.nslots, B,
+                       ci.as<int32_t>(), hash.as<uint64_t>());
     KH_HIP(hipGetLastError());
     std::vector<int32_t> wait(B), hpid(B);
     std::vector<uint64_t> hh(B);
-    KH_HIP(hipMemcpy(wait.data(), wait_row, (size_t)B * 4, hipMemcpyDeviceToHost));
+    int32_t* wcol = ci.as<int32_t>() + CI_WAIT;
+    KH_HIP(hipMemcpy2D(wait.data(), 4, wcol, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
     KH_HIP(hipMemcpy(hh.data(), hash.p, (size_t)B * 8, hipMemcpyDeviceToHost));
     KH_HIP(hipMemcpy(hpid.data(), pid.p, (size_t)B * 4, hipMemcpyDeviceToHost));
     std::unordered_map<uint64_t, int32_t> by_hash;
@@ -543,7 +589,7 @@ struct BatchRun {
         hpid[c] = it->second;
         continue;
       }
-      KH_HIP(hipMemcpy2D(col.data(), 1, present.as<uint8_t>() + c, (size_t)B, 1, T.nslots, hipMemcpyDeviceToHost));
+      KH_HIP(hipMemcpy(col.data(), present.as<uint8_t>() + c * T.nslots, T.nslots, hipMemcpyDeviceToHost));
       std::vector<int32_t> slots;
       for (int s = 0; s < T.nslots; s++)
         if (col[s]) slots.push_back(s);
@@ -559,99 +605,135 @@ struct BatchRun {
       }
       by_hash[hh[c]] = id;
       hpid[c] = id;
-      wait[c] = 0;
     }
     for (int64_t c = 0; c < B; c++) wait[c] = 0;
     stats.patterns = (int64_t)pats.host.desc.size();
     KH_HIP(pats.sync());
     KH_HIP(work.ensure(T.n, B, std::max(work.capL, lu_capacity(pats.host.max_fill, T.n))));
     KH_HIP(hipMemcpy(pid.p, hpid.data(), (size_t)B * 4, hipMemcpyHostToDevice));
-    KH_HIP(hipMemcpy(wait_row, wait.data(), (size_t)B * 4, hipMemcpyHostToDevice));
+    KH_HIP(hipMemcpy2D(wcol, (size_t)NCI * 4, wait.data(), 4, 4, B, hipMemcpyHostToDevice));
     return KH_OK;
   }
 
-  kh_status run(const double* d_params, const CtlParams& p) {
-    CtlArgs ca = ctl_args(d_params, p);
-    // Kernel durations from HIP events on the launch stream (stream 0).
-    hipEvent_t ev[4];
+  // Per-circuit results (caller's SoA layout) and batch counters.
+  kh_status outputs(double* d_x, int32_t* d_status, int32_t* d_iters, int32_t* d_acc, int32_t* d_rej,
+                    double* d_ledger) {
+    if (d_x) {
+      int64_t tot = (int64_t)n * B;
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, x.as<double>(),
+                         d_x, B, (int64_t)n);
+      KH_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_outputs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, ci.as<int32_t>(),
+                       cd.as<double>(), B, d_status, d_iters, d_acc, d_rej, d_ledger, solves.as<int32_t>());
+    KH_HIP(hipGetLastError());
+    std::vector<int32_t> w(NCI * B);
+    KH_HIP(hipMemcpy(w.data(), ci.p, (size_t)NCI * B * 4, hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < B; c++) {
+      const int32_t* v = &w[c * NCI];
+      stats.newton_iterations += v[CI_ITERS];
+      stats.accepted_steps += v[CI_ACCEPTED];
+      stats.rejected_steps += v[CI_REJECTED];
+      stats.solves += v[CI_SOLVES];
+    }
+    // Algorithmic bytes per executed solve: matrix values and right-hand side
+    // read once, solution written once, 8*(nnzA + 2n), plus the device
+    // parameters read by the stamp, 8*P (largest pattern of the group:
+    // patterns only grow).
+    int64_t nnz = 0;
+    for (const auto& d : pats.host.desc) nnz = std::max<int64_t>(nnz, d.nnz);
+    stats.bytes_algorithmic = stats.solves * 8 * (nnz + 2 * (int64_t)n + topo->t.nparams);
+    return KH_OK;
+  }
+};
+
+// Runs every group to completion with persistent launches.
+struct Runner {
+  std::vector<GroupRun*> groups;
+  DBuf dgroups, dbgroup, counters;
+  int32_t* hcount = nullptr;
+  kh_stats stats = {};
+  int32_t max_solves = 1 << 20;
+  int32_t cpb = 1;
+
+  ~Runner() {
+    if (hcount) (void)hipHostFree(hcount);
+  }
+
+  kh_status run(const CtlParams& p) {
+    size_t ng = groups.size();
+    std::vector<int32_t> bg;
+    std::vector<Group> hg(ng);
+    for (size_t g = 0; g < ng; g++) {
+      hg[g].block0 = (int64_t)bg.size();
+      hg[g].gid = (int32_t)g;
+      hg[g].cpb = cpb;
+      hg[g].npar = groups[g]->topo->t.nparams;
+      hg[g].nstate = std::max(groups[g]->topo->t.nstate, 1);
+      int64_t nb = (groups[g]->B + cpb - 1) / cpb;
+      for (int64_t k = 0; k < nb; k++) bg.push_back((int32_t)g);
+    }
+    KH_HIP(dbgroup.alloc(bg.size() * 4));
+    KH_HIP(hipMemcpy(dbgroup.p, bg.data(), bg.size() * 4, hipMemcpyHostToDevice));
+    KH_HIP(dgroups.alloc(ng * sizeof(Group)));
+    KH_HIP(counters.alloc(ng * 8));
+    KH_HIP(hipHostMalloc((void**)&hcount, ng * 8, hipHostMallocDefault));
+    hipEvent_t ev[2];
     for (auto& e : ev) KH_HIP(hipEventCreate(&e));
     struct EvGuard {
       hipEvent_t* e;
-      ~EvGuard() { for (int k = 0; k < 4; k++) (void)hipEventDestroy(e[k]); }
+      ~EvGuard() { for (int k = 0; k < 2; k++) (void)hipEventDestroy(e[k]); }
     } guard{ev};
-    bool lu_pending = false;
     auto t0 = std::chrono::steady_clock::now();
+    bool dirty = true;
     for (;;) {
-      KH_HIP(hipMemsetAsync(counters.p, 0, 16, 0));
+      if (dirty) {  // group descriptors: LU workspace pointers change when capacity grows
+        for (size_t g = 0; g < ng; g++) {
+          hg[g].ctl = groups[g]->ctl_args(p);
+          hg[g].lu = groups[g]->lu_args();
+        }
+        KH_HIP(hipMemcpy(dgroups.p, hg.data(), ng * sizeof(Group), hipMemcpyHostToDevice));
+        dirty = false;
+      }
+      KH_HIP(hipMemsetAsync(counters.p, 0, ng * 8, 0));
       KH_HIP(hipEventRecord(ev[0], 0));
-      hipLaunchKernelGGL(k_control, dim3(grid_for(B)), dim3(kBlock), 0, 0, ca);
+      hipLaunchKernelGGL(k_persist, dim3((unsigned)bg.size()), dim3(kBlock), 0, 0, dgroups.as<Group>(),
+                         dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>());
       KH_HIP(hipGetLastError());
       KH_HIP(hipEventRecord(ev[1], 0));
-      KH_HIP(hipMemcpyAsync(hcount, counters.p, 8, hipMemcpyDeviceToHost, 0));
+      KH_HIP(hipMemcpyAsync(hcount, counters.p, ng * 8, hipMemcpyDeviceToHost, 0));
       KH_HIP(hipStreamSynchronize(0));
       float ms = 0;
       KH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      stats.control_seconds += ms * 1e-3;
-      stats.control_launches++;
-      if (lu_pending) {
-        KH_HIP(hipEventElapsedTime(&ms, ev[2], ev[3]));
-        stats.lu_seconds += ms * 1e-3;
-        stats.lu_launches++;
-        lu_pending = false;
-      }
-      int32_t running = hcount[0], waiting = hcount[1];
-      if (running == 0) break;
-      if (waiting > 0) {
-        kh_status s = new_patterns();
-        if (s != KH_OK) return s;
-      }
-      LuArgs la = lu_args();
-      KH_HIP(hipEventRecord(ev[2], 0));
-      hipLaunchKernelGGL(k_lu, dim3(grid_for(B)), dim3(kBlock), 0, 0, la,
-                         ci.as<int32_t>() + (size_t)CI_NEED_SOLVE * B, ci.as<int32_t>() + (size_t)CI_WAIT * B,
-                         lustatus.as<int32_t>());
-      KH_HIP(hipGetLastError());
-      KH_HIP(hipEventRecord(ev[3], 0));
-      lu_pending = true;
+      stats.lu_seconds += ms * 1e-3;
+      stats.lu_launches++;
       stats.ticks++;
+      int64_t running = 0;
+      for (size_t g = 0; g < ng; g++) {
+        running += hcount[2 * g];
+        if (hcount[2 * g + 1] > 0) {
+          kh_status s = groups[g]->new_patterns();
+          if (s != KH_OK) return s;
+          dirty = true;
+        }
+      }
+      if (running == 0) break;
     }
     KH_HIP(hipDeviceSynchronize());
     stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return KH_OK;
   }
 
-  kh_status outputs(double* d_x, int32_t* d_status, int32_t* d_iters, int32_t* d_acc, int32_t* d_rej,
-                    double* d_ledger, kh_stats* out) {
-    if (d_x) KH_HIP(hipMemcpy(d_x, x.p, (size_t)n * B * 8, hipMemcpyDeviceToDevice));
-    const int32_t* cip = ci.as<int32_t>();
-    if (d_status) KH_HIP(hipMemcpy(d_status, cip + (size_t)CI_STATUS * B, (size_t)B * 4, hipMemcpyDeviceToDevice));
-    if (d_iters) KH_HIP(hipMemcpy(d_iters, cip + (size_t)CI_ITERS * B, (size_t)B * 4, hipMemcpyDeviceToDevice));
-    if (d_acc) KH_HIP(hipMemcpy(d_acc, cip + (size_t)CI_ACCEPTED * B, (size_t)B * 4, hipMemcpyDeviceToDevice));
-    if (d_rej) KH_HIP(hipMemcpy(d_rej, cip + (size_t)CI_REJECTED * B, (size_t)B * 4, hipMemcpyDeviceToDevice));
-    if (d_ledger) {
-      const double* cdp = cd.as<double>();
-      KH_HIP(hipMemcpy(d_ledger, cdp + (size_t)CD_MAXREL * B, (size_t)B * 8, hipMemcpyDeviceToDevice));
+  void total() {
+    for (GroupRun* g : groups) {
+      stats.solves += g->stats.solves;
+      stats.newton_iterations += g->stats.newton_iterations;
+      stats.accepted_steps += g->stats.accepted_steps;
+      stats.rejected_steps += g->stats.rejected_steps;
+      stats.analyses += g->stats.analyses;
+      stats.patterns += g->stats.patterns;
+      stats.bytes_algorithmic += g->stats.bytes_algorithmic;
     }
-    if (out) {
-      std::vector<int32_t> it(B), acc(B), rej(B), sol(B);
-      KH_HIP(hipMemcpy(it.data(), cip + (size_t)CI_ITERS * B, (size_t)B * 4, hipMemcpyDeviceToHost));
-      KH_HIP(hipMemcpy(acc.data(), cip + (size_t)CI_ACCEPTED * B, (size_t)B * 4, hipMemcpyDeviceToHost));
-      KH_HIP(hipMemcpy(rej.data(), cip + (size_t)CI_REJECTED * B, (size_t)B * 4, hipMemcpyDeviceToHost));
-      KH_HIP(hipMemcpy(sol.data(), cip + (size_t)CI_SOLVES * B, (size_t)B * 4, hipMemcpyDeviceToHost));
-      for (int64_t c = 0; c < B; c++) {
-        stats.newton_iterations += it[c];
-        stats.accepted_steps += acc[c];
-        stats.rejected_steps += rej[c];
-        stats.solves += sol[c];
-      }
-      // Algorithmic bytes per executed solve: 8*(nnzA + 2n), with nnzA the
-      // largest pattern's entry count (patterns only grow).
-      int64_t nnz = 0;
-      for (const auto& d : pats.host.desc) nnz = std::max<int64_t>(nnz, d.nnz);
-      stats.bytes_algorithmic = stats.solves * 8 * (nnz + 2 * (int64_t)n);
-      *out = stats;
-    }
-    return KH_OK;
   }
 };
 
@@ -672,41 +754,11 @@ CtlParams resolve(const kh_options* o, int analysis) {
   return p;
 }
 
-}  // namespace
-
-extern "C" {
-
-kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, double* d_x,
-                      int32_t* d_status, int32_t* d_iters, kh_stats* stats) {
-  if (!t || B <= 0 || !d_params) return fail(KH_BAD_ARGUMENT, "bad argument");
-  std::lock_guard<std::mutex> lk(t->mu);
-  BatchRun r;
-  r.topo = t;
-  r.B = B;
-  kh_status s = r.setup();
-  if (s != KH_OK) return s;
-  // polish (analysis/op.go:251-271) when the caller asks for round-off exact points
-  CtlParams p = resolve(opt, (opt && opt->polish) ? AN_OP_POLISH : AN_OP);
-  s = r.run(d_params, p);
-  if (s != KH_OK) return s;
-  return r.outputs(d_x, d_status, d_iters, nullptr, nullptr, nullptr, stats);
-}
-
-kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
-                        const kh_tran_spec* spec, const double* d_lane_spec, double* d_x, int32_t* d_status,
-                        int32_t* d_accepted, int32_t* d_rejected, double* d_ledger, kh_stats* stats) {
-  if (!t || B <= 0 || !d_params || !spec) return fail(KH_BAD_ARGUMENT, "bad argument");
+kh_status tran_params(const kh_tran_spec* spec, CtlParams& p) {
   // Tran's spec validation, analysis/tran.go:71-74
   if (!(spec->step > 0) || !(spec->stop > 0) || spec->start < 0 || spec->start >= spec->stop || spec->max_step < 0 ||
       std::isnan(spec->step + spec->stop + spec->start + spec->max_step) || std::isinf(spec->stop))
     return fail(KH_BAD_TRAN_SPEC, "invalid transient specification");
-  std::lock_guard<std::mutex> lk(t->mu);
-  BatchRun r;
-  r.topo = t;
-  r.B = B;
-  kh_status s = r.setup();
-  if (s != KH_OK) return s;
-  CtlParams p = resolve(opt, AN_TRAN);
   p.step = spec->step;
   p.stop = spec->stop;
   p.start = spec->start;
@@ -717,10 +769,69 @@ kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const 
   p.maxSteps = spec->max_steps > 0 ? spec->max_steps : 1000000;
   p.uic = spec->uic;
   p.method = spec->method == 1 ? BE : TRAP;
-  r.lane_spec = d_lane_spec;
-  s = r.run(d_params, p);
+  return KH_OK;
+}
+
+std::mutex g_run_mu;  // batch entry points run one at a time per process
+
+}  // namespace
+
+extern "C" {
+
+kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const kh_tran_spec* spec,
+                       kh_stats* stats) {
+  if (ngroups <= 0 || !groups) return fail(KH_BAD_ARGUMENT, "no groups");
+  CtlParams p = resolve(opt, spec ? AN_TRAN : ((opt && opt->polish) ? AN_OP_POLISH : AN_OP));
+  if (spec) {
+    kh_status s = tran_params(spec, p);
+    if (s != KH_OK) return s;
+  }
+  int32_t cpb = (opt && opt->circuits_per_wave > 0) ? opt->circuits_per_wave : 1;
+  if (cpb > kBlock) return fail(KH_BAD_ARGUMENT, "circuits_per_wave above 64");
+  std::lock_guard<std::mutex> lk(g_run_mu);
+  std::vector<std::unique_ptr<GroupRun>> runs;
+  Runner R;
+  R.cpb = cpb;
+  for (int32_t g = 0; g < ngroups; g++) {
+    const kh_group& G = groups[g];
+    if (!G.topology || G.B <= 0 || !G.d_params) return fail(KH_BAD_ARGUMENT, "bad group " + std::to_string(g));
+    auto r = std::make_unique<GroupRun>();
+    r->topo = G.topology;
+    r->B = G.B;
+    r->gid = g;
+    kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr);
+    if (s != KH_OK) return s;
+    R.groups.push_back(r.get());
+    runs.push_back(std::move(r));
+  }
+  kh_status s = R.run(p);
   if (s != KH_OK) return s;
-  return r.outputs(d_x, d_status, nullptr, d_accepted, d_rejected, d_ledger, stats);
+  for (int32_t g = 0; g < ngroups; g++) {
+    const kh_group& G = groups[g];
+    s = runs[g]->outputs(G.d_x, G.d_status, G.d_iters, G.d_accepted, G.d_rejected, G.d_ledger);
+    if (s != KH_OK) return s;
+  }
+  KH_HIP(hipDeviceSynchronize());
+  R.total();
+  if (stats) *stats = R.stats;
+  return KH_OK;
+}
+
+kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, double* d_x,
+                      int32_t* d_status, int32_t* d_iters, kh_stats* stats) {
+  kh_group g = {};
+  g.topology = t; g.B = B; g.d_params = d_params; g.d_x = d_x; g.d_status = d_status; g.d_iters = d_iters;
+  return kh_batch_run(1, &g, opt, nullptr, stats);
+}
+
+kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                        const kh_tran_spec* spec, const double* d_lane_spec, double* d_x, int32_t* d_status,
+                        int32_t* d_accepted, int32_t* d_rejected, double* d_ledger, kh_stats* stats) {
+  if (!spec) return fail(KH_BAD_ARGUMENT, "no transient spec");
+  kh_group g = {};
+  g.topology = t; g.B = B; g.d_params = d_params; g.d_lane_spec = d_lane_spec; g.d_x = d_x; g.d_status = d_status;
+  g.d_accepted = d_accepted; g.d_rejected = d_rejected; g.d_ledger = d_ledger;
+  return kh_batch_run(1, &g, opt, spec, stats);
 }
 
 }  // extern "C"

@@ -160,6 +160,100 @@ def rectifier_rc_ladders(B: int = 4096, sections: int = 32, seed: int = 2, perio
                  lane_spec=lane)
 
 
+def mosfet_chain_pattern(p: int, seed: int = 3) -> Tuple[int, List[int]]:
+    """Stage count and per-stage RC-ladder lengths of pattern p: target node
+    count log-uniform in [20, 2000], 1-50 stages of >= 3 nodes each."""
+    pa = np.array([p])
+    nodes = int(round(float(log_uniform(seed, pa, 0, 20.0, 2000.0)[0])))
+    smax = max(1, min(50, (nodes - 2) // 3))
+    stages = 1 + int(uniform(seed, pa, 1)[0] * smax)
+    stages = min(stages, smax)
+    rest = max(0, nodes - 2 - 2 * stages)
+    base, extra = divmod(rest, stages)
+    perm = np.argsort(uniform(seed, np.arange(stages) + 1000 * p, 2))
+    ladders = [base + (1 if k in set(perm[:extra].tolist()) else 0) for k in range(stages)]
+    return stages, ladders
+
+
+def mosfet_chain_batch(p: int, B: int, seed: int = 3, steps: int = 500, first_circuit: int = 0) -> Batch:
+    """One sparsity pattern of the uneven MOSFET-chain workload: a chain of
+    common-source level-1 NMOS stages (KP log-U(2e-5,2e-4), VTO U(0.4,1.0) V,
+    W/L U(1,100), LAMBDA 0.02), each biased by a resistive divider from a 5 V
+    supply, AC-coupled from the previous stage, with a drain load resistor and
+    capacitor and an RC ladder (R log-U(100,10k) ohm, C log-U(1p,1n) F) after
+    the drain. The input is a small sine (U(10,100) mV, f log-U(1k,100k) Hz);
+    each circuit runs the DC operating point and 2 input periods with max step
+    stop/steps."""
+    stages, ladders = mosfet_chain_pattern(p, seed)
+    b = kc.Builder(f"mos_chain_{p}")
+    b.vsource("VDD", "vdd", "0", 5.0)
+    b.vsource("VIN", "in", "0", 0.0, wave=kc.Wave(kc.WAVE_SIN, (0.0, 0.05, 1e4)))
+    idx: Dict[str, List[int]] = {k: [] for k in ("cc", "r1", "r2", "rd", "m", "cd", "rl", "cl")}
+    prev = "in"
+    for k in range(stages):
+        g, d = f"g{k}", f"d{k}"
+        b.capacitor(f"CC{k}", prev, g, 1e-7); idx["cc"].append(len(b.elems) - 1)
+        b.resistor(f"RA{k}", "vdd", g, 1e5); idx["r1"].append(len(b.elems) - 1)
+        b.resistor(f"RB{k}", g, "0", 1e5); idx["r2"].append(len(b.elems) - 1)
+        b.resistor(f"RD{k}", "vdd", d, 1e4); idx["rd"].append(len(b.elems) - 1)
+        b.mosfet(f"M{k}", d, g, "0", "0", "nch", {"l": 2e-6, "w": 20e-6}); idx["m"].append(len(b.elems) - 1)
+        b.capacitor(f"CD{k}", d, "0", 1e-11); idx["cd"].append(len(b.elems) - 1)
+        node = d
+        for j in range(ladders[k]):
+            nxt = f"l{k}_{j}"
+            b.resistor(f"RL{k}_{j}", node, nxt, 1e3); idx["rl"].append(len(b.elems) - 1)
+            b.capacitor(f"CL{k}_{j}", nxt, "0", 1e-10); idx["cl"].append(len(b.elems) - 1)
+            node = nxt
+        prev = node
+    b.model("nch", "nmos", {"level": 1, "kp": 1e-4, "vto": 0.7, "lambda": 0.02})
+    c = b.build()
+    cs = np.arange(first_circuit, first_circuit + B) + 100000 * p
+    f = log_uniform(seed, cs, 1, 1e3, 1e5)
+    stop = 2.0 / f
+    step = stop / steps
+    els, params = _template(c, B, float(step[0]), float(stop[0]))
+    pv = els[1]["poff"]
+    params[pv + 4] = lin_uniform(seed, cs, 0, 0.01, 0.1)
+    params[pv + 5] = f
+    fld = 10
+    vdd = 5.0
+    for k in range(stages):
+        kp = log_uniform(seed, cs, fld, 2e-5, 2e-4)
+        vto = lin_uniform(seed, cs, fld + 1, 0.4, 1.0)
+        wl = lin_uniform(seed, cs, fld + 2, 1.0, 100.0)
+        beta = kp * wl
+        pm = els[idx["m"][k]]["poff"]
+        params[pm + 1] = beta
+        params[pm + 2] = vto
+        params[pm + 3] = vto
+        vg = vto + lin_uniform(seed, cs, fld + 3, 0.1, 0.8)
+        rtot = log_uniform(seed, cs, fld + 4, 2e4, 2e6)
+        r2 = rtot * vg / vdd
+        _set_resistors(els, params, [idx["r1"][k], idx["r2"][k]], np.stack([rtot - r2, r2]))
+        idd = 0.5 * beta * (vg - vto) ** 2
+        vd = lin_uniform(seed, cs, fld + 5, 1.5, 3.5)
+        rd = np.clip((vdd - vd) / idd, 100.0, 1e6)
+        _set_resistors(els, params, [idx["rd"][k]], rd[None, :])
+        params[els[idx["cc"][k]]["poff"]] = log_uniform(seed, cs, fld + 6, 1e-8, 1e-6)
+        params[els[idx["cd"][k]]["poff"]] = log_uniform(seed, cs, fld + 7, 1e-12, 1e-10)
+        fld += 8
+    for j, e in enumerate(idx["rl"]):
+        _set_resistors(els, params, [e], log_uniform(seed, cs, 1000 + j, 100.0, 1e4)[None, :])
+    for j, e in enumerate(idx["cl"]):
+        params[els[e]["poff"]] = log_uniform(seed, cs, 5000 + j, 1e-12, 1e-9)
+    _refresh(c, els, params)
+    lane = np.stack([step, stop, step, 1e-12 * stop])
+    return Batch(f"mos_chain_{p}", c, els, params, c.layout().n,
+                 spec=dict(step=float(step[0]), stop=float(stop[0]), start=0.0, max_step=float(step[0])),
+                 lane_spec=lane)
+
+
+def mosfet_chains(B: int = 16384, patterns: int = 200, seed: int = 3, steps: int = 500) -> List[Batch]:
+    """The uneven-size workload: B circuits over `patterns` distinct topologies."""
+    per = [B // patterns + (1 if p < B % patterns else 0) for p in range(patterns)]
+    return [mosfet_chain_batch(p, per[p], seed, steps) for p in range(patterns) if per[p] > 0]
+
+
 def circuit_of(batch: Batch, c: int) -> Tuple[np.ndarray, np.ndarray]:
     """Element table and parameter vector of circuit c of a batch."""
     return batch.elements, batch.circuit_params(c)
