@@ -142,6 +142,8 @@ typedef struct kh_stats {
   double  control_seconds;   /* summed duration of the stamp/control launches (HIP events) */
   int64_t lu_launches;
   int64_t control_launches;
+  double  cycles_lu;         /* summed over circuits: shader clock cycles spent in factor+solve */
+  double  cycles_control;    /* ... and in the rest of the analysis (stamping, Newton, LTE, ledger) */
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/

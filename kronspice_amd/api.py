@@ -57,7 +57,7 @@ class Stats(C.Structure):
                 ("rejected_steps", C.c_int64), ("analyses", C.c_int64), ("patterns", C.c_int64),
                 ("ticks", C.c_int64), ("bytes_algorithmic", C.c_int64), ("seconds", C.c_double),
                 ("lu_seconds", C.c_double), ("control_seconds", C.c_double), ("lu_launches", C.c_int64),
-                ("control_launches", C.c_int64)]
+                ("control_launches", C.c_int64), ("cycles_lu", C.c_double), ("cycles_control", C.c_double)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -25,7 +25,7 @@ enum {
 enum {
   CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
   CD_SS_F, CD_PT_H, CD_TR_T, CD_TR_H, CD_TR_TN, CD_PEAK, CD_WORST, CD_MAXREL, CD_LEDGER, CD_MINSTEP_SEEN,
-  CD_MAXSTEP_SEEN, NCD
+  CD_MAXSTEP_SEEN, CD_CYC_LU, CD_CYC_CTL, NCD
 };
 
 enum { PH_INIT = 0, PH_RUN = 1, PH_DONE = 2 };

@@ -7,6 +7,12 @@
 #pragma once
 #include "kh_control.h"
 
+// Cycle counter for the per-circuit split of time between factorization and
+// the rest of the analysis (reported in kh_stats); 0 where none is provided.
+#ifndef KH_CLOCK
+#define KH_CLOCK() ((uint64_t)0)
+#endif
+
 namespace kh {
 
 struct Group {
@@ -45,6 +51,7 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   l.lp += c * (n + 1); l.up += c * (n + 1);
   l.pinv += c * n; l.prow += c * n; l.xi += c * n; l.pstack += c * n; l.mark += c * n;
   l.err_step += c;
+  l.rch += c * l.capR; l.rp += c * (n + 1); l.dk += c * n; l.cpid += c;
 }
 
 // One circuit's persistent loop: factor + solve the pending system, advance
@@ -57,19 +64,26 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
   Ctl ctl(a, 0);
   int32_t* lus = const_cast<int32_t*>(a.lu_status);
   int solves = 0;
+  uint64_t cyc_lu = 0, cyc_ctl = 0;
   for (;;) {
     if (ctl.ci[CI_PHASE] == PH_DONE || ctl.ci[CI_WAIT]) break;
     if (ctl.ci[CI_NEED_SOLVE]) {
       if (solves >= max_solves) break;
+      uint64_t t0 = KH_CLOCK();
       int st = lu_factor(la, 0);
       if (st == LU_OK) st = lu_solve(la, 0);
       lus[0] = st;
       solves++;
+      cyc_lu += KH_CLOCK() - t0;
     } else if (ctl.ci[CI_PHASE] != PH_INIT) {
       break;  // nothing pending (every path ends in a solve request or finish)
     }
+    uint64_t t1 = KH_CLOCK();
     ctl.step();
+    cyc_ctl += KH_CLOCK() - t1;
   }
+  ctl.cd[CD_CYC_LU] += (double)cyc_lu;
+  ctl.cd[CD_CYC_CTL] += (double)cyc_ctl;
   ctl.store();
   *running = ctl.ci[CI_PHASE] != PH_DONE;
   *waiting = ctl.ci[CI_WAIT] != 0;

@@ -67,6 +67,12 @@ struct LuArgs {
   int32_t* pstack;          // [n][B]
   int32_t* mark;            // [n][B]
   int32_t* err_step;        // [B] pivot step of a singular failure
+  // reach cache of the previous factorization (see lu_factor)
+  int64_t capR;             // entries of rch per lane (>= |L| + |U| + n)
+  int32_t* rch;             // [capR][B] reach lists of all columns, concatenated
+  int32_t* rp;              // [n+1][B] start of column k's reach in rch
+  int32_t* dk;              // [n][B] 1 if row k is in the reach of column k
+  int32_t* cpid;            // [B] pattern id the cache belongs to, -1: none
 };
 
 KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
@@ -116,10 +122,20 @@ KH_HD inline int lu_reach(const LuArgs& a, int64_t c, int k, const int32_t* ap, 
 }
 
 // factor, solver/sparse.go:199-293.
+//
+// Reach reuse: the reach of column k (its order and membership) depends only
+// on the pattern and on the pivot rows of columns < k. A circuit refactored
+// with the same pattern and, so far, the same pivot rows as its previous
+// factorization has the same reach lists, so they are replayed from the
+// per-circuit cache rch/rp/dk instead of re-running the depth-first search.
+// The pivot rule is still evaluated at every column exactly as the reference
+// does; at the first column whose pivot differs, the search resumes. Results
+// are identical to re-running the search every time.
 KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
   const int64_t B = a.B;
   const int n = a.n;
-  const PatDesc pd = a.pats[a.pid[c]];
+  const int32_t pidc = a.pid[c];
+  const PatDesc pd = a.pats[pidc];
   const int32_t* ap = a.pool + pd.ap;
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
@@ -134,16 +150,31 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
     a.err_step[c] = -1;
     return LU_SINGULAR;
   }
-  for (int i = 0; i < n; i++) {
-    a.pinv[(int64_t)i * B + c] = -1;
-    a.mark[(int64_t)i * B + c] = 0;
-  }
-  int64_t nl = 0, nu = 0;
+  bool reuse = a.cpid[c] == pidc;
+  a.cpid[c] = -1;  // the cache is valid again only once this factorization completes
+  for (int i = 0; i < n; i++) a.pinv[(int64_t)i * B + c] = -1;
+  if (!reuse)
+    for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
+  int64_t nl = 0, nu = 0, nr = 0;
   for (int k = 0; k < n; k++) {
     a.lp[(int64_t)k * B + c] = (int32_t)nl;
     a.up[(int64_t)k * B + c] = (int32_t)nu;
-    int top = lu_reach(a, c, k, ap, arow);
-    for (int p = top; p < n; p++) a.xw[(int64_t)a.xi[(int64_t)p * B + c] * B + c] = 0.0;
+    int64_t r0 = nr, r1;
+    bool diag_in_reach;
+    if (reuse) {
+      r1 = a.rp[(int64_t)(k + 1) * B + c];
+      diag_in_reach = a.dk[(int64_t)k * B + c] != 0;
+    } else {
+      int top = lu_reach(a, c, k, ap, arow);
+      r1 = r0 + (n - top);
+      if (r1 > a.capR) return LU_CAPACITY;
+      for (int p = top; p < n; p++) a.rch[(r0 + p - top) * B + c] = a.xi[(int64_t)p * B + c];
+      diag_in_reach = a.mark[(int64_t)k * B + c] == k + 1;
+      a.rp[(int64_t)(k + 1) * B + c] = (int32_t)r1;
+      a.dk[(int64_t)k * B + c] = diag_in_reach ? 1 : 0;
+    }
+    nr = r1;
+    for (int64_t p = r0; p < r1; p++) a.xw[(int64_t)a.rch[p * B + c] * B + c] = 0.0;
     double colMax = 0.0;
     for (int p = ap[k]; p < ap[k + 1]; p++) {
       double v = a.val[(int64_t)aslot[p] * B + c];
@@ -153,8 +184,8 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
     }
     double tiny = colMax * (double)n * 1e-14;
     // x = L \ A(:,k) in reach order.
-    for (int p = top; p < n; p++) {
-      int32_t r = a.xi[(int64_t)p * B + c];
+    for (int64_t p = r0; p < r1; p++) {
+      int32_t r = a.rch[p * B + c];
       int32_t j = a.pinv[(int64_t)r * B + c];
       if (j < 0) continue;
       double xj = a.xw[(int64_t)r * B + c];
@@ -169,8 +200,8 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
     // Pivot among the rows not yet pivotal; U entries in reach order.
     int32_t piv = -1;
     double best = -1.0;
-    for (int p = top; p < n; p++) {
-      int32_t r = a.xi[(int64_t)p * B + c];
+    for (int64_t p = r0; p < r1; p++) {
+      int32_t r = a.rch[p * B + c];
       int32_t pr = a.pinv[(int64_t)r * B + c];
       if (pr >= 0) {
         if (nu >= a.capU) return LU_CAPACITY;
@@ -190,16 +221,21 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
       return LU_SINGULAR;
     }
     // KLU diagonal preference (solver/sparse.go:274-277).
-    if (a.pinv[(int64_t)k * B + c] < 0 && a.mark[(int64_t)k * B + c] == k + 1) {
+    if (a.pinv[(int64_t)k * B + c] < 0 && diag_in_reach) {
       double ak = fabs(a.xw[(int64_t)k * B + c]);
       if (ak >= 1e-3 * best && ak > tiny) piv = k;
+    }
+    if (reuse && piv != a.prow[(int64_t)k * B + c]) {
+      // Pivot row differs from the cached factorization: later reaches change.
+      reuse = false;
+      for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
     }
     double pv = a.xw[(int64_t)piv * B + c];
     a.udiag[(int64_t)k * B + c] = pv;
     a.pinv[(int64_t)piv * B + c] = k;
     a.prow[(int64_t)k * B + c] = piv;
-    for (int p = top; p < n; p++) {
-      int32_t r = a.xi[(int64_t)p * B + c];
+    for (int64_t p = r0; p < r1; p++) {
+      int32_t r = a.rch[p * B + c];
       if (a.pinv[(int64_t)r * B + c] < 0) {
         if (nl >= a.capL) return LU_CAPACITY;
         a.li[nl * B + c] = r;
@@ -210,6 +246,8 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
   }
   a.lp[(int64_t)n * B + c] = (int32_t)nl;
   a.up[(int64_t)n * B + c] = (int32_t)nu;
+  a.rp[c] = 0;
+  a.cpid[c] = pidc;
   return LU_OK;
 }
 

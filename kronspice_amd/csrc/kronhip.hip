@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/kronhip.h"
+#define KH_CLOCK() ((uint64_t)clock64())
 #include "kh_group.h"
 #include "kh_symbolic.h"
 
@@ -143,14 +144,15 @@ struct DBuf {
 
 // LU workspace for B lanes of n unknowns with capL / capU entries.
 struct LuWork {
-  DBuf xw, w, udiag, lx, ux, li, ui, lp, up, pinv, prow, xi, pstack, mark, err;
-  int64_t capL = 0, capU = 0, B = 0;
+  DBuf xw, w, udiag, lx, ux, li, ui, lp, up, pinv, prow, xi, pstack, mark, err, rch, rp, dk, cpid;
+  int64_t capL = 0, capU = 0, capR = 0, B = 0;
   int n = 0;
   hipError_t ensure(int n_, int64_t B_, int64_t cap) {
     if (n_ == n && B_ == B && cap <= capL && xw.p) return hipSuccess;
     n = n_;
     B = B_;
     capL = capU = cap;
+    capR = capL + capU + n;
     size_t nd = (size_t)n * B * 8, ni = (size_t)n * B * 4;
     hipError_t e;
 #define KH_A(buf, bytes_) if ((e = buf.alloc(bytes_)) != hipSuccess) return e
@@ -160,11 +162,16 @@ struct LuWork {
     KH_A(lp, (size_t)(n + 1) * B * 4); KH_A(up, (size_t)(n + 1) * B * 4);
     KH_A(pinv, ni); KH_A(prow, ni); KH_A(xi, ni); KH_A(pstack, ni); KH_A(mark, ni);
     KH_A(err, (size_t)B * 4);
+    KH_A(rch, (size_t)capR * B * 4); KH_A(rp, (size_t)(n + 1) * B * 4); KH_A(dk, ni);
+    KH_A(cpid, (size_t)B * 4);
 #undef KH_A
-    return hipSuccess;
+    return invalidate();
   }
+  // Forget the cached reach lists (a pattern id now names a different pattern).
+  hipError_t invalidate() { return hipMemset(cpid.p, 0xFF, (size_t)B * 4); }  // -1: no cached reach
   void fill(LuArgs& a) const {
-    a.capL = capL; a.capU = capU;
+    a.capL = capL; a.capU = capU; a.capR = capR;
+    a.rch = rch.as<int32_t>(); a.rp = rp.as<int32_t>(); a.dk = dk.as<int32_t>(); a.cpid = cpid.as<int32_t>();
     a.xw = xw.as<double>(); a.w = w.as<double>(); a.udiag = udiag.as<double>();
     a.lx = lx.as<double>(); a.ux = ux.as<double>(); a.li = li.as<int32_t>(); a.ui = ui.as<int32_t>();
     a.lp = lp.as<int32_t>(); a.up = up.as<int32_t>(); a.pinv = pinv.as<int32_t>(); a.prow = prow.as<int32_t>();
@@ -323,6 +330,7 @@ static kh_status sys_prepare(kh_sys s) {
     s->pats.host.add(s->pat, slots_key(slots));
     KH_HIP(s->pats.sync());
     KH_HIP(s->work.ensure(s->n, 1, lu_capacity(s->pat.fill_estimate, s->n)));
+    KH_HIP(s->work.invalidate());  // pattern id 0 is reused for the new pattern
     KH_HIP(s->dval.alloc(std::max<size_t>(1, s->val.size()) * 8));
     s->dval_slots = s->val.size();
     KH_HIP(s->drhs.alloc((size_t)s->n * 8));
@@ -636,6 +644,12 @@ struct GroupRun {
       stats.rejected_steps += v[CI_REJECTED];
       stats.solves += v[CI_SOLVES];
     }
+    std::vector<double> dv(NCD * B);
+    KH_HIP(hipMemcpy(dv.data(), cd.p, (size_t)NCD * B * 8, hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < B; c++) {
+      stats.cycles_lu += dv[c * NCD + CD_CYC_LU];
+      stats.cycles_control += dv[c * NCD + CD_CYC_CTL];
+    }
     // Algorithmic bytes per executed solve: matrix values and right-hand side
     // read once, solution written once, 8*(nnzA + 2n), plus the device
     // parameters read by the stamp, 8*P (largest pattern of the group:
@@ -653,7 +667,11 @@ struct Runner {
   DBuf dgroups, dbgroup, counters;
   int32_t* hcount = nullptr;
   kh_stats stats = {};
-  int32_t max_solves = 1 << 20;
+  // Solves per circuit per launch. Launches are kept short (kLaunchTargetMs) so
+  // that no single dispatch occupies the queue for long: the budget starts
+  // small and adapts to the measured launch duration.
+  int32_t max_solves = 4;
+  static constexpr float kLaunchTargetMs = 25.0f;
   int32_t cpb = 1;
 
   ~Runner() {
@@ -708,6 +726,8 @@ struct Runner {
       stats.lu_seconds += ms * 1e-3;
       stats.lu_launches++;
       stats.ticks++;
+      if (ms < 0.5f * kLaunchTargetMs && max_solves < (1 << 14)) max_solves *= 2;
+      else if (ms > 2.0f * kLaunchTargetMs && max_solves > 1) max_solves /= 2;
       int64_t running = 0;
       for (size_t g = 0; g < ng; g++) {
         running += hcount[2 * g];
@@ -733,6 +753,8 @@ struct Runner {
       stats.analyses += g->stats.analyses;
       stats.patterns += g->stats.patterns;
       stats.bytes_algorithmic += g->stats.bytes_algorithmic;
+      stats.cycles_lu += g->stats.cycles_lu;
+      stats.cycles_control += g->stats.cycles_control;
     }
   }
 };
