@@ -9,9 +9,12 @@ already resident in HBM):
   trapezoidal integration with LTE step control, Newton to reltol 1e-3,
   abstol 1e-12, the reference's DC operating point first (analysis.Tran).
 
-Prints one JSON line. `roofline` is for the dominant kernel (the factor+solve
-kernel k_lu): algorithmic bytes 8*(nnzA + 2n) per executed solve divided by
-the summed HIP-event duration of its launches. `cpu_baseline` times the FP64
+Prints one JSON line. `roofline` is for the dominant kernel (k_persist, the
+persistent factor+solve+stamp kernel; every other kernel of the run is a
+layout change or a fingerprint at the run's edges): algorithmic bytes
+8*(nnzA + 2n + P) per executed solve, summed over the run, divided by the
+summed HIP-event duration of its launches (events recorded on the stream the
+library launches on). `cpu_baseline` times the FP64
 CPU restatement of the reference (oracle/, one worker thread per core, one
 task per circuit) on a bounded sample of the same batch.
 """
@@ -38,10 +41,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--parity-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: library default)")
+    ap.add_argument("--cpw", type=int, default=1, help="circuits per wavefront")
+    ap.add_argument("--staging", type=int, default=0, help="0: circuits worked on in place in HBM")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); circuits shard across GPUs with no "
@@ -54,7 +58,7 @@ def main():
     b = W.rectifier_rc_ladders(args.batch)
     topo = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
     spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
-    opts = api.Options(circuits_per_wave=args.cpw)
+    opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging)
     d_params = api.to_device(b.params)
     d_lane = api.to_device(b.lane_spec)
     torch.cuda.synchronize()
@@ -89,7 +93,7 @@ def main():
     ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
     k = min(args.parity_sample, b.B)
     idx = np.linspace(0, b.B - 1, k).astype(int)
-    threads = os.cpu_count() or 1
+    threads = min(16, os.cpu_count() or 1)  # the host cores one GPU's share of the machine has
     xo, so, ao, ro, lo = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, idx], b.n, ospec,
                                            threads=threads, lane_spec=b.lane_spec[:, idx])
     rel = np.max(np.abs(x_gpu[:, idx] - xo), axis=0) / np.maximum(np.max(np.abs(xo), axis=0), 1e-300)
@@ -129,14 +133,16 @@ def main():
                    "parallelism": "one wavefront per circuit (persistent kernel), 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_lu", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
+                     "kernel": "k_persist", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
                      "bytes_per_launch": bytes_alg / max(lu_n, 1)},
         "cpu_baseline": cpu,
         "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
                    "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),
                    "same_step_count": float((acc[idx] == ao).mean())},
         "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
-                   "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "solves_per_step": solves / args.steps,
+                   "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "staging": args.staging,
+                   "solves_per_step": solves / args.steps,
+                   "cycles_lu_fraction": res.stats["cycles_lu"] / max(res.stats["cycles_lu"] + res.stats["cycles_control"], 1.0),
                    "control_ms_per_step": ctl_s / args.steps * 1e3, "lu_ms_per_step": lu_s / args.steps * 1e3},
     }
     print(json.dumps(out))

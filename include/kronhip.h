@@ -113,7 +113,11 @@ typedef struct kh_options {
   int32_t polish;           /* kh_batch_op: polish the operating point to round-off
                                after convergence (Sim.polish, analysis/op.go:243-271) */
   int32_t circuits_per_wave; /* batched analyses: circuits sharing one wavefront
-                               (1..64; 0 -> the library's choice) */
+                               (0 or 1; other values -> KH_UNSUPPORTED for now) */
+  int32_t staging;          /* batched analyses: 0 -> circuits are worked on in place in
+                               device memory; 1 (each circuit's working set held in LDS
+                               during a launch) -> KH_UNSUPPORTED for now */
+  int32_t reserved;
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -210,6 +214,10 @@ typedef struct kh_group {
   int32_t* d_accepted;
   int32_t* d_rejected;
   double* d_ledger;           /* transient: MaxRelativeResidual per circuit */
+  const double* d_nodeset;    /* optional [(numNodes-1)*B]: Newton's starting node voltages
+                                 per circuit, node k at row k-1 (Options.NodeSet,
+                                 analysis/op.go:39-43, applied as analysis/newton.go:221-243:
+                                 the other unknowns start at 0, InitJct off) */
 } kh_group;
 
 /* Runs analysis.OP (spec == NULL) or analysis.Tran (spec != NULL) on every

@@ -42,7 +42,8 @@ class Options(C.Structure):
                 ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
-                ("polish", C.c_int32), ("circuits_per_wave", C.c_int32)]
+                ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
+                ("staging", C.c_int32), ("reserved", C.c_int32)]
 
 
 class TranSpec(C.Structure):
@@ -273,7 +274,8 @@ def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = Fa
 class Group(C.Structure):
     _fields_ = [("topology", C.c_void_p), ("B", C.c_int32), ("reserved", C.c_int32), ("d_params", C.c_void_p),
                 ("d_lane_spec", C.c_void_p), ("d_x", C.c_void_p), ("d_status", C.c_void_p), ("d_iters", C.c_void_p),
-                ("d_accepted", C.c_void_p), ("d_rejected", C.c_void_p), ("d_ledger", C.c_void_p)]
+                ("d_accepted", C.c_void_p), ("d_rejected", C.c_void_p), ("d_ledger", C.c_void_p),
+                ("d_nodeset", C.c_void_p)]
 
 
 def run_groups(topos, params_list, spec: Optional[TranSpec] = None, opts: Optional[Options] = None,

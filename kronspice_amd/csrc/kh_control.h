@@ -21,6 +21,8 @@ enum {
   CI_FIRST_ERR, CI_GM_OK, CI_GM_K, CI_SS_K, CI_PT_K, CI_PT_QUIET, CI_TR_ORDER, CI_TR_HIT, CI_ACCEPTED,
   CI_REJECTED, CI_WAIT, CI_NEED_SOLVE, CI_MODE, CI_METHOD, CI_LU, CI_DC_RET, CI_PL_K, CI_PID, CI_SOLVES, NCI
 };
+// CI_WAIT: why a circuit waits for the host between launches.
+enum { WAIT_NONE = 0, WAIT_PATTERN = 1, WAIT_CAPACITY = 2 };
 // Double control words (SoA [NCD][B]).
 enum {
   CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
@@ -79,6 +81,7 @@ struct CtlArgs {
   int32_t* ci;      // [NCI][B]
   double* cd;       // [NCD][B]
   const double* lane_spec;   // optional [4][B]: per-circuit step, stop, max step, min step
+  const double* nodeset;     // optional [numNodes-1][B]: per-circuit Newton start (Options.NodeSet)
   const int32_t* lu_status;  // [B] result of the last factor+solve
   int32_t* counters;         // [0] lanes still running, [1] lanes waiting for a new pattern
 };
@@ -349,7 +352,7 @@ struct Ctl {
         if (A.ckt.isV[i]) add_diag(diag_slot(i), g);
     }
     ci[CI_NEED_SOLVE] = 1;
-    if (L.grew) ci[CI_WAIT] = 1;
+    if (L.grew) ci[CI_WAIT] = WAIT_PATTERN;
   }
   // Slot of the diagonal entry (i, i): the device slot when a device stamps
   // it, else a slot reserved after the device slots.
@@ -407,7 +410,25 @@ struct Ctl {
   KH_HD void dc_begin(int ret) {
     ci[CI_DC_RET] = ret;
     ci[CI_INITJCT] = ci[CI_WARM] ? 0 : 1;
+    if (!ci[CI_WARM] && A.nodeset) {
+      apply_nodeset();
+      ci[CI_INITJCT] = 0;
+    }
     start_newton(P.itl1, CALLER_DC_PLAIN);
+  }
+  // applyNodeSet, analysis/newton.go:221-243: x <- the guess (node k is
+  // unknown k-1, the rest 0), then every nonlinear device seeds its limiting
+  // state from it.
+  KH_HD void apply_nodeset() {
+    for (int i = 0; i < n; i++) X(A.x, i) = 0;
+    for (int i = 0; i < A.ckt.numNodes - 1; i++) X(A.x, i) = A.nodeset[(int64_t)i * B + c];
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      switch (e[ET_KIND]) {
+        case K_D: accept_D(L, e, A.x, 0, MODE_OP); break;
+        case K_M: accept_M(L, e, A.x, 0, MODE_OP); break;
+      }
+    }
   }
   KH_HD void dc_return(int r) {
     if (ci[CI_DC_RET] == DCRET_OP) op_after_dc(r);
@@ -723,7 +744,7 @@ struct Ctl {
   // ---- one control step of the lane ------------------------------------------
   KH_HD void step() {
     if (ci[CI_PHASE] == PH_DONE) return;
-    if (ci[CI_WAIT]) return;  // waiting for the host to analyze a grown pattern
+    if (ci[CI_WAIT]) return;  // waiting for the host (a grown pattern, more L/U capacity)
     if (ci[CI_PHASE] == PH_INIT) {
       ci[CI_PHASE] = PH_RUN;
       if (P.analysis == AN_TRAN) tran_begin();

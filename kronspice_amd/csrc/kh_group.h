@@ -5,6 +5,8 @@
 // val[c*nslots ...], its unknowns x[c*n ...], and so on, so one circuit's
 // working set shares cache lines and its serial factorization hits L1/L2.
 #pragma once
+#include <type_traits>
+
 #include "kh_control.h"
 
 // Cycle counter for the per-circuit split of time between factorization and
@@ -22,7 +24,88 @@ struct Group {
   int32_t gid;
   int32_t cpb;     // circuits per workgroup
   int32_t npar, nstate;
+  int32_t lds_bytes;  // > 0: each circuit's working set is staged in this much LDS
+  int32_t pad;
 };
+
+// Visits the pointer field of every per-circuit array of a group (null until
+// the array is allocated; the LU workspace is allocated with the first pattern):
+// f(field, bytes of one circuit's slice, copy in, copy out). Circuit c's slice
+// starts at field + c * bytes. "Copy in" marks what a circuit reads before
+// writing within one launch (its state between launches), "copy out" what it
+// changes that outlives the launch; the rest is scratch of one factorization
+// or one convergence test. Parameters, pattern ids and per-circuit transient
+// specifications are read-only.
+template <typename G, typename F>
+KH_HD inline void for_each_array(G& g, F&& f) {
+  const int64_t n = g.ctl.ckt.n, ns = g.ctl.ckt.nslots;
+  const int64_t capL = g.lu.capL, capU = g.lu.capU, capR = g.lu.capR;
+  auto& a = g.ctl;
+  auto& l = g.lu;
+  f(a.par, g.npar * 8, true, false);
+  f(a.st, g.nstate * 8, true, true);
+  f(a.val, ns * 8, true, true);
+  f(a.rhs, n * 8, true, true);
+  f(a.x, n * 8, true, true);
+  f(a.xold, n * 8, true, true);
+  f(a.xsave, n * 8, true, true);
+  f(a.ptprev, n * 8, true, true);
+  f(a.xacc, n * 8, true, true);
+  f(a.absbuf, n * 8, false, false);
+  f(a.cd, (int64_t)NCD * 8, true, true);
+  if (a.lane_spec) f(a.lane_spec, 4 * 8, true, false);
+  if (a.nodeset) f(a.nodeset, (int64_t)(a.ckt.numNodes - 1) * 8, true, false);
+  f(l.xw, n * 8, false, false);
+  f(l.w, n * 8, false, false);
+  f(l.udiag, n * 8, false, false);
+  f(l.lx, capL * 8, false, false);
+  f(l.ux, capU * 8, false, false);
+  f(a.ci, (int64_t)NCI * 4, true, true);
+  f(a.lu_status, 4, true, true);
+  f(l.pid, 4, true, false);
+  f(l.li, capL * 4, false, false);
+  f(l.ui, capU * 4, false, false);
+  f(l.lp, (n + 1) * 4, false, false);
+  f(l.up, (n + 1) * 4, false, false);
+  f(l.pinv, n * 4, false, false);
+  f(l.prow, n * 4, true, true);
+  f(l.xi, n * 4, false, false);
+  f(l.pstack, n * 4, false, false);
+  f(l.mark, n * 4, false, false);
+  f(l.err_step, 4, true, true);
+  f(l.rch, capR * 4, true, true);
+  f(l.rp, (n + 1) * 4, true, true);
+  f(l.dk, n * 4, true, true);
+  f(l.cpid, 4, true, true);
+  f(a.present, ns, true, true);
+}
+
+KH_HD inline int64_t round8(int64_t b) { return (b + 7) & ~(int64_t)7; }
+
+// Bytes of LDS one circuit of the group needs (each array 8-byte aligned).
+KH_HD inline int64_t lds_need(const Group& g) {
+  int64_t total = 0;
+  for_each_array(g, [&](const auto&, int64_t bytes, bool, bool) { total += round8(bytes); });
+  return total;
+}
+
+// The group as one circuit sees it once its working set is copied to `base`
+// (arrays in for_each_array order, each 8-byte aligned): run_circuit(view, 0,
+// ...) then works on the copies.
+KH_HD inline Group lds_view(const Group& g, char* base) {
+  Group v = g;
+  int64_t off = 0;
+  for_each_array(v, [&](auto& p, int64_t bytes, bool, bool) {
+    p = (std::remove_reference_t<decltype(p)>)(base + off);
+    off += round8(bytes);
+  });
+  v.lu.val = v.ctl.val;
+  v.lu.rhs = v.ctl.rhs;
+  v.lu.x = v.ctl.x;
+  v.ctl.B = 1;
+  v.lu.B = 1;
+  return v;
+}
 
 // The argument blocks of circuit c: base pointers advanced to the circuit and
 // unit stride, so the per-circuit device code (kh_control.h, kh_lu.h) runs
@@ -42,6 +125,7 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   a.cd += c * NCD;
   a.lu_status += c;
   if (a.lane_spec) a.lane_spec += c * 4;
+  if (a.nodeset) a.nodeset += c * (a.ckt.numNodes - 1);
   l.B = 1;
   l.pid += c;
   l.val = a.val; l.rhs = a.rhs; l.x = a.x;
@@ -56,7 +140,8 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
 
 // One circuit's persistent loop: factor + solve the pending system, advance
 // the analysis to its next solve (stamping it), until the analysis ends, the
-// stamps need a new ordering, or max_solves solves were done.
+// stamps need a new ordering or the factors more capacity, or max_solves
+// solves were done.
 KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bool* running, bool* waiting) {
   CtlArgs a;
   LuArgs la;
@@ -71,6 +156,13 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
       if (solves >= max_solves) break;
       uint64_t t0 = KH_CLOCK();
       int st = lu_factor(la, 0);
+      if (st == LU_CAPACITY) {
+        // L/U outgrew the group's capacity (pivoting away from the diagonal
+        // fills more than the structural estimate): the host enlarges it and
+        // the factorization is redone in the next launch.
+        ctl.ci[CI_WAIT] = WAIT_CAPACITY;
+        break;
+      }
       if (st == LU_OK) st = lu_solve(la, 0);
       lus[0] = st;
       solves++;

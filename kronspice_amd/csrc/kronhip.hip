@@ -53,14 +53,59 @@ constexpr int kBlock = 64;  // one wavefront per workgroup: lanes run long, dive
 // workgroup, one per lane. With cpb = 1 every circuit has its own wavefront
 // and the many resident wavefronts hide the latency of each circuit's
 // dependent accesses.
+//
+// Groups with lds_bytes > 0 run staged: the wavefront copies the working set
+// of each of its circuits (every per-circuit array, kh_group.h) into LDS,
+// the circuits run on the LDS copies, and the wavefront copies back what
+// outlives the launch. Every dependent access of the serial factorization and
+// state machine is then an LDS access instead of a trip to L2/HBM.
+__device__ inline void copy_wave(char* dst, const char* src, int64_t bytes) {
+  const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
+  if ((al & 7) == 0) {
+    for (int64_t k = threadIdx.x; k < bytes / 8; k += kBlock) ((uint64_t*)dst)[k] = ((const uint64_t*)src)[k];
+  } else if ((al & 3) == 0) {
+    for (int64_t k = threadIdx.x; k < bytes / 4; k += kBlock) ((uint32_t*)dst)[k] = ((const uint32_t*)src)[k];
+  } else {
+    for (int64_t k = threadIdx.x; k < bytes; k += kBlock) dst[k] = src[k];
+  }
+}
+
 __global__ void __launch_bounds__(kBlock) k_persist(const Group* G, const int32_t* bgroup, int32_t max_solves,
                                                     int32_t* counters) {
+  extern __shared__ __align__(16) char lds[];
   const Group& g = G[bgroup[blockIdx.x]];
-  if ((int)threadIdx.x >= g.cpb) return;
-  int64_t cc = (int64_t)(blockIdx.x - g.block0) * g.cpb + threadIdx.x;
-  if (cc >= g.ctl.B) return;
-  bool running, waiting;
-  run_circuit(g, cc, max_solves, &running, &waiting);
+  const int64_t c0 = (int64_t)(blockIdx.x - g.block0) * g.cpb;
+  bool running = false, waiting = false;
+  if (g.lds_bytes > 0) {
+    const int nc = (int)min((int64_t)g.cpb, g.ctl.B - c0);  // uniform over the workgroup
+    for (int t = 0; t < nc; t++) {
+      char* base = lds + (int64_t)t * g.lds_bytes;
+      int64_t off = 0;
+      for_each_array(g, [&](const auto& p, int64_t bytes, bool in, bool) {
+        if (in && p) copy_wave(base + off, (const char*)p + (c0 + t) * bytes, bytes);
+        off += round8(bytes);
+      });
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+      Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes);
+      run_circuit(v, 0, max_solves, &running, &waiting);
+    }
+    __syncthreads();
+    for (int t = 0; t < nc; t++) {
+      char* base = lds + (int64_t)t * g.lds_bytes;
+      int64_t off = 0;
+      for_each_array(g, [&](const auto& p, int64_t bytes, bool, bool out) {
+        if (out && p) copy_wave((char*)p + (c0 + t) * bytes, base + off, bytes);
+        off += round8(bytes);
+      });
+    }
+  } else {
+    if ((int)threadIdx.x >= g.cpb) return;
+    const int64_t cc = c0 + threadIdx.x;
+    if (cc >= g.ctl.B) return;
+    run_circuit(g, cc, max_solves, &running, &waiting);
+  }
   if (running) atomicAdd(&counters[2 * g.gid], 1);
   if (waiting) atomicAdd(&counters[2 * g.gid + 1], 1);
 }
@@ -180,6 +225,10 @@ struct LuWork {
   }
 };
 
+// L and U capacity (entries per system) for a pattern with structural fill
+// max_fill = |L| + |U| + n under diagonal pivoting, with a wide margin for
+// pivoting away from the diagonal. Batched analyses that still overflow it
+// double it and refactor (WAIT_CAPACITY).
 int64_t lu_capacity(int64_t max_fill, int n) { return 2 * max_fill + 4 * (int64_t)n + 64; }
 
 // Pattern pool mirrored on the device.
@@ -501,13 +550,14 @@ struct GroupRun {
   int64_t B = 0;
   int n = 0;
   int32_t gid = 0;
-  DBuf par, lane, st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, pid, hash, solves;
+  DBuf par, lane, nodeset, st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, pid, hash, solves;
   DevPatterns pats;
   LuWork work;
   bool has_lane = false;
+  bool has_nodeset = false;
   kh_stats stats = {};
 
-  kh_status setup(const double* d_params, const double* d_lane_spec) {
+  kh_status setup(const double* d_params, const double* d_lane_spec, const double* d_nodeset) {
     const Topology& T = topo->t;
     n = T.n;
     size_t nd = (size_t)n * B * 8;
@@ -534,6 +584,14 @@ struct GroupRun {
                          lane.as<double>(), (int64_t)4, B);
       KH_HIP(hipGetLastError());
     }
+    has_nodeset = d_nodeset != nullptr && T.num_nodes > 1;
+    if (has_nodeset) {
+      int64_t rows = T.num_nodes - 1;
+      KH_HIP(nodeset.alloc((size_t)rows * B * 8));
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((rows * B + 255) / 256)), dim3(256), 0, 0, d_nodeset,
+                         nodeset.as<double>(), rows, B);
+      KH_HIP(hipGetLastError());
+    }
     return KH_OK;
   }
 
@@ -556,6 +614,7 @@ struct GroupRun {
     a.lu_status = lustatus.as<int32_t>();
     a.counters = nullptr;
     a.lane_spec = has_lane ? lane.as<double>() : nullptr;
+    a.nodeset = has_nodeset ? nodeset.as<double>() : nullptr;
     return a;
   }
 
@@ -590,8 +649,10 @@ struct GroupRun {
     KH_HIP(hipMemcpy(hpid.data(), pid.p, (size_t)B * 4, hipMemcpyDeviceToHost));
     std::unordered_map<uint64_t, int32_t> by_hash;
     std::vector<uint8_t> col(T.nslots);
+    bool grow = false;
     for (int64_t c = 0; c < B; c++) {
       if (!wait[c]) continue;
+      if (wait[c] == WAIT_CAPACITY) grow = true;  // same pattern, needs more L/U room
       auto it = by_hash.find(hh[c]);
       if (it != by_hash.end()) {
         hpid[c] = it->second;
@@ -617,7 +678,9 @@ struct GroupRun {
     for (int64_t c = 0; c < B; c++) wait[c] = 0;
     stats.patterns = (int64_t)pats.host.desc.size();
     KH_HIP(pats.sync());
-    KH_HIP(work.ensure(T.n, B, std::max(work.capL, lu_capacity(pats.host.max_fill, T.n))));
+    int64_t cap = std::max(work.capL, lu_capacity(pats.host.max_fill, T.n));
+    if (grow) cap = std::max(cap, 2 * work.capL);
+    KH_HIP(work.ensure(T.n, B, cap));
     KH_HIP(hipMemcpy(pid.p, hpid.data(), (size_t)B * 4, hipMemcpyHostToDevice));
     KH_HIP(hipMemcpy2D(wcol, (size_t)NCI * 4, wait.data(), 4, 4, B, hipMemcpyHostToDevice));
     return KH_OK;
@@ -673,6 +736,12 @@ struct Runner {
   int32_t max_solves = 4;
   static constexpr float kLaunchTargetMs = 25.0f;
   int32_t cpb = 1;
+  // LDS staging (k_persist): a group is staged when cpb circuits' working
+  // sets fit in kLdsLimit bytes (the per-workgroup LDS available without
+  // opting in to more).
+  bool lds_staging = false;
+  static constexpr int64_t kLdsLimit = 64 * 1024;
+  size_t lds_launch = 0;
 
   ~Runner() {
     if (hcount) (void)hipHostFree(hcount);
@@ -706,16 +775,28 @@ struct Runner {
     bool dirty = true;
     for (;;) {
       if (dirty) {  // group descriptors: LU workspace pointers change when capacity grows
+        lds_launch = 0;
         for (size_t g = 0; g < ng; g++) {
           hg[g].ctl = groups[g]->ctl_args(p);
           hg[g].lu = groups[g]->lu_args();
+          hg[g].lds_bytes = 0;
+          if (lds_staging) {
+            // An odd number of 8-byte words per circuit puts the circuits of
+            // one wavefront in different LDS banks.
+            int64_t b = round8(lds_need(hg[g]));
+            if (((b / 8) & 1) == 0) b += 8;
+            if (b * hg[g].cpb <= kLdsLimit) {
+              hg[g].lds_bytes = (int32_t)b;
+              lds_launch = std::max(lds_launch, (size_t)(b * hg[g].cpb));
+            }
+          }
         }
         KH_HIP(hipMemcpy(dgroups.p, hg.data(), ng * sizeof(Group), hipMemcpyHostToDevice));
         dirty = false;
       }
       KH_HIP(hipMemsetAsync(counters.p, 0, ng * 8, 0));
       KH_HIP(hipEventRecord(ev[0], 0));
-      hipLaunchKernelGGL(k_persist, dim3((unsigned)bg.size()), dim3(kBlock), 0, 0, dgroups.as<Group>(),
+      hipLaunchKernelGGL(k_persist, dim3((unsigned)bg.size()), dim3(kBlock), lds_launch, 0, dgroups.as<Group>(),
                          dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>());
       KH_HIP(hipGetLastError());
       KH_HIP(hipEventRecord(ev[1], 0));
@@ -810,10 +891,16 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
   }
   int32_t cpb = (opt && opt->circuits_per_wave > 0) ? opt->circuits_per_wave : 1;
   if (cpb > kBlock) return fail(KH_BAD_ARGUMENT, "circuits_per_wave above 64");
+  // Several circuits per wavefront and LDS staging have not passed on the
+  // GPU yet (a run that used them ended with a device fault whose cause was
+  // not found): both are refused until they do.
+  if (cpb != 1) return fail(KH_UNSUPPORTED, "circuits_per_wave other than 1 is disabled");
+  if (opt && opt->staging != 0) return fail(KH_UNSUPPORTED, "LDS staging is disabled");
   std::lock_guard<std::mutex> lk(g_run_mu);
   std::vector<std::unique_ptr<GroupRun>> runs;
   Runner R;
   R.cpb = cpb;
+  R.lds_staging = opt && opt->staging != 0;
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];
     if (!G.topology || G.B <= 0 || !G.d_params) return fail(KH_BAD_ARGUMENT, "bad group " + std::to_string(g));
@@ -821,7 +908,7 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
     r->topo = G.topology;
     r->B = G.B;
     r->gid = g;
-    kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr);
+    kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr, G.d_nodeset);
     if (s != KH_OK) return s;
     R.groups.push_back(r.get());
     runs.push_back(std::move(r));

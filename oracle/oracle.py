@@ -28,7 +28,8 @@ class KhOptions(C.Structure):
                 ("itl1", C.c_int32), ("itl4", C.c_int32), ("no_gmin_stepping", C.c_int32),
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
-                ("polish", C.c_int32), ("circuits_per_wave", C.c_int32)]
+                ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
+                ("staging", C.c_int32), ("reserved", C.c_int32)]
 
 
 class KhTranSpec(C.Structure):
