@@ -254,11 +254,15 @@ def to_device(a: np.ndarray, dtype=None):
     return t.to(device="cuda", dtype=dtype) if dtype is not None else t.to("cuda")
 
 
-def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = False) -> Result:
-    """analysis.OP for every circuit; params [P, B] (device tensor or numpy)."""
+def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = False, nodeset=None) -> Result:
+    """analysis.OP for every circuit; params [P, B] (device tensor or numpy).
+    nodeset [numNodes-1, B] (optional): Newton's starting node voltages per
+    circuit (Options.NodeSet, analysis/op.go:39-43)."""
     torch = _torch()
     if isinstance(params, np.ndarray):
         params = to_device(params)
+    if isinstance(nodeset, np.ndarray):
+        nodeset = to_device(nodeset)
     B = params.shape[1]
     x = torch.zeros((topo.n, B), dtype=torch.float64, device="cuda")
     st = torch.zeros(B, dtype=torch.int32, device="cuda")
@@ -267,7 +271,14 @@ def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = Fa
     o.polish = 1 if polish else o.polish
     s = Stats()
     torch.cuda.synchronize()
-    _check(lib().kh_batch_op(topo.h, B, _p(params), C.byref(o), _p(x), _p(st), _p(it), C.byref(s)))
+    if nodeset is None:
+        _check(lib().kh_batch_op(topo.h, B, _p(params), C.byref(o), _p(x), _p(st), _p(it), C.byref(s)))
+    else:
+        if tuple(nodeset.shape) != (topo.num_nodes - 1, B):
+            raise ValueError(f"nodeset shape {tuple(nodeset.shape)}, want {(topo.num_nodes - 1, B)}")
+        g = Group(topology=topo.h.value, B=B, d_params=params.data_ptr(), d_x=x.data_ptr(), d_status=st.data_ptr(),
+                  d_iters=it.data_ptr(), d_nodeset=nodeset.data_ptr())
+        _check(lib().kh_batch_run(1, C.byref(g), C.byref(o), None, C.byref(s)))
     return Result(x, st, iters=it, stats=s.as_dict())
 
 

@@ -257,3 +257,19 @@ def mosfet_chains(B: int = 16384, patterns: int = 200, seed: int = 3, steps: int
 def circuit_of(batch: Batch, c: int) -> Tuple[np.ndarray, np.ndarray]:
     """Element table and parameter vector of circuit c of a batch."""
     return batch.elements, batch.circuit_params(c)
+
+
+def cmos_latch() -> kc.Circuit:
+    """Two cross-coupled level-1 CMOS inverters (a bistable latch) with 1 MΩ
+    loads, 5 V supply. Three operating points: the metastable q = qb, and the
+    two mirrored stable states; a cold start lands on the metastable one, so
+    the starting guess (Options.NodeSet) decides which state Newton reaches."""
+    nm = {"level": 1, "tox": 20e-9, "nsub": 3e16, "uo": 550, "lambda": 0.03}
+    pm = {"level": 1, "tox": 20e-9, "nsub": 1e16, "uo": 200, "lambda": 0.05}
+    return (kc.Builder("latch").vsource("VDD", "vdd", "0", 5)
+            .mosfet("MN1", "qb", "q", "0", "0", "nm", {"l": 1e-6, "w": 10e-6})
+            .mosfet("MP1", "qb", "q", "vdd", "vdd", "pm", {"l": 1e-6, "w": 25e-6})
+            .mosfet("MN2", "q", "qb", "0", "0", "nm", {"l": 1e-6, "w": 10e-6})
+            .mosfet("MP2", "q", "qb", "vdd", "vdd", "pm", {"l": 1e-6, "w": 25e-6})
+            .resistor("R1", "q", "0", 1e6).resistor("R2", "qb", "0", 1e6)
+            .model("nm", "nmos", nm).model("pm", "pmos", pm)).build()

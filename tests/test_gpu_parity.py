@@ -255,3 +255,54 @@ def test_rectifier_batch_tran_matches_oracle():
     rel = relerr(host(r.x), xo)
     assert rel[same].max() <= 1e-9, rel[same].max()
     assert host(r.ledger).max() <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["diode_fwd", "mos1_op", "mos1_cmos_vtc", "mos1_cs_ac"])
+def test_nodeset_warm_start_matches_oracle(name):
+    """Options.NodeSet (analysis/newton.go:221-243): Newton starts from a
+    per-circuit guess (node voltages only; branch currents and internal nodes
+    start at 0) with InitJct off and device limiting seeded from it. Guesses
+    are the converged node voltages scaled by 0.9 .. 1.1; each circuit must
+    reach the oracle's status, iteration count and solution from the same
+    guess."""
+    c = gr.load_deck(name).circuit
+    els, p = c.flatten()
+    e0, x0, info0 = oracle.op(c.num_nodes, els, p)
+    assert e0 == 0
+    B = 16
+    scale = np.linspace(0.9, 1.1, B)
+    scale[B // 2] = 1.0
+    guess = x0[: c.num_nodes - 1, None] * scale[None, :]
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.op(t, np.repeat(p[:, None], B, axis=1), nodeset=np.ascontiguousarray(guess))
+    x, st, it = host(r.x), host(r.status), host(r.iters)
+    for k in range(B):
+        e, xo, info = oracle.op(c.num_nodes, els, p, guess=np.ascontiguousarray(guess[:, k]))
+        assert st[k] == e, (k, st[k], e)
+        assert it[k] == info["iters"], (k, it[k], info["iters"])
+        assert np.max(np.abs(x[:, k] - xo)) <= 1e-9 * np.max(np.abs(xo)), k
+
+
+def test_nodeset_latch_states_match_oracle():
+    """A bistable latch: the guess decides which operating point Newton
+    reaches (cold start: the metastable point; Newton's basins are not
+    monotonic in the guess). Guesses sweep q over 0..5 V with qb = 5 - q;
+    each circuit must end where the oracle does from the same guess, with the
+    same iteration count, and both stable states must be reached."""
+    c = W.cmos_latch()
+    els, p = c.flatten()
+    vdd, qb, q = c.node("vdd") - 1, c.node("qb") - 1, c.node("q") - 1
+    B = 64
+    guess = np.zeros((c.num_nodes - 1, B))
+    guess[vdd] = 5.0
+    hi = np.linspace(3.0, 5.0, B)
+    guess[q] = np.where(np.arange(B) % 2 == 0, hi, 5.0 - hi)
+    guess[qb] = 5.0 - guess[q]
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.op(t, np.repeat(p[:, None], B, axis=1), nodeset=guess)
+    x, st, it = host(r.x), host(r.status), host(r.iters)
+    for k in range(B):
+        e, xo, info = oracle.op(c.num_nodes, els, p, guess=np.ascontiguousarray(guess[:, k]))
+        assert st[k] == e == 0 and it[k] == info["iters"], (k, st[k], e, it[k], info["iters"])
+        assert np.max(np.abs(x[:, k] - xo)) <= 1e-9 * np.max(np.abs(xo)), k
+    assert (x[q] > 4.9).any() and (x[qb] > 4.9).any()  # both stable states are reached

@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle
-from kronspice_amd import circuit as kc
+from kronspice_amd import circuit as kc, workloads as W
 from tests import golden_runner as gr
 
 
@@ -260,3 +260,29 @@ def test_no_convergence_without_fallbacks():
     o = oracle.options(itl1=2, no_gmin_stepping=1, no_source_stepping=1, no_pseudo_transient=1)
     st, _, _ = oracle.op(c.num_nodes, els, p, opts=o)
     assert st == 4
+
+
+def test_nodeset_selects_latch_state():
+    """applyNodeSet (analysis/newton.go:221-243), pinned the way the
+    reference's multiple-operating-point test pins it
+    (analysis/nonlinear_test.go:27-60): a bistable latch has one operating
+    point near each guess. A cold start ends on the metastable point
+    (q = qb by symmetry); the guess (q high, qb low) must end on that stable
+    state and the mirrored guess on the mirrored state (to round-off: the
+    stamp order differs), each with a closed ledger. An ignored guess would give all three runs the same
+    answer; a guess applied to the wrong unknowns would break the mirror."""
+    c = W.cmos_latch()
+    els, p = c.flatten()
+    vdd, qb, q = c.node("vdd") - 1, c.node("qb") - 1, c.node("q") - 1
+    e0, x0, _ = oracle.op(c.num_nodes, els, p)
+    assert e0 == 0 and abs(x0[q] - x0[qb]) <= 1e-9 and 0.5 < x0[q] < 4.5
+    ga = np.zeros(c.num_nodes - 1)
+    ga[vdd], ga[q], ga[qb] = 5.0, 5.0, 0.0
+    gb = ga.copy()
+    gb[q], gb[qb] = 0.0, 5.0
+    ea, xa, ia = oracle.op(c.num_nodes, els, p, guess=ga)
+    eb, xb, ib = oracle.op(c.num_nodes, els, p, guess=gb)
+    assert ea == 0 and eb == 0
+    assert xa[q] > 4.9 and xa[qb] < 1e-6
+    assert abs(xa[q] - xb[qb]) <= 1e-9 * xa[q] and abs(xa[qb] - xb[q]) <= 1e-9 * xa[q]
+    assert ia["ledger_rel"] <= 1e-12 and ib["ledger_rel"] <= 1e-12
