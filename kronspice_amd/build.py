@@ -1,4 +1,5 @@
-"""Builds libkronhip.so (gfx950 kernels + host symbolic phase + C ABI) in-tree."""
+"""Builds libkronhip.so (gfx950 kernels + host symbolic phase + C ABI) and the
+kh_bench driver (plain C++ over the C ABI) in-tree."""
 from __future__ import annotations
 
 import os
@@ -18,6 +19,7 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 SOURCES = ["kronhip.hip", "kh_symbolic.cpp"]
+BENCH = os.path.join(HERE, "kh_bench")
 
 
 def build(verbose: bool = False) -> str:
@@ -30,6 +32,14 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr:
         print(r.stderr)
     os.replace(tmp, LIB)
+    # kh_bench: host code only, linked against the library next to it.
+    tmpb = BENCH + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(HERE, "kh_bench.cpp"), "-L" + HERE, "-lkronhip",
+           "-Wl,-rpath,$ORIGIN", "-o", tmpb]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc (kh_bench) failed:\n" + r.stdout + r.stderr)
+    os.replace(tmpb, BENCH)
     return LIB
 
 

@@ -273,3 +273,50 @@ def cmos_latch() -> kc.Circuit:
             .mosfet("MP2", "q", "qb", "vdd", "vdd", "pm", {"l": 1e-6, "w": 25e-6})
             .resistor("R1", "q", "0", 1e6).resistor("R2", "qb", "0", 1e6)
             .model("nm", "nmos", nm).model("pm", "pmos", pm)).build()
+
+
+def write_batch_file(path: str, batches: List[Batch], transient: bool = True) -> int:
+    """Writes batches (one group each) in the KHB1 format read by kh_bench:
+    "KHB1", int32 groups, int32 transient, double step, stop, max_step (of the
+    first batch), then per group int32 num_nodes, nel, nparams, B, has_lane,
+    the element table, params [P, B] and lane_spec [4, B] if present.
+    Returns the number of circuits written."""
+    import struct
+    spec = batches[0].spec or {"step": 0.0, "stop": 0.0, "max_step": 0.0}
+    total = 0
+    with open(path, "wb") as f:
+        f.write(b"KHB1")
+        f.write(struct.pack("<ii3d", len(batches), 1 if transient else 0, spec["step"], spec["stop"],
+                            spec["max_step"]))
+        for b in batches:
+            has_lane = transient and b.lane_spec is not None
+            f.write(struct.pack("<5i", b.circuit.num_nodes, len(b.elements), b.params.shape[0], b.B,
+                                1 if has_lane else 0))
+            f.write(np.ascontiguousarray(b.elements).tobytes())
+            f.write(np.ascontiguousarray(b.params, np.float64).tobytes())
+            if has_lane:
+                f.write(np.ascontiguousarray(b.lane_spec, np.float64).tobytes())
+            total += b.B
+    return total
+
+
+def _main(argv=None) -> int:
+    import argparse
+    ap = argparse.ArgumentParser(description="write a benchmark workload as a KHB1 batch file")
+    ap.add_argument("workload", choices=["rectifier", "ladders", "mosfet_chains"])
+    ap.add_argument("path")
+    ap.add_argument("--batch", type=int, default=0, help="circuits (0: the workload's default)")
+    ap.add_argument("--patterns", type=int, default=200)
+    a = ap.parse_args(argv)
+    if a.workload == "rectifier":
+        bs, tran = [rectifier_rc_ladders(a.batch or 4096)], True
+    elif a.workload == "ladders":
+        bs, tran = [resistor_ladders(a.batch or 1024)], False
+    else:
+        bs, tran = mosfet_chains(a.batch or 16384, patterns=a.patterns), True
+    print(write_batch_file(a.path, bs, tran), "circuits written to", a.path)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
