@@ -32,9 +32,10 @@ def build(verbose: bool = False) -> str:
     if verbose and r.stderr:
         print(r.stderr)
     os.replace(tmp, LIB)
-    # kh_bench: host code only, linked against the library next to it.
+    # kh_bench: the C++ driver (and one copy kernel), linked against the library next to it.
     tmpb = BENCH + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", os.path.join(HERE, "kh_bench.cpp"), "-L" + HERE, "-lkronhip",
+    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+           "-packed-fp32-ops", "-x", "hip", os.path.join(HERE, "kh_bench.cpp"), "-x", "none", "-L" + HERE, "-lkronhip",
            "-Wl,-rpath,$ORIGIN", "-o", tmpb]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:

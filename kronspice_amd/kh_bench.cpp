@@ -2,6 +2,8 @@
 // Python, no PyTorch in the process), for timing and for profilers.
 //
 //   kh_bench <batch file> [steps] [warmup]
+//   kh_bench --copy <MiB>    one device-to-device copy kernel of that size:
+//                            the reference point for hardware byte counters
 //
 // The batch file is written by kronspice_amd.workloads.write_batch_file:
 //   "KHB1", int32 ngroups, int32 transient, double step, stop, max_step,
@@ -54,10 +56,32 @@ int die(const char* what) {
     }                                                                                \
   } while (0)
 
+__global__ void __launch_bounds__(256) k_copy(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+int copy_mode(int mib) {
+  const int64_t n = (int64_t)mib * (1 << 20) / 8;
+  double *a = nullptr, *b = nullptr;
+  CHECK_HIP(hipMalloc(&a, n * 8));
+  CHECK_HIP(hipMalloc(&b, n * 8));
+  CHECK_HIP(hipMemset(a, 0, n * 8));
+  CHECK_HIP(hipDeviceSynchronize());
+  hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, 0, a, b, n);
+  CHECK_HIP(hipGetLastError());
+  CHECK_HIP(hipDeviceSynchronize());
+  printf("{\"copy_bytes_read\": %lld, \"copy_bytes_written\": %lld}\n", (long long)(n * 8), (long long)(n * 8));
+  CHECK_HIP(hipFree(a));
+  CHECK_HIP(hipFree(b));
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
-  if (argc < 2) return die("usage: kh_bench <batch file> [steps] [warmup]");
+  if (argc < 2) return die("usage: kh_bench <batch file> [steps] [warmup] | kh_bench --copy <MiB>");
+  if (strcmp(argv[1], "--copy") == 0) return copy_mode(argc > 2 ? atoi(argv[2]) : 256);
   const int steps = argc > 2 ? atoi(argv[2]) : 1;
   const int warmup = argc > 3 ? atoi(argv[3]) : 0;
   FILE* f = fopen(argv[1], "rb");
