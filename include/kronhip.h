@@ -113,10 +113,10 @@ typedef struct kh_options {
   int32_t polish;           /* kh_batch_op: polish the operating point to round-off
                                after convergence (Sim.polish, analysis/op.go:243-271) */
   int32_t circuits_per_wave; /* batched analyses: circuits sharing one wavefront
-                               (0 or 1; other values -> KH_UNSUPPORTED for now) */
+                               (1..64; 0 -> 1) */
   int32_t staging;          /* batched analyses: 0 -> circuits are worked on in place in
-                               device memory; 1 (each circuit's working set held in LDS
-                               during a launch) -> KH_UNSUPPORTED for now */
+                               device memory; 1 -> each circuit's working set is held in
+                               LDS during a launch when it fits (64 KiB per workgroup) */
   int32_t reserved;
 } kh_options;
 
@@ -224,6 +224,32 @@ typedef struct kh_group {
  * circuit of every group. */
 kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt,
                        const kh_tran_spec* spec, kh_stats* stats);
+
+/* DC sweep request: analysis.DC with one or two swept sources
+ * (analysis/dc.go:22-28 DCSweep, :51-130 DC). Sweep 0 is the inner loop;
+ * its points are start + k*step, k = 0 .. floor((stop-start)/step + 1e-9)
+ * (DCSweep.values, :31-46); element[] are indices of V or I sources in the
+ * topology's element list (the source named by DCSweep.Source). */
+typedef struct kh_dc_spec {
+  int32_t nsweeps;          /* 1 or 2 */
+  int32_t element[2];
+  int32_t reserved;
+  double start[2], stop[2], step[2];
+} kh_dc_spec;
+
+/* analysis.DC for B circuits of one topology: at every point the swept
+ * sources' DC values are overridden (SetDCOverride, device/source.go:179-203)
+ * and analysis.OP runs on the same circuit state, so every point after the
+ * first starts warm from the previous one; the sweep stops at a circuit's
+ * first failing point with that point's status. d_sweep_x [points*n*B]
+ * (point-major, then unknown, then circuit); d_points [B] points solved;
+ * d_iters (optional) [points*B] Newton iterations per point; d_ledger
+ * (optional) [B] largest ledger residual over the points; d_nodeset
+ * (optional) [(numNodes-1)*B] the first point's Newton start. Replaces the
+ * reference's per-circuit DC (analysis/dc.go:51). */
+kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                      const kh_dc_spec* spec, const double* d_nodeset, double* d_sweep_x, int32_t* d_status,
+                      int32_t* d_points, int32_t* d_iters, double* d_ledger, kh_stats* stats);
 
 /* analysis.OP for B circuits (analysis/op.go:331-357): dcSolve with the
  * gmin / source / pseudo-transient fallbacks (analysis/newton.go:95-311),

@@ -104,6 +104,7 @@ def lib():
         L.kh_batch_run.argtypes = [i32, P, P, P, P]
         L.kh_batch_op.argtypes = [P, i32, P, P, P, P, P, P]
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
+        L.kh_batch_dc.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         _lib = L
         st = L.kh_init()
         if st != OK:
@@ -246,6 +247,7 @@ class Result:
     rejected: object = None
     ledger: object = None
     stats: dict = field(default_factory=dict)
+    points: object = None  # DC sweep: [B] points solved
 
 
 def to_device(a: np.ndarray, dtype=None):
@@ -280,6 +282,53 @@ def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = Fa
                   d_iters=it.data_ptr(), d_nodeset=nodeset.data_ptr())
         _check(lib().kh_batch_run(1, C.byref(g), C.byref(o), None, C.byref(s)))
     return Result(x, st, iters=it, stats=s.as_dict())
+
+
+class DcSpec(C.Structure):
+    """DCSweep x 1 or 2 (analysis/dc.go:22-28); element = index of the swept source."""
+    _fields_ = [("nsweeps", C.c_int32), ("element", C.c_int32 * 2), ("reserved", C.c_int32),
+                ("start", C.c_double * 2), ("stop", C.c_double * 2), ("step", C.c_double * 2)]
+
+
+def dc_spec(sweeps) -> DcSpec:
+    """sweeps: [(element index, start, stop, step)], inner first."""
+    s = DcSpec()
+    s.nsweeps = len(sweeps)
+    for k, (e, a, b, h) in enumerate(sweeps):
+        s.element[k], s.start[k], s.stop[k], s.step[k] = e, a, b, h
+    return s
+
+
+def sweep_count(start: float, stop: float, step: float) -> int:
+    """Number of points of DCSweep.values (analysis/dc.go:31-46)."""
+    return int(np.floor((stop - start) / step + 1e-9)) + 1
+
+
+def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=None) -> Result:
+    """analysis.DC for every circuit; sweeps [(element index, start, stop, step)],
+    inner first. Result.x is [points, n, B]; iters [points, B] Newton iterations per
+    point; points [B] points solved; ledger [B] largest ledger residual."""
+    torch = _torch()
+    if isinstance(params, np.ndarray):
+        params = to_device(params)
+    if isinstance(nodeset, np.ndarray):
+        nodeset = to_device(nodeset)
+    B = params.shape[1]
+    pts = 1
+    for _, a, b, h in sweeps:
+        pts *= sweep_count(a, b, h)
+    x = torch.zeros((pts, topo.n, B), dtype=torch.float64, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    done = torch.zeros(B, dtype=torch.int32, device="cuda")
+    it = torch.zeros((pts, B), dtype=torch.int32, device="cuda")
+    led = torch.zeros(B, dtype=torch.float64, device="cuda")
+    o = Options() if opts is None else opts
+    spec = dc_spec(sweeps)
+    s = Stats()
+    torch.cuda.synchronize()
+    _check(lib().kh_batch_dc(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(nodeset), _p(x), _p(st), _p(done),
+                             _p(it), _p(led), C.byref(s)))
+    return Result(x, st, iters=it, ledger=led, stats=s.as_dict(), points=done)
 
 
 class Group(C.Structure):

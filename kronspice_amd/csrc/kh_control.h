@@ -19,7 +19,8 @@ namespace kh {
 enum {
   CI_PHASE = 0, CI_CALLER, CI_NT_IT, CI_NT_MAX, CI_INITJCT, CI_WARM, CI_SRCSTEP, CI_STATUS, CI_ITERS,
   CI_FIRST_ERR, CI_GM_OK, CI_GM_K, CI_SS_K, CI_PT_K, CI_PT_QUIET, CI_TR_ORDER, CI_TR_HIT, CI_ACCEPTED,
-  CI_REJECTED, CI_WAIT, CI_NEED_SOLVE, CI_MODE, CI_METHOD, CI_LU, CI_DC_RET, CI_PL_K, CI_PID, CI_SOLVES, NCI
+  CI_REJECTED, CI_WAIT, CI_NEED_SOLVE, CI_MODE, CI_METHOD, CI_LU, CI_DC_RET, CI_PL_K, CI_PID, CI_SOLVES,
+  CI_SW_I, CI_SW_J, CI_SW_IT0, CI_SW_DONE, NCI
 };
 // CI_WAIT: why a circuit waits for the host between launches.
 enum { WAIT_NONE = 0, WAIT_PATTERN = 1, WAIT_CAPACITY = 2 };
@@ -31,7 +32,7 @@ enum {
 };
 
 enum { PH_INIT = 0, PH_RUN = 1, PH_DONE = 2 };
-enum { AN_OP = 0, AN_TRAN = 1, AN_OP_POLISH = 2 };
+enum { AN_OP = 0, AN_TRAN = 1, AN_OP_POLISH = 2, AN_DC = 3 };
 enum {
   CALLER_DC_PLAIN = 1, CALLER_GMIN, CALLER_SRC0, CALLER_SRC, CALLER_PT, CALLER_PT_FINAL, CALLER_UIC, CALLER_TRAN,
   CALLER_POLISH
@@ -51,6 +52,10 @@ struct CtlParams {
   double step, stop, start, maxStep, chgTol, trTol, minStep;
   int64_t maxSteps;
   int32_t uic, method;
+  // DC sweep (analysis/dc.go:22-46): sweep 0 is the inner loop; point k of
+  // sweep s is swStart[s] + k * swStep[s], k < swCount[s]
+  int32_t swN, swElem[2], swCount[2];
+  double swStart[2], swStep[2];
 };
 
 struct Circ {
@@ -82,6 +87,8 @@ struct CtlArgs {
   double* cd;       // [NCD][B]
   const double* lane_spec;   // optional [4][B]: per-circuit step, stop, max step, min step
   const double* nodeset;     // optional [numNodes-1][B]: per-circuit Newton start (Options.NodeSet)
+  double* sw_x;              // DC sweep: per circuit [points][n] solutions
+  int32_t* sw_it;            // DC sweep: per circuit [points] Newton iterations
   const int32_t* lu_status;  // [B] result of the last factor+solve
   int32_t* counters;         // [0] lanes still running, [1] lanes waiting for a new pattern
 };
@@ -600,6 +607,7 @@ struct Ctl {
     dc_begin(DCRET_OP);
   }
   KH_HD void op_after_dc(int r) {
+    if (P.analysis == AN_DC) { sweep_point_done(r); return; }
     if (r != ST_OK) { finish(r); return; }
     accept(0, MODE_OP);
     double rr = ledger_op();
@@ -627,6 +635,52 @@ struct Ctl {
       return;
     }
     request_solve();
+  }
+
+  // ---- DC sweep, analysis/dc.go:51-130 ------------------------------------------
+  // Every point overrides the swept sources' DC values (SetDCOverride,
+  // device/source.go:179-203: in ModeOP the source takes the override, which is
+  // the parameter pair dc = v, hasDC = 1) and runs Sim.OP on the same circuit
+  // state, so every point after the first Newton-starts warm from the last.
+  KH_HD void sweep_set_sources() {
+    const int idx[2] = {ci[CI_SW_I], ci[CI_SW_J]};
+    for (int s = 0; s < P.swN; s++) {
+      const int32_t* e = el(P.swElem[s]);
+      double* par = const_cast<double*>(A.par);
+      par[(int64_t)(e[ET_POFF] + 0) * B + c] = P.swStart[s] + (double)idx[s] * P.swStep[s];
+      par[(int64_t)(e[ET_POFF] + 1) * B + c] = 1.0;
+    }
+  }
+  KH_HD void sweep_begin() {
+    ci[CI_SW_I] = 0;
+    ci[CI_SW_J] = 0;
+    ci[CI_SW_DONE] = 0;
+    cd[CD_MAXREL] = 0;
+    sweep_point_begin();
+  }
+  KH_HD void sweep_point_begin() {
+    sweep_set_sources();
+    ci[CI_SW_IT0] = ci[CI_ITERS];
+    ci[CI_MODE] = MODE_OP;
+    cd[CD_TIME] = 0;
+    dc_begin(DCRET_OP);
+  }
+  KH_HD void sweep_point_done(int r) {
+    const int point = ci[CI_SW_J] * P.swCount[0] + ci[CI_SW_I];
+    A.sw_it[point] = ci[CI_ITERS] - ci[CI_SW_IT0];
+    if (r != ST_OK) { finish(r); return; }
+    accept(0, MODE_OP);
+    double rr = ledger_op();
+    cd[CD_LEDGER] = rr;
+    if (rr > cd[CD_MAXREL]) cd[CD_MAXREL] = rr;
+    if (!P.skipEnergy && rr > P.ledgerTol) { finish(ST_LEDGER); return; }
+    for (int i = 0; i < n; i++) A.sw_x[(int64_t)point * n + i] = X(A.x, i);
+    ci[CI_SW_DONE]++;
+    if (++ci[CI_SW_I] == P.swCount[0]) {
+      ci[CI_SW_I] = 0;
+      if (P.swN < 2 || ++ci[CI_SW_J] == P.swCount[1]) { finish(ST_OK); return; }
+    }
+    sweep_point_begin();
   }
 
   // ---- transient, analysis/tran.go:95-231 --------------------------------------
@@ -748,6 +802,7 @@ struct Ctl {
     if (ci[CI_PHASE] == PH_INIT) {
       ci[CI_PHASE] = PH_RUN;
       if (P.analysis == AN_TRAN) tran_begin();
+      else if (P.analysis == AN_DC) sweep_begin();
       else op_begin();
       return;
     }

@@ -34,15 +34,17 @@ struct Group {
 // starts at field + c * bytes. "Copy in" marks what a circuit reads before
 // writing within one launch (its state between launches), "copy out" what it
 // changes that outlives the launch; the rest is scratch of one factorization
-// or one convergence test. Parameters, pattern ids and per-circuit transient
-// specifications are read-only.
+// or one convergence test. Pattern ids and per-circuit transient
+// specifications are read-only; parameters change only where a DC sweep
+// overrides a source's value. The DC sweep's per-point outputs are not here:
+// they are written once per point, straight to device memory.
 template <typename G, typename F>
 KH_HD inline void for_each_array(G& g, F&& f) {
   const int64_t n = g.ctl.ckt.n, ns = g.ctl.ckt.nslots;
   const int64_t capL = g.lu.capL, capU = g.lu.capU, capR = g.lu.capR;
   auto& a = g.ctl;
   auto& l = g.lu;
-  f(a.par, g.npar * 8, true, false);
+  f(a.par, g.npar * 8, true, true);
   f(a.st, g.nstate * 8, true, true);
   f(a.val, ns * 8, true, true);
   f(a.rhs, n * 8, true, true);
@@ -89,11 +91,18 @@ KH_HD inline int64_t lds_need(const Group& g) {
   return total;
 }
 
-// The group as one circuit sees it once its working set is copied to `base`
+KH_HD inline int64_t sweep_points(const CtlParams& p) {
+  return p.analysis == AN_DC ? (int64_t)p.swCount[0] * (p.swN > 1 ? p.swCount[1] : 1) : 0;
+}
+
+// The group as circuit cc sees it once its working set is copied to `base`
 // (arrays in for_each_array order, each 8-byte aligned): run_circuit(view, 0,
 // ...) then works on the copies.
-KH_HD inline Group lds_view(const Group& g, char* base) {
+KH_HD inline Group lds_view(const Group& g, char* base, int64_t cc) {
   Group v = g;
+  const int64_t pts = sweep_points(g.ctl.p);
+  if (v.ctl.sw_x) v.ctl.sw_x += cc * pts * g.ctl.ckt.n;
+  if (v.ctl.sw_it) v.ctl.sw_it += cc * pts;
   int64_t off = 0;
   for_each_array(v, [&](auto& p, int64_t bytes, bool, bool) {
     p = (std::remove_reference_t<decltype(p)>)(base + off);
@@ -126,6 +135,9 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   a.lu_status += c;
   if (a.lane_spec) a.lane_spec += c * 4;
   if (a.nodeset) a.nodeset += c * (a.ckt.numNodes - 1);
+  const int64_t pts = sweep_points(a.p);
+  if (a.sw_x) a.sw_x += c * pts * n;
+  if (a.sw_it) a.sw_it += c * pts;
   l.B = 1;
   l.pid += c;
   l.val = a.val; l.rhs = a.rhs; l.x = a.x;

@@ -88,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) k_persist(const Group* G, const int32_
     }
     __syncthreads();
     if ((int)threadIdx.x < nc) {
-      Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes);
+      Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes, c0 + threadIdx.x);
       run_circuit(v, 0, max_solves, &running, &waiting);
     }
     __syncthreads();
@@ -151,7 +151,7 @@ __global__ void __launch_bounds__(256) k_transpose(const T* src, T* dst, int64_t
 // Per-circuit results into the caller's SoA arrays.
 __global__ void __launch_bounds__(256) k_outputs(const int32_t* ci, const double* cd, int64_t B, int32_t* status,
                                                  int32_t* iters, int32_t* acc, int32_t* rej, double* ledger,
-                                                 int32_t* solves) {
+                                                 int32_t* solves, int32_t* points) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= B) return;
   const int32_t* w = ci + c * NCI;
@@ -161,6 +161,7 @@ __global__ void __launch_bounds__(256) k_outputs(const int32_t* ci, const double
   if (rej) rej[c] = w[CI_REJECTED];
   if (ledger) ledger[c] = cd[c * NCD + CD_MAXREL];
   if (solves) solves[c] = w[CI_SOLVES];
+  if (points) points[c] = w[CI_SW_DONE];
 }
 
 // ---- device buffers -----------------------------------------------------------------
@@ -551,6 +552,8 @@ struct GroupRun {
   int n = 0;
   int32_t gid = 0;
   DBuf par, lane, nodeset, st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, pid, hash, solves;
+  DBuf swx, swit;           // DC sweep: per circuit [points][n] solutions, [points] Newton iterations
+  int64_t sw_points = 0;
   DevPatterns pats;
   LuWork work;
   bool has_lane = false;
@@ -584,6 +587,10 @@ struct GroupRun {
                          lane.as<double>(), (int64_t)4, B);
       KH_HIP(hipGetLastError());
     }
+    if (sw_points > 0) {
+      KH_HIP(swx.alloc((size_t)B * sw_points * T.n * 8));
+      KH_HIP(swit.alloc((size_t)B * sw_points * 4));
+    }
     has_nodeset = d_nodeset != nullptr && T.num_nodes > 1;
     if (has_nodeset) {
       int64_t rows = T.num_nodes - 1;
@@ -615,6 +622,8 @@ struct GroupRun {
     a.counters = nullptr;
     a.lane_spec = has_lane ? lane.as<double>() : nullptr;
     a.nodeset = has_nodeset ? nodeset.as<double>() : nullptr;
+    a.sw_x = sw_points ? swx.as<double>() : nullptr;
+    a.sw_it = sw_points ? swit.as<int32_t>() : nullptr;
     return a;
   }
 
@@ -688,7 +697,20 @@ struct GroupRun {
 
   // Per-circuit results (caller's SoA layout) and batch counters.
   kh_status outputs(double* d_x, int32_t* d_status, int32_t* d_iters, int32_t* d_acc, int32_t* d_rej,
-                    double* d_ledger) {
+                    double* d_ledger, double* d_sw_x = nullptr, int32_t* d_sw_it = nullptr,
+                    int32_t* d_points = nullptr) {
+    if (sw_points > 0 && d_sw_x) {  // [B][points*n] -> [points*n][B]
+      int64_t tot = (int64_t)sw_points * n * B;
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, swx.as<double>(),
+                         d_sw_x, B, (int64_t)sw_points * n);
+      KH_HIP(hipGetLastError());
+    }
+    if (sw_points > 0 && d_sw_it) {
+      int64_t tot = (int64_t)sw_points * B;
+      hipLaunchKernelGGL(k_transpose<int32_t>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0,
+                         swit.as<int32_t>(), d_sw_it, B, (int64_t)sw_points);
+      KH_HIP(hipGetLastError());
+    }
     if (d_x) {
       int64_t tot = (int64_t)n * B;
       hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, x.as<double>(),
@@ -696,7 +718,7 @@ struct GroupRun {
       KH_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_outputs, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, ci.as<int32_t>(),
-                       cd.as<double>(), B, d_status, d_iters, d_acc, d_rej, d_ledger, solves.as<int32_t>());
+                       cd.as<double>(), B, d_status, d_iters, d_acc, d_rej, d_ledger, solves.as<int32_t>(), d_points);
     KH_HIP(hipGetLastError());
     std::vector<int32_t> w(NCI * B);
     KH_HIP(hipMemcpy(w.data(), ci.p, (size_t)NCI * B * 4, hipMemcpyDeviceToHost));
@@ -877,25 +899,19 @@ kh_status tran_params(const kh_tran_spec* spec, CtlParams& p) {
 
 std::mutex g_run_mu;  // batch entry points run one at a time per process
 
-}  // namespace
+// Outputs of a DC sweep (one group).
+struct SweepOut {
+  double* d_x = nullptr;
+  int32_t* d_iters = nullptr;
+  int32_t* d_points = nullptr;
+};
 
-extern "C" {
-
-kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const kh_tran_spec* spec,
-                       kh_stats* stats) {
+// Runs every group of a batch to completion; p is the resolved analysis.
+kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const CtlParams& p,
+                    const kh_tran_spec* spec, const SweepOut* sw, kh_stats* stats) {
   if (ngroups <= 0 || !groups) return fail(KH_BAD_ARGUMENT, "no groups");
-  CtlParams p = resolve(opt, spec ? AN_TRAN : ((opt && opt->polish) ? AN_OP_POLISH : AN_OP));
-  if (spec) {
-    kh_status s = tran_params(spec, p);
-    if (s != KH_OK) return s;
-  }
   int32_t cpb = (opt && opt->circuits_per_wave > 0) ? opt->circuits_per_wave : 1;
   if (cpb > kBlock) return fail(KH_BAD_ARGUMENT, "circuits_per_wave above 64");
-  // Several circuits per wavefront and LDS staging have not passed on the
-  // GPU yet (a run that used them ended with a device fault whose cause was
-  // not found): both are refused until they do.
-  if (cpb != 1) return fail(KH_UNSUPPORTED, "circuits_per_wave other than 1 is disabled");
-  if (opt && opt->staging != 0) return fail(KH_UNSUPPORTED, "LDS staging is disabled");
   std::lock_guard<std::mutex> lk(g_run_mu);
   std::vector<std::unique_ptr<GroupRun>> runs;
   Runner R;
@@ -908,6 +924,7 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
     r->topo = G.topology;
     r->B = G.B;
     r->gid = g;
+    r->sw_points = sweep_points(p);
     kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr, G.d_nodeset);
     if (s != KH_OK) return s;
     R.groups.push_back(r.get());
@@ -917,13 +934,63 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
   if (s != KH_OK) return s;
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];
-    s = runs[g]->outputs(G.d_x, G.d_status, G.d_iters, G.d_accepted, G.d_rejected, G.d_ledger);
+    s = runs[g]->outputs(G.d_x, G.d_status, G.d_iters, G.d_accepted, G.d_rejected, G.d_ledger,
+                         sw ? sw->d_x : nullptr, sw ? sw->d_iters : nullptr, sw ? sw->d_points : nullptr);
     if (s != KH_OK) return s;
   }
   KH_HIP(hipDeviceSynchronize());
   R.total();
   if (stats) *stats = R.stats;
   return KH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const kh_tran_spec* spec,
+                       kh_stats* stats) {
+  CtlParams p = resolve(opt, spec ? AN_TRAN : ((opt && opt->polish) ? AN_OP_POLISH : AN_OP));
+  if (spec) {
+    kh_status s = tran_params(spec, p);
+    if (s != KH_OK) return s;
+  }
+  return batch_run(ngroups, groups, opt, p, spec, nullptr, stats);
+}
+
+kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                      const kh_dc_spec* spec, const double* d_nodeset, double* d_sweep_x, int32_t* d_status,
+                      int32_t* d_points, int32_t* d_iters, double* d_ledger, kh_stats* stats) {
+  if (!t || !spec) return fail(KH_BAD_ARGUMENT, "no topology or sweep");
+  if (spec->nsweeps < 1 || spec->nsweeps > 2) return fail(KH_BAD_ARGUMENT, "need 1 or 2 sweeps");
+  CtlParams p = resolve(opt, AN_DC);
+  p.swN = spec->nsweeps;
+  int64_t total = 1;
+  for (int s = 0; s < spec->nsweeps; s++) {
+    // DCSweep.values, analysis/dc.go:31-46
+    const double a = spec->start[s], b = spec->stop[s], h = spec->step[s];
+    if (h == 0 || std::isnan(a + b + h) || std::isinf(a + b + h)) return fail(KH_BAD_ARGUMENT, "invalid DC sweep");
+    const double cnt = (b - a) / h;
+    if (cnt < 0 || cnt > 1000000) return fail(KH_BAD_ARGUMENT, "invalid DC sweep");
+    p.swCount[s] = (int32_t)std::floor(cnt + 1e-9) + 1;
+    p.swStart[s] = a;
+    p.swStep[s] = h;
+    const int32_t e = spec->element[s];
+    if (e < 0 || e >= t->t.nel) return fail(KH_BAD_ARGUMENT, "swept element out of range");
+    const int32_t kind = t->t.et[(size_t)e * ET_W + ET_KIND];
+    if (kind != K_V && kind != K_I) return fail(KH_BAD_ARGUMENT, "swept element is not a V or I source");
+    p.swElem[s] = e;
+    total *= p.swCount[s];
+  }
+  if (total > 1000000) return fail(KH_BAD_ARGUMENT, "invalid DC sweep: too many points");
+  kh_group g = {};
+  g.topology = t; g.B = B; g.d_params = d_params; g.d_nodeset = d_nodeset; g.d_status = d_status;
+  g.d_ledger = d_ledger;
+  SweepOut sw;
+  sw.d_x = d_sweep_x;
+  sw.d_iters = d_iters;
+  sw.d_points = d_points;
+  return batch_run(1, &g, opt, p, nullptr, &sw, stats);
 }
 
 kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, double* d_x,

@@ -1,7 +1,7 @@
 // kh_bench — runs a batch file through kh_batch_run from plain C++ (no
 // Python, no PyTorch in the process), for timing and for profilers.
 //
-//   kh_bench <batch file> [steps] [warmup]
+//   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave]
 //   kh_bench --copy <MiB>    one device-to-device copy kernel of that size:
 //                            the reference point for hardware byte counters
 //
@@ -84,6 +84,8 @@ int main(int argc, char** argv) {
   if (strcmp(argv[1], "--copy") == 0) return copy_mode(argc > 2 ? atoi(argv[2]) : 256);
   const int steps = argc > 2 ? atoi(argv[2]) : 1;
   const int warmup = argc > 3 ? atoi(argv[3]) : 0;
+  const int staging = argc > 4 ? atoi(argv[4]) : 0;
+  const int cpw = argc > 5 ? atoi(argv[5]) : 0;
   FILE* f = fopen(argv[1], "rb");
   if (!f) return die("cannot open the batch file");
   char magic[4];
@@ -156,6 +158,8 @@ int main(int argc, char** argv) {
   }
   kh_options opt;
   memset(&opt, 0, sizeof(opt));
+  opt.staging = staging;
+  opt.circuits_per_wave = cpw;
   kh_tran_spec spec;
   memset(&spec, 0, sizeof(spec));
   spec.step = spec3[0];
@@ -188,10 +192,11 @@ int main(int argc, char** argv) {
   std::string h = "{";
   for (auto& kv : hist) h += (h.size() > 1 ? ", \"" : "\"") + std::to_string(kv.first) + "\": " + std::to_string(kv.second);
   h += "}";
-  printf("{\"circuits\": %lld, \"groups\": %d, \"steps\": %d, \"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
+  printf("{\"circuits\": %lld, \"groups\": %d, \"staging\": %d, \"circuits_per_wave\": %d, \"steps\": %d, "
+         "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
          "\"algorithmic_GBps\": %.4f, \"status_histogram\": %s}\n",
-         (long long)circuits, ngroups, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
+         (long long)circuits, ngroups, staging, cpw, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, h.c_str());
   for (void* p : owned) (void)hipFree(p);
   for (auto t : topo) kh_topology_destroy(t);

@@ -435,12 +435,15 @@ struct SourceValue {
   bool hasDC = false;
   Wave w;
   cplx ac = 0;
+  double override_ = 0;  // DC value forced by a .DC sweep (SetDCOverride, device/source.go:179-203)
+  bool overridden = false;
   double at(const LoadContext& lc) const {
     double v = raw(lc);
     if (lc.sourceStepping && lc.mode != ModeTran) v *= lc.srcFact;
     return v;
   }
   double raw(const LoadContext& lc) const {
+    if (overridden && lc.mode == ModeOP) return override_;
     if (w.kind == 0) return dc;
     if (lc.mode == ModeOP && hasDC) return dc;
     double t = lc.time;

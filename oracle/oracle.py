@@ -81,6 +81,7 @@ def lib():
         _lib.oracle_op_polished.argtypes = [C.c_int32, P, C.c_int32, P, P, P, P, P]
         _lib.oracle_tran.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
         _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
+        _lib.oracle_dc.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P, P, P]
         _lib.oracle_batch_op.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int32,
                                          C.c_int32, P, P, P]
         _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
@@ -210,6 +211,35 @@ def tran(num_nodes: int, els: np.ndarray, params: np.ndarray, spec: KhTranSpec, 
     c = info.count
     d.update(times=times[:c], xs=xs[:c], current=cur[:c])
     return e, x, d
+
+
+def sweep_values(start: float, stop: float, step: float) -> np.ndarray:
+    """DCSweep.values (analysis/dc.go:31-46): start to stop inclusive by step."""
+    cnt = (stop - start) / step
+    count = int(np.floor(cnt + 1e-9)) + 1
+    return start + np.arange(count, dtype=np.float64) * step
+
+
+def dc(num_nodes: int, els: np.ndarray, params: np.ndarray, sweeps, opts: Optional[KhOptions] = None):
+    """analysis.DC restatement. sweeps: [(element index, start, stop, step)], inner
+    first (one or two). Returns (status, x [points, n], iters [points], points done,
+    max ledger residual)."""
+    n = size(num_nodes, els, params)
+    total = 1
+    for _, a, b, s in sweeps:
+        total *= len(sweep_values(a, b, s))
+    x = np.zeros((total, n))
+    it = np.zeros(total, np.int32)
+    src = np.array([s[0] for s in sweeps], np.int32)
+    st = np.array([s[1] for s in sweeps], np.float64)
+    sp = np.array([s[2] for s in sweeps], np.float64)
+    sd = np.array([s[3] for s in sweeps], np.float64)
+    done = C.c_int32()
+    mr = C.c_double()
+    o = opts if opts is not None else KhOptions()
+    e = lib().oracle_dc(num_nodes, _p(els), len(els), _p(np.ascontiguousarray(params, np.float64)), C.byref(o),
+                        len(sweeps), _p(src), _p(st), _p(sp), _p(sd), _p(x), _p(it), C.byref(done), C.byref(mr))
+    return e, x, it, done.value, mr.value
 
 
 def ac(num_nodes: int, els: np.ndarray, params: np.ndarray, freqs, opts: Optional[KhOptions] = None):

@@ -597,6 +597,67 @@ struct Sim {
     return E_OK;
   }
 
+  // DCSweep.values, analysis/dc.go:31-46: start to stop inclusive by step.
+  static int sweepValues(double start, double stop, double step, std::vector<double>& out) {
+    if (step == 0 || std::isnan(start + stop + step) || std::isinf(start + stop + step)) return E_BADSPEC;
+    double cnt = (stop - start) / step;
+    if (cnt < 0 || cnt > 1000000) return E_BADSPEC;
+    int count = (int)std::floor(cnt + 1e-9) + 1;
+    out.resize(count);
+    for (int k = 0; k < count; k++) out[k] = start + (double)k * step;
+    return E_OK;
+  }
+
+  SourceValue* sourceOf(int e) {
+    Device* d = devs[e].get();
+    if (d->kind == KH_ELEM_V) return &static_cast<VoltageSource*>(d)->src;
+    if (d->kind == KH_ELEM_I) return &static_cast<CurrentSource*>(d)->src;
+    return nullptr;
+  }
+
+  // DC, analysis/dc.go:51-130: sweeps[0] is the inner loop; at every point
+  // the swept sources' DC values are overridden and Sim.OP runs on the same
+  // Sim, so every point after the first starts warm from the previous one.
+  // x of each point goes to xout[point][n], its Newton iterations to
+  // iters[point]; *done counts the points that succeeded.
+  int dcSweep(int nsw, const int* elem, const double* start, const double* stop, const double* step,
+              const Options& o, double* xout, int* iters, int* done, double* maxRel) {
+    if (nsw < 1 || nsw > 2) return E_BADSPEC;
+    std::vector<double> vals[2];
+    SourceValue* srcs[2] = {nullptr, nullptr};
+    size_t total = 1;
+    for (int i = 0; i < nsw; i++) {
+      int e = sweepValues(start[i], stop[i], step[i], vals[i]);
+      if (e != E_OK) return e;
+      total *= vals[i].size();
+      if (elem[i] < 0 || elem[i] >= (int)devs.size() || !(srcs[i] = sourceOf(elem[i]))) return E_BADSPEC;
+    }
+    if (total > 1000000) return E_BADSPEC;
+    std::vector<double> outer = nsw == 2 ? vals[1] : std::vector<double>{0.0};
+    *done = 0;
+    *maxRel = 0;
+    int point = 0, ret = E_OK;
+    for (double vo : outer) {
+      if (nsw == 2) { srcs[1]->override_ = vo; srcs[1]->overridden = true; }
+      for (double vi : vals[0]) {
+        srcs[0]->override_ = vi;
+        srcs[0]->overridden = true;
+        int it = 0;
+        double lr = 0;
+        int e = op(o, &it, &lr, nullptr);
+        if (iters) iters[point] = it;
+        if (e != E_OK) { ret = e; goto out; }
+        if (xout) std::copy(x.begin(), x.end(), xout + (size_t)point * n);
+        if (lr > *maxRel) *maxRel = lr;
+        ++*done;
+        ++point;
+      }
+    }
+  out:
+    for (int i = 0; i < nsw; i++) srcs[i]->overridden = false;  // ClearDCOverride
+    return ret;
+  }
+
   // AC sweep, analysis/ac.go:76-193: OP + polish, then per frequency LoadAC,
   // complex GP solve and complex-Tellegen check.
   int acSweep(const double* freqs, int nf, const Options& o, cplx* xout, double* maxRel, int* iters) {
@@ -801,6 +862,24 @@ int oracle_tran(int32_t num_nodes, const kh_element* el, int32_t nel, const doub
     info->max_rel = out.maxRel;
     info->t_end = out.tEnd;
   }
+  return e;
+}
+
+// DC sweep of one or two sources (element indices src[0] inner, src[1]
+// outer); xout [points][n], iters [points] (both sized by the caller for every
+// point of the sweep), *done = points solved.
+int oracle_dc(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params, const kh_options* ko,
+              int32_t nsweeps, const int32_t* src, const double* start, const double* stop, const double* step,
+              double* xout, int32_t* iters, int32_t* done, double* max_rel) {
+  Sim s;
+  int e = s.build(num_nodes, el, nel, params, 2, 0, 0);
+  if (e != E_OK) return e;
+  Options o;
+  o.from(ko);
+  int el2[2] = {src[0], nsweeps > 1 ? src[1] : -1};
+  int d = 0;
+  e = s.dcSweep(nsweeps, el2, start, stop, step, o, xout, iters, &d, max_rel);
+  *done = d;
   return e;
 }
 
