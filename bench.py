@@ -33,6 +33,22 @@ sys.path.insert(0, ROOT)
 
 METRIC = "circuits solved/sec at ≤1e-9 rel err vs FP64 oracle: chosen form vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_persist_rectifier4096_hbm.md")
+
+
+def measured_traffic_per_solve(staging: int, cpw: int):
+    """HBM bytes per executed solve of k_persist on this workload, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the same launch
+    configuration (None when that configuration was not measured)."""
+    key = f"traffic_bytes_per_solve[staging={staging},cpw={cpw}]:"
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith(key):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -114,6 +130,8 @@ def main():
                          f"{threads} worker threads, one task per circuit ({tcpu:.2f} s)"}
 
     achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
+    per_solve = measured_traffic_per_solve(args.staging, max(args.cpw, 1))
+    traffic = per_solve * solves / max(lu_n, 1) if per_solve is not None else None
     out = {
         "metric": METRIC,
         "value": value,
@@ -132,9 +150,11 @@ def main():
                    "circuits": b.B, "unknowns": b.n, "form": "plain FP64 GPU path",
                    "parallelism": "one wavefront per circuit (persistent kernel), 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_persist", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
-                     "bytes_per_launch": bytes_alg / max(lu_n, 1)},
+                     "bytes_per_launch": bytes_alg / max(lu_n, 1),
+                     "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per solve "
+                                       "(copy-kernel calibrated), profiles/k_persist_rectifier4096_hbm.md"},
         "cpu_baseline": cpu,
         "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
                    "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),

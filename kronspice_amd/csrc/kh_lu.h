@@ -26,6 +26,58 @@ namespace kh {
 
 enum LuStatus { LU_OK = 0, LU_SINGULAR = 1, LU_NONFINITE = 2, LU_CAPACITY = 14 };
 
+// Complex values (CSparse, solver/sparse.go:495-531) with Go's complex128
+// arithmetic: componentwise + and -, (ac - bd) + (ad + bc)i products, Smith's
+// division (R. L. Smith, Algorithm 116, Commun. ACM 5(8):435, 1962) and
+// magnitudes as math.Hypot computes them.
+struct kcplx {
+  double re, im;
+};
+KH_HD inline kcplx operator+(kcplx a, kcplx b) { return {a.re + b.re, a.im + b.im}; }
+KH_HD inline kcplx operator-(kcplx a, kcplx b) { return {a.re - b.re, a.im - b.im}; }
+KH_HD inline kcplx operator-(kcplx a) { return {-a.re, -a.im}; }
+KH_HD inline kcplx operator*(kcplx a, kcplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+KH_HD inline kcplx conj(kcplx a) { return {a.re, -a.im}; }
+KH_HD inline double go_hypot(double p, double q) {  // math.Hypot: p * sqrt(1 + (q/p)^2), p >= q
+  p = fabs(p);
+  q = fabs(q);
+  if (isinf(p) || isinf(q)) return __builtin_huge_val();
+  if (isnan(p) || isnan(q)) return __builtin_nan("");
+  if (p < q) {
+    double t = p;
+    p = q;
+    q = t;
+  }
+  if (p == 0) return 0;
+  q = q / p;
+  return p * sqrt(1 + q * q);
+}
+KH_HD inline double lu_mag(double v) { return fabs(v); }
+KH_HD inline double lu_mag(kcplx v) { return go_hypot(v.re, v.im); }
+KH_HD inline bool lu_zero(double v) { return v == 0; }
+KH_HD inline bool lu_zero(kcplx v) { return v.re == 0 && v.im == 0; }
+KH_HD inline bool lu_finite(double v) { return !isnan(v) && !isinf(v); }
+KH_HD inline bool lu_finite(kcplx v) { return lu_finite(v.re) && lu_finite(v.im); }
+KH_HD inline double lu_div(double a, double b) { return a / b; }
+KH_HD inline kcplx lu_div(kcplx n, kcplx m) {
+  double e, f;
+  if (fabs(m.re) >= fabs(m.im)) {
+    const double ratio = m.im / m.re;
+    const double denom = m.re + ratio * m.im;
+    e = (n.re + n.im * ratio) / denom;
+    f = (n.im - n.re * ratio) / denom;
+  } else {
+    const double ratio = m.re / m.im;
+    const double denom = m.im + ratio * m.re;
+    e = (n.re * ratio + n.im) / denom;
+    f = (n.im * ratio - n.re) / denom;
+  }
+  return {e, f};
+}
+template <typename T> KH_HD inline T lu_zero_value();
+template <> KH_HD inline double lu_zero_value<double>() { return 0.0; }
+template <> KH_HD inline kcplx lu_zero_value<kcplx>() { return {0.0, 0.0}; }
+
 // Permuted column structure of one pattern (solver/sparse.go:154-187):
 // column k holds rows arow[ap[k]..ap[k+1]) (permuted ids, ascending) with
 // values at slot aslot[...] of the lane's value array.
@@ -40,7 +92,9 @@ struct PatDesc {
 };
 
 // Everything a lane needs to factor and solve. Pointers are device memory.
-struct LuArgs {
+// T = double (real systems) or kcplx (AC systems).
+template <typename T>
+struct LuArgsT {
   int32_t n;          // unknowns
   int32_t nslots;     // value slots per lane
   int64_t B;          // lanes in the batch (SoA stride)
@@ -48,15 +102,15 @@ struct LuArgs {
   const PatDesc* pats;
   const int32_t* pool;
   const int32_t* pid;       // [B] pattern id per lane
-  const double* val;        // [nslots][B]
-  const double* rhs;        // [n][B]
-  double* x;                // [n][B] solution (written on success)
+  const T* val;             // [nslots][B]
+  const T* rhs;             // [n][B]
+  T* x;                     // [n][B] solution (written on success)
   // workspace
-  double* xw;               // [n][B] dense working column
-  double* w;                // [n][B] permuted right-hand side
-  double* udiag;            // [n][B]
-  double* lx;               // [capL][B]
-  double* ux;               // [capU][B]
+  T* xw;                    // [n][B] dense working column
+  T* w;                     // [n][B] permuted right-hand side
+  T* udiag;                 // [n][B]
+  T* lx;                    // [capL][B]
+  T* ux;                    // [capU][B]
   int32_t* li;              // [capL][B]
   int32_t* ui;              // [capU][B]
   int32_t* lp;              // [n+1][B]
@@ -74,6 +128,7 @@ struct LuArgs {
   int32_t* dk;              // [n][B] 1 if row k is in the reach of column k
   int32_t* cpid;            // [B] pattern id the cache belongs to, -1: none
 };
+using LuArgs = LuArgsT<double>;
 
 KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
 
@@ -81,7 +136,8 @@ KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
 // A(:,k) through the graph of L; returns top, the reach is xi[top..n).
 // mark[r] == k + 1 means "visited while processing column k" (the reference's
 // generation counter, restarted per factorization).
-KH_HD inline int lu_reach(const LuArgs& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
+template <typename T>
+KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
   const int64_t B = a.B;
   const int n = a.n;
   const int32_t gen = k + 1;
@@ -131,7 +187,8 @@ KH_HD inline int lu_reach(const LuArgs& a, int64_t c, int k, const int32_t* ap, 
 // The pivot rule is still evaluated at every column exactly as the reference
 // does; at the first column whose pivot differs, the search resumes. Results
 // are identical to re-running the search every time.
-KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
+template <typename T>
+KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
   const int64_t B = a.B;
   const int n = a.n;
   const int32_t pidc = a.pid[c];
@@ -141,9 +198,9 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
   const int32_t* aslot = a.pool + pd.aslot;
   double scale = 0.0;
   for (int s = 0; s < a.nslots; s++) {
-    double v = a.val[(int64_t)s * B + c];
-    if (!kh_finite(v)) return LU_NONFINITE;
-    double av = fabs(v);
+    T v = a.val[(int64_t)s * B + c];
+    if (!lu_finite(v)) return LU_NONFINITE;
+    double av = lu_mag(v);
     if (av > scale) scale = av;
   }
   if (scale == 0) {
@@ -174,12 +231,12 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
       a.dk[(int64_t)k * B + c] = diag_in_reach ? 1 : 0;
     }
     nr = r1;
-    for (int64_t p = r0; p < r1; p++) a.xw[(int64_t)a.rch[p * B + c] * B + c] = 0.0;
+    for (int64_t p = r0; p < r1; p++) a.xw[(int64_t)a.rch[p * B + c] * B + c] = lu_zero_value<T>();
     double colMax = 0.0;
     for (int p = ap[k]; p < ap[k + 1]; p++) {
-      double v = a.val[(int64_t)aslot[p] * B + c];
+      T v = a.val[(int64_t)aslot[p] * B + c];
       a.xw[(int64_t)arow[p] * B + c] = v;
-      double av = fabs(v);
+      double av = lu_mag(v);
       if (av > colMax) colMax = av;
     }
     double tiny = colMax * (double)n * 1e-14;
@@ -188,12 +245,12 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
       int32_t r = a.rch[p * B + c];
       int32_t j = a.pinv[(int64_t)r * B + c];
       if (j < 0) continue;
-      double xj = a.xw[(int64_t)r * B + c];
-      if (xj == 0) continue;
+      T xj = a.xw[(int64_t)r * B + c];
+      if (lu_zero(xj)) continue;
       int32_t q1 = a.lp[(int64_t)(j + 1) * B + c];
       for (int32_t q = a.lp[(int64_t)j * B + c]; q < q1; q++) {
         int64_t ix = (int64_t)a.li[(int64_t)q * B + c] * B + c;
-        double prod = a.lx[(int64_t)q * B + c] * xj;
+        T prod = a.lx[(int64_t)q * B + c] * xj;
         a.xw[ix] = a.xw[ix] - prod;
       }
     }
@@ -210,7 +267,7 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
         nu++;
         continue;
       }
-      double av = fabs(a.xw[(int64_t)r * B + c]);
+      double av = lu_mag(a.xw[(int64_t)r * B + c]);
       if (av > best || (av == best && r < piv)) {
         best = av;
         piv = r;
@@ -222,7 +279,7 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
     }
     // KLU diagonal preference (solver/sparse.go:274-277).
     if (a.pinv[(int64_t)k * B + c] < 0 && diag_in_reach) {
-      double ak = fabs(a.xw[(int64_t)k * B + c]);
+      double ak = lu_mag(a.xw[(int64_t)k * B + c]);
       if (ak >= 1e-3 * best && ak > tiny) piv = k;
     }
     if (reuse && piv != a.prow[(int64_t)k * B + c]) {
@@ -230,7 +287,7 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
       reuse = false;
       for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
     }
-    double pv = a.xw[(int64_t)piv * B + c];
+    T pv = a.xw[(int64_t)piv * B + c];
     a.udiag[(int64_t)k * B + c] = pv;
     a.pinv[(int64_t)piv * B + c] = k;
     a.prow[(int64_t)k * B + c] = piv;
@@ -239,7 +296,7 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
       if (a.pinv[(int64_t)r * B + c] < 0) {
         if (nl >= a.capL) return LU_CAPACITY;
         a.li[nl * B + c] = r;
-        a.lx[nl * B + c] = a.xw[(int64_t)r * B + c] / pv;
+        a.lx[nl * B + c] = lu_div(a.xw[(int64_t)r * B + c], pv);
         nl++;
       }
     }
@@ -254,39 +311,40 @@ KH_HD inline int lu_factor(const LuArgs& a, int64_t c) {
 // solve, solver/sparse.go:355-401 (the factorization is held in the lane's
 // workspace). x is written in original unknown order, stopping at the first
 // non-finite entry as the reference does.
-KH_HD inline int lu_solve(const LuArgs& a, int64_t c) {
+template <typename T>
+KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
   const int64_t B = a.B;
   const int n = a.n;
   const PatDesc pd = a.pats[a.pid[c]];
   const int32_t* q = a.pool + pd.q;
   const int32_t* qinv = a.pool + pd.qinv;
   for (int i = 0; i < n; i++) a.w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
-  double* y = a.xw;
+  T* y = a.xw;
   for (int k = 0; k < n; k++) {
-    double yk = a.w[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
+    T yk = a.w[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
     y[(int64_t)k * B + c] = yk;
-    if (yk == 0) continue;
+    if (lu_zero(yk)) continue;
     int32_t p1 = a.lp[(int64_t)(k + 1) * B + c];
     for (int32_t p = a.lp[(int64_t)k * B + c]; p < p1; p++) {
       int64_t ix = (int64_t)a.li[(int64_t)p * B + c] * B + c;
-      double prod = a.lx[(int64_t)p * B + c] * yk;
+      T prod = a.lx[(int64_t)p * B + c] * yk;
       a.w[ix] = a.w[ix] - prod;
     }
   }
   for (int k = n - 1; k >= 0; k--) {
-    double zk = y[(int64_t)k * B + c] / a.udiag[(int64_t)k * B + c];
+    T zk = lu_div(y[(int64_t)k * B + c], a.udiag[(int64_t)k * B + c]);
     y[(int64_t)k * B + c] = zk;
-    if (zk == 0) continue;
+    if (lu_zero(zk)) continue;
     int32_t p1 = a.up[(int64_t)(k + 1) * B + c];
     for (int32_t p = a.up[(int64_t)k * B + c]; p < p1; p++) {
       int64_t ix = (int64_t)a.ui[(int64_t)p * B + c] * B + c;
-      double prod = a.ux[(int64_t)p * B + c] * zk;
+      T prod = a.ux[(int64_t)p * B + c] * zk;
       y[ix] = y[ix] - prod;
     }
   }
   for (int k = 0; k < n; k++) {
-    double v = y[(int64_t)k * B + c];
-    if (!kh_finite(v)) return LU_NONFINITE;
+    T v = y[(int64_t)k * B + c];
+    if (!lu_finite(v)) return LU_NONFINITE;
     a.x[(int64_t)q[k] * B + c] = v;
   }
   return LU_OK;

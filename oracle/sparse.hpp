@@ -19,8 +19,43 @@ namespace oracle {
 
 enum Err { OK = 0, SINGULAR = 1, NONFINITE = 2, BADSIZE = 3 };
 
+// cmplx.Abs is math.Hypot (Go 1.24 math/hypot.go): the larger magnitude times
+// sqrt(1 + (smaller/larger)^2), not a correctly rounded hypot.
+inline double go_hypot(double p, double q) {
+  p = std::fabs(p);
+  q = std::fabs(q);
+  if (std::isinf(p) || std::isinf(q)) return std::numeric_limits<double>::infinity();
+  if (std::isnan(p) || std::isnan(q)) return std::numeric_limits<double>::quiet_NaN();
+  if (p < q) std::swap(p, q);
+  if (p == 0) return 0;
+  q = q / p;
+  return p * std::sqrt(1 + q * q);
+}
 inline double absval(double v) { return std::fabs(v); }
-inline double absval(std::complex<double> v) { return std::hypot(v.real(), v.imag()); }  // cmplx.Abs
+inline double absval(std::complex<double> v) { return go_hypot(v.real(), v.imag()); }  // cmplx.Abs
+
+// Division as Go's complex128 division does it: Smith's algorithm (R. L.
+// Smith, Algorithm 116, Commun. ACM 5(8):435, 1962), scaling by the larger
+// part of the divisor. (Go's fix-up of the both-NaN results of infinite or
+// zero operands is not needed here: every divisor is a pivot that passed the
+// singularity test, and a non-finite quotient is caught downstream.)
+inline double sdiv(double a, double b) { return a / b; }
+inline std::complex<double> sdiv(std::complex<double> n, std::complex<double> m) {
+  const double a = n.real(), b = n.imag(), c = m.real(), d = m.imag();
+  double e, f;
+  if (std::fabs(c) >= std::fabs(d)) {
+    const double ratio = d / c;
+    const double denom = c + ratio * d;
+    e = (a + b * ratio) / denom;
+    f = (b - a * ratio) / denom;
+  } else {
+    const double ratio = c / d;
+    const double denom = d + ratio * c;
+    e = (a * ratio + b) / denom;
+    f = (b * ratio - a) / denom;
+  }
+  return {e, f};
+}
 inline bool finite(double v) { return !std::isnan(v) && !std::isinf(v); }
 inline bool finite(std::complex<double> v) { return finite(v.real()) && finite(v.imag()); }
 
@@ -293,7 +328,7 @@ struct SparseLU {
         int32_t r = xi[p];
         if (pinv[r] < 0) {
           li.push_back(r);
-          lx.push_back(x[r] / pv);
+          lx.push_back(sdiv(x[r], pv));
         }
       }
     }
@@ -318,7 +353,7 @@ struct SparseLU {
       for (int p = lp[k]; p < lp[k + 1]; p++) w[li[p]] -= lx[p] * yk;
     }
     for (int k = n - 1; k >= 0; k--) {
-      T zk = y[k] / udiag[k];
+      T zk = sdiv(y[k], udiag[k]);
       y[k] = zk;
       if (zk == T(0)) continue;
       for (int p = up[k]; p < up[k + 1]; p++) y[ui[p]] -= ux[p] * zk;
