@@ -251,6 +251,19 @@ kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh
                       const kh_dc_spec* spec, const double* d_nodeset, double* d_sweep_x, int32_t* d_status,
                       int32_t* d_points, int32_t* d_iters, double* d_ledger, kh_stats* stats);
 
+/* analysis.AC for B circuits of one topology (analysis/ac.go:76-193): the
+ * operating point and polish, then at every frequency the devices' LoadAC
+ * stamps, a complex Gilbert–Peierls solve (CSparse, solver/sparse.go:495-531:
+ * pivoting on cmplx.Abs) and the complex-Tellegen ledger check. freqs [nf] is
+ * host memory (ACSpec.Frequencies' output); d_xac [nf*n*B*2] receives the
+ * solutions, frequency-major, then unknown, then circuit, each a (re, im)
+ * pair; d_points [B] (optional) the frequencies solved before a circuit's
+ * first failure; d_ledger [B] (optional) the largest ledger residual;
+ * d_nodeset as kh_batch_dc. */
+kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
+                      int32_t nf, const double* d_nodeset, double* d_xac, int32_t* d_status, int32_t* d_points,
+                      double* d_ledger, kh_stats* stats);
+
 /* analysis.OP for B circuits (analysis/op.go:331-357): dcSolve with the
  * gmin / source / pseudo-transient fallbacks (analysis/newton.go:95-311),
  * Accept, ledger check. x_out is [n*B] SoA on the device. iters (optional) the

@@ -105,6 +105,7 @@ def lib():
         L.kh_batch_op.argtypes = [P, i32, P, P, P, P, P, P]
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_dc.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
+        L.kh_batch_ac.argtypes = [P, i32, P, P, P, i32, P, P, P, P, P, P]
         _lib = L
         st = L.kh_init()
         if st != OK:
@@ -329,6 +330,30 @@ def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=N
     _check(lib().kh_batch_dc(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(nodeset), _p(x), _p(st), _p(done),
                              _p(it), _p(led), C.byref(s)))
     return Result(x, st, iters=it, ledger=led, stats=s.as_dict(), points=done)
+
+
+def ac(topo: Topology, params, freqs, opts: Optional[Options] = None, nodeset=None) -> Result:
+    """analysis.AC for every circuit at the host frequencies freqs [nf].
+    Result.x is a complex128 tensor [nf, n, B]; points [B] frequencies solved;
+    ledger [B] largest ledger residual."""
+    torch = _torch()
+    if isinstance(params, np.ndarray):
+        params = to_device(params)
+    if isinstance(nodeset, np.ndarray):
+        nodeset = to_device(nodeset)
+    B = params.shape[1]
+    f = np.ascontiguousarray(freqs, np.float64)
+    nf = len(f)
+    x = torch.zeros((nf, topo.n, B, 2), dtype=torch.float64, device="cuda")
+    st = torch.zeros(B, dtype=torch.int32, device="cuda")
+    done = torch.zeros(B, dtype=torch.int32, device="cuda")
+    led = torch.zeros(B, dtype=torch.float64, device="cuda")
+    o = Options() if opts is None else opts
+    s = Stats()
+    torch.cuda.synchronize()
+    _check(lib().kh_batch_ac(topo.h, B, _p(params), C.byref(o), _p(f), nf, _p(nodeset), _p(x), _p(st), _p(done),
+                             _p(led), C.byref(s)))
+    return Result(torch.view_as_complex(x), st, ledger=led, stats=s.as_dict(), points=done)
 
 
 class Group(C.Structure):

@@ -24,6 +24,7 @@
 
 #include "../../include/kronhip.h"
 #define KH_CLOCK() ((uint64_t)clock64())
+#include "kh_ac.h"
 #include "kh_group.h"
 #include "kh_symbolic.h"
 
@@ -164,6 +165,124 @@ __global__ void __launch_bounds__(256) k_outputs(const int32_t* ci, const double
   if (points) points[c] = w[CI_SW_DONE];
 }
 
+// ---- AC systems (kh_ac.h) ----------------------------------------------------------
+// Lane l of a chunk of AC systems is frequency f0 + l / B of circuit l % B:
+// consecutive lanes are consecutive circuits at one frequency.
+struct AcArgs {
+  Circ ckt;
+  const double* par;     // [B][npar] per-circuit parameters
+  const double* st;      // [B][nstate] device state accepted at the operating point
+  const int32_t* ci;     // [B][NCI] control words (status of the operating point)
+  int32_t npar, nstate;
+  int64_t B, BL;         // circuits; lanes of this chunk
+  int32_t f0;            // first frequency of the chunk
+  const double* freqs;   // [nf]
+  kcplx* val;            // [nslots][BL]
+  kcplx* rhs;            // [n][BL]
+  uint8_t* present;      // [B][nslots] stamped-slot marks (first chunk), or null
+  int32_t* lstatus;      // [BL]: -1 skipped (no operating point), else status
+  double* lrr;           // [BL] ledger relative residual
+};
+
+KH_HD inline ACLane ac_lane(const AcArgs& a, int64_t l) {
+  const int64_t c = l % a.B;
+  const int32_t f = a.f0 + (int32_t)(l / a.B);
+  ACLane A;
+  A.dev = Lane{};
+  A.dev.c = 0;
+  A.dev.B = 1;
+  A.dev.n = a.ckt.n;
+  A.dev.et = a.ckt.et;
+  A.dev.par = a.par + c * a.npar;
+  A.dev.st = const_cast<double*>(a.st) + c * a.nstate;
+  A.l = l;
+  A.BL = a.BL;
+  A.val = a.val;
+  A.rhs = a.rhs;
+  A.present = nullptr;
+  A.omega = 2 * 3.14159265358979323846 * a.freqs[f];
+  return A;
+}
+
+// Stamps every lane's AC system (LoadAC of each element at omega).
+__global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= a.BL) return;
+  for (int s = 0; s < a.ckt.nslots; s++) a.val[(int64_t)s * a.BL + l] = kcplx{0, 0};
+  for (int i = 0; i < a.ckt.n; i++) a.rhs[(int64_t)i * a.BL + l] = kcplx{0, 0};
+  const int64_t c = l % a.B;
+  if (a.ci[c * NCI + CI_STATUS] != ST_OK) {
+    a.lstatus[l] = -1;
+    return;
+  }
+  ACLane A = ac_lane(a, l);
+  if (a.present && l < a.B) A.present = a.present + c * a.ckt.nslots;
+  ac_load(A, a.ckt.nel);
+  a.lstatus[l] = ST_OK;
+}
+
+// Factors and solves every stamped lane (complex GP LU), evaluates its
+// ledger (analysis/ac.go:148-190) and writes its solution to the caller's
+// [nf][n][B] complex array.
+__global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu, double ledger_tol,
+                                                     int32_t skip_energy, double* xout) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= a.BL || a.lstatus[l] < 0) return;
+  int st = lu_factor(lu, l);
+  if (st == LU_OK) st = lu_solve(lu, l);
+  if (st != LU_OK) {
+    a.lstatus[l] = st;
+    return;
+  }
+  const int n = a.ckt.n;
+  const int64_t BL = a.BL;
+  ACLane A = ac_lane(a, l);
+  kcplx sumS;
+  double scaleS;
+  ac_power(A, a.ckt.nel, lu.x, &sumS, &scaleS);
+  // Round-off floor (output.Ledger.Roundoff): 64 eps sum |x_i| (sum_j |A_ij||x_j| + |b_i|)
+  kcplx* y = lu.w;  // free after the solve: y[i].re accumulates AbsProduct
+  for (int i = 0; i < n; i++) y[(int64_t)i * BL + l] = kcplx{0, 0};
+  for (int s = 0; s < a.ckt.nslots; s++) {
+    const int i = a.ckt.slot_i[s], j = a.ckt.slot_j[s];
+    y[(int64_t)i * BL + l].re += lu_mag(a.val[(int64_t)s * BL + l]) * lu_mag(lu.x[(int64_t)j * BL + l]);
+  }
+  double ro = 0;
+  for (int i = 0; i < n; i++)
+    ro += lu_mag(lu.x[(int64_t)i * BL + l]) * (y[(int64_t)i * BL + l].re + lu_mag(a.rhs[(int64_t)i * BL + l]));
+  ro *= 64 * 0x1p-52;
+  const double mag = go_hypot(sumS.re, sumS.im);
+  const double rr = (scaleS > 0 && mag > ro) ? mag / scaleS : 0.0;
+  a.lrr[l] = rr;
+  a.lstatus[l] = (!skip_energy && rr > ledger_tol) ? ST_LEDGER : ST_OK;
+  const int64_t c = l % a.B, f = a.f0 + l / a.B;
+  for (int i = 0; i < n; i++) {
+    const kcplx v = lu.x[(int64_t)i * BL + l];
+    const int64_t o = ((f * n + i) * a.B + c) * 2;
+    xout[o] = v.re;
+    xout[o + 1] = v.im;
+  }
+}
+
+// Per circuit, walks the chunk's frequencies in order as the sweep does: the
+// first failing frequency sets the circuit's status and stops it.
+__global__ void __launch_bounds__(256) k_ac_reduce(AcArgs a, int32_t* done, int32_t* status, int32_t* points,
+                                                   double* maxrr) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.B || done[c]) return;
+  for (int64_t l = c; l < a.BL; l += a.B) {
+    const int s = a.lstatus[l];
+    if (s < 0) { done[c] = 1; return; }  // the operating point failed: its status stands
+    if (s == ST_OK || s == ST_LEDGER) maxrr[c] = fmax(maxrr[c], a.lrr[l]);
+    if (s != ST_OK) {
+      status[c] = s;
+      done[c] = 1;
+      return;
+    }
+    points[c]++;
+  }
+}
+
 // ---- device buffers -----------------------------------------------------------------
 struct DBuf {
   void* p = nullptr;
@@ -188,8 +307,10 @@ struct DBuf {
   template <typename T> T* as() const { return static_cast<T*>(p); }
 };
 
-// LU workspace for B lanes of n unknowns with capL / capU entries.
-struct LuWork {
+// LU workspace for B lanes of n unknowns with capL / capU entries, values of
+// type T (double, or kcplx for AC systems).
+template <typename T>
+struct LuWorkT {
   DBuf xw, w, udiag, lx, ux, li, ui, lp, up, pinv, prow, xi, pstack, mark, err, rch, rp, dk, cpid;
   int64_t capL = 0, capU = 0, capR = 0, B = 0;
   int n = 0;
@@ -199,11 +320,11 @@ struct LuWork {
     B = B_;
     capL = capU = cap;
     capR = capL + capU + n;
-    size_t nd = (size_t)n * B * 8, ni = (size_t)n * B * 4;
+    size_t nd = (size_t)n * B * sizeof(T), ni = (size_t)n * B * 4;
     hipError_t e;
 #define KH_A(buf, bytes_) if ((e = buf.alloc(bytes_)) != hipSuccess) return e
     KH_A(xw, nd); KH_A(w, nd); KH_A(udiag, nd);
-    KH_A(lx, (size_t)capL * B * 8); KH_A(ux, (size_t)capU * B * 8);
+    KH_A(lx, (size_t)capL * B * sizeof(T)); KH_A(ux, (size_t)capU * B * sizeof(T));
     KH_A(li, (size_t)capL * B * 4); KH_A(ui, (size_t)capU * B * 4);
     KH_A(lp, (size_t)(n + 1) * B * 4); KH_A(up, (size_t)(n + 1) * B * 4);
     KH_A(pinv, ni); KH_A(prow, ni); KH_A(xi, ni); KH_A(pstack, ni); KH_A(mark, ni);
@@ -215,21 +336,24 @@ struct LuWork {
   }
   // Forget the cached reach lists (a pattern id now names a different pattern).
   hipError_t invalidate() { return hipMemset(cpid.p, 0xFF, (size_t)B * 4); }  // -1: no cached reach
-  void fill(LuArgs& a) const {
+  void fill(LuArgsT<T>& a) const {
     a.capL = capL; a.capU = capU; a.capR = capR;
     a.rch = rch.as<int32_t>(); a.rp = rp.as<int32_t>(); a.dk = dk.as<int32_t>(); a.cpid = cpid.as<int32_t>();
-    a.xw = xw.as<double>(); a.w = w.as<double>(); a.udiag = udiag.as<double>();
-    a.lx = lx.as<double>(); a.ux = ux.as<double>(); a.li = li.as<int32_t>(); a.ui = ui.as<int32_t>();
+    a.xw = xw.as<T>(); a.w = w.as<T>(); a.udiag = udiag.as<T>();
+    a.lx = lx.as<T>(); a.ux = ux.as<T>(); a.li = li.as<int32_t>(); a.ui = ui.as<int32_t>();
     a.lp = lp.as<int32_t>(); a.up = up.as<int32_t>(); a.pinv = pinv.as<int32_t>(); a.prow = prow.as<int32_t>();
     a.xi = xi.as<int32_t>(); a.pstack = pstack.as<int32_t>(); a.mark = mark.as<int32_t>();
     a.err_step = err.as<int32_t>();
   }
 };
 
+using LuWork = LuWorkT<double>;
+
 // L and U capacity (entries per system) for a pattern with structural fill
 // max_fill = |L| + |U| + n under diagonal pivoting, with a wide margin for
 // pivoting away from the diagonal. Batched analyses that still overflow it
 // double it and refactor (WAIT_CAPACITY).
+
 int64_t lu_capacity(int64_t max_fill, int n) { return 2 * max_fill + 4 * (int64_t)n + 64; }
 
 // Pattern pool mirrored on the device.
@@ -907,12 +1031,16 @@ struct SweepOut {
 };
 
 // Runs every group of a batch to completion; p is the resolved analysis.
+// The group runs are handed to `keep` (when given) for a further phase on
+// their per-circuit state; the caller then holds g_run_mu itself.
 kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const CtlParams& p,
-                    const kh_tran_spec* spec, const SweepOut* sw, kh_stats* stats) {
+                    const kh_tran_spec* spec, const SweepOut* sw, kh_stats* stats,
+                    std::vector<std::unique_ptr<GroupRun>>* keep = nullptr) {
   if (ngroups <= 0 || !groups) return fail(KH_BAD_ARGUMENT, "no groups");
   int32_t cpb = (opt && opt->circuits_per_wave > 0) ? opt->circuits_per_wave : 1;
   if (cpb > kBlock) return fail(KH_BAD_ARGUMENT, "circuits_per_wave above 64");
-  std::lock_guard<std::mutex> lk(g_run_mu);
+  std::unique_lock<std::mutex> lk(g_run_mu, std::defer_lock);
+  if (!keep) lk.lock();
   std::vector<std::unique_ptr<GroupRun>> runs;
   Runner R;
   R.cpb = cpb;
@@ -941,6 +1069,7 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
   KH_HIP(hipDeviceSynchronize());
   R.total();
   if (stats) *stats = R.stats;
+  if (keep) *keep = std::move(runs);
   return KH_OK;
 }
 
@@ -991,6 +1120,122 @@ kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh
   sw.d_iters = d_iters;
   sw.d_points = d_points;
   return batch_run(1, &g, opt, p, nullptr, &sw, stats);
+}
+
+kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
+                      int32_t nf, const double* d_nodeset, double* d_xac, int32_t* d_status, int32_t* d_points,
+                      double* d_ledger, kh_stats* stats) {
+  if (!t || B <= 0 || !d_params || !freqs || nf <= 0 || !d_xac || !d_status)
+    return fail(KH_BAD_ARGUMENT, "bad AC arguments");
+  for (int32_t k = 0; k < nf; k++)
+    if (!(freqs[k] >= 0) || std::isinf(freqs[k])) return fail(KH_BAD_ARGUMENT, "bad AC frequency");
+  std::lock_guard<std::mutex> lk(g_run_mu);
+  // analysis.AC: the operating point, then polish (analysis/ac.go:85-90).
+  CtlParams p = resolve(opt, AN_OP_POLISH);
+  kh_group g = {};
+  g.topology = t; g.B = B; g.d_params = d_params; g.d_nodeset = d_nodeset; g.d_status = d_status;
+  std::vector<std::unique_ptr<GroupRun>> keep;
+  kh_status s = batch_run(1, &g, opt, p, nullptr, nullptr, stats, &keep);
+  if (s != KH_OK) return s;
+  GroupRun& G = *keep[0];
+  const Topology& T = t->t;
+  const int n = T.n;
+  // Frequencies in chunks of at most kAcLanes systems.
+  constexpr int64_t kAcLanes = 1 << 18;
+  const int32_t fchunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nf, kAcLanes / B));
+  const int64_t BLmax = (int64_t)fchunk * B;
+  DBuf dfreq, val, rhs, x, present, lst, lrr, pid, done, points, maxrr;
+  KH_HIP(dfreq.alloc((size_t)nf * 8));
+  KH_HIP(hipMemcpy(dfreq.p, freqs, (size_t)nf * 8, hipMemcpyHostToDevice));
+  KH_HIP(val.alloc((size_t)T.nslots * BLmax * sizeof(kcplx)));
+  KH_HIP(rhs.alloc((size_t)n * BLmax * sizeof(kcplx)));
+  KH_HIP(x.alloc((size_t)n * BLmax * sizeof(kcplx)));
+  KH_HIP(present.alloc((size_t)B * T.nslots));
+  KH_HIP(lst.alloc((size_t)BLmax * 4));
+  KH_HIP(lrr.alloc((size_t)BLmax * 8));
+  KH_HIP(pid.alloc((size_t)BLmax * 4));
+  KH_HIP(done.alloc((size_t)B * 4));
+  KH_HIP(points.alloc((size_t)B * 4));
+  KH_HIP(maxrr.alloc((size_t)B * 8));
+  DevPatterns pats;
+  LuWorkT<kcplx> work;
+  std::vector<int32_t> cpid(B, -1);
+  AcArgs a = {};
+  a.ckt = G.ctl_args(p).ckt;
+  a.par = G.par.as<double>();
+  a.st = G.st.as<double>();
+  a.ci = G.ci.as<int32_t>();
+  a.npar = T.nparams;
+  a.nstate = std::max(T.nstate, 1);
+  a.B = B;
+  a.freqs = dfreq.as<double>();
+  a.val = val.as<kcplx>();
+  a.rhs = rhs.as<kcplx>();
+  a.lstatus = lst.as<int32_t>();
+  a.lrr = lrr.as<double>();
+  auto t0 = std::chrono::steady_clock::now();
+  for (int32_t f0 = 0; f0 < nf; f0 += fchunk) {
+    const int32_t nfc = std::min(fchunk, nf - f0);
+    a.f0 = f0;
+    a.BL = (int64_t)nfc * B;
+    a.present = f0 == 0 ? present.as<uint8_t>() : nullptr;
+    const unsigned grid = (unsigned)((a.BL + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_ac_stamp, dim3(grid), dim3(kBlock), 0, 0, a);
+    KH_HIP(hipGetLastError());
+    if (f0 == 0) {
+      // The AC pattern of each circuit (LoadAC skips zero channel conductances,
+      // so it depends on the operating point): one ordering per distinct pattern.
+      std::vector<uint8_t> pr((size_t)B * T.nslots);
+      KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
+      std::vector<int32_t> wst(B);
+      KH_HIP(hipMemcpy2D(wst.data(), 4, G.ci.as<int32_t>() + CI_STATUS, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
+      for (int64_t c = 0; c < B; c++) {
+        if (wst[c] != ST_OK) continue;
+        std::vector<int32_t> slots;
+        for (int s2 = 0; s2 < T.nslots; s2++)
+          if (pr[(size_t)c * T.nslots + s2]) slots.push_back(s2);
+        std::string key = slots_key(slots);
+        auto f = pats.host.by_key.find(key);
+        if (f != pats.host.by_key.end()) {
+          cpid[c] = f->second;
+        } else {
+          cpid[c] = pats.host.add(analyze_pattern(n, T.slot_i, T.slot_j, slots), key);
+        }
+      }
+      KH_HIP(pats.sync());
+      if (pats.host.desc.empty()) break;  // no operating point succeeded
+      KH_HIP(work.ensure(n, BLmax, lu_capacity(pats.host.max_fill, n)));
+      std::vector<int32_t> lp((size_t)BLmax);
+      for (int64_t l = 0; l < BLmax; l++) lp[l] = std::max(cpid[l % B], 0);
+      KH_HIP(hipMemcpy(pid.p, lp.data(), lp.size() * 4, hipMemcpyHostToDevice));
+      KH_HIP(hipMemset(done.p, 0, (size_t)B * 4));
+      KH_HIP(hipMemset(points.p, 0, (size_t)B * 4));
+      KH_HIP(hipMemset(maxrr.p, 0, (size_t)B * 8));
+    }
+    LuArgsT<kcplx> lu = {};
+    lu.n = n;
+    lu.nslots = T.nslots;
+    lu.B = a.BL;
+    lu.pats = pats.desc.as<PatDesc>();
+    lu.pool = pats.pool.as<int32_t>();
+    lu.pid = pid.as<int32_t>();
+    lu.val = a.val;
+    lu.rhs = a.rhs;
+    lu.x = x.as<kcplx>();
+    work.fill(lu);
+    KH_HIP(work.invalidate());
+    hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac);
+    KH_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),
+                       d_status, points.as<int32_t>(), maxrr.as<double>());
+    KH_HIP(hipGetLastError());
+    if (stats) stats->solves += a.BL;
+  }
+  KH_HIP(hipDeviceSynchronize());
+  if (d_points) KH_HIP(hipMemcpy(d_points, points.p, (size_t)B * 4, hipMemcpyDeviceToDevice));
+  if (d_ledger) KH_HIP(hipMemcpy(d_ledger, maxrr.p, (size_t)B * 8, hipMemcpyDeviceToDevice));
+  if (stats) stats->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return KH_OK;
 }
 
 kh_status kh_batch_op(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, double* d_x,

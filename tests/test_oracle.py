@@ -335,3 +335,27 @@ def test_dc_sweep_warm_matches_golden(name):
         _, _, info = oracle.op(c.num_nodes, els, pv)
         cold.append(info["iters"])
     assert np.mean(it[1:6]) < np.mean(cold)
+
+
+def test_ac_rc_lowpass_closed_form():
+    """analysis.AC on RC low-pass circuits with random R and C: v(out) for a
+    unit AC source is 1/(1 + j w R C) (textbook transfer function), which
+    fixes the complex stamps of R, C and V, the complex LU and the
+    unknown numbering."""
+    B, nf = 8, 64
+    rng = np.random.default_rng(5)
+    R = np.exp(rng.uniform(np.log(10.0), np.log(1e5), B))
+    Cv = np.exp(rng.uniform(np.log(1e-12), np.log(1e-6), B))
+    base = (kc.Builder("rc").vsource("V1", "in", "0", 0.0, ac=1.0).resistor("R1", "in", "out", 1e3)
+            .capacitor("C1", "out", "0", 1e-9).build())
+    els, p0 = base.flatten()
+    kr, kcap = base.element_index("R1"), base.element_index("C1")
+    freqs = np.logspace(1, 9, nf)
+    out = base.node("out") - 1
+    for b in range(B):
+        p = p0.copy()
+        p[els[kr]["poff"]], p[els[kr]["poff"] + 1], p[els[kcap]["poff"]] = R[b], 1.0 / R[b], Cv[b]
+        e, xo, mr = oracle.ac(base.num_nodes, els, p, freqs)
+        assert e == 0 and mr <= 1e-9
+        H = 1.0 / (1.0 + 1j * 2 * np.pi * freqs * R[b] * Cv[b])
+        assert np.max(np.abs(xo[:, out] - H)) <= 1e-12
