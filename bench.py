@@ -9,6 +9,12 @@ already resident in HBM):
   trapezoidal integration with LTE step control, Newton to reltol 1e-3,
   abstol 1e-12, the reference's DC operating point first (analysis.Tran).
 
+Forms: the plain FP64 GPU path (in place in HBM, one circuit per wavefront)
+runs the batch once; the chosen form is the launch configuration the
+autotuner (kronspice_amd/autotune.py) picks on a sample of the batch
+(LDS staging and circuits per wavefront), and it is what `value` times.
+Both compute the same results bit for bit.
+
 Prints one JSON line. `roofline` is for the dominant kernel (k_persist, the
 persistent factor+solve+stamp kernel; every other kernel of the run is a
 layout change or a fingerprint at the run's edges): algorithmic bytes
@@ -36,11 +42,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_persist_rectifier4096_hbm.md")
 
 
-def measured_traffic_per_solve(staging: int, cpw: int):
+def measured_traffic_per_solve(staging: int, cpw: int, waves: int):
     """HBM bytes per executed solve of k_persist on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the same launch
     configuration (None when that configuration was not measured)."""
-    key = f"traffic_bytes_per_solve[staging={staging},cpw={cpw}]:"
+    key = f"traffic_bytes_per_solve[staging={staging},cpw={cpw},waves={waves}]:"
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
@@ -60,8 +66,11 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=4096)
     ap.add_argument("--parity-sample", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpw", type=int, default=1, help="circuits per wavefront")
-    ap.add_argument("--staging", type=int, default=0, help="0: circuits worked on in place in HBM")
+    ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: autotuned)")
+    ap.add_argument("--staging", type=int, default=-1, help="0 in place, 1 LDS staged (-1: autotuned)")
+    ap.add_argument("--waves", type=int, default=0, help="resident wavefronts per SIMD (0: autotuned)")
+    ap.add_argument("--tune-sample", type=int, default=1024)
+    ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); circuits shard across GPUs with no "
@@ -69,15 +78,32 @@ def main():
         sys.exit(2)
 
     import torch
-    from kronspice_amd import api, workloads as W
+    from kronspice_amd import api, autotune, workloads as W
 
     b = W.rectifier_rc_ladders(args.batch)
     topo = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
     spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
-    opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging)
     d_params = api.to_device(b.params)
     d_lane = api.to_device(b.lane_spec)
     torch.cuda.synchronize()
+
+    # Plain FP64 GPU form: in place, one circuit per wavefront.
+    plain_s = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        api.tran(topo, d_params, spec, opts=api.Options(staging=0, circuits_per_wave=1), lane_spec=d_lane)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+
+    # Chosen form: the autotuner's pick on a sample of the batch.
+    tuned = None
+    if args.staging < 0 or args.cpw <= 0 or args.waves <= 0:
+        cols = autotune.sample_columns(b.B, args.tune_sample)
+        sp = api.to_device(np.ascontiguousarray(b.params[:, cols]))
+        sl = api.to_device(np.ascontiguousarray(b.lane_spec[:, cols]))
+        tuned = autotune.tune(lambda o: api.tran(topo, sp, spec, opts=o, lane_spec=sl))
+        args.staging, args.cpw, args.waves = tuned.staging, tuned.circuits_per_wave, tuned.waves_per_simd
+    opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging, waves_per_simd=args.waves)
 
     res = None
     for _ in range(args.warmup):
@@ -130,7 +156,7 @@ def main():
                          f"{threads} worker threads, one task per circuit ({tcpu:.2f} s)"}
 
     achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
-    per_solve = measured_traffic_per_solve(args.staging, max(args.cpw, 1))
+    per_solve = measured_traffic_per_solve(args.staging, max(args.cpw, 1), max(args.waves, 1))
     traffic = per_solve * solves / max(lu_n, 1) if per_solve is not None else None
     out = {
         "metric": METRIC,
@@ -147,8 +173,10 @@ def main():
         "data": "synthetic (seeded counter-based generator)",
         "config": {"workload": "rectifier_rc_ladder_tran: half-wave rectifier + 32-section RC ladder, "
                                "4 source periods, max step stop/2000, trapezoidal + LTE, FP64 GP LU",
-                   "circuits": b.B, "unknowns": b.n, "form": "plain FP64 GPU path",
-                   "parallelism": "one wavefront per circuit (persistent kernel), 1 GPU"},
+                   "circuits": b.B, "unknowns": b.n,
+                   "form": f"chosen: autotuned launch (staging={args.staging}, circuits per wavefront={args.cpw}, "
+                           f"wavefronts per SIMD={args.waves}), FP64 GP LU",
+                   "parallelism": f"persistent kernel, {args.cpw} circuit(s) per wavefront, 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_persist", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
@@ -159,6 +187,13 @@ def main():
         "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
                    "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),
                    "same_step_count": float((acc[idx] == ao).mean())},
+        "forms": {"plain_fp64_gpu": ({"circuits_per_s": b.B / plain_s, "seconds": plain_s} if plain_s else None),
+                  "chosen": {"circuits_per_s": b.B / (ms_per_step / 1e3), "staging": args.staging,
+                             "circuits_per_wave": args.cpw, "waves_per_simd": args.waves},
+                  "chosen_vs_plain": (plain_s / (ms_per_step / 1e3)) if plain_s else None,
+                  "autotune": ({"sample": args.tune_sample,
+                                "seconds": {f"staging={a},cpw={c},waves={w}": t for a, c, w, t in tuned.timings}}
+                               if tuned else None)},
         "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
                    "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "staging": args.staging,
                    "solves_per_step": solves / args.steps,

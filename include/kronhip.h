@@ -117,7 +117,9 @@ typedef struct kh_options {
   int32_t staging;          /* batched analyses: 0 -> circuits are worked on in place in
                                device memory; 1 -> each circuit's working set is held in
                                LDS during a launch when it fits (64 KiB per workgroup) */
-  int32_t reserved;
+  int32_t waves_per_simd;   /* batched analyses: register budget of the persistent kernel,
+                               as resident wavefronts per SIMD: 0 or 1 -> unconstrained,
+                               2 or 4 -> fewer registers (spilled), more circuits in flight */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */

@@ -11,35 +11,55 @@ LIB = os.path.join(HERE, "libkronhip.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
-    "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-O3", "-std=c++17", "-fPIC",
     "--offload-arch=gfx950",
     # Bit-exact with the reference: every a - b*c rounds twice, as in Go.
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-SOURCES = ["kronhip.hip", "kh_symbolic.cpp"]
+# (source, extra flags, object): the persistent kernel once per register
+# budget (resident wavefronts per SIMD), each in its own translation unit.
+UNITS = [
+    ("kronhip.hip", [], "kronhip.o"),
+    ("kh_symbolic.cpp", [], "kh_symbolic.o"),
+    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=1"], "kh_persist_w1.o"),
+    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=2"], "kh_persist_w2.o"),
+    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=4"], "kh_persist_w4.o"),
+]
 BENCH = os.path.join(HERE, "kh_bench")
+OBJ = os.path.join(HERE, "_obj")
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc ({what}) failed:\n" + r.stdout + r.stderr)
+    return r
 
 
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = []
+    for src, extra, obj in UNITS:
+        cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
+        jobs.append((cmd, src + " " + " ".join(extra)))
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        results = list(ex.map(lambda j: _run(*j), jobs))
+    if verbose:
+        for r in results:
+            if r.stderr:
+                print(r.stderr)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and r.stderr:
-        print(r.stderr)
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950"] + [os.path.join(OBJ, o) for _, _, o in UNITS] +
+         ["-o", tmp], "link")
     os.replace(tmp, LIB)
     # kh_bench: the C++ driver (and one copy kernel), linked against the library next to it.
     tmpb = BENCH + ".tmp"
-    cmd = [HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
-           "-packed-fp32-ops", "-x", "hip", os.path.join(HERE, "kh_bench.cpp"), "-x", "none", "-L" + HERE, "-lkronhip",
-           "-Wl,-rpath,$ORIGIN", "-o", tmpb]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("hipcc (kh_bench) failed:\n" + r.stdout + r.stderr)
+    _run([HIPCC, "-O2", "-std=c++17", "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", "-x", "hip", os.path.join(HERE, "kh_bench.cpp"), "-x", "none", "-L" + HERE, "-lkronhip",
+          "-Wl,-rpath,$ORIGIN", "-o", tmpb], "kh_bench")
     os.replace(tmpb, BENCH)
     return LIB
 

@@ -15,7 +15,7 @@
 
 namespace kh {
 
-// Integer control words (SoA [NCI][B]).
+// Integer control words ([NCI] per circuit).
 enum {
   CI_PHASE = 0, CI_CALLER, CI_NT_IT, CI_NT_MAX, CI_INITJCT, CI_WARM, CI_SRCSTEP, CI_STATUS, CI_ITERS,
   CI_FIRST_ERR, CI_GM_OK, CI_GM_K, CI_SS_K, CI_PT_K, CI_PT_QUIET, CI_TR_ORDER, CI_TR_HIT, CI_ACCEPTED,
@@ -24,7 +24,7 @@ enum {
 };
 // CI_WAIT: why a circuit waits for the host between launches.
 enum { WAIT_NONE = 0, WAIT_PATTERN = 1, WAIT_CAPACITY = 2 };
-// Double control words (SoA [NCD][B]).
+// Double control words ([NCD] per circuit).
 enum {
   CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
   CD_SS_F, CD_PT_H, CD_TR_T, CD_TR_H, CD_TR_TN, CD_PEAK, CD_WORST, CD_MAXREL, CD_LEDGER, CD_MINSTEP_SEEN,
@@ -95,35 +95,31 @@ struct CtlArgs {
 
 KH_HD inline bool retryable(int e) { return e == ST_NOCONV || e == ST_NONFINITE || e == ST_SINGULAR; }
 
+// One circuit's state machine over its own (unit-stride) arrays; the control
+// words are worked on in place, so they live wherever the circuit's arrays do
+// (device memory, or LDS when the launch stages the circuit).
 struct Ctl {
   const CtlArgs& A;
-  int64_t c, B;
   int n;
-  int32_t ci[NCI];
-  double cd[NCD];
+  int32_t* ci;  // [NCI]
+  double* cd;   // [NCD]
   Lane L;
   CtlParams P;  // options and transient spec as seen by this circuit
 
-  KH_HD Ctl(const CtlArgs& a, int64_t lane) : A(a), c(lane), B(a.B), n(a.ckt.n) {
-    for (int k = 0; k < NCI; k++) ci[k] = a.ci[(int64_t)k * B + c];
-    for (int k = 0; k < NCD; k++) cd[k] = a.cd[(int64_t)k * B + c];
-    L.c = c; L.B = B; L.n = n;
+  KH_HD Ctl(const CtlArgs& a) : A(a), n(a.ckt.n), ci(a.ci), cd(a.cd) {
+    L.n = n;
     L.et = a.ckt.et; L.par = a.par; L.st = a.st; L.val = a.val; L.present = a.present; L.rhs = a.rhs;
     L.X = a.xold;
     L.grew = false;
     P = a.p;
     if (a.lane_spec) {  // per-circuit time scale: step, stop, max step, min step
-      P.step = a.lane_spec[c];
-      P.stop = a.lane_spec[B + c];
-      P.maxStep = a.lane_spec[2 * B + c];
-      P.minStep = a.lane_spec[3 * B + c];
+      P.step = a.lane_spec[0];
+      P.stop = a.lane_spec[1];
+      P.maxStep = a.lane_spec[2];
+      P.minStep = a.lane_spec[3];
     }
   }
-  KH_HD void store() {
-    for (int k = 0; k < NCI; k++) A.ci[(int64_t)k * B + c] = ci[k];
-    for (int k = 0; k < NCD; k++) A.cd[(int64_t)k * B + c] = cd[k];
-  }
-  KH_HD double& X(double* v, int i) { return v[(int64_t)i * B + c]; }
+  KH_HD double& X(double* v, int i) { return v[i]; }
   KH_HD const int32_t* el(int k) const { return A.ckt.et + (int64_t)k * ET_W; }
 
   KH_HD void finish(int status) {
@@ -241,7 +237,7 @@ struct Ctl {
   KH_HD double roundoff() {
     for (int i = 0; i < n; i++) X(A.absbuf, i) = 0;
     for (int s = 0; s < A.ckt.nslots; s++) {
-      int64_t ix = (int64_t)s * B + c;
+      int64_t ix = s;
       if (!A.present[ix]) continue;
       X(A.absbuf, A.ckt.slot_i[s]) += fabs(A.val[ix]) * fabs(X(A.x, A.ckt.slot_j[s]));
     }
@@ -343,7 +339,7 @@ struct Ctl {
       X(A.xold, i) = X(A.x, i);
       X(A.rhs, i) = 0;
     }
-    for (int s = 0; s < A.ckt.nslots; s++) A.val[(int64_t)s * B + c] = 0;
+    for (int s = 0; s < A.ckt.nslots; s++) A.val[s] = 0;
     load_all();
     double ptG = cd[CD_PTG];
     if (ptG > 0) {
@@ -365,7 +361,7 @@ struct Ctl {
   // it, else a slot reserved after the device slots.
   KH_HD int diag_slot(int i) const { return A.ckt.diag_slot[i]; }
   KH_HD void add_diag(int s, double v) {
-    int64_t ix = (int64_t)s * B + c;
+    int64_t ix = s;
     A.val[ix] = A.val[ix] + v;
     if (!A.present[ix]) {
       A.present[ix] = 1;
@@ -428,7 +424,7 @@ struct Ctl {
   // state from it.
   KH_HD void apply_nodeset() {
     for (int i = 0; i < n; i++) X(A.x, i) = 0;
-    for (int i = 0; i < A.ckt.numNodes - 1; i++) X(A.x, i) = A.nodeset[(int64_t)i * B + c];
+    for (int i = 0; i < A.ckt.numNodes - 1; i++) X(A.x, i) = A.nodeset[i];
     for (int k = 0; k < A.ckt.nel; k++) {
       const int32_t* e = el(k);
       switch (e[ET_KIND]) {
@@ -647,8 +643,8 @@ struct Ctl {
     for (int s = 0; s < P.swN; s++) {
       const int32_t* e = el(P.swElem[s]);
       double* par = const_cast<double*>(A.par);
-      par[(int64_t)(e[ET_POFF] + 0) * B + c] = P.swStart[s] + (double)idx[s] * P.swStep[s];
-      par[(int64_t)(e[ET_POFF] + 1) * B + c] = 1.0;
+      par[e[ET_POFF] + 0] = P.swStart[s] + (double)idx[s] * P.swStep[s];
+      par[e[ET_POFF] + 1] = 1.0;
     }
   }
   KH_HD void sweep_begin() {
@@ -807,7 +803,7 @@ struct Ctl {
       return;
     }
     if (!ci[CI_NEED_SOLVE]) return;
-    int e = A.lu_status[c];
+    int e = A.lu_status[0];
     ci[CI_NEED_SOLVE] = 0;
     if (e == ST_CAPACITY) { finish(ST_CAPACITY); return; }
     after_solve(e);

@@ -38,30 +38,31 @@ constexpr double kInf = __builtin_huge_val();
 constexpr double kPi = 3.14159265358979323846264338327950288419716939937510582097494459;
 constexpr double kE = 2.71828182845904523536028747135266249775724709369995957496696763;
 
+// One circuit's device view: its own parameter, state, value and vector
+// arrays, unit stride.
 struct Lane {
-  int64_t c, B;
   int n;
   const int32_t* et;     // element table
-  const double* par;     // [P][B]
-  double* st;            // [S][B]
-  double* val;           // [nslots][B]
-  uint8_t* present;      // [nslots][B]
-  double* rhs;           // [n][B]
-  const double* X;       // [n][B] linearization point (xOld)
+  const double* par;     // [P]
+  double* st;            // [S]
+  double* val;           // [nslots]
+  uint8_t* present;      // [nslots]
+  double* rhs;           // [n]
+  const double* X;       // [n] linearization point (xOld)
   bool grew;
   // LoadContext (device/device.go:127-148)
   int mode, method;
   double time, H, gmin, srcFact;
   bool initJct, srcStepping;
 
-  KH_HD double xat(const double* v, int i) const { return (i < 0 || i >= n) ? 0.0 : v[(int64_t)i * B + c]; }
-  KH_HD double P(const int32_t* e, int k) const { return par[(int64_t)(e[ET_POFF] + k) * B + c]; }
-  KH_HD double& S(const int32_t* e, int k) const { return st[(int64_t)(e[ET_SOFF] + k) * B + c]; }
+  KH_HD double xat(const double* v, int i) const { return (i < 0 || i >= n) ? 0.0 : v[i]; }
+  KH_HD double P(const int32_t* e, int k) const { return par[e[ET_POFF] + k]; }
+  KH_HD double& S(const int32_t* e, int k) const { return st[e[ET_SOFF] + k]; }
   KH_HD int T(const int32_t* e, int k) const { return e[ET_TERM + k]; }
   KH_HD void add(const int32_t* e, int ti, int tj, double v) {
     int s = e[ET_SLOT + ti * 6 + tj];
     if (s < 0) return;
-    int64_t ix = (int64_t)s * B + c;
+    int64_t ix = s;
     val[ix] = val[ix] + v;
     if (!present[ix]) {
       present[ix] = 1;
@@ -69,13 +70,13 @@ struct Lane {
     }
   }
   KH_HD void addRHS(int i, double v) {
-    if (i >= 0 && i < n) rhs[(int64_t)i * B + c] += v;
+    if (i >= 0 && i < n) rhs[i] += v;
   }
   KH_HD double volt(const int32_t* e, const double* x) const {  // Base.voltage, device/device.go:310-319
     double v = 0;
     int a = T(e, 0), b = T(e, 1);
-    if (a >= 0) v += x[(int64_t)a * B + c];
-    if (b >= 0) v -= x[(int64_t)b * B + c];
+    if (a >= 0) v += x[a];
+    if (b >= 0) v -= x[b];
     return v;
   }
 };
@@ -300,7 +301,7 @@ KH_HD inline void load_L(Lane& L, const int32_t* e) {
 KH_HD inline OpPt op_L(const Lane& L, const int32_t* e, const double* x) {
   OpPt o;
   o.voltage = L.volt(e, x);
-  o.current = x[(int64_t)L.T(e, 2) * L.B + L.c];
+  o.current = x[L.T(e, 2)];
   o.power = o.voltage * o.current;
   o.storage = o.power;
   return o;
@@ -311,7 +312,7 @@ KH_HD inline void accept_L(const Lane& L, const int32_t* e, const double* x, dou
     hist_reset(L, e, 2);
   } else if (mode == MODE_OP || mode == MODE_TRANOP) {
     L.S(e, 0) = 0;
-    L.S(e, 1) = x[(int64_t)L.T(e, 2) * L.B + L.c];
+    L.S(e, 1) = x[L.T(e, 2)];
     hist_reset(L, e, 2);
   } else {
     OpPt o = op_L(L, e, x);
@@ -338,7 +339,7 @@ KH_HD inline void load_V(Lane& L, const int32_t* e) {
 KH_HD inline OpPt op_V(const Lane& L, const int32_t* e, const double* x) {
   OpPt o;
   o.voltage = L.volt(e, x);
-  o.current = x[(int64_t)L.T(e, 2) * L.B + L.c];
+  o.current = x[L.T(e, 2)];
   o.power = o.voltage * o.current;
   o.storage = 0;
   return o;

@@ -158,7 +158,7 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
   CtlArgs a;
   LuArgs la;
   circuit_view(g, cc, a, la);
-  Ctl ctl(a, 0);
+  Ctl ctl(a);
   int32_t* lus = const_cast<int32_t*>(a.lu_status);
   int solves = 0;
   uint64_t cyc_lu = 0, cyc_ctl = 0;
@@ -167,7 +167,7 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
     if (ctl.ci[CI_NEED_SOLVE]) {
       if (solves >= max_solves) break;
       uint64_t t0 = KH_CLOCK();
-      int st = lu_factor(la, 0);
+      int st = lu_factor<double, true>(la, 0);
       if (st == LU_CAPACITY) {
         // L/U outgrew the group's capacity (pivoting away from the diagonal
         // fills more than the structural estimate): the host enlarges it and
@@ -175,7 +175,7 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
         ctl.ci[CI_WAIT] = WAIT_CAPACITY;
         break;
       }
-      if (st == LU_OK) st = lu_solve(la, 0);
+      if (st == LU_OK) st = lu_solve<double, true>(la, 0);
       lus[0] = st;
       solves++;
       cyc_lu += KH_CLOCK() - t0;
@@ -188,7 +188,6 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
   }
   ctl.cd[CD_CYC_LU] += (double)cyc_lu;
   ctl.cd[CD_CYC_CTL] += (double)cyc_ctl;
-  ctl.store();
   *running = ctl.ci[CI_PHASE] != PH_DONE;
   *waiting = ctl.ci[CI_WAIT] != 0;
 }

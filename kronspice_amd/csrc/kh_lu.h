@@ -136,9 +136,10 @@ KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
 // A(:,k) through the graph of L; returns top, the reach is xi[top..n).
 // mark[r] == k + 1 means "visited while processing column k" (the reference's
 // generation counter, restarted per factorization).
-template <typename T>
+template <typename T, bool Unit = false>
 KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
-  const int64_t B = a.B;
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
   const int n = a.n;
   const int32_t gen = k + 1;
   int top = n;
@@ -177,7 +178,8 @@ KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* 
   return top;
 }
 
-// factor, solver/sparse.go:199-293.
+// factor, solver/sparse.go:199-293. Unit = true: a single system stored with
+// unit stride (one circuit's own arrays), B = 1 and c = 0 known at compile time.
 //
 // Reach reuse: the reach of column k (its order and membership) depends only
 // on the pattern and on the pivot rows of columns < k. A circuit refactored
@@ -187,9 +189,10 @@ KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* 
 // The pivot rule is still evaluated at every column exactly as the reference
 // does; at the first column whose pivot differs, the search resumes. Results
 // are identical to re-running the search every time.
-template <typename T>
+template <typename T, bool Unit = false>
 KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
-  const int64_t B = a.B;
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
   const int n = a.n;
   const int32_t pidc = a.pid[c];
   const PatDesc pd = a.pats[pidc];
@@ -222,7 +225,7 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
       r1 = a.rp[(int64_t)(k + 1) * B + c];
       diag_in_reach = a.dk[(int64_t)k * B + c] != 0;
     } else {
-      int top = lu_reach(a, c, k, ap, arow);
+      int top = lu_reach<T, Unit>(a, c, k, ap, arow);
       r1 = r0 + (n - top);
       if (r1 > a.capR) return LU_CAPACITY;
       for (int p = top; p < n; p++) a.rch[(r0 + p - top) * B + c] = a.xi[(int64_t)p * B + c];
@@ -311,9 +314,10 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
 // solve, solver/sparse.go:355-401 (the factorization is held in the lane's
 // workspace). x is written in original unknown order, stopping at the first
 // non-finite entry as the reference does.
-template <typename T>
+template <typename T, bool Unit = false>
 KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
-  const int64_t B = a.B;
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
   const int n = a.n;
   const PatDesc pd = a.pats[a.pid[c]];
   const int32_t* q = a.pool + pd.q;

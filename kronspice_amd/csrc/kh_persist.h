@@ -1,0 +1,33 @@
+// kh_persist.h — the persistent batched-analysis kernel (k_persist).
+//
+// A launch covers every group of an uneven batch: workgroup w runs circuits
+// of group bgroup[w] (kh_group.h); `cpb` circuits run in each one-wavefront
+// workgroup, one per lane. Each circuit loops: factor + solve its stamped
+// system, then advance its analysis to the next solve (run_circuit).
+//
+// Groups with lds_bytes > 0 run staged: the wavefront copies the working set
+// of each of its circuits (every per-circuit array, kh_group.h) into LDS,
+// the circuits run on the LDS copies, and the wavefront copies back what
+// outlives the launch. Every dependent access of the serial factorization and
+// state machine is then an LDS access instead of a trip to L2/HBM.
+//
+// The kernel is compiled once per register budget W (resident wavefronts per
+// SIMD the allocation must allow): 1 leaves the allocator free; 2 and 4 trade
+// spills for more circuits in flight, whose latencies then overlap. Each
+// budget is its own translation unit (kh_persist.hip with KH_PERSIST_WAVES),
+// so the device functions the kernel calls are allocated under that budget.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kh_group.h"
+
+namespace kh {
+
+constexpr int kBlock = 64;  // one wavefront per workgroup: lanes run long, divergent programs
+
+// Launches k_persist compiled for `waves` (1, 2 or 4) resident wavefronts per
+// SIMD on the null stream; returns the launch error.
+hipError_t kh_launch_persist(int waves, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup,
+                             int32_t max_solves, int32_t* counters);
+
+}  // namespace kh

@@ -1,0 +1,89 @@
+// kh_persist.hip — k_persist compiled for one register budget,
+// KH_PERSIST_WAVES resident wavefronts per SIMD (see kh_persist.h).
+#include <hip/hip_runtime.h>
+
+#define KH_CLOCK() ((uint64_t)clock64())
+#include "kh_persist.h"
+
+#ifndef KH_PERSIST_WAVES
+#define KH_PERSIST_WAVES 1
+#endif
+#define KH_CAT2(a, b) a##b
+#define KH_CAT(a, b) KH_CAT2(a, b)
+
+namespace kh {
+namespace {
+
+__device__ inline void copy_wave(char* dst, const char* src, int64_t bytes) {
+  const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
+  if ((al & 7) == 0) {
+    for (int64_t k = threadIdx.x; k < bytes / 8; k += kBlock) ((uint64_t*)dst)[k] = ((const uint64_t*)src)[k];
+  } else if ((al & 3) == 0) {
+    for (int64_t k = threadIdx.x; k < bytes / 4; k += kBlock) ((uint32_t*)dst)[k] = ((const uint32_t*)src)[k];
+  } else {
+    for (int64_t k = threadIdx.x; k < bytes; k += kBlock) dst[k] = src[k];
+  }
+}
+
+__global__ void __launch_bounds__(kBlock, KH_PERSIST_WAVES) k_persist(const Group* G, const int32_t* bgroup,
+                                                                       int32_t max_solves, int32_t* counters) {
+  extern __shared__ __align__(16) char lds[];
+  const Group& g = G[bgroup[blockIdx.x]];
+  const int64_t c0 = (int64_t)(blockIdx.x - g.block0) * g.cpb;
+  bool running = false, waiting = false;
+  if (g.lds_bytes > 0) {
+    const int nc = (int)min((int64_t)g.cpb, g.ctl.B - c0);  // uniform over the workgroup
+    for (int t = 0; t < nc; t++) {
+      char* base = lds + (int64_t)t * g.lds_bytes;
+      int64_t off = 0;
+      for_each_array(g, [&](const auto& p, int64_t bytes, bool in, bool) {
+        if (in && p) copy_wave(base + off, (const char*)p + (c0 + t) * bytes, bytes);
+        off += round8(bytes);
+      });
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nc) {
+      Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes, c0 + threadIdx.x);
+      run_circuit(v, 0, max_solves, &running, &waiting);
+    }
+    __syncthreads();
+    for (int t = 0; t < nc; t++) {
+      char* base = lds + (int64_t)t * g.lds_bytes;
+      int64_t off = 0;
+      for_each_array(g, [&](const auto& p, int64_t bytes, bool, bool out) {
+        if (out && p) copy_wave((char*)p + (c0 + t) * bytes, base + off, bytes);
+        off += round8(bytes);
+      });
+    }
+  } else {
+    if ((int)threadIdx.x >= g.cpb) return;
+    const int64_t cc = c0 + threadIdx.x;
+    if (cc >= g.ctl.B) return;
+    run_circuit(g, cc, max_solves, &running, &waiting);
+  }
+  if (running) atomicAdd(&counters[2 * g.gid], 1);
+  if (waiting) atomicAdd(&counters[2 * g.gid + 1], 1);
+}
+
+}  // namespace
+
+hipError_t KH_CAT(kh_launch_persist_w, KH_PERSIST_WAVES)(unsigned grid, size_t lds_bytes, const Group* G,
+                                                          const int32_t* bgroup, int32_t max_solves,
+                                                          int32_t* counters) {
+  hipLaunchKernelGGL(k_persist, dim3(grid), dim3(kBlock), lds_bytes, 0, G, bgroup, max_solves, counters);
+  return hipGetLastError();
+}
+
+#if KH_PERSIST_WAVES == 1
+hipError_t kh_launch_persist_w2(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
+hipError_t kh_launch_persist_w4(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
+
+hipError_t kh_launch_persist(int waves, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup,
+                             int32_t max_solves, int32_t* counters) {
+  if (waves == 4) return kh_launch_persist_w4(grid, lds_bytes, G, bgroup, max_solves, counters);
+  if (waves == 2) return kh_launch_persist_w2(grid, lds_bytes, G, bgroup, max_solves, counters);
+  return kh_launch_persist_w1(grid, lds_bytes, G, bgroup, max_solves, counters);
+}
+#endif
+
+}  // namespace kh

@@ -26,6 +26,7 @@
 #define KH_CLOCK() ((uint64_t)clock64())
 #include "kh_ac.h"
 #include "kh_group.h"
+#include "kh_persist.h"
 #include "kh_symbolic.h"
 
 using namespace kh;
@@ -46,70 +47,10 @@ kh_status fail(kh_status s, const std::string& msg) {
       return fail(KH_HIP_ERROR, std::string(#call) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
-constexpr int kBlock = 64;  // one wavefront per workgroup: lanes run long, divergent programs
 
 // ---- kernels ----------------------------------------------------------------------
-// A launch covers every group of an uneven batch: workgroup w runs circuits
-// of group bgroup[w] (kh_group.h); `cpb` circuits run in each one-wavefront
-// workgroup, one per lane. With cpb = 1 every circuit has its own wavefront
-// and the many resident wavefronts hide the latency of each circuit's
-// dependent accesses.
-//
-// Groups with lds_bytes > 0 run staged: the wavefront copies the working set
-// of each of its circuits (every per-circuit array, kh_group.h) into LDS,
-// the circuits run on the LDS copies, and the wavefront copies back what
-// outlives the launch. Every dependent access of the serial factorization and
-// state machine is then an LDS access instead of a trip to L2/HBM.
-__device__ inline void copy_wave(char* dst, const char* src, int64_t bytes) {
-  const uintptr_t al = (uintptr_t)dst | (uintptr_t)src | (uintptr_t)bytes;
-  if ((al & 7) == 0) {
-    for (int64_t k = threadIdx.x; k < bytes / 8; k += kBlock) ((uint64_t*)dst)[k] = ((const uint64_t*)src)[k];
-  } else if ((al & 3) == 0) {
-    for (int64_t k = threadIdx.x; k < bytes / 4; k += kBlock) ((uint32_t*)dst)[k] = ((const uint32_t*)src)[k];
-  } else {
-    for (int64_t k = threadIdx.x; k < bytes; k += kBlock) dst[k] = src[k];
-  }
-}
-
-__global__ void __launch_bounds__(kBlock) k_persist(const Group* G, const int32_t* bgroup, int32_t max_solves,
-                                                    int32_t* counters) {
-  extern __shared__ __align__(16) char lds[];
-  const Group& g = G[bgroup[blockIdx.x]];
-  const int64_t c0 = (int64_t)(blockIdx.x - g.block0) * g.cpb;
-  bool running = false, waiting = false;
-  if (g.lds_bytes > 0) {
-    const int nc = (int)min((int64_t)g.cpb, g.ctl.B - c0);  // uniform over the workgroup
-    for (int t = 0; t < nc; t++) {
-      char* base = lds + (int64_t)t * g.lds_bytes;
-      int64_t off = 0;
-      for_each_array(g, [&](const auto& p, int64_t bytes, bool in, bool) {
-        if (in && p) copy_wave(base + off, (const char*)p + (c0 + t) * bytes, bytes);
-        off += round8(bytes);
-      });
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nc) {
-      Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes, c0 + threadIdx.x);
-      run_circuit(v, 0, max_solves, &running, &waiting);
-    }
-    __syncthreads();
-    for (int t = 0; t < nc; t++) {
-      char* base = lds + (int64_t)t * g.lds_bytes;
-      int64_t off = 0;
-      for_each_array(g, [&](const auto& p, int64_t bytes, bool, bool out) {
-        if (out && p) copy_wave((char*)p + (c0 + t) * bytes, base + off, bytes);
-        off += round8(bytes);
-      });
-    }
-  } else {
-    if ((int)threadIdx.x >= g.cpb) return;
-    const int64_t cc = c0 + threadIdx.x;
-    if (cc >= g.ctl.B) return;
-    run_circuit(g, cc, max_solves, &running, &waiting);
-  }
-  if (running) atomicAdd(&counters[2 * g.gid], 1);
-  if (waiting) atomicAdd(&counters[2 * g.gid + 1], 1);
-}
+// k_persist lives in kh_persist.hip, compiled once per register budget
+// (kh_persist.h); the host launches it through kh_launch_persist.
 
 // Factor + solve the lanes that requested a solve and have a structure.
 // need == nullptr: every lane.
@@ -189,8 +130,6 @@ KH_HD inline ACLane ac_lane(const AcArgs& a, int64_t l) {
   const int32_t f = a.f0 + (int32_t)(l / a.B);
   ACLane A;
   A.dev = Lane{};
-  A.dev.c = 0;
-  A.dev.B = 1;
   A.dev.n = a.ckt.n;
   A.dev.et = a.ckt.et;
   A.dev.par = a.par + c * a.npar;
@@ -886,6 +825,7 @@ struct Runner {
   // sets fit in kLdsLimit bytes (the per-workgroup LDS available without
   // opting in to more).
   bool lds_staging = false;
+  int32_t waves = 1;  // k_persist variant: resident wavefronts per SIMD (1, 2 or 4)
   static constexpr int64_t kLdsLimit = 64 * 1024;
   size_t lds_launch = 0;
 
@@ -942,9 +882,8 @@ struct Runner {
       }
       KH_HIP(hipMemsetAsync(counters.p, 0, ng * 8, 0));
       KH_HIP(hipEventRecord(ev[0], 0));
-      hipLaunchKernelGGL(k_persist, dim3((unsigned)bg.size()), dim3(kBlock), lds_launch, 0, dgroups.as<Group>(),
-                         dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>());
-      KH_HIP(hipGetLastError());
+      KH_HIP(kh_launch_persist(waves, (unsigned)bg.size(), lds_launch, dgroups.as<Group>(), dbgroup.as<int32_t>(),
+                               max_solves, counters.as<int32_t>()));
       KH_HIP(hipEventRecord(ev[1], 0));
       KH_HIP(hipMemcpyAsync(hcount, counters.p, ng * 8, hipMemcpyDeviceToHost, 0));
       KH_HIP(hipStreamSynchronize(0));
@@ -1045,6 +984,7 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
   Runner R;
   R.cpb = cpb;
   R.lds_staging = opt && opt->staging != 0;
+  R.waves = (opt && (opt->waves_per_simd == 2 || opt->waves_per_simd == 4)) ? opt->waves_per_simd : 1;
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];
     if (!G.topology || G.B <= 0 || !G.d_params) return fail(KH_BAD_ARGUMENT, "bad group " + std::to_string(g));

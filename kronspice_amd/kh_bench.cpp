@@ -1,7 +1,7 @@
 // kh_bench — runs a batch file through kh_batch_run from plain C++ (no
 // Python, no PyTorch in the process), for timing and for profilers.
 //
-//   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave]
+//   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave] [waves per SIMD]
 //   kh_bench --copy <MiB>    one device-to-device copy kernel of that size:
 //                            the reference point for hardware byte counters
 //
@@ -86,6 +86,7 @@ int main(int argc, char** argv) {
   const int warmup = argc > 3 ? atoi(argv[3]) : 0;
   const int staging = argc > 4 ? atoi(argv[4]) : 0;
   const int cpw = argc > 5 ? atoi(argv[5]) : 0;
+  const int waves = argc > 6 ? atoi(argv[6]) : 0;
   FILE* f = fopen(argv[1], "rb");
   if (!f) return die("cannot open the batch file");
   char magic[4];
@@ -160,6 +161,7 @@ int main(int argc, char** argv) {
   memset(&opt, 0, sizeof(opt));
   opt.staging = staging;
   opt.circuits_per_wave = cpw;
+  opt.waves_per_simd = waves;
   kh_tran_spec spec;
   memset(&spec, 0, sizeof(spec));
   spec.step = spec3[0];
@@ -192,11 +194,12 @@ int main(int argc, char** argv) {
   std::string h = "{";
   for (auto& kv : hist) h += (h.size() > 1 ? ", \"" : "\"") + std::to_string(kv.first) + "\": " + std::to_string(kv.second);
   h += "}";
-  printf("{\"circuits\": %lld, \"groups\": %d, \"staging\": %d, \"circuits_per_wave\": %d, \"steps\": %d, "
+  printf("{\"circuits\": %lld, \"groups\": %d, \"staging\": %d, \"circuits_per_wave\": %d, \"waves_per_simd\": %d, "
+         "\"steps\": %d, "
          "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
          "\"algorithmic_GBps\": %.4f, \"status_histogram\": %s}\n",
-         (long long)circuits, ngroups, staging, cpw, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
+         (long long)circuits, ngroups, staging, cpw, waves, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, h.c_str());
   for (void* p : owned) (void)hipFree(p);
   for (auto t : topo) kh_topology_destroy(t);

@@ -62,3 +62,19 @@ def test_rectifier_batch_staged(cpw):
     same = (acc == ao) & (rej == ro)
     assert same.mean() >= 0.99, same.mean()
     assert relerr(host(r.x), xo)[same].max() <= 1e-9
+
+
+@pytest.mark.parametrize("waves,staging,cpw", [(2, 0, 1), (4, 0, 1), (2, 1, 4), (4, 1, 1)])
+def test_register_budgets_bit_identical(waves, staging, cpw):
+    """The persistent kernel compiled for 2 or 4 resident wavefronts per SIMD
+    (fewer registers, spills) computes exactly what the unconstrained build
+    computes."""
+    b = W.rectifier_rc_ladders(256)
+    spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    t = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    r0 = api.tran(t, b.params, spec, lane_spec=b.lane_spec)
+    r = api.tran(t, b.params, spec, opts=api.Options(staging=staging, circuits_per_wave=cpw, waves_per_simd=waves),
+                 lane_spec=b.lane_spec)
+    assert np.array_equal(host(r.status), host(r0.status))
+    assert np.array_equal(host(r.accepted), host(r0.accepted)) and np.array_equal(host(r.rejected), host(r0.rejected))
+    assert np.array_equal(host(r.x), host(r0.x))
