@@ -69,7 +69,8 @@ def main():
     ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: autotuned)")
     ap.add_argument("--staging", type=int, default=-1, help="0 in place, 1 LDS staged (-1: autotuned)")
     ap.add_argument("--waves", type=int, default=0, help="resident wavefronts per SIMD (0: autotuned)")
-    ap.add_argument("--tune-sample", type=int, default=1024)
+    ap.add_argument("--tune-fraction", type=float, default=0.05,
+                    help="fraction of each circuit's interval the autotuner's probe simulates")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 form")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -98,10 +99,8 @@ def main():
     # Chosen form: the autotuner's pick on a sample of the batch.
     tuned = None
     if args.staging < 0 or args.cpw <= 0 or args.waves <= 0:
-        cols = autotune.sample_columns(b.B, args.tune_sample)
-        sp = api.to_device(np.ascontiguousarray(b.params[:, cols]))
-        sl = api.to_device(np.ascontiguousarray(b.lane_spec[:, cols]))
-        tuned = autotune.tune(lambda o: api.tran(topo, sp, spec, opts=o, lane_spec=sl))
+        sl = api.to_device(autotune.short_lane_spec(b.lane_spec, args.tune_fraction))
+        tuned = autotune.tune(lambda o: api.tran(topo, d_params, spec, opts=o, lane_spec=sl))
         args.staging, args.cpw, args.waves = tuned.staging, tuned.circuits_per_wave, tuned.waves_per_simd
     opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging, waves_per_simd=args.waves)
 
@@ -191,7 +190,7 @@ def main():
                   "chosen": {"circuits_per_s": b.B / (ms_per_step / 1e3), "staging": args.staging,
                              "circuits_per_wave": args.cpw, "waves_per_simd": args.waves},
                   "chosen_vs_plain": (plain_s / (ms_per_step / 1e3)) if plain_s else None,
-                  "autotune": ({"sample": args.tune_sample,
+                  "autotune": ({"probe": f"whole batch, first {args.tune_fraction:g} of each interval",
                                 "seconds": {f"staging={a},cpw={c},waves={w}": t for a, c, w, t in tuned.timings}}
                                if tuned else None)},
         "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},

@@ -5,9 +5,10 @@ its working set staged in LDS, with one or several circuits per wavefront,
 and compiled for 1, 2 or 4 resident wavefronts per SIMD (register budget).
 Which is fastest depends on the circuit size (LDS footprint), the number of
 circuits and the divergence between them, so the choice is
-measured: each candidate runs a sample of the batch, and the fastest
-candidate (circuits per second on the sample) is the configuration for the
-whole batch. Results do not depend on the choice — every configuration runs
+measured: each candidate runs a probe — the whole batch (so the number of
+circuits in flight, which decides how well latencies overlap, is the real
+one) over a short stretch of the analysis — and the fastest candidate is the
+configuration for the full run. Results do not depend on the choice — every configuration runs
 the same device code on the same values — only time does.
 """
 from __future__ import annotations
@@ -21,10 +22,8 @@ import numpy as np
 from . import api
 
 # (staging, circuits per wavefront, resident wavefronts per SIMD) candidates.
-CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
-    (0, 1, 1), (1, 1, 1), (1, 4, 1), (0, 4, 1),
-    (0, 1, 2), (1, 1, 2), (1, 4, 2), (0, 4, 2),
-    (0, 1, 4), (1, 1, 4), (0, 4, 4))
+CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
+    (s, c, w) for w in (1, 2, 4) for c in (1, 2, 4, 8, 16, 32) for s in (0, 1) if not (s == 1 and c > 4))
 
 
 @dataclass
@@ -66,6 +65,10 @@ def tune(run: Callable[[api.Options], object], candidates: Sequence[Tuple[int, i
     return Choice(s, c, w, t, timings)
 
 
-def sample_columns(B: int, k: int) -> np.ndarray:
-    """k circuits spread over the batch (the sample the candidates run)."""
-    return np.linspace(0, B - 1, min(k, B)).astype(int)
+def short_lane_spec(lane_spec: np.ndarray, fraction: float) -> np.ndarray:
+    """Per-circuit transient specs [4, B] (step, stop, max step, min step)
+    cut to the first `fraction` of each circuit's interval: the probe."""
+    s = np.array(lane_spec, dtype=np.float64, copy=True)
+    s[1] *= fraction
+    s[2] = np.minimum(s[2], s[1] / 50)
+    return s
