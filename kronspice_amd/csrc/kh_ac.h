@@ -25,7 +25,7 @@ struct ACLane {
   double omega;
 
   KH_HD void add(const int32_t* e, int ti, int tj, kcplx v) {
-    int s = e[ET_SLOT + ti * 6 + tj];
+    int s = e[ET_SLOT + ti * KH_NT + tj];
     if (s < 0) return;  // ground (solver/sparse.go:115-117)
     int64_t ix = (int64_t)s * BL + l;
     val[ix] = val[ix] + v;

@@ -16,14 +16,16 @@
 
 namespace kh {
 
-// Element table layout (int32 words per element).
+// Element table layout (int32 words per element): up to KH_NT terminals
+// (external nodes, branches and internal nodes; the BJT has 7).
+constexpr int KH_NT = 8;
 enum {
   ET_KIND = 0,
-  ET_TERM = 1,   // 6 unknown indices (-1 = ground / unused)
-  ET_POFF = 7,
-  ET_SOFF = 8,
-  ET_SLOT = 9,   // 6x6 slot table
-  ET_W = 45
+  ET_TERM = 1,                          // KH_NT unknown indices (-1 = ground / unused)
+  ET_POFF = ET_TERM + KH_NT,
+  ET_SOFF = ET_POFF + 1,
+  ET_SLOT = ET_SOFF + 1,                // KH_NT x KH_NT slot table
+  ET_W = ET_SLOT + KH_NT * KH_NT
 };
 
 enum { K_R = 1, K_C = 2, K_L = 3, K_V = 4, K_I = 5, K_D = 6, K_M = 7 };
@@ -60,7 +62,7 @@ struct Lane {
   KH_HD double& S(const int32_t* e, int k) const { return st[e[ET_SOFF] + k]; }
   KH_HD int T(const int32_t* e, int k) const { return e[ET_TERM + k]; }
   KH_HD void add(const int32_t* e, int ti, int tj, double v) {
-    int s = e[ET_SLOT + ti * 6 + tj];
+    int s = e[ET_SLOT + ti * KH_NT + tj];
     if (s < 0) return;
     int64_t ix = s;
     val[ix] = val[ix] + v;

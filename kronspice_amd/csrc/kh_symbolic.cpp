@@ -203,7 +203,7 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     int32_t* t = &et[(size_t)e * ET_W];
     t[ET_KIND] = E.kind;
     t[ET_POFF] = E.poff;
-    for (int k = 0; k < 6; k++) t[ET_TERM + k] = -1;
+    for (int k = 0; k < KH_NT; k++) t[ET_TERM + k] = -1;
     auto unk = [&](int node) { return node - 1; };
     int ns = 0, npar = 0;
     switch (E.kind) {
@@ -290,12 +290,12 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
       case K_M: pairs = kPairsM; np2 = (int)(sizeof(kPairsM) / sizeof(kPairsM[0])); break;
       default: break;
     }
-    for (int k = 0; k < 36; k++) t[ET_SLOT + k] = -1;
+    for (int k = 0; k < KH_NT * KH_NT; k++) t[ET_SLOT + k] = -1;
     for (int p = 0; p < np2; p++) {
       int ti = pairs[p][0], tj = pairs[p][1];
       int ui = t[ET_TERM + ti], uj = t[ET_TERM + tj];
       if (ui < 0 || uj < 0 || ui >= n || uj >= n) continue;
-      t[ET_SLOT + ti * 6 + tj] = get(ui, uj);
+      t[ET_SLOT + ti * KH_NT + tj] = get(ui, uj);
     }
   }
   diag_slot.assign(n, -1);
