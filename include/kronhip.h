@@ -58,7 +58,7 @@ enum {
  * node[] per kind (as circuit.Element Nodes + Extra, circuit/builder.go:101-118):
  *   R C L V I D : node[0] = n+, node[1] = n-
  *   M (MOSFET)  : node[0] = drain, node[1] = source, node[2] = gate, node[3] = bulk
- *   Q (BJT)     : node[0] = collector, node[1] = emitter, node[2] = base, node[3] = substrate (-1: none)
+ *   Q (BJT)     : node[0] = collector, node[1] = emitter, node[2] = base, node[3] = substrate (0: none)
  *
  * Each element reads its parameters at params[poff ...]; the per-kind layout
  * is below (already resolved by the host setup step: defaults, area factors,
@@ -88,6 +88,13 @@ typedef struct kh_element {
  *    13 czbd, 14 czbdsw, 15 czbs, 16 czbssw, 17 PB, 18 MJ, 19 MJSW, 20 FC,
  *    21..23 fbt[3], 24..26 fsw[3], 27 cgso, 28 cgdo, 29 cgbo, 30 oxideCap,
  *    31 vt, 32 hasJcap, 33 hasGcap                        (device/mosfet.go:218-323)
+ * Q (Gummel-Poon, area applied): 0 type (+1 NPN, -1 PNP), 1 IS, 2 BF, 3 NF,
+ *    4 VAF, 5 IKF, 6 ISE, 7 NE, 8 BR, 9 NR, 10 VAR, 11 IKR, 12 ISC, 13 NC,
+ *    14 RB, 15 IRB, 16 RBM, 17 RE, 18 RC, 19 CJE, 20 VJE, 21 MJE, 22 TF,
+ *    23 XTF, 24 VTF, 25 ITF, 26 CJC, 27 VJC, 28 MJC, 29 XCJC, 30 TR, 31 CJS,
+ *    32 VJS, 33 MJS, 34 FC, 35 ISS, 36 NS, 37 SUBS (+1 vertical, -1 lateral),
+ *    38 vt, 39 vcrit, 40 rbpr, 41 rbpi, 42 gcpr, 43 gepr, 44 ovtf,
+ *    45..47 f1e f2e f3e, 48..50 f1c f2c f3c, 51 hasCharge  (device/bjt.go:142-242)
  */
 #define KH_NPAR_R 2
 #define KH_NPAR_C 3
@@ -95,6 +102,7 @@ typedef struct kh_element {
 #define KH_NPAR_SRC 12
 #define KH_NPAR_D 16
 #define KH_NPAR_M 34
+#define KH_NPAR_Q 52
 
 /* Analysis options: analysis.Options (analysis/op.go:22-62) with the same
  * zero-value defaults (analysis/op.go:91-110). */

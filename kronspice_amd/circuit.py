@@ -25,7 +25,7 @@ import numpy as np
 # Element kinds (include/kronhip.h)
 R, C, L, V, I, D, M, Q = 1, 2, 3, 4, 5, 6, 7, 8
 KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q"}
-NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 34}
+NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 34, Q: 52}
 
 ELEMENT_DTYPE = np.dtype([("kind", "<i4"), ("node", "<i4", (4,)), ("poff", "<i4")])
 
@@ -33,9 +33,23 @@ ELEMENT_DTYPE = np.dtype([("kind", "<i4"), ("node", "<i4", (4,)), ("poff", "<i4"
 CHARGE_Q = 1.602176634e-19
 BOLTZMANN_K = 1.380649e-23
 TNOM_K = 300.15
-EPS_OX = 3.9 * 8.854214871e-12
-EPS_SI = 11.7 * 8.854214871e-12
 NI_SI = 1.45e16
+
+
+def _exact(expr: str) -> float:
+    """A constant expression over the decimal constants above, evaluated
+    exactly and rounded once, as Go folds untyped constant expressions."""
+    from fractions import Fraction as F
+    env = {"q": F("1.602176634e-19"), "k": F("1.380649e-23"), "T": F("300.15"), "F": F}
+    return float(eval(expr, {"__builtins__": {}}, env))
+
+
+EPS_OX = _exact("F('3.9') * F('8.854214871e-12')")
+EPS_SI = _exact("F('11.7') * F('8.854214871e-12')")
+VT_NOM = _exact("k * T / q")                          # BoltzmanK * TNomK / ChargeQ
+EG_NOM = _exact("F('1.16') - F('7.02e-4') * T * T / (T + 1108)")
+TWO_EPS_SI_Q = _exact("2 * F('11.7') * F('8.854214871e-12') * q")   # 2*epsSi*ChargeQ
+EPS_SI_Q = _exact("F('11.7') * F('8.854214871e-12') * q")           # epsSi * ChargeQ
 
 WAVE_NONE, WAVE_PULSE, WAVE_SIN, WAVE_EXP = 0, 1, 2, 3
 
@@ -129,6 +143,14 @@ class Circuit:
                     j = alloc(); internal.append(j); extra.append(j)
                 if extra:
                     internal_of[k] = tuple(extra)
+            elif e.kind == Q:  # cp, bp, ep behind RC, RB, RE (device/bjt.go:250-268)
+                p = bjt_params(self, e)
+                extra = []
+                for g in (p[18], p[14], p[17]):
+                    if g > 0:
+                        j = alloc(); internal.append(j); extra.append(j)
+                if extra:
+                    internal_of[k] = tuple(extra)
         n = nn - 1 + branches
         is_v = np.zeros(n, dtype=bool)
         is_v[: nn - 1] = True
@@ -146,9 +168,7 @@ class Circuit:
         params: List[float] = []
         for k, e in enumerate(self.elements):
             els[k]["kind"] = e.kind
-            nodes = list(e.nodes) + [0] * (4 - len(e.nodes))
-            if e.kind == Q and len(e.nodes) < 4:
-                nodes[3] = -1
+            nodes = list(e.nodes) + [0] * (4 - len(e.nodes))  # a BJT without substrate: ground
             els[k]["node"] = nodes[:4]
             els[k]["poff"] = len(params)
             params.extend(element_params(self, e, step, stop))
@@ -217,6 +237,13 @@ class Builder:
         # AddMulti(e, drain, source, gate, bulk): circuit/builder.go:113-118
         e = Element(name, M, (), model=model.lower(), params={k.lower(): float(v) for k, v in (params or {}).items()})
         return self.add(e, drain, source, gate, bulk)
+
+    def bjt(self, name, collector, base, emitter, model, substrate: Optional[str] = None, area: float = 0.0):
+        # AddMulti(e, collector, emitter, base[, substrate]): circuit/builder.go:99-103, netlist
+        # element 'Q' (circuit/builder.go:458-461)
+        e = Element(name, Q, (), model=model.lower(), area=area)
+        names = (collector, emitter, base) + ((substrate,) if substrate is not None else ())
+        return self.add(e, *names)
 
     def build(self) -> Circuit:
         return Circuit(self.title, list(self.names), list(self.elems), dict(self.models))
@@ -350,7 +377,7 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
         kk = "as_" if k == "as" else k
         if kk in inst:
             inst[kk] = v
-    vt = BOLTZMANN_K * TNOM_K / CHARGE_Q
+    vt = VT_NOM
     leff = inst["l"] - 2 * p["LD"]
     weff = inst["w"]
     if not (leff > 0) or not (weff > 0):
@@ -368,9 +395,9 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
             if "phi" not in given:
                 p["PHI"] = max(0.1, 2 * vt * math.log(nsub / ni))
             if "gamma" not in given:
-                p["GAMMA"] = math.sqrt(2 * EPS_SI * CHARGE_Q * nsub) / cox
+                p["GAMMA"] = math.sqrt(TWO_EPS_SI_Q * nsub) / cox
             if "vto" not in given:
-                eg = 1.16 - 7.02e-4 * TNOM_K * TNOM_K / (TNOM_K + 1108)
+                eg = EG_NOM
                 fermis = typ * 0.5 * p["PHI"]
                 wkfng = 3.2
                 if p["TPG"] != 0:
@@ -380,7 +407,7 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
                 vfb = wkfngs - p["NSS"] * 1e4 * CHARGE_Q / cox
                 p["VTO"] = vfb + typ * (p["GAMMA"] * math.sqrt(p["PHI"]) + p["PHI"])
             if "cj" not in given:
-                p["CJ"] = math.sqrt(EPS_SI * CHARGE_Q * nsub / (2 * p["PB"]))
+                p["CJ"] = math.sqrt(EPS_SI_Q * nsub / (2 * p["PB"]))
     mult = inst["m"]
     oxide_cap = cox * leff * weff * mult
     beta = p["KP"] * weff / leff * mult
@@ -413,6 +440,65 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
             fsw[2], cgso, cgdo, cgbo, oxide_cap, vt, 1.0 if has_j else 0.0, 1.0 if has_g else 0.0]
 
 
+BJT_DEFAULTS = dict(IS=1e-16, BF=100.0, NF=1.0, VAF=0.0, IKF=0.0, ISE=0.0, NE=1.5, BR=1.0, NR=1.0, VAR=0.0,
+                    IKR=0.0, ISC=0.0, NC=2.0, RB=0.0, IRB=0.0, RBM=0.0, RE=0.0, RC=0.0, CJE=0.0, VJE=0.75,
+                    MJE=0.33, TF=0.0, XTF=0.0, VTF=0.0, ITF=0.0, CJC=0.0, VJC=0.75, MJC=0.33, XCJC=1.0, TR=0.0,
+                    CJS=0.0, VJS=0.75, MJS=0.0, FC=0.5, ISS=0.0, NS=1.0, SUBS=0.0)
+BJT_KEYS = (("IS", ("is",)), ("BF", ("bf",)), ("NF", ("nf",)), ("VAF", ("vaf", "va")), ("IKF", ("ikf", "ik")),
+            ("ISE", ("ise",)), ("NE", ("ne",)), ("BR", ("br",)), ("NR", ("nr",)), ("VAR", ("var", "vb")),
+            ("IKR", ("ikr",)), ("ISC", ("isc",)), ("NC", ("nc",)), ("RB", ("rb",)), ("IRB", ("irb",)))
+BJT_KEYS2 = (("RE", ("re",)), ("RC", ("rc",)), ("CJE", ("cje",)), ("VJE", ("vje", "pe")), ("MJE", ("mje", "me")),
+             ("TF", ("tf",)), ("XTF", ("xtf",)), ("VTF", ("vtf",)), ("ITF", ("itf",)), ("CJC", ("cjc",)),
+             ("VJC", ("vjc", "pc")), ("MJC", ("mjc", "mc")), ("XCJC", ("xcjc",)), ("TR", ("tr",)),
+             ("CJS", ("cjs", "ccs")), ("VJS", ("vjs", "ps")), ("MJS", ("mjs", "ms")), ("FC", ("fc",)),
+             ("ISS", ("iss",)), ("NS", ("ns",)), ("SUBS", ("subs",)))
+BJT_LAYOUT = ("IS", "BF", "NF", "VAF", "IKF", "ISE", "NE", "BR", "NR", "VAR", "IKR", "ISC", "NC", "RB", "IRB", "RBM",
+              "RE", "RC", "CJE", "VJE", "MJE", "TF", "XTF", "VTF", "ITF", "CJC", "VJC", "MJC", "XCJC", "TR", "CJS",
+              "VJS", "MJS", "FC", "ISS", "NS", "SUBS")
+
+
+def bjt_params(c: Circuit, e: Element) -> List[float]:
+    """NewBJT (device/bjt.go:142-242) -> the Q layout of include/kronhip.h."""
+    mod = _model(c, e)
+    mp = mod.params
+    if mod.type not in ("npn", "pnp"):
+        raise ValueError(f"{e.name} needs an NPN or PNP model, {mod.name!r} is type {mod.type}")
+    p = dict(BJT_DEFAULTS)
+    typ = -1.0 if mod.type == "pnp" else 1.0
+
+    def get(keys):
+        for name, ks in keys:
+            for k in ks:
+                if k in mp:
+                    p[name] = mp[k]
+                    break
+
+    get(BJT_KEYS)
+    p["RBM"] = p["RB"]
+    get((("RBM", ("rbm",)),))
+    get(BJT_KEYS2)
+    if p["SUBS"] == 0:  # vertical NPN, lateral PNP
+        p["SUBS"] = -1.0 if typ < 0 else 1.0
+    if p["RBM"] > p["RB"]:
+        p["RBM"] = p["RB"]
+    area = e.area if e.area != 0 else 1.0
+    for k in ("IS", "ISE", "ISC", "IKF", "IKR", "IRB", "ITF", "CJE", "CJC", "CJS", "ISS"):
+        p[k] *= area
+    for k in ("RB", "RBM", "RE", "RC"):
+        p[k] /= area
+    vt = VT_NOM
+    vcrit = vt * math.log(vt / (math.sqrt(2) * p["IS"]))
+    rbpr, rbpi = p["RBM"], p["RB"] - p["RBM"]
+    gcpr = 1 / p["RC"] if p["RC"] > 0 else 0.0
+    gepr = 1 / p["RE"] if p["RE"] > 0 else 0.0
+    ovtf = 1 / (1.44 * p["VTF"]) if p["VTF"] > 0 else 0.0
+    fe = depl_f(p["VJE"], p["MJE"], p["FC"])
+    fcc = depl_f(p["VJC"], p["MJC"], p["FC"])
+    has_charge = p["CJE"] > 0 or p["CJC"] > 0 or p["CJS"] > 0 or p["TF"] > 0 or p["TR"] > 0
+    return ([typ] + [p[k] for k in BJT_LAYOUT] + [vt, vcrit, rbpr, rbpi, gcpr, gepr, ovtf] + list(fe) + list(fcc)
+            + [1.0 if has_charge else 0.0])
+
+
 def source_params(e: Element, step: float, stop: float) -> List[float]:
     """sourceValue (device/source.go:21-66) with the waveform resolved."""
     kind, a = resolve_wave(e.wave, step, stop)
@@ -430,6 +516,8 @@ def element_params(c: Circuit, e: Element, step: float, stop: float) -> List[flo
         return diode_params(c, e)
     if e.kind == M:
         return mosfet_params(c, e)
+    if e.kind == Q:
+        return bjt_params(c, e)
     raise NotImplementedError(f"element kind {e.kind} ({e.name}) is not on this path")
 
 

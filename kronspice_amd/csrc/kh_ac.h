@@ -84,8 +84,43 @@ KH_HD inline MosAC mos_ac(const Lane& L, const int32_t* e) {
   return a;
 }
 
+// BJT small-signal admittances at the accepted point (device/bjt.go:690-705).
+struct BjtAC {
+  double itBE, itBC, gpi, gmu, gx;
+  kcplx ybe, ybeBC, ybc, ybx, ycs;
+};
+KH_HD inline BjtAC bjt_ac(const Lane& L, const int32_t* e, double omega) {
+  double v[4] = {L.S(e, QS_ACC + 0), L.S(e, QS_ACC + 1), L.S(e, QS_ACC + 2), L.S(e, QS_ACC + 3)};
+  double gmin = L.S(e, QS_GMIN);
+  BjtDC d = bjt_eval_dc(L, e, v[0], v[1], gmin);
+  BjtQ ch = bjt_charges(L, e, v, d);
+  double isub, gsub;
+  bjt_junction(L.P(e, QP_ISS), v[3], L.P(e, QP_NS) * L.P(e, QP_VT), &isub, &gsub);
+  gsub += gmin;
+  BjtAC a;
+  a.itBE = d.itBE; a.itBC = d.itBC; a.gpi = d.gpi; a.gmu = d.gmu; a.gx = d.gx;
+  a.ybe = {0, omega * ch.c[0]};
+  a.ybeBC = {0, omega * ch.dqbeDvbc};
+  a.ybc = {0, omega * ch.c[1]};
+  a.ybx = {0, omega * ch.c[2]};
+  a.ycs = {gsub, omega * ch.c[3]};
+  return a;
+}
+KH_HD inline bool cnonzero(kcplx y) { return y.re != 0 || y.im != 0; }
+// acBranch, device/bjt.go:718-731 (terminal indices)
+KH_HD inline void bjt_ac_branch(ACLane& A, const int32_t* e, int ta, int tb, kcplx y1, int p1, int n1, kcplx y2,
+                                int p2, int n2) {
+  if (cnonzero(y1)) {
+    A.add(e, ta, p1, y1); A.add(e, ta, n1, -y1); A.add(e, tb, p1, -y1); A.add(e, tb, n1, y1);
+  }
+  if (cnonzero(y2)) {
+    A.add(e, ta, p2, y2); A.add(e, ta, n2, -y2); A.add(e, tb, p2, -y2); A.add(e, tb, n2, y2);
+  }
+}
+
 // LoadAC of every element (device/passive.go, reactive.go:355-401,
-// source.go:186-217, diode.go:409-420, mosfet.go:779-814), in element order.
+// source.go:186-217, diode.go:409-420, mosfet.go:779-814, bjt.go:690-716), in
+// element order.
 KH_HD inline void ac_load(ACLane& A, int nel) {
   const Lane& L = A.dev;
   for (int k = 0; k < nel; k++) {
@@ -140,6 +175,20 @@ KH_HD inline void ac_load(ACLane& A, int nel) {
         if (L.T(e, 5) != L.T(e, 1)) A.stampY(e, 1, 5, {L.P(e, 8), 0});
         break;
       }
+      case K_Q: {  // terms: 0 c, 1 e, 2 b, 3 s, 4 cp, 5 bp, 6 ep (device/bjt.go:690-716)
+        BjtAC a = bjt_ac(L, e, A.omega);
+        const int sa = bjt_sa(L, e), sb = bjt_sb(L, e);
+        const kcplx z = {0, 0};
+        bjt_ac_branch(A, e, 4, 6, {a.itBE, 0}, 5, 6, {a.itBC, 0}, 5, 4);
+        bjt_ac_branch(A, e, 5, 6, kcplx{a.gpi, 0} + a.ybe, 5, 6, a.ybeBC, 5, 4);
+        bjt_ac_branch(A, e, 5, 4, kcplx{a.gmu, 0} + a.ybc, 5, 4, z, 0, 0);
+        bjt_ac_branch(A, e, 2, 4, a.ybx, 2, 4, z, 0, 0);
+        bjt_ac_branch(A, e, sa, sb, a.ycs, sa, sb, z, 0, 0);
+        if (L.T(e, 5) != L.T(e, 2)) A.stampY(e, 2, 5, {a.gx, 0});
+        if (L.T(e, 4) != L.T(e, 0)) A.stampY(e, 0, 4, {L.P(e, QP_GCPR), 0});
+        if (L.T(e, 6) != L.T(e, 1)) A.stampY(e, 1, 6, {L.P(e, QP_GEPR), 0});
+        break;
+      }
     }
   }
 }
@@ -170,6 +219,24 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
       kcplx ig = iqs[0] + iqs[1] + iqs[2];
       kcplx ib = ibs + ibd - iqs[2] + iqs[3] + iqs[4];
       s = A.xat(x, L.T(e, 0)) * conj(id) + gg * conj(ig) + A.xat(x, L.T(e, 1)) * conj(is) + bb * conj(ib);
+      s = {s.re / 2.0, s.im / 2.0};
+    } else if (e[ET_KIND] == K_Q) {  // acCurrents + ACPower, device/bjt.go:734-765
+      BjtAC a = bjt_ac(L, e, A.omega);
+      kcplx xb = A.xat(x, L.T(e, 2)), xcp = A.xat(x, L.T(e, 4)), xbp = A.xat(x, L.T(e, 5)), xep = A.xat(x, L.T(e, 6));
+      kcplx vbe = xbp - xep, vbc = xbp - xcp, vbx = xb - xcp;
+      kcplx vcs = A.xat(x, L.T(e, bjt_sa(L, e))) - A.xat(x, L.T(e, bjt_sb(L, e)));
+      kcplx it = kcplx{a.itBE, 0} * vbe + kcplx{a.itBC, 0} * vbc;
+      kcplx ibe = (kcplx{a.gpi, 0} + a.ybe) * vbe + a.ybeBC * vbc;
+      kcplx ibc = (kcplx{a.gmu, 0} + a.ybc) * vbc;
+      kcplx ix = a.ybx * vbx, isb = a.ycs * vcs;
+      kcplx ic, ib, ie = -it - ibe, is;
+      if (L.P(e, QP_SUBS) > 0) {
+        ic = it - ibc - ix - isb; ib = ibe + ibc + ix; is = isb;
+      } else {
+        ic = it - ibc - ix; ib = ibe + ibc + ix + isb; is = -isb;
+      }
+      s = A.xat(x, L.T(e, 0)) * conj(ic) + xb * conj(ib) + A.xat(x, L.T(e, 1)) * conj(ie) +
+          A.xat(x, L.T(e, 3)) * conj(is);
       s = {s.re / 2.0, s.im / 2.0};
     } else {
       kcplx v = A.xat(x, L.T(e, 0)) - A.xat(x, L.T(e, 1));

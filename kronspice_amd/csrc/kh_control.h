@@ -148,6 +148,7 @@ struct Ctl {
         case K_I: load_I(L, e); break;
         case K_D: load_D(L, e); break;
         case K_M: load_M(L, e); break;
+        case K_Q: load_Q(L, e); break;
       }
     }
   }
@@ -167,6 +168,7 @@ struct Ctl {
       case K_I: return op_I(L, e, x);
       case K_D: return op_D(L, e, x);
       case K_M: return op_M(L, e, x);
+      case K_Q: return op_Q(L, e, x);
     }
     OpPt o = {0, 0, 0, 0};
     return o;
@@ -179,6 +181,7 @@ struct Ctl {
         case K_L: accept_L(L, e, A.x, t, mode); break;
         case K_D: accept_D(L, e, A.x, t, mode); break;
         case K_M: accept_M(L, e, A.x, t, mode); break;
+        case K_Q: accept_Q(L, e, A.x, t, mode); break;
       }
     }
   }
@@ -187,6 +190,7 @@ struct Ctl {
       const int32_t* e = el(k);
       if (e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
       if (e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
+      if (e[ET_KIND] == K_Q && L.S(e, QS_LIM) != 0) return false;
     }
     for (int i = 0; i < n; i++) {
       double xv = X(A.x, i), old = X(A.xold, i);
@@ -198,6 +202,7 @@ struct Ctl {
       const int32_t* e = el(k);
       if (e[ET_KIND] == K_D && !conv_D(L, e, A.x, P.relTol, P.absTol)) return false;
       if (e[ET_KIND] == K_M && !conv_M(L, e, A.x, P.relTol, P.absTol)) return false;
+      if (e[ET_KIND] == K_Q && !conv_Q(L, e, A.x, P.relTol, P.absTol)) return false;
     }
     return true;
   }
@@ -226,6 +231,7 @@ struct Ctl {
         case K_L: pk = trunc_L(L, e, A.x, tc); break;
         case K_D: pk = trunc_D(L, e, A.x, tc); break;
         case K_M: pk = trunc_M(L, e, A.x, tc); break;
+        case K_Q: pk = trunc_Q(L, e, A.x, tc); break;
         default: continue;
       }
       if (pk < prop) prop = pk;
@@ -259,7 +265,7 @@ struct Ctl {
         if (L.P(e, 0) > 0) { pw = -o.power; ch = 2; }
       } else if (kind == K_V || kind == K_I) {
         pw = -o.power; ch = 0;
-      } else if (kind == K_D || kind == K_M) {
+      } else if (kind == K_D || kind == K_M || kind == K_Q) {
         pw = -(o.power - o.storage); ch = 2;
       }
       if (ch >= 0 && pw != 0) {
@@ -268,7 +274,7 @@ struct Ctl {
         else if (pw > 0) inflow += pw;
         else outflow += -pw;
       }
-      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M) {
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) {
         stor += o.storage;
         scale += fabs(o.storage);
       }
@@ -291,7 +297,7 @@ struct Ctl {
       int kind = e[ET_KIND];
       double pw = 0;
       int ch = -1;
-      if (kind == K_R || kind == K_V || kind == K_I || kind == K_D || kind == K_M) {
+      if (kind == K_R || kind == K_V || kind == K_I || kind == K_D || kind == K_M || kind == K_Q) {
         OpPt o = operating(e, A.x);
         if (kind == K_R) { if (L.P(e, 0) > 0) { pw = -o.power; ch = 2; } }
         else if (kind == K_V || kind == K_I) { pw = -o.power; ch = 0; }
@@ -315,7 +321,7 @@ struct Ctl {
     for (int k = 0; k < A.ckt.nel; k++) {  // storage of reactive devices
       const int32_t* e = el(k);
       int kind = e[ET_KIND];
-      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M) {
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) {
         OpPt o = operating(e, A.x);
         stor += o.storage;
         scale += fabs(o.storage);
@@ -430,6 +436,7 @@ struct Ctl {
       switch (e[ET_KIND]) {
         case K_D: accept_D(L, e, A.x, 0, MODE_OP); break;
         case K_M: accept_M(L, e, A.x, 0, MODE_OP); break;
+        case K_Q: accept_Q(L, e, A.x, 0, MODE_OP); break;
       }
     }
   }

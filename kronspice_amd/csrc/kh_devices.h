@@ -28,12 +28,12 @@ enum {
   ET_W = ET_SLOT + KH_NT * KH_NT
 };
 
-enum { K_R = 1, K_C = 2, K_L = 3, K_V = 4, K_I = 5, K_D = 6, K_M = 7 };
+enum { K_R = 1, K_C = 2, K_L = 3, K_V = 4, K_I = 5, K_D = 6, K_M = 7, K_Q = 8 };
 enum { MODE_OP = 0, MODE_TRANOP = 1, MODE_UICINIT = 2, MODE_TRAN = 3 };
 enum { BE = 1, TRAP = 2 };
 
 // State sizes (doubles per element).
-enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96 };
+enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96, NS_Q = 66 };
 
 // history (device/lte.go:23-37) lives in 7 state words: t0 t1 t2 q0 q1 q2 n.
 constexpr double kInf = __builtin_huge_val();
@@ -121,12 +121,15 @@ KH_HD inline double lte_step(const Lane& L, const int32_t* e, int h, double qn, 
   double fact = 2.0, cc = 0.5;
   if (k == 2) { fact = 6; cc = 1.0 / 12; }
   double hh = tc.H;
-  double errI = cc * pow(hh, (double)k) * fabs(deriv * fact);
+  // math.Pow(h, k) and math.Pow(r, 1/k) for k = 1, 2: Go's Pow returns x for
+  // y = 1, Sqrt(x) for y = 0.5 and squares the mantissa exactly once for y = 2
+  // (math/pow.go), so h, h*h, r and sqrt(r) are its results bit for bit.
+  double errI = cc * (k == 1 ? hh : hh * hh) * fabs(deriv * fact);
   if (errI == 0 || isnan(errI)) return kInf;
   double qmax = fmax(fabs(qn), fabs(L.S(e, h + 3)));
   double allowed = tc.trTol * fmax(tc.relTol * fmax(fabs(in), fabs(inPrev)) + absTol,
                                    tc.relTol * fmax(qmax, tc.chgTol) / hh);
-  return hh * pow(allowed / errI, 1 / (double)k);
+  return hh * (k == 1 ? allowed / errI : sqrt(allowed / errI));
 }
 
 // ---- sources (device/source.go:43-66, device/waveform.go:69-239) -----------
@@ -944,6 +947,355 @@ KH_HD inline double trunc_M(const Lane& L, const int32_t* e, const double* x, co
     if (qk == 0 && L.S(e, MS_QPREV + k) == 0) continue;
     double i = L.S(e, MS_IQ + k) + L.S(e, MS_AG) * mos_cap(L, e, k) * (v[k] - L.S(e, MS_DV + k));
     h = fmin(h, lte_step(L, e, MS_HIST + 7 * k, qk, i, L.S(e, MS_IQPREV + k), tc.absTol, tc));
+  }
+  return h;
+}
+
+// ---- Gummel–Poon BJT (device/bjt.go:272-765) --------------------------------
+// terms: 0 c, 1 e, 2 b, 3 s (external), 4 cp, 5 bp, 6 ep (internal, or the
+// external node when the series resistance is zero). The substrate junction
+// runs from terminal sa to sb: (s, cp) vertical (SUBS > 0), (bp, s) lateral.
+// params: the Q layout of include/kronhip.h (0 type, 1 IS, ... 51 hasCharge).
+enum {
+  QP_TYPE = 0, QP_IS = 1, QP_BF, QP_NF, QP_VAF, QP_IKF, QP_ISE, QP_NE, QP_BR, QP_NR, QP_VAR, QP_IKR, QP_ISC, QP_NC,
+  QP_RB, QP_IRB, QP_RBM, QP_RE, QP_RC, QP_CJE, QP_VJE, QP_MJE, QP_TF, QP_XTF, QP_VTF, QP_ITF, QP_CJC, QP_VJC, QP_MJC,
+  QP_XCJC, QP_TR, QP_CJS, QP_VJS, QP_MJS, QP_FC, QP_ISS, QP_NS, QP_SUBS, QP_VT, QP_VCRIT, QP_RBPR, QP_RBPI, QP_GCPR,
+  QP_GEPR, QP_OVTF, QP_F1E, QP_F2E, QP_F3E, QP_F1C, QP_F2C, QP_F3C, QP_HASQ
+};
+// state: linearization (vbe vbc vbx vcs, the DC currents and slopes the stamp
+// used, substrate current/slope, capacitances, ag, iq[4], flags, gmin),
+// accepted voltages, qPrev[4], iqPrev[4], four histories.
+enum {
+  QS_V = 0, QS_IT = 4, QS_ITBE, QS_ITBC, QS_IBE, QS_GPI, QS_IBC, QS_GMU, QS_ISUB, QS_GSUB, QS_C = 13, QS_DQ = 17,
+  QS_AG = 18, QS_IQ = 19, QS_TRAN = 23, QS_LIM = 24, QS_GMIN = 25, QS_ACC = 26, QS_QPREV = 30, QS_IQPREV = 34,
+  QS_HIST = 38
+};
+static_assert(QS_HIST + 4 * 7 == NS_Q, "BJT state size");
+
+KH_HD inline int bjt_sa(const Lane& L, const int32_t* e) { return L.P(e, QP_SUBS) > 0 ? 3 : 5; }
+KH_HD inline int bjt_sb(const Lane& L, const int32_t* e) { return L.P(e, QP_SUBS) > 0 ? 4 : 3; }
+
+// junction, device/bjt.go:272-283
+KH_HD inline void bjt_junction(double is, double v, double nvt, double* i, double* g) {
+  if (is == 0) { *i = 0; *g = 0; return; }
+  if (v >= -3 * nvt) {
+    double ex = exp(fmin(v / nvt, 700.0));
+    *i = is * (ex - 1);
+    *g = is * ex / nvt;
+    return;
+  }
+  double a = 3 * nvt / (v * kE);
+  a = a * a * a;
+  *i = -is * (1 + a);
+  *g = is * 3 * a / v;
+}
+
+struct BjtDC {  // bjtDC, device/bjt.go:118-127
+  double cbe, gbe, cbc, gbc, it, itBE, itBC, ibe, gpi, ibc, gmu, qb, dqbdve, dqbdvc, gx;
+};
+
+// evalDC, device/bjt.go:294-355
+KH_HD inline BjtDC bjt_eval_dc(const Lane& L, const int32_t* e, double vbe, double vbc, double gmin) {
+  BjtDC d;
+  double vt = L.P(e, QP_VT);
+  bjt_junction(L.P(e, QP_IS), vbe, L.P(e, QP_NF) * vt, &d.cbe, &d.gbe);
+  double cben, gben, cbcn, gbcn;
+  bjt_junction(L.P(e, QP_ISE), vbe, L.P(e, QP_NE) * vt, &cben, &gben);
+  cben += gmin * vbe;
+  gben += gmin;
+  bjt_junction(L.P(e, QP_IS), vbc, L.P(e, QP_NR) * vt, &d.cbc, &d.gbc);
+  bjt_junction(L.P(e, QP_ISC), vbc, L.P(e, QP_NC) * vt, &cbcn, &gbcn);
+  cbcn += gmin * vbc;
+  gbcn += gmin;
+  double VAF = L.P(e, QP_VAF), VAR = L.P(e, QP_VAR), IKF = L.P(e, QP_IKF), IKR = L.P(e, QP_IKR);
+  double ivaf = VAF > 0 ? 1 / VAF : 0.0, ivar = VAR > 0 ? 1 / VAR : 0.0;
+  double oik = IKF > 0 ? 1 / IKF : 0.0, oikr = IKR > 0 ? 1 / IKR : 0.0;
+  double q1 = 1 / (1 - ivaf * vbc - ivar * vbe);
+  if (oik == 0 && oikr == 0) {
+    d.qb = q1;
+    d.dqbdve = q1 * d.qb * ivar;
+    d.dqbdvc = q1 * d.qb * ivaf;
+  } else {
+    double q2 = oik * d.cbe + oikr * d.cbc;
+    double arg = fmax(0.0, 1 + 4 * q2);
+    double sq = arg != 0 ? sqrt(arg) : 1.0;
+    d.qb = q1 * (1 + sq) / 2;
+    d.dqbdve = q1 * (d.qb * ivar + oik * d.gbe / sq);
+    d.dqbdvc = q1 * (d.qb * ivaf + oikr * d.gbc / sq);
+  }
+  d.it = (d.cbe - d.cbc) / d.qb;
+  d.itBE = (d.gbe - d.it * d.dqbdve) / d.qb;
+  d.itBC = (-d.gbc - d.it * d.dqbdvc) / d.qb;
+  d.ibe = d.cbe / L.P(e, QP_BF) + cben;
+  d.gpi = d.gbe / L.P(e, QP_BF) + gben;
+  d.ibc = d.cbc / L.P(e, QP_BR) + cbcn;
+  d.gmu = d.gbc / L.P(e, QP_BR) + gbcn;
+  double rbpr = L.P(e, QP_RBPR), rbpi = L.P(e, QP_RBPI), IRB = L.P(e, QP_IRB);
+  double gx = rbpr + rbpi / d.qb;
+  if (IRB > 0 && rbpi != 0) {  // current crowding
+    double cb = d.ibe + d.ibc;
+    double a1 = fmax(cb / IRB, 1e-9);
+    double z = (-1 + sqrt(1 + 14.59025 * a1)) / 2.4317 / sqrt(a1);
+    double tz = tan(z);
+    gx = rbpr + 3 * rbpi * (tz - z) / z / tz / tz;
+  }
+  if (gx != 0) gx = 1 / gx;
+  d.gx = gx;
+  return d;
+}
+
+// depletion with exp(-m log arg), device/bjt.go:358-370
+KH_HD inline void bjt_depletion(double v, double cz, double vj, double m, double fc, double f1, double f2, double f3,
+                                double* qv, double* c) {
+  if (cz == 0) { *qv = 0; *c = 0; return; }
+  double fcv = fc * vj;
+  if (v < fcv) {
+    double arg = 1 - v / vj;
+    double sarg = exp(-m * log(arg));
+    *qv = vj * cz * (1 - arg * sarg) / (1 - m);
+    *c = cz * sarg;
+    return;
+  }
+  double czf2 = cz / f2;
+  *qv = cz * f1 + czf2 * (f3 * (v - fcv) + (m / (vj + vj)) * (v * v - fcv * fcv));
+  *c = czf2 * (f3 + m * v / vj);
+}
+
+struct BjtQ {  // bjtCharge, device/bjt.go:130-134
+  double q[4], c[4], dqbeDvbc;
+};
+
+// charges, device/bjt.go:373-410
+KH_HD inline BjtQ bjt_charges(const Lane& L, const int32_t* e, const double v[4], const BjtDC& d) {
+  BjtQ ch;
+  ch.dqbeDvbc = 0;
+  double cbe = d.cbe, gbe = d.gbe;
+  double TF = L.P(e, QP_TF);
+  if (TF != 0 && v[0] > 0) {
+    double argtf = 0, arg2 = 0, arg3 = 0, XTF = L.P(e, QP_XTF), ovtf = L.P(e, QP_OVTF), ITF = L.P(e, QP_ITF);
+    if (XTF != 0) {
+      argtf = XTF;
+      if (ovtf != 0) argtf *= exp(v[1] * ovtf);
+      arg2 = argtf;
+      if (ITF != 0) {
+        double t = cbe / (cbe + ITF);
+        argtf *= t * t;
+        arg2 = argtf * (3 - t - t);
+      }
+      arg3 = cbe * argtf * ovtf;
+    }
+    cbe = cbe * (1 + argtf) / d.qb;
+    gbe = (gbe * (1 + arg2) - cbe * d.dqbdve) / d.qb;
+    ch.dqbeDvbc = TF * (arg3 - cbe * d.dqbdvc) / d.qb;
+  }
+  double FC = L.P(e, QP_FC), qd, cdp;
+  bjt_depletion(v[0], L.P(e, QP_CJE), L.P(e, QP_VJE), L.P(e, QP_MJE), FC, L.P(e, QP_F1E), L.P(e, QP_F2E),
+                L.P(e, QP_F3E), &qd, &cdp);
+  ch.q[0] = TF * cbe + qd;
+  ch.c[0] = TF * gbe + cdp;
+  double CJC = L.P(e, QP_CJC), czbc = CJC * L.P(e, QP_XCJC), VJC = L.P(e, QP_VJC), MJC = L.P(e, QP_MJC);
+  double f1c = L.P(e, QP_F1C), f2c = L.P(e, QP_F2C), f3c = L.P(e, QP_F3C), TR = L.P(e, QP_TR);
+  bjt_depletion(v[1], czbc, VJC, MJC, FC, f1c, f2c, f3c, &qd, &cdp);
+  ch.q[1] = TR * d.cbc + qd;
+  ch.c[1] = TR * d.gbc + cdp;
+  bjt_depletion(v[2], CJC - czbc, VJC, MJC, FC, f1c, f2c, f3c, &ch.q[2], &ch.c[2]);
+  double CJS = L.P(e, QP_CJS);
+  ch.q[3] = 0;
+  ch.c[3] = 0;
+  if (CJS != 0) {
+    double VJS = L.P(e, QP_VJS), MJS = L.P(e, QP_MJS), vcs = v[3];
+    if (vcs < 0) {
+      double arg = 1 - vcs / VJS;
+      double sarg = exp(-MJS * log(arg));
+      ch.q[3] = VJS * CJS * (1 - arg * sarg) / (1 - MJS);
+      ch.c[3] = CJS * sarg;
+    } else {
+      ch.q[3] = vcs * CJS * (1 + MJS * vcs / (2 * VJS));
+      ch.c[3] = CJS * (1 + MJS * vcs / VJS);
+    }
+  }
+  return ch;
+}
+
+// voltages, device/bjt.go:413-418: polarity-corrected vbe, vbc, vbx, vcs.
+KH_HD inline void bjt_voltages(const Lane& L, const int32_t* e, const double* x, double v[4]) {
+  double t = L.P(e, QP_TYPE);
+  double vb = L.xat(x, L.T(e, 2)), vbp = L.xat(x, L.T(e, 5)), vcp = L.xat(x, L.T(e, 4)), vep = L.xat(x, L.T(e, 6));
+  v[0] = t * (vbp - vep);
+  v[1] = t * (vbp - vcp);
+  v[2] = t * (vb - vcp);
+  v[3] = t * (L.xat(x, L.T(e, bjt_sa(L, e))) - L.xat(x, L.T(e, bjt_sb(L, e))));
+}
+
+// branch, device/bjt.go:447-469 (terminal indices into the element's terms):
+// a current i0 + g1 dv1 + g2 dv2 from ta to tb; g2 = 0 stamps nothing more.
+KH_HD inline void bjt_branch(Lane& L, const int32_t* e, int ta, int tb, double i0, double g1, int p1, int n1,
+                             double v10, double g2, int p2, int n2, double v20) {
+  if (g1 != 0) {
+    L.add(e, ta, p1, g1); L.add(e, ta, n1, -g1); L.add(e, tb, p1, -g1); L.add(e, tb, n1, g1);
+  }
+  if (g2 != 0) {
+    L.add(e, ta, p2, g2); L.add(e, ta, n2, -g2); L.add(e, tb, p2, -g2); L.add(e, tb, n2, g2);
+  }
+  double ieq = L.P(e, QP_TYPE) * (i0 - g1 * v10 - g2 * v20);
+  L.addRHS(L.T(e, ta), -ieq);
+  L.addRHS(L.T(e, tb), ieq);
+}
+
+// Load + stamp, device/bjt.go:421-515
+KH_HD inline void load_Q(Lane& L, const int32_t* e) {
+  double gmin = L.gmin;
+  L.S(e, QS_GMIN) = gmin;
+  double v[4];
+  bjt_voltages(L, e, L.X, v);
+  bool lim = false;
+  if (L.initJct) {
+    v[0] = L.P(e, QP_VCRIT); v[1] = 0; v[2] = 0; v[3] = 0;
+    lim = true;
+  } else {
+    bool l1, l2;
+    double vt = L.P(e, QP_VT), vcrit = L.P(e, QP_VCRIT);
+    v[0] = pnjlim(v[0], L.S(e, QS_ACC + 0), vt, vcrit, &l1);
+    v[1] = pnjlim(v[1], L.S(e, QS_ACC + 1), vt, vcrit, &l2);
+    lim = l1 || l2;
+  }
+  L.S(e, QS_ACC + 0) = v[0];
+  L.S(e, QS_ACC + 1) = v[1];
+  for (int k = 0; k < 4; k++) L.S(e, QS_V + k) = v[k];
+  BjtDC d = bjt_eval_dc(L, e, v[0], v[1], gmin);
+  L.S(e, QS_IT) = d.it; L.S(e, QS_ITBE) = d.itBE; L.S(e, QS_ITBC) = d.itBC; L.S(e, QS_IBE) = d.ibe;
+  L.S(e, QS_GPI) = d.gpi; L.S(e, QS_IBC) = d.ibc; L.S(e, QS_GMU) = d.gmu;
+  double isub, gsub;  // substrate, device/bjt.go:288-291
+  bjt_junction(L.P(e, QP_ISS), v[3], L.P(e, QP_NS) * L.P(e, QP_VT), &isub, &gsub);
+  isub += gmin * v[3];
+  gsub += gmin;
+  L.S(e, QS_ISUB) = isub;
+  L.S(e, QS_GSUB) = gsub;
+  bool tran = false;
+  double ag = 0, iq[4] = {0, 0, 0, 0};
+  BjtQ ch;
+  if (L.mode == MODE_TRAN && L.P(e, QP_HASQ) != 0) {
+    ch = bjt_charges(L, e, v, d);
+    tran = true;
+    if (L.method == TRAP) {
+      ag = 2 / L.H;
+      for (int k = 0; k < 4; k++) iq[k] = ag * (ch.q[k] - L.S(e, QS_QPREV + k)) - L.S(e, QS_IQPREV + k);
+    } else {
+      ag = 1 / L.H;
+      for (int k = 0; k < 4; k++) iq[k] = ag * (ch.q[k] - L.S(e, QS_QPREV + k));
+    }
+    for (int k = 0; k < 4; k++) L.S(e, QS_C + k) = ch.c[k];
+    L.S(e, QS_DQ) = ch.dqbeDvbc;
+  }
+  L.S(e, QS_AG) = ag;
+  for (int k = 0; k < 4; k++) L.S(e, QS_IQ + k) = iq[k];
+  L.S(e, QS_TRAN) = tran ? 1 : 0;
+  L.S(e, QS_LIM) = lim ? 1 : 0;
+  const int sa = bjt_sa(L, e), sb = bjt_sb(L, e);
+  bjt_branch(L, e, 4, 6, d.it, d.itBE, 5, 6, v[0], d.itBC, 5, 4, v[1]);
+  bjt_branch(L, e, 5, 6, d.ibe, d.gpi, 5, 6, v[0], 0, 0, 0, 0);
+  bjt_branch(L, e, 5, 4, d.ibc, d.gmu, 5, 4, v[1], 0, 0, 0, 0);
+  bjt_branch(L, e, sa, sb, isub, gsub, sa, sb, v[3], 0, 0, 0, 0);
+  if (tran) {
+    bjt_branch(L, e, 5, 6, iq[0], ag * ch.c[0], 5, 6, v[0], ag * ch.dqbeDvbc, 5, 4, v[1]);
+    bjt_branch(L, e, 5, 4, iq[1], ag * ch.c[1], 5, 4, v[1], 0, 0, 0, 0);
+    bjt_branch(L, e, 2, 4, iq[2], ag * ch.c[2], 2, 4, v[2], 0, 0, 0, 0);
+    bjt_branch(L, e, sa, sb, iq[3], ag * ch.c[3], sa, sb, v[3], 0, 0, 0, 0);
+  }
+  if (L.T(e, 5) != L.T(e, 2)) mos_stampG(L, e, 2, 5, d.gx);
+  if (L.T(e, 4) != L.T(e, 0)) mos_stampG(L, e, 0, 4, L.P(e, QP_GCPR));
+  if (L.T(e, 6) != L.T(e, 1)) mos_stampG(L, e, 1, 6, L.P(e, QP_GEPR));
+}
+
+// ConvTest, device/bjt.go:521-532
+KH_HD inline bool conv_Q(const Lane& L, const int32_t* e, const double* x, double reltol, double abstol) {
+  double v[4];
+  bjt_voltages(L, e, x, v);
+  double dbe = v[0] - L.S(e, QS_V + 0), dbc = v[1] - L.S(e, QS_V + 1);
+  double it = L.S(e, QS_IT), ibc = L.S(e, QS_IBC), gmu = L.S(e, QS_GMU);
+  double ichat = it - ibc + L.S(e, QS_ITBE) * dbe + (L.S(e, QS_ITBC) - gmu) * dbc;
+  double ibhat = L.S(e, QS_IBE) + ibc + L.S(e, QS_GPI) * dbe + gmu * dbc;
+  BjtDC n = bjt_eval_dc(L, e, v[0], v[1], L.S(e, QS_GMIN));
+  double ic = n.it - n.ibc, ib = n.ibe + n.ibc;
+  return fabs(ichat - ic) <= reltol * fmax(fabs(ichat), fabs(ic)) + abstol &&
+         fabs(ibhat - ib) <= reltol * fmax(fabs(ibhat), fabs(ib)) + abstol;
+}
+
+// terminalCurrents + Operating, device/bjt.go:537-579
+KH_HD inline OpPt op_Q(const Lane& L, const int32_t* e, const double* x) {
+  double v[4];
+  bjt_voltages(L, e, x, v);
+  double t = L.P(e, QP_TYPE);
+  double dbe = v[0] - L.S(e, QS_V + 0), dbc = v[1] - L.S(e, QS_V + 1);
+  double it = t * (L.S(e, QS_IT) + L.S(e, QS_ITBE) * dbe + L.S(e, QS_ITBC) * dbc);
+  double ibe = t * (L.S(e, QS_IBE) + L.S(e, QS_GPI) * dbe);
+  double ibc = t * (L.S(e, QS_IBC) + L.S(e, QS_GMU) * dbc);
+  double iqbe = 0, iqbc = 0, iqbx = 0, iqcs = 0, ps = 0;
+  if (L.S(e, QS_TRAN) != 0) {
+    double ag = L.S(e, QS_AG);
+    iqbe = t * (L.S(e, QS_IQ + 0) + ag * L.S(e, QS_C + 0) * dbe + ag * L.S(e, QS_DQ) * dbc);
+    iqbc = t * (L.S(e, QS_IQ + 1) + ag * L.S(e, QS_C + 1) * dbc);
+    iqbx = t * (L.S(e, QS_IQ + 2) + ag * L.S(e, QS_C + 2) * (v[2] - L.S(e, QS_V + 2)));
+    iqcs = t * (L.S(e, QS_IQ + 3) + ag * L.S(e, QS_C + 3) * (v[3] - L.S(e, QS_V + 3)));
+    ps = v[0] * t * iqbe + v[1] * t * iqbc + v[2] * t * iqbx + v[3] * t * iqcs;
+  }
+  double isb = t * (L.S(e, QS_ISUB) + L.S(e, QS_GSUB) * (v[3] - L.S(e, QS_V + 3))) + iqcs;
+  double ic = it - ibc - iqbc - iqbx;
+  double ib = ibe + ibc + iqbe + iqbc + iqbx;
+  double ie = -it - ibe - iqbe;
+  double is;
+  if (L.P(e, QP_SUBS) > 0) {
+    is = isb;
+    ic = ic - isb;
+  } else {
+    ib = ib + isb;
+    is = -isb;
+  }
+  double vc = L.xat(x, L.T(e, 0)), ve = L.xat(x, L.T(e, 1));
+  OpPt o;
+  o.power = vc * ic + L.xat(x, L.T(e, 2)) * ib + ve * ie + L.xat(x, L.T(e, 3)) * is;
+  o.voltage = vc - ve;
+  o.current = ic;
+  o.storage = ps;
+  return o;
+}
+
+// Accept, device/bjt.go:582-605
+KH_HD inline void accept_Q(const Lane& L, const int32_t* e, const double* x, double t, int mode) {
+  double v[4];
+  bjt_voltages(L, e, x, v);
+  BjtDC d = bjt_eval_dc(L, e, v[0], v[1], L.S(e, QS_GMIN));
+  BjtQ ch = bjt_charges(L, e, v, d);
+  bool tr = mode == MODE_TRAN && L.S(e, QS_TRAN) != 0;
+  double ag = L.S(e, QS_AG);
+  for (int k = 0; k < 4; k++) {
+    double i = 0;
+    if (tr) {
+      i = L.S(e, QS_IQ + k) + ag * L.S(e, QS_C + k) * (v[k] - L.S(e, QS_V + k));
+      if (k == 0) i += ag * L.S(e, QS_DQ) * (v[1] - L.S(e, QS_V + 1));
+    }
+    L.S(e, QS_IQPREV + k) = i;
+    if (mode != MODE_TRAN) hist_reset(L, e, QS_HIST + 7 * k);
+    L.S(e, QS_QPREV + k) = ch.q[k];
+    hist_push(L, e, QS_HIST + 7 * k, t, ch.q[k]);
+  }
+  for (int k = 0; k < 4; k++) L.S(e, QS_ACC + k) = v[k];
+}
+
+// Truncate, device/bjt.go:655-675
+KH_HD inline double trunc_Q(const Lane& L, const int32_t* e, const double* x, const TruncCtx& tc) {
+  if (L.P(e, QP_HASQ) == 0 || L.S(e, QS_TRAN) == 0) return kInf;
+  double v[4];
+  bjt_voltages(L, e, x, v);
+  BjtDC d = bjt_eval_dc(L, e, v[0], v[1], L.S(e, QS_GMIN));
+  BjtQ ch = bjt_charges(L, e, v, d);
+  double ag = L.S(e, QS_AG);
+  double h = kInf;
+  for (int k = 0; k < 4; k++) {
+    if (ch.q[k] == 0 && L.S(e, QS_QPREV + k) == 0) continue;
+    double i = L.S(e, QS_IQ + k) + ag * L.S(e, QS_C + k) * (v[k] - L.S(e, QS_V + k));
+    if (k == 0) i += ag * L.S(e, QS_DQ) * (v[1] - L.S(e, QS_V + 1));
+    h = fmin(h, lte_step(L, e, QS_HIST + 7 * k, ch.q[k], i, L.S(e, QS_IQPREV + k), tc.absTol, tc));
   }
   return h;
 }

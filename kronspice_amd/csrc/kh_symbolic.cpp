@@ -188,6 +188,19 @@ static const int kPairsM[][2] = {
     {3, 3}, {3, 5}, {5, 3}, {5, 5}, {3, 3}, {3, 4}, {4, 3}, {4, 4},                                  // junction charges
     {0, 0}, {4, 4}, {0, 4}, {4, 0}, {1, 1}, {5, 5}, {1, 5}, {5, 1}};                                 // RD, RS
 
+// BJT terms: 0 c, 1 e, 2 b, 3 s, 4 cp, 5 bp, 6 ep (device/bjt.go:492-515, in Load's Add order).
+static const int kPairsQ[][2] = {
+    {4, 5}, {4, 6}, {6, 5}, {6, 6}, {4, 5}, {4, 4}, {6, 5}, {6, 4},  // transport (cp -> ep)
+    {5, 5}, {5, 6}, {6, 5}, {6, 6},                                  // base-emitter
+    {5, 5}, {5, 4}, {4, 5}, {4, 4},                                  // base-collector
+    {3, 3}, {3, 4}, {4, 3}, {4, 4},                                  // substrate, vertical (s -> cp)
+    {5, 5}, {5, 3}, {3, 5}, {3, 3},                                  // substrate, lateral (bp -> s)
+    {5, 5}, {5, 6}, {6, 5}, {6, 6}, {5, 5}, {5, 4}, {6, 5}, {6, 4},  // base-emitter charge
+    {2, 2}, {2, 4}, {4, 2}, {4, 4},                                  // external base-collector charge
+    {2, 2}, {5, 5}, {2, 5}, {5, 2},                                  // RB
+    {0, 0}, {4, 4}, {0, 4}, {4, 0},                                  // RC
+    {1, 1}, {6, 6}, {1, 6}, {6, 1}};                                 // RE
+
 bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0) {
   num_nodes = numNodes;
   nel = n_elems;
@@ -244,6 +257,19 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         nonlin = true;
         break;
       }
+      case K_Q: {  // cp, bp, ep behind RC, RB, RE (device/bjt.go:250-268)
+        int c = unk(E.node[0]), em = unk(E.node[1]), b = unk(E.node[2]);
+        t[ET_TERM] = c; t[ET_TERM + 1] = em; t[ET_TERM + 2] = b; t[ET_TERM + 3] = unk(E.node[3]);
+        int cp = c, bp = b, ep = em;
+        if (params_c0[E.poff + 18] > 0) { cp = alloc(); internal.push_back(cp); }
+        if (params_c0[E.poff + 14] > 0) { bp = alloc(); internal.push_back(bp); }
+        if (params_c0[E.poff + 17] > 0) { ep = alloc(); internal.push_back(ep); }
+        t[ET_TERM + 4] = cp; t[ET_TERM + 5] = bp; t[ET_TERM + 6] = ep;
+        ns = NS_Q;
+        npar = KH_NPAR_Q;
+        nonlin = true;
+        break;
+      }
       default:
         error = "element " + std::to_string(e) + ": kind " + std::to_string(E.kind) + " is not supported";
         return false;
@@ -288,6 +314,7 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
       case K_V: pairs = kPairsV; np2 = 4; break;
       case K_D: pairs = kPairsD; np2 = 7; break;
       case K_M: pairs = kPairsM; np2 = (int)(sizeof(kPairsM) / sizeof(kPairsM[0])); break;
+      case K_Q: pairs = kPairsQ; np2 = (int)(sizeof(kPairsQ) / sizeof(kPairsQ[0])); break;
       default: break;
     }
     for (int k = 0; k < KH_NT * KH_NT; k++) t[ET_SLOT + k] = -1;

@@ -1,11 +1,11 @@
 """SPICE netlist subset reader for the solve path's element set.
 
 Follows kronspice's netlist package for the cards this path runs
-(netlist/parser.go, netlist/units.go): R, C, L, V, I, D and M elements,
+(netlist/parser.go, netlist/units.go): R, C, L, V, I, D, M and Q elements,
 .model, .op, .tran, .ac, .dc and .end; comments (* lines and ; tails) and +
 continuations.  Elements are handed to the circuit builder in file order, with
 multi-terminal node order as circuit.FromDeck (circuit/builder.go:307-470):
-MOSFET drain, source, gate, bulk.
+MOSFET drain, source, gate, bulk; BJT collector, emitter, base[, substrate].
 """
 from __future__ import annotations
 
@@ -187,6 +187,22 @@ def parse(text: str) -> Deck:
                     kk, v = t.split("=", 1)
                     params[kk.lower()] = parse_value(v)
             b.mosfet(name, tok[1], tok[2], tok[3], tok[4], tok[5], params)
+        elif k == "Q":  # Qname c b e [s] model [area], netlist/parser.go:661-695
+            t = tok[1:]
+            if not 4 <= len(t) <= 6:
+                raise ValueError(f"{name}: BJT needs: Qname c b e [s] model [area]")
+            area, sub = 0.0, None
+            if len(t) == 4:
+                model = t[3]
+            elif len(t) == 5:
+                try:
+                    area = parse_value(t[4])
+                    model = t[3]
+                except ValueError:
+                    sub, model = t[3], t[4]
+            else:
+                sub, model, area = t[3], t[4], parse_value(t[5])
+            b.bjt(name, t[0], t[1], t[2], model, substrate=sub, area=area)
         else:
             raise NotImplementedError(f"element {name} is not on this path")
     return Deck(title, b.build(), analyses)

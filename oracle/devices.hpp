@@ -3,7 +3,8 @@
 // CPU restatement of kronspice's device models for the elements the solve path
 // uses: R (device/passive.go), C and L (device/reactive.go), V and I
 // (device/source.go, device/waveform.go), the SPICE3 diode (device/diode.go),
-// the level-1 MOSFET (device/mosfet.go, device/mosfet_levels.go) and the LTE
+// the level-1 MOSFET (device/mosfet.go, device/mosfet_levels.go), the
+// Gummel–Poon BJT (device/bjt.go) and the LTE
 // estimate (device/lte.go). Parameters arrive already resolved by the host
 // setup step in the layout of include/kronhip.h.
 #pragma once
@@ -96,12 +97,14 @@ inline double lteStep(const History& hist, double qn, double in, double inPrev, 
   double fact = 2.0, c = 0.5;
   if (k == 2) { fact = 6; c = 1.0 / 12; }
   double h = tc.H;
-  double errI = c * std::pow(h, (double)k) * std::fabs(deriv * fact);
+  // Go's math.Pow(x, 1) is x, Pow(x, 0.5) is Sqrt(x) and Pow(x, 2) squares
+  // the mantissa once then scales by a power of two (math/pow.go): x*x.
+  double errI = c * (k == 1 ? h : h * h) * std::fabs(deriv * fact);
   if (errI == 0 || std::isnan(errI)) return kInf;
   double qmax = std::max(std::fabs(qn), std::fabs(hist.q[0]));
   double allowed = tc.trTol * std::max(tc.relTol * std::max(std::fabs(in), std::fabs(inPrev)) + absTol,
                                        tc.relTol * std::max(qmax, tc.chgTol) / h);
-  return h * std::pow(allowed / errI, 1 / (double)k);
+  return h * (k == 1 ? allowed / errI : std::sqrt(allowed / errI));
 }
 
 inline double at(const double* x, int nx, int i) { return (i < 0 || i >= nx) ? 0.0 : x[i]; }  // device/diode.go:140-145
@@ -1112,6 +1115,372 @@ struct Mosfet : Device {
   }
   void ports(const OperatingPoint& op, Accumulator& accu) override { accu.addPort(2, -(op.power - op.storage)); }
   void loadAC(ACContext& ctx) override;
+};
+
+// Gummel–Poon BJT, device/bjt.go:14-765 (transport and base currents,
+// base-resistance modulation, transit-time and depletion charges, substrate
+// junction, pnjlim limiting, LoadAC).
+struct Bjt : Device {
+  // Parameters (area-scaled) and derived constants, NewBJT :142-242.
+  double typ = 1, IS = 1e-16, BF = 100, NF = 1, VAF = 0, IKF = 0, ISE = 0, NE = 1.5, BR = 1, NR = 1, VAR = 0, IKR = 0;
+  double ISC = 0, NC = 2, RB = 0, IRB = 0, RBM = 0, RE = 0, RC = 0, CJE = 0, VJE = 0.75, MJE = 0.33, TF = 0, XTF = 0;
+  double VTF = 0, ITF = 0, CJC = 0, VJC = 0.75, MJC = 0.33, XCJC = 1, TR = 0, CJS = 0, VJS = 0.75, MJS = 0, FC = 0.5;
+  double ISS = 0, NS = 1, SUBS = 1;
+  double vt = 0, vcrit = 0, rbpr = 0, rbpi = 0, gcpr = 0, gepr = 0, ovtf = 0, f1e = 0, f2e = 0, f3e = 0, f1c = 0,
+         f2c = 0, f3c = 0;
+  bool hasCharge = false;
+  int b = -1, s = -1, cp = -1, bp = -1, ep = -1, sa = -1, sb = -1;
+
+  struct DC {  // bjtDC, :118-127
+    double cbe = 0, gbe = 0, cbc = 0, gbc = 0, it = 0, itBE = 0, itBC = 0, ibe = 0, gpi = 0, ibc = 0, gmu = 0;
+    double qb = 0, dqbdve = 0, dqbdvc = 0, gx = 0;
+  };
+  struct Charge {  // bjtCharge, :130-134
+    double q[4] = {0, 0, 0, 0}, c[4] = {0, 0, 0, 0}, dqbeDvbc = 0;
+  };
+  // Linearization of the most recent Load.
+  double vbe = 0, vbc = 0, vbx = 0, vcs = 0, isub = 0, gsub = 0, ag = 0, iq[4] = {0, 0, 0, 0}, gmin = 0;
+  DC dc;
+  Charge ch;
+  bool lim = false, tran = false;
+  // Accepted state.
+  double vbeAcc = 0, vbcAcc = 0, vbxAcc = 0, vcsAcc = 0, qPrev[4] = {0, 0, 0, 0}, iqPrev[4] = {0, 0, 0, 0};
+  History hist[4];
+  double omega = 0;
+  struct { double gx = 0, itBE = 0, itBC = 0, gpi = 0, gmu = 0; cplx ybe, ybeBC, ybc, ybx, ycs; } ac;
+
+  void setup(SetupContext& c) override {  // :250-268
+    setupNodes(c);
+    b = c.unknown(nodes[2]);
+    s = c.unknown(nodes[3]);
+    cp = u[0]; bp = b; ep = u[1];
+    if (RC > 0) cp = c.allocUnknown();
+    if (RB > 0) bp = c.allocUnknown();
+    if (RE > 0) ep = c.allocUnknown();
+    sa = s; sb = cp;
+    if (SUBS < 0) { sa = bp; sb = s; }
+  }
+  // junction, :272-283: IS(e^(v/nvt) - 1) with the cubic reverse tail below -3 nvt.
+  static void junction(double is, double v, double nvt, double* i, double* g) {
+    if (is == 0) { *i = 0; *g = 0; return; }
+    if (v >= -3 * nvt) {
+      double e = std::exp(std::min(v / nvt, 700.0));
+      *i = is * (e - 1);
+      *g = is * e / nvt;
+      return;
+    }
+    double a = 3 * nvt / (v * kE);
+    a = a * a * a;
+    *i = -is * (1 + a);
+    *g = is * 3 * a / v;
+  }
+  void substrate(double v, double* i, double* g) const {  // :288-291
+    junction(ISS, v, NS * vt, i, g);
+    *i += gmin * v;
+    *g += gmin;
+  }
+  DC evalDC(double a_vbe, double a_vbc) const {  // :294-355
+    DC d;
+    junction(IS, a_vbe, NF * vt, &d.cbe, &d.gbe);
+    double cben, gben, cbcn, gbcn;
+    junction(ISE, a_vbe, NE * vt, &cben, &gben);
+    cben += gmin * a_vbe;
+    gben += gmin;
+    junction(IS, a_vbc, NR * vt, &d.cbc, &d.gbc);
+    junction(ISC, a_vbc, NC * vt, &cbcn, &gbcn);
+    cbcn += gmin * a_vbc;
+    gbcn += gmin;
+    double ivaf = VAF > 0 ? 1 / VAF : 0, ivar = VAR > 0 ? 1 / VAR : 0;
+    double oik = IKF > 0 ? 1 / IKF : 0, oikr = IKR > 0 ? 1 / IKR : 0;
+    double q1 = 1 / (1 - ivaf * a_vbc - ivar * a_vbe);
+    if (oik == 0 && oikr == 0) {
+      d.qb = q1;
+      d.dqbdve = q1 * d.qb * ivar;
+      d.dqbdvc = q1 * d.qb * ivaf;
+    } else {
+      double q2 = oik * d.cbe + oikr * d.cbc;
+      double arg = std::max(0.0, 1 + 4 * q2);
+      double sq = arg != 0 ? std::sqrt(arg) : 1.0;
+      d.qb = q1 * (1 + sq) / 2;
+      d.dqbdve = q1 * (d.qb * ivar + oik * d.gbe / sq);
+      d.dqbdvc = q1 * (d.qb * ivaf + oikr * d.gbc / sq);
+    }
+    d.it = (d.cbe - d.cbc) / d.qb;
+    d.itBE = (d.gbe - d.it * d.dqbdve) / d.qb;
+    d.itBC = (-d.gbc - d.it * d.dqbdvc) / d.qb;
+    d.ibe = d.cbe / BF + cben;
+    d.gpi = d.gbe / BF + gben;
+    d.ibc = d.cbc / BR + cbcn;
+    d.gmu = d.gbc / BR + gbcn;
+    double gx = rbpr + rbpi / d.qb;
+    if (IRB > 0 && rbpi != 0) {  // current crowding
+      double cb = d.ibe + d.ibc;
+      double a1 = std::max(cb / IRB, 1e-9);
+      double z = (-1 + std::sqrt(1 + 14.59025 * a1)) / 2.4317 / std::sqrt(a1);
+      double tz = std::tan(z);
+      gx = rbpr + 3 * rbpi * (tz - z) / z / tz / tz;
+    }
+    if (gx != 0) gx = 1 / gx;
+    d.gx = gx;
+    return d;
+  }
+  // depletionExp: the BJT's depletion charge (:358-370), arg^-m as exp(-m log arg).
+  Charge charges(double a_vbe, double a_vbc, double a_vbx, double a_vcs, const DC& d) const {  // :373-410
+    Charge c;
+    double cbe = d.cbe, gbe = d.gbe;
+    if (TF != 0 && a_vbe > 0) {
+      double argtf = 0, arg2 = 0, arg3 = 0;
+      if (XTF != 0) {
+        argtf = XTF;
+        if (ovtf != 0) argtf *= std::exp(a_vbc * ovtf);
+        arg2 = argtf;
+        if (ITF != 0) {
+          double t = cbe / (cbe + ITF);
+          argtf *= t * t;
+          arg2 = argtf * (3 - t - t);
+        }
+        arg3 = cbe * argtf * ovtf;
+      }
+      cbe = cbe * (1 + argtf) / d.qb;
+      gbe = (gbe * (1 + arg2) - cbe * d.dqbdve) / d.qb;
+      c.dqbeDvbc = TF * (arg3 - cbe * d.dqbdvc) / d.qb;
+    }
+    double qd, cd;
+    depletion(a_vbe, CJE, VJE, MJE, FC, f1e, f2e, f3e, &qd, &cd);
+    c.q[0] = TF * cbe + qd;
+    c.c[0] = TF * gbe + cd;
+    double czbc = CJC * XCJC;
+    depletion(a_vbc, czbc, VJC, MJC, FC, f1c, f2c, f3c, &qd, &cd);
+    c.q[1] = TR * d.cbc + qd;
+    c.c[1] = TR * d.gbc + cd;
+    depletion(a_vbx, CJC - czbc, VJC, MJC, FC, f1c, f2c, f3c, &c.q[2], &c.c[2]);
+    if (CJS != 0) {
+      if (a_vcs < 0) {
+        double arg = 1 - a_vcs / VJS;
+        double sarg = std::exp(-MJS * std::log(arg));
+        c.q[3] = VJS * CJS * (1 - arg * sarg) / (1 - MJS);
+        c.c[3] = CJS * sarg;
+      } else {
+        c.q[3] = a_vcs * CJS * (1 + MJS * a_vcs / (2 * VJS));
+        c.c[3] = CJS * (1 + MJS * a_vcs / VJS);
+      }
+    }
+    return c;
+  }
+  void voltages(const double* x, int nx, double v[4]) const {  // :413-418
+    double vb = at(x, nx, b), vbp = at(x, nx, bp), vcp = at(x, nx, cp), vep = at(x, nx, ep);
+    v[0] = typ * (vbp - vep);
+    v[1] = typ * (vbp - vcp);
+    v[2] = typ * (vb - vcp);
+    v[3] = typ * (at(x, nx, sa) - at(x, nx, sb));
+  }
+  // branch, :447-469: current i0 + g1 dv1 + g2 dv2 from a to b.
+  void branch(LoadContext& lc, int a, int bb, double i0, double g1, int p1, int n1, double v10, double g2, int p2, int n2,
+              double v20) {
+    Stamper* m = lc.M;
+    if (g1 != 0) {
+      m->add(a, p1, g1); m->add(a, n1, -g1); m->add(bb, p1, -g1); m->add(bb, n1, g1);
+    }
+    if (g2 != 0) {
+      m->add(a, p2, g2); m->add(a, n2, -g2); m->add(bb, p2, -g2); m->add(bb, n2, g2);
+    }
+    double ieq = typ * (i0 - g1 * v10 - g2 * v20);
+    lc.addRHS(a, -ieq);
+    lc.addRHS(bb, ieq);
+  }
+  void stamp(LoadContext& lc) {  // :492-515
+    const DC& d = dc;
+    branch(lc, cp, ep, d.it, d.itBE, bp, ep, vbe, d.itBC, bp, cp, vbc);
+    branch(lc, bp, ep, d.ibe, d.gpi, bp, ep, vbe, 0, 0, 0, 0);
+    branch(lc, bp, cp, d.ibc, d.gmu, bp, cp, vbc, 0, 0, 0, 0);
+    branch(lc, sa, sb, isub, gsub, sa, sb, vcs, 0, 0, 0, 0);
+    if (tran) {
+      branch(lc, bp, ep, iq[0], ag * ch.c[0], bp, ep, vbe, ag * ch.dqbeDvbc, bp, cp, vbc);
+      branch(lc, bp, cp, iq[1], ag * ch.c[1], bp, cp, vbc, 0, 0, 0, 0);
+      branch(lc, b, cp, iq[2], ag * ch.c[2], b, cp, vbx, 0, 0, 0, 0);
+      branch(lc, sa, sb, iq[3], ag * ch.c[3], sa, sb, vcs, 0, 0, 0, 0);
+    }
+    if (bp != b) Mosfet::stampG(lc.M, b, bp, d.gx);
+    if (cp != u[0]) Mosfet::stampG(lc.M, u[0], cp, gcpr);
+    if (ep != u[1]) Mosfet::stampG(lc.M, u[1], ep, gepr);
+  }
+  void load(LoadContext& lc) override {  // :421-444
+    gmin = lc.gmin;
+    double v[4];
+    voltages(lc.X, lc.nx, v);
+    lim = false;
+    if (lc.initJct) {
+      v[0] = vcrit; v[1] = 0; v[2] = 0; v[3] = 0;
+      lim = true;
+    } else {
+      bool l1, l2;
+      v[0] = pnjlim(v[0], vbeAcc, vt, vcrit, &l1);
+      v[1] = pnjlim(v[1], vbcAcc, vt, vcrit, &l2);
+      lim = l1 || l2;
+    }
+    vbeAcc = v[0]; vbcAcc = v[1];
+    vbe = v[0]; vbc = v[1]; vbx = v[2]; vcs = v[3];
+    dc = evalDC(vbe, vbc);
+    substrate(vcs, &isub, &gsub);
+    tran = false;
+    ag = 0;
+    for (double& q : iq) q = 0;
+    if (lc.mode == ModeTran && hasCharge) {
+      ch = charges(vbe, vbc, vbx, vcs, dc);
+      tran = true;
+      if (lc.method == Trapezoidal) {
+        ag = 2 / lc.H;
+        for (int k = 0; k < 4; k++) iq[k] = ag * (ch.q[k] - qPrev[k]) - iqPrev[k];
+      } else {
+        ag = 1 / lc.H;
+        for (int k = 0; k < 4; k++) iq[k] = ag * (ch.q[k] - qPrev[k]);
+      }
+    }
+    stamp(lc);
+  }
+  bool nonlinear() const override { return true; }
+  bool limited() const override { return lim; }
+  bool convTest(const double* x, int nx, double reltol, double abstol) override {  // :521-532
+    double v[4];
+    voltages(x, nx, v);
+    const DC& d = dc;
+    double dbe = v[0] - vbe, dbc = v[1] - vbc;
+    double ichat = d.it - d.ibc + d.itBE * dbe + (d.itBC - d.gmu) * dbc;
+    double ibhat = d.ibe + d.ibc + d.gpi * dbe + d.gmu * dbc;
+    DC nw = evalDC(v[0], v[1]);
+    double ic = nw.it - nw.ibc, ib = nw.ibe + nw.ibc;
+    return std::fabs(ichat - ic) <= reltol * std::max(std::fabs(ichat), std::fabs(ic)) + abstol &&
+           std::fabs(ibhat - ib) <= reltol * std::max(std::fabs(ibhat), std::fabs(ib)) + abstol;
+  }
+  // terminalCurrents, :537-570: currents into c, b, e, s from the linearization.
+  void terminalCurrents(const double* x, int nx, double* ic, double* ib, double* ie, double* is, double* ps) const {
+    double v[4];
+    voltages(x, nx, v);
+    double t = typ;
+    const DC& d = dc;
+    double dbe = v[0] - vbe, dbc = v[1] - vbc;
+    double it = t * (d.it + d.itBE * dbe + d.itBC * dbc);
+    double ibe = t * (d.ibe + d.gpi * dbe);
+    double ibc = t * (d.ibc + d.gmu * dbc);
+    double iqbe = 0, iqbc = 0, iqbx = 0, iqcs = 0;
+    *ps = 0;
+    if (tran) {
+      iqbe = t * (iq[0] + ag * ch.c[0] * dbe + ag * ch.dqbeDvbc * dbc);
+      iqbc = t * (iq[1] + ag * ch.c[1] * dbc);
+      iqbx = t * (iq[2] + ag * ch.c[2] * (v[2] - vbx));
+      iqcs = t * (iq[3] + ag * ch.c[3] * (v[3] - vcs));
+      *ps = v[0] * t * iqbe + v[1] * t * iqbc + v[2] * t * iqbx + v[3] * t * iqcs;
+    }
+    double isb = t * (isub + gsub * (v[3] - vcs)) + iqcs;
+    *ic = it - ibc - iqbc - iqbx;
+    *ib = ibe + ibc + iqbe + iqbc + iqbx;
+    *ie = -it - ibe - iqbe;
+    if (SUBS > 0) {
+      *is = isb;
+      *ic = *ic - isb;
+    } else {
+      *ib = *ib + isb;
+      *is = -isb;
+    }
+  }
+  OperatingPoint operating(const double* x, int nx) override {  // :574-579
+    double ic, ib, ie, is, ps;
+    terminalCurrents(x, nx, &ic, &ib, &ie, &is, &ps);
+    double vc = at(x, nx, u[0]), ve = at(x, nx, u[1]);
+    OperatingPoint op;
+    op.power = vc * ic + at(x, nx, b) * ib + ve * ie + at(x, nx, s) * is;
+    op.voltage = vc - ve;
+    op.current = ic;
+    op.storage = ps;
+    return op;
+  }
+  void accept(const double* x, int nx, double t, int mode) override {  // :582-605
+    double v[4];
+    voltages(x, nx, v);
+    vbeAcc = v[0]; vbcAcc = v[1]; vbxAcc = v[2]; vcsAcc = v[3];
+    DC d = evalDC(v[0], v[1]);
+    Charge c = charges(v[0], v[1], v[2], v[3], d);
+    double dv[4] = {v[0] - vbe, v[1] - vbc, v[2] - vbx, v[3] - vcs};
+    for (int k = 0; k < 4; k++) {
+      if (mode == ModeTran && tran) {
+        double i = iq[k] + ag * ch.c[k] * dv[k];
+        if (k == 0) i += ag * ch.dqbeDvbc * (v[1] - vbc);
+        iqPrev[k] = i;
+      } else {
+        iqPrev[k] = 0;
+      }
+      if (mode != ModeTran) hist[k].reset();
+      qPrev[k] = c.q[k];
+      hist[k].push(t, c.q[k]);
+    }
+  }
+  bool reactive() const override { return true; }
+  double truncate(const double* x, int nx, const TruncContext& tc) override {  // :655-675
+    if (!hasCharge || !tran) return kInf;
+    double v[4];
+    voltages(x, nx, v);
+    DC d = evalDC(v[0], v[1]);
+    Charge c = charges(v[0], v[1], v[2], v[3], d);
+    double dv[4] = {v[0] - vbe, v[1] - vbc, v[2] - vbx, v[3] - vcs};
+    double h = kInf;
+    for (int k = 0; k < 4; k++) {
+      if (c.q[k] == 0 && qPrev[k] == 0) continue;
+      double i = iq[k] + ag * ch.c[k] * dv[k];
+      if (k == 0) i += ag * ch.dqbeDvbc * dv[1];
+      h = std::min(h, lteStep(hist[k], c.q[k], i, iqPrev[k], tc.absTol, tc));
+    }
+    return h;
+  }
+  void ports(const OperatingPoint& op, Accumulator& accu) override { accu.addPort(2, -(op.power - op.storage)); }
+  // acBranch, :718-731
+  static void acBranch(CStamper* m, int a, int bb, cplx y1, int p1, int n1, cplx y2, int p2, int n2) {
+    if (y1 != cplx(0)) {
+      m->add(a, p1, y1); m->add(a, n1, -y1); m->add(bb, p1, -y1); m->add(bb, n1, y1);
+    }
+    if (y2 != cplx(0)) {
+      m->add(a, p2, y2); m->add(a, n2, -y2); m->add(bb, p2, -y2); m->add(bb, n2, y2);
+    }
+  }
+  void loadAC(ACContext& ctx) override {  // :690-716
+    DC d = evalDC(vbeAcc, vbcAcc);
+    Charge c = charges(vbeAcc, vbcAcc, vbxAcc, vcsAcc, d);
+    double w = ctx.omega;
+    omega = w;
+    ac.itBE = d.itBE; ac.itBC = d.itBC; ac.gpi = d.gpi; ac.gmu = d.gmu; ac.gx = d.gx;
+    ac.ybe = cplx(0, w * c.c[0]);
+    ac.ybeBC = cplx(0, w * c.dqbeDvbc);
+    double is_, gs;
+    substrate(vcsAcc, &is_, &gs);
+    ac.ybc = cplx(0, w * c.c[1]);
+    ac.ybx = cplx(0, w * c.c[2]);
+    ac.ycs = cplx(gs, w * c.c[3]);
+    CStamper* m = ctx.M;
+    acBranch(m, cp, ep, cplx(ac.itBE, 0), bp, ep, cplx(ac.itBC, 0), bp, cp);
+    acBranch(m, bp, ep, cplx(ac.gpi, 0) + ac.ybe, bp, ep, ac.ybeBC, bp, cp);
+    acBranch(m, bp, cp, cplx(ac.gmu, 0) + ac.ybc, bp, cp, 0, 0, 0);
+    acBranch(m, b, cp, ac.ybx, b, cp, 0, 0, 0);
+    acBranch(m, sa, sb, ac.ycs, sa, sb, 0, 0, 0);
+    if (bp != b) stampY(m, b, bp, cplx(ac.gx, 0));
+    if (cp != u[0]) stampY(m, u[0], cp, cplx(gcpr, 0));
+    if (ep != u[1]) stampY(m, u[1], ep, cplx(gepr, 0));
+  }
+  // ACPower, :759-765 (acCurrents :734-751): half the sum of V_k conj(I_k).
+  cplx acPower(const cplx* x, int nx) const {
+    auto cat = [&](int i) { return (i < 0 || i >= nx) ? cplx(0) : x[i]; };
+    cplx vbe_ = cat(bp) - cat(ep), vbc_ = cat(bp) - cat(cp), vbx_ = cat(b) - cat(cp), vcs_ = cat(sa) - cat(sb);
+    cplx it = cplx(ac.itBE, 0) * vbe_ + cplx(ac.itBC, 0) * vbc_;
+    cplx ibe = (cplx(ac.gpi, 0) + ac.ybe) * vbe_ + ac.ybeBC * vbc_;
+    cplx ibc = (cplx(ac.gmu, 0) + ac.ybc) * vbc_;
+    cplx ix = ac.ybx * vbx_, isb = ac.ycs * vcs_;
+    cplx ic, ib, ie = -it - ibe, is;
+    if (SUBS > 0) {
+      ic = it - ibc - ix - isb; ib = ibe + ibc + ix; is = isb;
+    } else {
+      ic = it - ibc - ix; ib = ibe + ibc + ix + isb; is = -isb;
+    }
+    cplx sum = cat(u[0]) * std::conj(ic) + cat(b) * std::conj(ib) + cat(u[1]) * std::conj(ie) + cat(s) * std::conj(is);
+    return cplx(sum.real() / 2, sum.imag() / 2);
+  }
 };
 
 }  // namespace oracle

@@ -155,6 +155,20 @@ struct Sim {
           d.reset(q);
           break;
         }
+        case KH_ELEM_Q: {
+          auto q = new Bjt();
+          double* f[37] = {&q->IS, &q->BF, &q->NF, &q->VAF, &q->IKF, &q->ISE, &q->NE, &q->BR, &q->NR, &q->VAR,
+                           &q->IKR, &q->ISC, &q->NC, &q->RB, &q->IRB, &q->RBM, &q->RE, &q->RC, &q->CJE, &q->VJE,
+                           &q->MJE, &q->TF, &q->XTF, &q->VTF, &q->ITF, &q->CJC, &q->VJC, &q->MJC, &q->XCJC, &q->TR,
+                           &q->CJS, &q->VJS, &q->MJS, &q->FC, &q->ISS, &q->NS, &q->SUBS};
+          q->typ = p[0];
+          for (int k = 0; k < 37; k++) *f[k] = p[1 + k];
+          q->vt = p[38]; q->vcrit = p[39]; q->rbpr = p[40]; q->rbpi = p[41]; q->gcpr = p[42]; q->gepr = p[43];
+          q->ovtf = p[44]; q->f1e = p[45]; q->f2e = p[46]; q->f3e = p[47]; q->f1c = p[48]; q->f2c = p[49];
+          q->f3c = p[50]; q->hasCharge = p[51] != 0;
+          d.reset(q);
+          break;
+        }
         default:
           return E_UNSUPPORTED;
       }
@@ -705,6 +719,12 @@ struct Sim {
           cplx s = cat(q->u[0]) * std::conj(id) + cat(q->g) * std::conj(ig) + cat(q->u[1]) * std::conj(is) +
                    cat(q->b) * std::conj(ib);
           s /= 2.0;
+          sumS += s;
+          scaleS += absval(s);
+          continue;
+        }
+        if (d->kind == KH_ELEM_Q) {
+          cplx s = static_cast<Bjt*>(d.get())->acPower(cx.data(), n);
           sumS += s;
           scaleS += absval(s);
           continue;

@@ -203,7 +203,7 @@ def test_op_goldens_bitexact_with_oracle(name):
         assert (st == e0).all(), (v, st[0], e0)
         assert (it == info["iters"]).all(), (v, it[0], info["iters"])
         rel = relerr(x, np.repeat(x0[:, None], B, axis=1))
-        linear = all(e.kind not in (6, 7) for e in c.elements)
+        linear = all(e.kind not in (6, 7, 8) for e in c.elements)
         if linear:
             assert (x == x0[:, None]).all(), v
         else:
@@ -257,7 +257,7 @@ def test_rectifier_batch_tran_matches_oracle():
     assert host(r.ledger).max() <= 1e-9
 
 
-@pytest.mark.parametrize("name", ["diode_fwd", "mos1_op", "mos1_cmos_vtc", "mos1_cs_ac"])
+@pytest.mark.parametrize("name", ["diode_fwd", "mos1_op", "mos1_cmos_vtc", "mos1_cs_ac", "bjt_ce_op"])
 def test_nodeset_warm_start_matches_oracle(name):
     """Options.NodeSet (analysis/newton.go:221-243): Newton starts from a
     per-circuit guess (node voltages only; branch currents and internal nodes
@@ -308,7 +308,8 @@ def test_nodeset_latch_states_match_oracle():
     assert (x[q] > 4.9).any() and (x[qb] > 4.9).any()  # both stable states are reached
 
 
-@pytest.mark.parametrize("name", ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc"])
+@pytest.mark.parametrize("name", ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc", "bjt_output_dc",
+                                  "bjt_pnp_dc"])
 def test_dc_sweep_matches_oracle(name):
     """analysis.DC (analysis/dc.go:51-130) through kh_batch_dc: every point
     warm-started from the previous one on the same circuit state; the same
@@ -336,7 +337,11 @@ def test_dc_sweep_matches_oracle(name):
     assert host(r.ledger).max() <= 1e-9
 
 
-AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac"]
+AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac"]
+# Decks whose operating point involves no transcendental function of a value
+# that differs between the device library and glibc: bit for bit. The BJT's
+# exponentials differ in the last bit: within 1e-9.
+AC_BITEXACT = {"rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac"}
 
 
 @pytest.mark.parametrize("name", AC_DECKS)
@@ -344,7 +349,8 @@ def test_ac_goldens_match_oracle(name):
     """analysis.AC (analysis/ac.go:76-193) through kh_batch_ac: operating point,
     polish, then per frequency the LoadAC stamps, the complex GP solve (CSparse:
     cmplx.Abs pivoting, Go's complex division) and the complex-Tellegen ledger.
-    Every frequency's solution matches the oracle's bit for bit."""
+    Every frequency's solution matches the oracle's bit for bit (the BJT deck:
+    within 1e-9, its exponentials being the device library's)."""
     from kronspice_amd import netlist
     deck = gr.load_deck(name)
     c = deck.circuit
@@ -359,7 +365,10 @@ def test_ac_goldens_match_oracle(name):
     x, st, pts = host(r.x), host(r.status), host(r.points)
     assert (st == 0).all() and (pts == len(freqs)).all()
     for b in range(B):
-        assert np.array_equal(x[:, :, b], xo), b
+        if name in AC_BITEXACT:
+            assert np.array_equal(x[:, :, b], xo), b
+        else:
+            assert np.max(np.abs(x[:, :, b] - xo)) <= 1e-9 * np.max(np.abs(xo)), b
     assert host(r.ledger).max() <= 1e-9
 
 

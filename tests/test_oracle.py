@@ -289,7 +289,7 @@ def test_nodeset_selects_latch_state():
     assert ia["ledger_rel"] <= 1e-12 and ib["ledger_rel"] <= 1e-12
 
 
-DC_DECKS = ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc"]
+DC_DECKS = ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc", "bjt_output_dc", "bjt_pnp_dc"]
 
 
 @pytest.mark.parametrize("name", DC_DECKS)
@@ -359,3 +359,57 @@ def test_ac_rc_lowpass_closed_form():
         assert e == 0 and mr <= 1e-9
         H = 1.0 / (1.0 + 1j * 2 * np.pi * freqs * R[b] * Cv[b])
         assert np.max(np.abs(xo[:, out] - H)) <= 1e-12
+
+
+class _MutatedBackend(OracleBackend):
+    """The oracle with one BJT parameter altered after setup: a planted
+    mistake (a dropped term or a swapped constant) that the reference's
+    goldens must expose."""
+
+    def __init__(self, index, value):
+        self.index, self.value = index, value
+
+    def _mut(self, c, step=0.0, stop=0.0):
+        els, p = c.flatten(step, stop)
+        p = p.copy()
+        for e in els:
+            if e["kind"] == kc.Q:
+                p[e["poff"] + self.index] = self.value(p[e["poff"] + self.index], p[e["poff"]:e["poff"] + kc.NPAR[kc.Q]])
+        return els, p
+
+    def op(self, c):
+        els, p = self._mut(c)
+        st, x, info = oracle.op(c.num_nodes, els, p)
+        return st, x, info.get("current")
+
+    def tran(self, c, a):
+        els, p = self._mut(c, a["step"], a["stop"])
+        spec = oracle.KhTranSpec(step=a["step"], stop=a["stop"], start=a["start"], max_step=a["max_step"],
+                                 uic=1 if a["uic"] else 0)
+        st, x, info = oracle.tran(c.num_nodes, els, p, spec, cap=200000)
+        return st, info["times"], info["xs"], info["current"], info["max_rel"]
+
+    def ac(self, c, freqs):
+        els, p = self._mut(c)
+        st, X, mr = oracle.ac(c.num_nodes, els, p, freqs)
+        return st, X
+
+
+@pytest.mark.parametrize("deck,index,value,what", [
+    ("bjt_ce_op", 4, lambda v, p: 0.0, "Early effect (VAF) dropped"),
+    ("bjt_ce_op", 5, lambda v, p: 0.0, "high injection (IKF) dropped"),
+    ("bjt_ce_op", 3, lambda v, p: 1.0, "emission coefficient NF ignored"),
+    ("bjt_ce_op", 6, lambda v, p: 0.0, "B-E leakage (ISE) dropped"),
+    ("bjt_output_dc", 8, lambda v, p: p[2], "BR set to BF"),
+    ("bjt_pnp_dc", 0, lambda v, p: -v, "polarity flipped"),
+    ("bjt_ce_ac", 29, lambda v, p: 1.0, "XCJC split ignored"),
+    ("bjt_ce_ac", 22, lambda v, p: 0.0, "transit time (TF) dropped"),
+    ("bjt_switch_tran", 30, lambda v, p: 0.0, "reverse transit time (TR) dropped"),
+])
+def test_bjt_goldens_expose_planted_mistakes(deck, index, value, what):
+    assert not gr.check(deck, OracleBackend()), "the unaltered oracle must pass"
+    try:
+        msgs = gr.check(deck, _MutatedBackend(index, value))
+    except Exception as ex:  # a mutation may also break the analysis outright
+        msgs = [repr(ex)]
+    assert msgs, f"{deck}: golden does not notice: {what}"

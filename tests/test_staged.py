@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("cpw", [1, 4])
-@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "zener_dc", "mos1_cmos_vtc"])
+@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "zener_dc", "mos1_cmos_vtc", "bjt_ce_op"])
 def test_op_goldens_staged(name, cpw):
     """Staged runs give the in-place runs' results bit for bit (the same
     device code on copies of the same arrays), hence the oracle's."""
@@ -29,7 +29,7 @@ def test_op_goldens_staged(name, cpw):
 
 
 @pytest.mark.parametrize("cpw", [1, 4])
-@pytest.mark.parametrize("name", ["rc_step", "diode_charge_tran", "mos1_inv_tran", "rectifier_rc"])
+@pytest.mark.parametrize("name", ["rc_step", "diode_charge_tran", "mos1_inv_tran", "rectifier_rc", "bjt_switch_tran"])
 def test_tran_goldens_staged(name, cpw):
     deck = gr.load_deck(name)
     c = deck.circuit
