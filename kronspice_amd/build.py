@@ -19,14 +19,15 @@ FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 # (source, extra flags, object): the persistent kernel once per register
-# budget (resident wavefronts per SIMD), each in its own translation unit.
+# budget (resident wavefronts per SIMD) and element-kind set, each in its own
+# translation unit.
+PERSIST_WAVES = (1, 2, 4)
+PERSIST_KINDS = (126, 190, 318, 1022)  # diodes, level-1 MOSFETs, BJTs, everything (kh_persist.h)
 UNITS = [
     ("kronhip.hip", [], "kronhip.o"),
     ("kh_symbolic.cpp", [], "kh_symbolic.o"),
-    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=1"], "kh_persist_w1.o"),
-    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=2"], "kh_persist_w2.o"),
-    ("kh_persist.hip", ["-DKH_PERSIST_WAVES=4"], "kh_persist_w4.o"),
-]
+] + [("kh_persist.hip", [f"-DKH_PERSIST_WAVES={w}", f"-DKH_KINDS={k}"], f"kh_persist_w{w}_k{k}.o")
+     for w in PERSIST_WAVES for k in PERSIST_KINDS]
 BENCH = os.path.join(HERE, "kh_bench")
 OBJ = os.path.join(HERE, "_obj")
 
@@ -45,7 +46,7 @@ def build(verbose: bool = False) -> str:
     for src, extra, obj in UNITS:
         cmd = [HIPCC] + FLAGS + extra + ["-c", os.path.join(CSRC, src), "-o", os.path.join(OBJ, obj)]
         jobs.append((cmd, src + " " + " ".join(extra)))
-    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+    with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 8)) as ex:
         results = list(ex.map(lambda j: _run(*j), jobs))
     if verbose:
         for r in results:

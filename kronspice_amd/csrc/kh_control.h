@@ -95,6 +95,10 @@ struct CtlArgs {
 
 KH_HD inline bool retryable(int e) { return e == ST_NOCONV || e == ST_NONFINITE || e == ST_SINGULAR; }
 
+// The state machine's device dispatch depends on the element kinds compiled
+// in (KH_KINDS), so each kind set gets its own namespace.
+inline namespace KH_CAT(kinds_, KH_KINDS) {
+
 // One circuit's state machine over its own (unit-stride) arrays; the control
 // words are worked on in place, so they live wherever the circuit's arrays do
 // (device memory, or LDS when the launch stages the circuit).
@@ -146,9 +150,9 @@ struct Ctl {
         case K_L: load_L(L, e); break;
         case K_V: load_V(L, e); break;
         case K_I: load_I(L, e); break;
-        case K_D: load_D(L, e); break;
-        case K_M: load_M(L, e); break;
-        case K_Q: load_Q(L, e); break;
+        case K_D: if (KH_HAS(K_D)) load_D(L, e); break;
+        case K_M: if (KH_HAS(K_M)) load_M(L, e); break;
+        case K_Q: if (KH_HAS(K_Q)) load_Q(L, e); break;
       }
     }
   }
@@ -166,9 +170,9 @@ struct Ctl {
       case K_L: return op_L(L, e, x);
       case K_V: return op_V(L, e, x);
       case K_I: return op_I(L, e, x);
-      case K_D: return op_D(L, e, x);
-      case K_M: return op_M(L, e, x);
-      case K_Q: return op_Q(L, e, x);
+      case K_D: if (KH_HAS(K_D)) return op_D(L, e, x); break;
+      case K_M: if (KH_HAS(K_M)) return op_M(L, e, x); break;
+      case K_Q: if (KH_HAS(K_Q)) return op_Q(L, e, x); break;
     }
     OpPt o = {0, 0, 0, 0};
     return o;
@@ -179,18 +183,18 @@ struct Ctl {
       switch (e[ET_KIND]) {
         case K_C: accept_C(L, e, A.x, t, mode); break;
         case K_L: accept_L(L, e, A.x, t, mode); break;
-        case K_D: accept_D(L, e, A.x, t, mode); break;
-        case K_M: accept_M(L, e, A.x, t, mode); break;
-        case K_Q: accept_Q(L, e, A.x, t, mode); break;
+        case K_D: if (KH_HAS(K_D)) accept_D(L, e, A.x, t, mode); break;
+        case K_M: if (KH_HAS(K_M)) accept_M(L, e, A.x, t, mode); break;
+        case K_Q: if (KH_HAS(K_Q)) accept_Q(L, e, A.x, t, mode); break;
       }
     }
   }
   KH_HD bool converged() {  // analysis/newton.go:59-82
     for (int k = 0; k < A.ckt.nel; k++) {
       const int32_t* e = el(k);
-      if (e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
-      if (e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
-      if (e[ET_KIND] == K_Q && L.S(e, QS_LIM) != 0) return false;
+      if (KH_HAS(K_D) && e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
+      if (KH_HAS(K_M) && e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
+      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && L.S(e, QS_LIM) != 0) return false;
     }
     for (int i = 0; i < n; i++) {
       double xv = X(A.x, i), old = X(A.xold, i);
@@ -200,9 +204,9 @@ struct Ctl {
     }
     for (int k = 0; k < A.ckt.nel; k++) {
       const int32_t* e = el(k);
-      if (e[ET_KIND] == K_D && !conv_D(L, e, A.x, P.relTol, P.absTol)) return false;
-      if (e[ET_KIND] == K_M && !conv_M(L, e, A.x, P.relTol, P.absTol)) return false;
-      if (e[ET_KIND] == K_Q && !conv_Q(L, e, A.x, P.relTol, P.absTol)) return false;
+      if (KH_HAS(K_D) && e[ET_KIND] == K_D && !conv_D(L, e, A.x, P.relTol, P.absTol)) return false;
+      if (KH_HAS(K_M) && e[ET_KIND] == K_M && !conv_M(L, e, A.x, P.relTol, P.absTol)) return false;
+      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && !conv_Q(L, e, A.x, P.relTol, P.absTol)) return false;
     }
     return true;
   }
@@ -229,9 +233,9 @@ struct Ctl {
       switch (e[ET_KIND]) {
         case K_C: pk = trunc_C(L, e, A.x, tc); break;
         case K_L: pk = trunc_L(L, e, A.x, tc); break;
-        case K_D: pk = trunc_D(L, e, A.x, tc); break;
-        case K_M: pk = trunc_M(L, e, A.x, tc); break;
-        case K_Q: pk = trunc_Q(L, e, A.x, tc); break;
+        case K_D: if (KH_HAS(K_D)) pk = trunc_D(L, e, A.x, tc); break;
+        case K_M: if (KH_HAS(K_M)) pk = trunc_M(L, e, A.x, tc); break;
+        case K_Q: if (KH_HAS(K_Q)) pk = trunc_Q(L, e, A.x, tc); break;
         default: continue;
       }
       if (pk < prop) prop = pk;
@@ -434,9 +438,9 @@ struct Ctl {
     for (int k = 0; k < A.ckt.nel; k++) {
       const int32_t* e = el(k);
       switch (e[ET_KIND]) {
-        case K_D: accept_D(L, e, A.x, 0, MODE_OP); break;
-        case K_M: accept_M(L, e, A.x, 0, MODE_OP); break;
-        case K_Q: accept_Q(L, e, A.x, 0, MODE_OP); break;
+        case K_D: if (KH_HAS(K_D)) accept_D(L, e, A.x, 0, MODE_OP); break;
+        case K_M: if (KH_HAS(K_M)) accept_M(L, e, A.x, 0, MODE_OP); break;
+        case K_Q: if (KH_HAS(K_Q)) accept_Q(L, e, A.x, 0, MODE_OP); break;
       }
     }
   }
@@ -817,4 +821,5 @@ struct Ctl {
   }
 };
 
+}  // inline namespace kinds_<KH_KINDS>
 }  // namespace kh

@@ -29,6 +29,19 @@ enum {
 };
 
 enum { K_R = 1, K_C = 2, K_L = 3, K_V = 4, K_I = 5, K_D = 6, K_M = 7, K_Q = 8 };
+// K_M23: not an element kind, the bit of the MOSFET level 2 and 3 currents.
+enum { K_M23 = 9 };
+
+// Element kinds a translation unit's circuit code handles (bit k: kind k).
+// The persistent kernel is compiled once per set, so a batch runs code for
+// the devices it has and nothing else (smaller code, fewer registers).
+#define KH_KINDS_ALL 1022
+#ifndef KH_KINDS
+#define KH_KINDS KH_KINDS_ALL
+#endif
+#define KH_HAS(k) ((((unsigned)(KH_KINDS)) >> (k)) & 1u)
+#define KH_CAT2(a, b) a##b
+#define KH_CAT(a, b) KH_CAT2(a, b)
 enum { MODE_OP = 0, MODE_TRANOP = 1, MODE_UICINIT = 2, MODE_TRAN = 3 };
 enum { BE = 1, TRAP = 2 };
 

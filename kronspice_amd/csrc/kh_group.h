@@ -150,6 +150,8 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   l.rch += c * l.capR; l.rp += c * (n + 1); l.dk += c * n; l.cpid += c;
 }
 
+inline namespace KH_CAT(kinds_, KH_KINDS) {
+
 // One circuit's persistent loop: factor + solve the pending system, advance
 // the analysis to its next solve (stamping it), until the analysis ends, the
 // stamps need a new ordering or the factors more capacity, or max_solves
@@ -192,4 +194,5 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
   *waiting = ctl.ci[CI_WAIT] != 0;
 }
 
+}  // inline namespace kinds_<KH_KINDS>
 }  // namespace kh

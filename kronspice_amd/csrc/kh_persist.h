@@ -16,6 +16,8 @@
 // spills for more circuits in flight, whose latencies then overlap. Each
 // budget is its own translation unit (kh_persist.hip with KH_PERSIST_WAVES),
 // so the device functions the kernel calls are allocated under that budget.
+// It is also compiled once per set of element kinds (KH_KINDS, kh_devices.h):
+// a batch of diode circuits runs a kernel without the transistor models.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -25,9 +27,21 @@ namespace kh {
 
 constexpr int kBlock = 64;  // one wavefront per workgroup: lanes run long, divergent programs
 
+// Kind sets k_persist is compiled for (R, C, L, V and I are in every one):
+// diodes, level-1 MOSFETs, BJTs, and everything.
+constexpr unsigned kKindsD = 126, kKindsM = 190, kKindsQ = 318, kKindsAll = KH_KINDS_ALL;
+
+// The smallest compiled kind set holding `kinds`.
+inline unsigned persist_kinds(unsigned kinds) {
+  for (unsigned v : {kKindsD, kKindsM, kKindsQ})
+    if ((kinds & ~v) == 0) return v;
+  return kKindsAll;
+}
+
 // Launches k_persist compiled for `waves` (1, 2 or 4) resident wavefronts per
-// SIMD on the null stream; returns the launch error.
-hipError_t kh_launch_persist(int waves, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup,
-                             int32_t max_solves, int32_t* counters);
+// SIMD and the kind set persist_kinds(kinds) on the null stream; returns the
+// launch error.
+hipError_t kh_launch_persist(int waves, unsigned kinds, unsigned grid, size_t lds_bytes, const Group* G,
+                             const int32_t* bgroup, int32_t max_solves, int32_t* counters);
 
 }  // namespace kh

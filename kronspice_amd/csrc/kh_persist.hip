@@ -1,5 +1,6 @@
 // kh_persist.hip — k_persist compiled for one register budget,
-// KH_PERSIST_WAVES resident wavefronts per SIMD (see kh_persist.h).
+// KH_PERSIST_WAVES resident wavefronts per SIMD, and one set of element
+// kinds, KH_KINDS (see kh_persist.h).
 #include <hip/hip_runtime.h>
 
 #define KH_CLOCK() ((uint64_t)clock64())
@@ -8,8 +9,6 @@
 #ifndef KH_PERSIST_WAVES
 #define KH_PERSIST_WAVES 1
 #endif
-#define KH_CAT2(a, b) a##b
-#define KH_CAT(a, b) KH_CAT2(a, b)
 
 namespace kh {
 namespace {
@@ -67,23 +66,10 @@ __global__ void __launch_bounds__(kBlock, KH_PERSIST_WAVES) k_persist(const Grou
 
 }  // namespace
 
-hipError_t KH_CAT(kh_launch_persist_w, KH_PERSIST_WAVES)(unsigned grid, size_t lds_bytes, const Group* G,
-                                                          const int32_t* bgroup, int32_t max_solves,
-                                                          int32_t* counters) {
+hipError_t KH_CAT(KH_CAT(kh_launch_persist_w, KH_PERSIST_WAVES), KH_CAT(_k, KH_KINDS))(
+    unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup, int32_t max_solves, int32_t* counters) {
   hipLaunchKernelGGL(k_persist, dim3(grid), dim3(kBlock), lds_bytes, 0, G, bgroup, max_solves, counters);
   return hipGetLastError();
 }
-
-#if KH_PERSIST_WAVES == 1
-hipError_t kh_launch_persist_w2(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
-hipError_t kh_launch_persist_w4(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
-
-hipError_t kh_launch_persist(int waves, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup,
-                             int32_t max_solves, int32_t* counters) {
-  if (waves == 4) return kh_launch_persist_w4(grid, lds_bytes, G, bgroup, max_solves, counters);
-  if (waves == 2) return kh_launch_persist_w2(grid, lds_bytes, G, bgroup, max_solves, counters);
-  return kh_launch_persist_w1(grid, lds_bytes, G, bgroup, max_solves, counters);
-}
-#endif
 
 }  // namespace kh

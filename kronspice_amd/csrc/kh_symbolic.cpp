@@ -211,6 +211,7 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
   auto alloc = [&]() { return numNodes - 1 + branches++; };
   int soff = 0;
   nonlin = false;
+  kinds = 0;
   for (int e = 0; e < nel; e++) {
     const kh_element& E = elems[e];
     int32_t* t = &et[(size_t)e * ET_W];
@@ -280,6 +281,7 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     }
     t[ET_SOFF] = soff;
     soff += ns;
+    kinds |= 1u << E.kind;
   }
   nstate = soff;
   n = numNodes - 1 + branches;

@@ -47,6 +47,7 @@ struct Topology {
   std::vector<int32_t> slot_i, slot_j, diag_slot, isV;
   int32_t nslots = 0;
   bool nonlin = false;
+  uint32_t kinds = 0;  // bit k set when an element of kind k is present (kh_devices.h)
   std::string error;
   bool build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0);
 };

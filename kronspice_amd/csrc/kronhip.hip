@@ -606,6 +606,34 @@ void kh_topology_destroy(kh_topology t) { delete t; }
 }  // extern "C"
 
 // ---- the batch driver -----------------------------------------------------------------------
+// The compiled k_persist variants (kh_persist.hip, one translation unit per
+// register budget and kind set).
+namespace kh {
+#define KH_PERSIST_VARIANT(W, K)                                                                          \
+  hipError_t kh_launch_persist_w##W##_k##K(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
+#define KH_PERSIST_KINDS(W) KH_PERSIST_VARIANT(W, 126) KH_PERSIST_VARIANT(W, 190) KH_PERSIST_VARIANT(W, 318) \
+  KH_PERSIST_VARIANT(W, 1022)
+KH_PERSIST_KINDS(1)
+KH_PERSIST_KINDS(2)
+KH_PERSIST_KINDS(4)
+#undef KH_PERSIST_KINDS
+#undef KH_PERSIST_VARIANT
+
+hipError_t kh_launch_persist(int waves, unsigned kinds, unsigned grid, size_t lds_bytes, const Group* G,
+                             const int32_t* bgroup, int32_t max_solves, int32_t* counters) {
+  using L = hipError_t (*)(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
+  static const L table[3][4] = {
+      {kh_launch_persist_w1_k126, kh_launch_persist_w1_k190, kh_launch_persist_w1_k318, kh_launch_persist_w1_k1022},
+      {kh_launch_persist_w2_k126, kh_launch_persist_w2_k190, kh_launch_persist_w2_k318, kh_launch_persist_w2_k1022},
+      {kh_launch_persist_w4_k126, kh_launch_persist_w4_k190, kh_launch_persist_w4_k318, kh_launch_persist_w4_k1022}};
+  const unsigned v = persist_kinds(kinds);
+  const int kv = v == kKindsD ? 0 : v == kKindsM ? 1 : v == kKindsQ ? 2 : 3;
+  const int wv = waves == 4 ? 2 : waves == 2 ? 1 : 0;
+  return table[wv][kv](grid, lds_bytes, G, bgroup, max_solves, counters);
+}
+}  // namespace kh
+
+
 namespace {
 
 // Device state of the circuits of one topology, stored per circuit.
@@ -835,6 +863,8 @@ struct Runner {
 
   kh_status run(const CtlParams& p) {
     size_t ng = groups.size();
+    unsigned kinds = 0;  // element kinds of the whole batch: picks the k_persist variant
+    for (auto* gr : groups) kinds |= gr->topo->t.kinds;
     std::vector<int32_t> bg;
     std::vector<Group> hg(ng);
     for (size_t g = 0; g < ng; g++) {
@@ -882,8 +912,8 @@ struct Runner {
       }
       KH_HIP(hipMemsetAsync(counters.p, 0, ng * 8, 0));
       KH_HIP(hipEventRecord(ev[0], 0));
-      KH_HIP(kh_launch_persist(waves, (unsigned)bg.size(), lds_launch, dgroups.as<Group>(), dbgroup.as<int32_t>(),
-                               max_solves, counters.as<int32_t>()));
+      KH_HIP(kh_launch_persist(waves, kinds, (unsigned)bg.size(), lds_launch, dgroups.as<Group>(),
+                               dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>()));
       KH_HIP(hipEventRecord(ev[1], 0));
       KH_HIP(hipMemcpyAsync(hcount, counters.p, ng * 8, hipMemcpyDeviceToHost, 0));
       KH_HIP(hipStreamSynchronize(0));
