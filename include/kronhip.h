@@ -128,6 +128,8 @@ typedef struct kh_options {
   int32_t waves_per_simd;   /* batched analyses: register budget of the persistent kernel,
                                as resident wavefronts per SIMD: 0 or 1 -> unconstrained,
                                2 or 4 -> fewer registers (spilled), more circuits in flight */
+  int32_t backend;          /* Options.Backend (solver/matrix.go:49-93): 0 auto (dense LU below
+                               30 unknowns, sparse GP LU from 30), 1 dense, 2 sparse */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */

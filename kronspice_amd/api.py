@@ -43,7 +43,9 @@ class Options(C.Structure):
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
                 ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
-                ("staging", C.c_int32), ("waves_per_simd", C.c_int32)]
+                ("staging", C.c_int32), ("waves_per_simd", C.c_int32),
+                # Options.Backend: 0 auto (dense below 30 unknowns), 1 dense, 2 sparse
+                ("backend", C.c_int32)]
 
 
 class TranSpec(C.Structure):

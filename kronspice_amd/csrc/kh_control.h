@@ -66,6 +66,9 @@ struct Circ {
   const int32_t* slot_j;    // [nslots] column
   const int32_t* diag_slot; // [n] slot of (i, i)
   int32_t nonlin;           // any nonlinear device
+  int32_t dense;            // solved by the dense backend (lu_dense): no sparsity patterns
+  const int32_t* order;     // [nslots] slot order of AbsProduct: row-major for the dense
+                            // backend (solver/dense.go:63-73), null: slot order
 };
 
 struct CtlArgs {
@@ -246,7 +249,8 @@ struct Ctl {
   // accumulated over the stamped slots in slot order, then the weighted sum.
   KH_HD double roundoff() {
     for (int i = 0; i < n; i++) X(A.absbuf, i) = 0;
-    for (int s = 0; s < A.ckt.nslots; s++) {
+    for (int k = 0; k < A.ckt.nslots; k++) {
+      const int s = A.ckt.order ? A.ckt.order[k] : k;
       int64_t ix = s;
       if (!A.present[ix]) continue;
       X(A.absbuf, A.ckt.slot_i[s]) += fabs(A.val[ix]) * fabs(X(A.x, A.ckt.slot_j[s]));
@@ -365,7 +369,7 @@ struct Ctl {
         if (A.ckt.isV[i]) add_diag(diag_slot(i), g);
     }
     ci[CI_NEED_SOLVE] = 1;
-    if (L.grew) ci[CI_WAIT] = WAIT_PATTERN;
+    if (L.grew && !A.ckt.dense) ci[CI_WAIT] = WAIT_PATTERN;  // the dense LU has no pattern
   }
   // Slot of the diagonal entry (i, i): the device slot when a device stamps
   // it, else a slot reserved after the device slots.

@@ -57,6 +57,17 @@ KH_HD inline void for_each_array(G& g, F&& f) {
   f(a.cd, (int64_t)NCD * 8, true, true);
   if (a.lane_spec) f(a.lane_spec, 4 * 8, true, false);
   if (a.nodeset) f(a.nodeset, (int64_t)(a.ckt.numNodes - 1) * 8, true, false);
+  if (a.ckt.dense) {  // lu_dense's workspace: matrix, column maxima, y, pivot rows
+    f(l.lx, n * n * 8, false, false);
+    f(l.udiag, n * 8, false, false);
+    f(l.w, n * 8, false, false);
+    f(a.ci, (int64_t)NCI * 4, true, true);
+    f(a.lu_status, 4, true, true);
+    f(l.prow, n * 4, false, false);
+    f(l.err_step, 4, true, true);
+    f(a.present, ns, true, true);
+    return;
+  }
   f(l.xw, n * 8, false, false);
   f(l.w, n * 8, false, false);
   f(l.udiag, n * 8, false, false);
@@ -169,15 +180,20 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
     if (ctl.ci[CI_NEED_SOLVE]) {
       if (solves >= max_solves) break;
       uint64_t t0 = KH_CLOCK();
-      int st = lu_factor<double, true>(la, 0);
-      if (st == LU_CAPACITY) {
-        // L/U outgrew the group's capacity (pivoting away from the diagonal
-        // fills more than the structural estimate): the host enlarges it and
-        // the factorization is redone in the next launch.
-        ctl.ci[CI_WAIT] = WAIT_CAPACITY;
-        break;
+      int st;
+      if (a.ckt.dense) {
+        st = lu_dense<double, true>(la, 0, a.ckt.slot_i, a.ckt.slot_j);
+      } else {
+        st = lu_factor<double, true>(la, 0);
+        if (st == LU_CAPACITY) {
+          // L/U outgrew the group's capacity (pivoting away from the diagonal
+          // fills more than the structural estimate): the host enlarges it and
+          // the factorization is redone in the next launch.
+          ctl.ci[CI_WAIT] = WAIT_CAPACITY;
+          break;
+        }
+        if (st == LU_OK) st = lu_solve<double, true>(la, 0);
       }
-      if (st == LU_OK) st = lu_solve<double, true>(la, 0);
       lus[0] = st;
       solves++;
       cyc_lu += KH_CLOCK() - t0;

@@ -74,6 +74,8 @@ KH_HD inline kcplx lu_div(kcplx n, kcplx m) {
   }
   return {e, f};
 }
+KH_HD inline double& lu_re(double& v) { return v; }
+KH_HD inline double& lu_re(kcplx& v) { return v.re; }
 template <typename T> KH_HD inline T lu_zero_value();
 template <> KH_HD inline double lu_zero_value<double>() { return 0.0; }
 template <> KH_HD inline kcplx lu_zero_value<kcplx>() { return {0.0, 0.0}; }
@@ -351,6 +353,108 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
     if (!lu_finite(v)) return LU_NONFINITE;
     a.x[(int64_t)q[k] * B + c] = v;
   }
+  return LU_OK;
+}
+
+// Dense backend: Dense.Solve / CDense.Factor + SolveFactored
+// (solver/dense.go:80-178, solver/cdense.go:92-193), which Options.Backend
+// Auto picks below 30 unknowns (solver/matrix.go:77-85). The lane's slot
+// values are scattered into a row-major matrix (lx), which is factored in
+// place by Doolittle elimination with partial pivoting: the largest magnitude
+// at or below the diagonal, the first one on ties; a pivot at most n·1e-14 of
+// its column's largest original entry is singular. Forward substitution with
+// the unit lower factor, back substitution with the upper one; the solution is
+// written to x before its finiteness is checked, as Dense.Solve does.
+// Workspace: lx [n*n], udiag (column maxima), w (y), prow (pivot rows).
+template <typename T, bool Unit = false>
+KH_HD inline int lu_dense(const LuArgsT<T>& a, int64_t c, const int32_t* slot_i, const int32_t* slot_j) {
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
+  const int n = a.n;
+  T* m = a.lx;
+  auto M = [&](int i, int j) -> T& { return m[((int64_t)i * n + j) * B + c]; };
+  for (int64_t k = 0; k < (int64_t)n * n; k++) m[k * B + c] = lu_zero_value<T>();
+  for (int s = 0; s < a.nslots; s++) M(slot_i[s], slot_j[s]) = a.val[(int64_t)s * B + c];
+  double scale = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      const T v = M(i, j);
+      if (!lu_finite(v)) return LU_NONFINITE;
+      const double mv = lu_mag(v);
+      if (mv > scale) scale = mv;
+    }
+  if (scale == 0) {
+    a.err_step[c] = 0;
+    return LU_SINGULAR;
+  }
+  // Column maxima of the original matrix, in the real parts of udiag.
+  auto COL = [&](int j) -> double& { return lu_re(a.udiag[(int64_t)j * B + c]); };
+  for (int j = 0; j < n; j++) a.udiag[(int64_t)j * B + c] = lu_zero_value<T>();
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++) {
+      const double mv = lu_mag(M(i, j));
+      if (mv > COL(j)) COL(j) = mv;
+    }
+  for (int k = 0; k < n; k++) {
+    int p = k;
+    double maxv = lu_mag(M(k, k));
+    for (int i = k + 1; i < n; i++) {
+      const double v = lu_mag(M(i, k));
+      if (v > maxv) {
+        maxv = v;
+        p = i;
+      }
+    }
+    if (maxv <= COL(k) * (double)n * 1e-14) {
+      a.err_step[c] = k;
+      return LU_SINGULAR;
+    }
+    a.prow[(int64_t)k * B + c] = p;
+    if (p != k)
+      for (int j = 0; j < n; j++) {
+        const T t = M(k, j);
+        M(k, j) = M(p, j);
+        M(p, j) = t;
+      }
+    const T pivot = M(k, k);
+    for (int i = k + 1; i < n; i++) {
+      const T f = lu_div(M(i, k), pivot);
+      M(i, k) = f;
+      if (lu_zero(f)) continue;
+      for (int j = k + 1; j < n; j++) {
+        const T prod = f * M(k, j);
+        M(i, j) = M(i, j) - prod;
+      }
+    }
+  }
+  T* y = a.w;
+  for (int i = 0; i < n; i++) y[(int64_t)i * B + c] = a.rhs[(int64_t)i * B + c];
+  for (int k = 0; k < n; k++) {
+    const int p = a.prow[(int64_t)k * B + c];
+    if (p != k) {
+      const T t = y[(int64_t)k * B + c];
+      y[(int64_t)k * B + c] = y[(int64_t)p * B + c];
+      y[(int64_t)p * B + c] = t;
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    T sum = y[(int64_t)i * B + c];
+    for (int j = 0; j < i; j++) {
+      const T prod = M(i, j) * y[(int64_t)j * B + c];
+      sum = sum - prod;
+    }
+    y[(int64_t)i * B + c] = sum;
+  }
+  for (int i = n - 1; i >= 0; i--) {
+    T sum = y[(int64_t)i * B + c];
+    for (int j = i + 1; j < n; j++) {
+      const T prod = M(i, j) * a.x[(int64_t)j * B + c];
+      sum = sum - prod;
+    }
+    a.x[(int64_t)i * B + c] = lu_div(sum, M(i, i));
+  }
+  for (int i = 0; i < n; i++)
+    if (!lu_finite(a.x[(int64_t)i * B + c])) return LU_NONFINITE;
   return LU_OK;
 }
 

@@ -167,8 +167,13 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
                                                      int32_t skip_energy, double* xout) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= a.BL || a.lstatus[l] < 0) return;
-  int st = lu_factor(lu, l);
-  if (st == LU_OK) st = lu_solve(lu, l);
+  int st;
+  if (a.ckt.dense) {
+    st = lu_dense(lu, l, a.ckt.slot_i, a.ckt.slot_j);
+  } else {
+    st = lu_factor(lu, l);
+    if (st == LU_OK) st = lu_solve(lu, l);
+  }
   if (st != LU_OK) {
     a.lstatus[l] = st;
     return;
@@ -182,7 +187,8 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
   // Round-off floor (output.Ledger.Roundoff): 64 eps sum |x_i| (sum_j |A_ij||x_j| + |b_i|)
   kcplx* y = lu.w;  // free after the solve: y[i].re accumulates AbsProduct
   for (int i = 0; i < n; i++) y[(int64_t)i * BL + l] = kcplx{0, 0};
-  for (int s = 0; s < a.ckt.nslots; s++) {
+  for (int k = 0; k < a.ckt.nslots; k++) {  // row by row for the dense backend (solver/cdense.go:57-67)
+    const int s = a.ckt.order ? a.ckt.order[k] : k;
     const int i = a.ckt.slot_i[s], j = a.ckt.slot_j[s];
     y[(int64_t)i * BL + l].re += lu_mag(a.val[(int64_t)s * BL + l]) * lu_mag(lu.x[(int64_t)j * BL + l]);
   }
@@ -273,6 +279,23 @@ struct LuWorkT {
 #undef KH_A
     return invalidate();
   }
+  // Workspace of the dense backend (lu_dense): an n*n matrix in lx, udiag,
+  // w, prow and err; nothing else is used.
+  hipError_t ensure_dense(int n_, int64_t B_) {
+    if (n_ == n && B_ == B && capL == (int64_t)n_ * n_ && capU == 0 && lx.p) return hipSuccess;
+    n = n_;
+    B = B_;
+    capL = (int64_t)n * n;
+    capU = capR = 0;
+    size_t nd = (size_t)n * B * sizeof(T), ni = (size_t)n * B * 4;
+    hipError_t e;
+    if ((e = w.alloc(nd)) != hipSuccess) return e;
+    if ((e = udiag.alloc(nd)) != hipSuccess) return e;
+    if ((e = lx.alloc((size_t)capL * B * sizeof(T))) != hipSuccess) return e;
+    if ((e = prow.alloc(ni)) != hipSuccess) return e;
+    if ((e = err.alloc((size_t)B * 4)) != hipSuccess) return e;
+    return hipSuccess;
+  }
   // Forget the cached reach lists (a pattern id now names a different pattern).
   hipError_t invalidate() { return hipMemset(cpid.p, 0xFF, (size_t)B * 4); }  // -1: no cached reach
   void fill(LuArgsT<T>& a) const {
@@ -327,7 +350,7 @@ inline unsigned grid_for(int64_t B) { return (unsigned)((B + kBlock - 1) / kBloc
 // ---- handles -------------------------------------------------------------------------
 struct kh_topology_s {
   Topology t;
-  DBuf et, isV, slot_i, slot_j, diag;
+  DBuf et, isV, slot_i, slot_j, diag, rowmajor;  // rowmajor: slots sorted by (row, column)
   std::mutex mu;  // batch entry points on one topology run one at a time
 };
 
@@ -595,6 +618,13 @@ kh_status kh_topology_create(int32_t num_nodes, const kh_element* elems, int32_t
   KH_HIP(hipMemcpy(t->slot_j.p, T.slot_j.data(), T.slot_j.size() * 4, hipMemcpyHostToDevice));
   KH_HIP(t->diag.alloc(T.diag_slot.size() * 4));
   KH_HIP(hipMemcpy(t->diag.p, T.diag_slot.data(), T.diag_slot.size() * 4, hipMemcpyHostToDevice));
+  std::vector<int32_t> rm(T.nslots);  // the dense backend's AbsProduct order (row by row)
+  for (int32_t k = 0; k < T.nslots; k++) rm[k] = k;
+  std::sort(rm.begin(), rm.end(), [&](int32_t a, int32_t b) {
+    return T.slot_i[a] != T.slot_i[b] ? T.slot_i[a] < T.slot_i[b] : T.slot_j[a] < T.slot_j[b];
+  });
+  KH_HIP(t->rowmajor.alloc(rm.size() * 4));
+  KH_HIP(hipMemcpy(t->rowmajor.p, rm.data(), rm.size() * 4, hipMemcpyHostToDevice));
   *out = t.release();
   return KH_OK;
 }
@@ -649,11 +679,13 @@ struct GroupRun {
   LuWork work;
   bool has_lane = false;
   bool has_nodeset = false;
+  bool dense = false;  // Options.Backend resolved for this topology's size
   kh_stats stats = {};
 
   kh_status setup(const double* d_params, const double* d_lane_spec, const double* d_nodeset) {
     const Topology& T = topo->t;
     n = T.n;
+    if (dense) KH_HIP(work.ensure_dense(n, B));
     size_t nd = (size_t)n * B * 8;
     KH_HIP(par.alloc((size_t)T.nparams * B * 8));
     KH_HIP(st.alloc((size_t)std::max(T.nstate, 1) * B * 8));
@@ -701,6 +733,8 @@ struct GroupRun {
     a.ckt.slot_i = topo->slot_i.as<int32_t>(); a.ckt.slot_j = topo->slot_j.as<int32_t>();
     a.ckt.diag_slot = topo->diag.as<int32_t>();
     a.ckt.nonlin = T.nonlin ? 1 : 0;
+    a.ckt.dense = dense ? 1 : 0;
+    a.ckt.order = dense ? topo->rowmajor.as<int32_t>() : nullptr;
     a.p = p;
     a.B = B;
     a.par = par.as<double>();
@@ -1002,6 +1036,13 @@ struct SweepOut {
 // Runs every group of a batch to completion; p is the resolved analysis.
 // The group runs are handed to `keep` (when given) for a further phase on
 // their per-circuit state; the caller then holds g_run_mu itself.
+// Backend.Resolve (solver/matrix.go:77-85): Auto is dense below 30 unknowns.
+constexpr int kDenseThreshold = 30, kMaxDense = 4096;  // DenseThreshold, MaxDense (solver/dense.go:12)
+bool use_dense(const kh_options* opt, int n) {
+  const int b = opt ? opt->backend : 0;
+  return b == 1 || (b == 0 && n < kDenseThreshold);
+}
+
 kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt, const CtlParams& p,
                     const kh_tran_spec* spec, const SweepOut* sw, kh_stats* stats,
                     std::vector<std::unique_ptr<GroupRun>>* keep = nullptr) {
@@ -1023,6 +1064,8 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
     r->B = G.B;
     r->gid = g;
     r->sw_points = sweep_points(p);
+    r->dense = use_dense(opt, G.topology->t.n);
+    if (r->dense && G.topology->t.n > kMaxDense) return fail(KH_BAD_ARGUMENT, "dense backend above 4096 unknowns");
     kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr, G.d_nodeset);
     if (s != KH_OK) return s;
     R.groups.push_back(r.get());
@@ -1159,7 +1202,10 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
       KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
       std::vector<int32_t> wst(B);
       KH_HIP(hipMemcpy2D(wst.data(), 4, G.ci.as<int32_t>() + CI_STATUS, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
-      for (int64_t c = 0; c < B; c++) {
+      bool any_ok = false;
+      for (int64_t c = 0; c < B; c++) any_ok = any_ok || wst[c] == ST_OK;
+      if (!any_ok) break;  // no operating point succeeded
+      for (int64_t c = 0; c < B && !G.dense; c++) {  // the dense backend has no pattern
         if (wst[c] != ST_OK) continue;
         std::vector<int32_t> slots;
         for (int s2 = 0; s2 < T.nslots; s2++)
@@ -1173,8 +1219,8 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
         }
       }
       KH_HIP(pats.sync());
-      if (pats.host.desc.empty()) break;  // no operating point succeeded
-      KH_HIP(work.ensure(n, BLmax, lu_capacity(pats.host.max_fill, n)));
+      if (G.dense) KH_HIP(work.ensure_dense(n, BLmax));
+      else KH_HIP(work.ensure(n, BLmax, lu_capacity(pats.host.max_fill, n)));
       std::vector<int32_t> lp((size_t)BLmax);
       for (int64_t l = 0; l < BLmax; l++) lp[l] = std::max(cpid[l % B], 0);
       KH_HIP(hipMemcpy(pid.p, lp.data(), lp.size() * 4, hipMemcpyHostToDevice));
@@ -1193,7 +1239,7 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     lu.rhs = a.rhs;
     lu.x = x.as<kcplx>();
     work.fill(lu);
-    KH_HIP(work.invalidate());
+    if (!G.dense) KH_HIP(work.invalidate());
     hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac);
     KH_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),

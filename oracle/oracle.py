@@ -29,7 +29,7 @@ class KhOptions(C.Structure):
                 ("no_source_stepping", C.c_int32), ("no_pseudo_transient", C.c_int32),
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
                 ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
-                ("staging", C.c_int32), ("waves_per_simd", C.c_int32)]
+                ("staging", C.c_int32), ("waves_per_simd", C.c_int32), ("backend", C.c_int32)]
 
 
 class KhTranSpec(C.Structure):
@@ -166,9 +166,11 @@ def size(num_nodes: int, els: np.ndarray, params: np.ndarray) -> int:
     return lib().oracle_size(num_nodes, _p(els), len(els), _p(params))
 
 
-def op(num_nodes: int, els: np.ndarray, params: np.ndarray, opts: Optional[KhOptions] = None, backend: int = 2,
-       guess: Optional[np.ndarray] = None):
-    """analysis.OP restatement. Returns (status, x, info dict with q/prow/currents)."""
+def op(num_nodes: int, els: np.ndarray, params: np.ndarray, opts: Optional[KhOptions] = None,
+       backend: Optional[int] = None, guess: Optional[np.ndarray] = None):
+    """analysis.OP restatement. Returns (status, x, info dict with q/prow/currents).
+    backend (0 auto, 1 dense, 2 sparse) defaults to opts.backend, whose zero
+    value is Auto as in analysis.Options."""
     n = size(num_nodes, els, params)
     if n <= 0:
         return -n, None, {}
@@ -179,7 +181,8 @@ def op(num_nodes: int, els: np.ndarray, params: np.ndarray, opts: Optional[KhOpt
     info = OpInfo()
     o = opts if opts is not None else KhOptions()
     g = None if guess is None else np.ascontiguousarray(guess, np.float64)
-    e = lib().oracle_op(num_nodes, _p(els), len(els), _p(params), C.byref(o), backend, _p(g), _p(x), _p(q),
+    b = o.backend if backend is None else backend
+    e = lib().oracle_op(num_nodes, _p(els), len(els), _p(params), C.byref(o), b, _p(g), _p(x), _p(q),
                         _p(prow), _p(cur), C.byref(info))
     d = {f[0]: getattr(info, f[0]) for f in OpInfo._fields_}
     d.update(q=q, prow=prow, current=cur)
@@ -197,7 +200,7 @@ def op_polished(num_nodes, els, params, opts=None, guess=None):
 
 
 def tran(num_nodes: int, els: np.ndarray, params: np.ndarray, spec: KhTranSpec, opts: Optional[KhOptions] = None,
-         backend: int = 2, cap: int = 0):
+         backend: Optional[int] = None, cap: int = 0):
     n = size(num_nodes, els, params)
     x = np.zeros(n)
     times = np.zeros(max(cap, 1))
@@ -205,7 +208,8 @@ def tran(num_nodes: int, els: np.ndarray, params: np.ndarray, spec: KhTranSpec, 
     cur = np.zeros((max(cap, 1), len(els)))
     info = TranInfo()
     o = opts if opts is not None else KhOptions()
-    e = lib().oracle_tran(num_nodes, _p(els), len(els), _p(params), C.byref(o), C.byref(spec), backend, _p(x),
+    b = o.backend if backend is None else backend
+    e = lib().oracle_tran(num_nodes, _p(els), len(els), _p(params), C.byref(o), C.byref(spec), b, _p(x),
                           _p(times), _p(xs), _p(cur), cap, C.byref(info))
     d = {f[0]: getattr(info, f[0]) for f in TranInfo._fields_}
     c = info.count

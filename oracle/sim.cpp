@@ -76,10 +76,23 @@ struct RealMatrix : Stamper {
   void absProduct(const double* x, double* y) const { if (sparse) s.absProduct(x, y); else d.absProduct(x, y); }
 };
 
+// Complex system: CSparse or CDense (solver/matrix.go:95-101).
 struct ComplexMatrix : CStamper {
+  bool sparse = true;
   SparseLU<cplx> s;
-  void add(int i, int j, cplx v) override { s.add(i, j, v); }
+  CDense d;
+  int init(int n_, bool sp) {
+    sparse = sp;
+    return sp ? s.init(n_) : d.init(n_);
+  }
+  void add(int i, int j, cplx v) override { if (sparse) s.add(i, j, v); else d.add(i, j, v); }
+  void clear() { if (sparse) s.clear(); else d.clear(); }
+  int solve(const cplx* b, cplx* x) { return sparse ? s.solve(b, x) : d.solve(b, x); }
+  void absProduct(const cplx* x, double* y) const { if (sparse) s.absProduct(x, y); else d.absProduct(x, y); }
 };
+
+// Backend.Resolve, solver/matrix.go:77-85: 0 auto, 1 dense, 2 sparse.
+static bool resolve_sparse(int backend, int n) { return backend == 2 || (backend == 0 && n >= 30); }
 
 struct Options {  // analysis.Options with defaults(), analysis/op.go:91-110
   double relTol = 1e-3, absTol = 1e-12, vnTol = 1e-6, gmin = 1e-12, ledgerTol = 1e-9;
@@ -109,6 +122,7 @@ static bool retryable(int e) { return e == E_NOCONV || e == E_NONFINITE || e == 
 
 struct Sim {
   int numNodes = 1;
+  int backendSel = 0;  // Options.Backend the Sim was built with (the AC matrix resolves it too)
   std::vector<std::unique_ptr<Device>> devs;
   std::vector<int> reactive, nonlin, breakers;
   std::vector<char> isV;
@@ -184,7 +198,8 @@ struct Sim {
       if (devs[i]->breakpointer()) breakers.push_back((int)i);
     }
     n = ctx.size();
-    bool sp = backend == 2 || (backend == 0 && n >= 30);  // Resolve, solver/matrix.go:77-85
+    bool sp = resolve_sparse(backend, n);
+    backendSel = backend;
     int e = m.init(n, sp);
     if (e != OK) return E_BADSIZE;
     rhs.assign(n, 0); x.assign(n, 0); xOld.assign(n, 0); absBuf.assign(n, 0);
@@ -681,7 +696,7 @@ struct Sim {
     e = polish();
     if (e != E_OK) return e;
     ComplexMatrix cm;
-    if (cm.s.init(n) != OK) return E_BADSIZE;
+    if (cm.init(n, resolve_sparse(backendSel, n)) != OK) return E_BADSIZE;
     std::vector<cplx> crhs(n), cx(n);
     std::vector<double> ab(n);
     ACContext actx;
@@ -689,11 +704,11 @@ struct Sim {
     actx.RHS = &crhs;
     *maxRel = 0;
     for (int k = 0; k < nf; k++) {
-      cm.s.clear();
+      cm.clear();
       std::fill(crhs.begin(), crhs.end(), cplx(0));
       actx.omega = 2 * kPi * freqs[k];
       for (auto& d : devs) d->loadAC(actx);
-      int se = cm.s.solve(crhs.data(), cx.data());
+      int se = cm.solve(crhs.data(), cx.data());
       if (se != OK) return se;
       std::copy(cx.begin(), cx.end(), xout + (size_t)k * n);
       cplx sumS = 0;
@@ -753,7 +768,7 @@ struct Sim {
         sumS += s;
         scaleS += absval(s);
       }
-      cm.s.absProduct(cx.data(), ab.data());
+      cm.absProduct(cx.data(), ab.data());
       double ro = 0;
       for (int i = 0; i < n; i++) ro += absval(cx[i]) * (ab[i] + absval(crhs[i]));
       ro *= 64 * 0x1p-52;
@@ -835,7 +850,7 @@ int oracle_op(int32_t num_nodes, const kh_element* el, int32_t nel, const double
 int oracle_op_polished(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params,
                        const kh_options* ko, const double* guess, double* x_out, int32_t* iters_out) {
   Sim s;
-  int e = s.build(num_nodes, el, nel, params, 2, 0, 0);
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
   if (e != E_OK) return e;
   Options o;
   o.from(ko);
@@ -892,7 +907,7 @@ int oracle_dc(int32_t num_nodes, const kh_element* el, int32_t nel, const double
               int32_t nsweeps, const int32_t* src, const double* start, const double* stop, const double* step,
               double* xout, int32_t* iters, int32_t* done, double* max_rel) {
   Sim s;
-  int e = s.build(num_nodes, el, nel, params, 2, 0, 0);
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
   if (e != E_OK) return e;
   Options o;
   o.from(ko);
@@ -907,7 +922,7 @@ int oracle_dc(int32_t num_nodes, const kh_element* el, int32_t nel, const double
 int oracle_ac(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params, const kh_options* ko,
               const double* freqs, int32_t nf, double* xout, double* max_rel) {
   Sim s;
-  int e = s.build(num_nodes, el, nel, params, 2, 0, 0);
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
   if (e != E_OK) return e;
   Options o;
   o.from(ko);
@@ -1007,7 +1022,7 @@ int oracle_batch_op(int32_t num_nodes, const kh_element* el, int32_t nel, int32_
     std::vector<double> p(nparams);
     for (int k = 0; k < nparams; k++) p[k] = params[(size_t)k * B + c];
     Sim s;
-    int e = s.build(num_nodes, el, nel, p.data(), 2, 0, 0);
+    int e = s.build(num_nodes, el, nel, p.data(), ko ? ko->backend : 0, 0, 0);
     int it = 0;
     if (e == E_OK) {
       Options o;
@@ -1039,7 +1054,7 @@ int oracle_batch_tran(int32_t num_nodes, const kh_element* el, int32_t nel, int3
       sp.max_step = lane_spec[(size_t)2 * B + c];
       sp.min_step = lane_spec[(size_t)3 * B + c];
     }
-    int e = s.build(num_nodes, el, nel, p.data(), 2, sp.step, sp.stop);
+    int e = s.build(num_nodes, el, nel, p.data(), ko ? ko->backend : 0, sp.step, sp.stop);
     Sim::TranOut out;
     if (e == E_OK) {
       Options o;

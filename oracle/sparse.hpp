@@ -380,10 +380,13 @@ struct SparseLU {
   }
 };
 
-// Dense, solver/dense.go:21-178 (Doolittle LU with partial pivoting).
-struct Dense {
+// Dense and CDense, solver/dense.go:21-178 and solver/cdense.go:92-193
+// (Doolittle LU with partial pivoting; complex magnitudes are cmplx.Abs).
+template <typename T>
+struct DenseT {
   int n = 0;
-  std::vector<double> a, lu, y, col;
+  std::vector<T> a, lu, y;
+  std::vector<double> col;
   std::vector<int> piv;
   int errStep = -1;
   int init(int n_) {
@@ -397,48 +400,48 @@ struct Dense {
     return OK;
   }
   void clear() { std::fill(a.begin(), a.end(), 0.0); }
-  void add(int i, int j, double v) {
+  void add(int i, int j, T v) {
     if (i < 0 || j < 0 || i >= n || j >= n) return;
     a[(size_t)i * n + j] += v;
   }
-  void absProduct(const double* x, double* yv) const {
+  void absProduct(const T* x, double* yv) const {
     for (int i = 0; i < n; i++) {
       double s = 0;
-      for (int j = 0; j < n; j++) s += std::fabs(a[(size_t)i * n + j]) * std::fabs(x[j]);
+      for (int j = 0; j < n; j++) s += absval(a[(size_t)i * n + j]) * absval(x[j]);
       yv[i] = s;
     }
   }
-  int solve(const double* b, double* x) {
+  int solve(const T* b, T* x) {
     lu = a;
     double scale = 0;
-    for (double v : lu) {
+    for (const T& v : lu) {
       if (!finite(v)) return NONFINITE;
-      double av = std::fabs(v);
+      double av = absval(v);
       if (av > scale) scale = av;
     }
     if (scale == 0) return SINGULAR;
     std::fill(col.begin(), col.end(), 0.0);
     for (int i = 0; i < n; i++)
       for (int j = 0; j < n; j++) {
-        double av = std::fabs(lu[(size_t)i * n + j]);
+        double av = absval(lu[(size_t)i * n + j]);
         if (av > col[j]) col[j] = av;
       }
     for (int k = 0; k < n; k++) {
       int p = k;
-      double maxv = std::fabs(lu[(size_t)k * n + k]);
+      double maxv = absval(lu[(size_t)k * n + k]);
       for (int i = k + 1; i < n; i++) {
-        double v = std::fabs(lu[(size_t)i * n + k]);
+        double v = absval(lu[(size_t)i * n + k]);
         if (v > maxv) { maxv = v; p = i; }
       }
       if (maxv <= col[k] * (double)n * 1e-14) { errStep = k; return SINGULAR; }
       piv[k] = p;
       if (p != k)
         for (int j = 0; j < n; j++) std::swap(lu[(size_t)k * n + j], lu[(size_t)p * n + j]);
-      double pivot = lu[(size_t)k * n + k];
+      T pivot = lu[(size_t)k * n + k];
       for (int i = k + 1; i < n; i++) {
-        double f = lu[(size_t)i * n + k] / pivot;
+        T f = sdiv(lu[(size_t)i * n + k], pivot);
         lu[(size_t)i * n + k] = f;
-        if (f == 0) continue;
+        if (f == T(0)) continue;
         for (int j = k + 1; j < n; j++) lu[(size_t)i * n + j] -= f * lu[(size_t)k * n + j];
       }
     }
@@ -446,19 +449,21 @@ struct Dense {
     for (int k = 0; k < n; k++)
       if (piv[k] != k) std::swap(y[k], y[piv[k]]);
     for (int i = 0; i < n; i++) {
-      double s = y[i];
+      T s = y[i];
       for (int j = 0; j < i; j++) s -= lu[(size_t)i * n + j] * y[j];
       y[i] = s;
     }
     for (int i = n - 1; i >= 0; i--) {
-      double s = y[i];
+      T s = y[i];
       for (int j = i + 1; j < n; j++) s -= lu[(size_t)i * n + j] * x[j];
-      x[i] = s / lu[(size_t)i * n + i];
+      x[i] = sdiv(s, lu[(size_t)i * n + i]);
     }
     for (int i = 0; i < n; i++)
       if (!finite(x[i])) return NONFINITE;
     return OK;
   }
 };
+using Dense = DenseT<double>;
+using CDense = DenseT<std::complex<double>>;  // solver/cdense.go:92-193 (cmplx.Abs pivots)
 
 }  // namespace oracle

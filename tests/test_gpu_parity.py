@@ -401,3 +401,66 @@ def test_ac_rc_lowpass_batch_closed_form():
     for b in range(0, B, 37):
         e, xo, _ = oracle.ac(base.num_nodes, els, np.ascontiguousarray(params[:, b]), freqs)
         assert e == 0 and np.array_equal(x[:, :, b], xo), b
+
+
+# ---------------------------------------------------------------------------- backends
+@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "mos1_cmos_vtc", "bjt_ce_op", "zener_dc"])
+def test_op_goldens_forced_sparse(name):
+    """Options.Backend = SparseLU on circuits Auto would give the dense LU:
+    the GP path against the oracle's sparse restatement, same status and
+    iterations, solutions within 1e-9 (bit for bit where linear)."""
+    c = gr.load_deck(name).circuit
+    els, p = c.flatten()
+    o = api.Options()
+    o.backend = 2
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.op(t, np.repeat(p[:, None], 8, axis=1), opts=o)
+    x, st, it = host(r.x), host(r.status), host(r.iters)
+    e0, x0, info = oracle.op(c.num_nodes, els, p, backend=2)
+    assert (st == e0).all() and (it == info["iters"]).all()
+    assert relerr(x, np.repeat(x0[:, None], 8, axis=1)).max() <= 1e-9
+    if all(e.kind not in (6, 7, 8) for e in c.elements):
+        assert (x == x0[:, None]).all()
+
+
+def test_rectifier_batch_forced_dense_matches_oracle():
+    """Options.Backend = DenseLU on the 35-unknown rectifier (Auto would pick
+    the sparse LU): lu_dense (solver/dense.go:80-178) in the persistent
+    kernel against the oracle's Dense, same status and step counts,
+    solutions within 1e-9."""
+    b = W.rectifier_rc_ladders(64)
+    spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    o = api.Options()
+    o.backend = 1
+    t = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    r = api.tran(t, b.params, spec, opts=o, lane_spec=b.lane_spec)
+    ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+    xo, so, ao, ro, lo = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params, b.n, ospec, threads=8,
+                                           lane_spec=b.lane_spec, opts=oracle.options(backend=1))
+    st, acc, rej = host(r.status), host(r.accepted), host(r.rejected)
+    assert (st == so).all() and (st == 0).all()
+    same = (acc == ao) & (rej == ro)
+    assert same.mean() >= 0.99, same.mean()
+    assert relerr(host(r.x), xo)[same].max() <= 1e-9
+    assert r.stats["patterns"] == 0  # the dense LU needs no ordering
+
+
+def test_ac_forced_sparse_matches_oracle():
+    """AC with Options.Backend = SparseLU on the BJT amplifier (Auto: CDense):
+    the complex GP path (CSparse) against the oracle's, within 1e-9."""
+    from kronspice_amd import netlist
+    deck = gr.load_deck("bjt_ce_ac")
+    c = deck.circuit
+    a = [a for a in deck.analyses if a.kind == "ac"][0].args
+    freqs = np.array(netlist.ac_frequencies(a["variation"], a["points"], a["fstart"], a["fstop"]))
+    els, p = c.flatten()
+    e0, xo, mro = oracle.ac(c.num_nodes, els, p, freqs, opts=oracle.options(backend=2))
+    assert e0 == 0
+    o = api.Options()
+    o.backend = 2
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.ac(t, np.repeat(p[:, None], 4, axis=1), freqs, opts=o)
+    x, st, pts = host(r.x), host(r.status), host(r.points)
+    assert (st == 0).all() and (pts == len(freqs)).all()
+    for k in range(4):
+        assert np.max(np.abs(x[:, :, k] - xo)) <= 1e-9 * np.max(np.abs(xo)), k

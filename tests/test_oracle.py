@@ -201,28 +201,59 @@ def test_pivot_rule_diagonal_preference_and_ties():
 
 # ----------------------------------------------------------------------------- analyses
 class OracleBackend:
+    """The oracle's analyses; backend as Options.Backend (0 auto: dense LU
+    below 30 unknowns, 1 dense, 2 sparse GP)."""
+
+    def __init__(self, backend=0):
+        self.o = oracle.options(backend=backend)
+
     def op(self, c):
         els, p = c.flatten()
-        st, x, info = oracle.op(c.num_nodes, els, p)
+        st, x, info = oracle.op(c.num_nodes, els, p, opts=self.o)
         return st, x, info.get("current")
 
     def tran(self, c, a):
         els, p = c.flatten(a["step"], a["stop"])
         spec = oracle.KhTranSpec(step=a["step"], stop=a["stop"], start=a["start"], max_step=a["max_step"],
                                  uic=1 if a["uic"] else 0)
-        st, x, info = oracle.tran(c.num_nodes, els, p, spec, cap=200000)
+        st, x, info = oracle.tran(c.num_nodes, els, p, spec, opts=self.o, cap=200000)
         return st, info["times"], info["xs"], info["current"], info["max_rel"]
 
     def ac(self, c, freqs):
         els, p = c.flatten()
-        st, X, mr = oracle.ac(c.num_nodes, els, p, freqs)
+        st, X, mr = oracle.ac(c.num_nodes, els, p, freqs, opts=self.o)
         return st, X
 
 
+@pytest.mark.parametrize("backend", [0, 2])
 @pytest.mark.parametrize("name", gr.names())
-def test_golden_independent(name):
-    msgs = gr.check(name, OracleBackend())
+def test_golden_independent(name, backend):
+    """Every golden under the default backend (Auto: the dense LU for these
+    small circuits) and under the sparse GP LU forced."""
+    msgs = gr.check(name, OracleBackend(backend))
     assert not msgs, msgs[:5]
+
+
+@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "bjt_ce_op", "rectifier_rc", "rlc_step"])
+def test_dense_and_sparse_backends_agree(name):
+    """solver/matrix.go:40-46: the backends agree to round-off. Auto resolves
+    to the dense LU below 30 unknowns (Backend.Resolve, :77-85); the forced
+    sparse GP LU gives the same analysis to 1e-12."""
+    deck = gr.load_deck(name)
+    c = deck.circuit
+    kinds = [a.kind for a in deck.analyses]
+    if "tran" in kinds:
+        a = [a for a in deck.analyses if a.kind == "tran"][0].args
+        els, p = c.flatten(a["step"], a["stop"])
+        spec = oracle.KhTranSpec(step=a["step"], stop=a["stop"], max_step=a["max_step"])
+        ed, xd, _ = oracle.tran(c.num_nodes, els, p, spec, backend=1)
+        es, xs, _ = oracle.tran(c.num_nodes, els, p, spec, backend=2)
+    else:
+        els, p = c.flatten()
+        ed, xd, _ = oracle.op(c.num_nodes, els, p, backend=1)
+        es, xs, _ = oracle.op(c.num_nodes, els, p, backend=2)
+    assert ed == es == 0
+    assert np.max(np.abs(xd - xs)) <= 1e-12 * max(1.0, np.max(np.abs(xs)))
 
 
 NGSPICE = [n for n in gr.names() if __import__("os").path.exists(__import__("os").path.join(gr.REF, "ngspice", n + ".csv"))]
@@ -367,6 +398,7 @@ class _MutatedBackend(OracleBackend):
     goldens must expose."""
 
     def __init__(self, index, value):
+        super().__init__(0)
         self.index, self.value = index, value
 
     def _mut(self, c, step=0.0, stop=0.0):
