@@ -83,11 +83,15 @@ typedef struct kh_element {
  *       10 AC magnitude, 11 AC phase [deg]                (device/source.go:21-35)
  * D: 0 IS, 1 N, 2 RS, 3 CJO, 4 VJ, 5 M, 6 TT, 7 BV, 8 IBV, 9 FC, 10 hasBV,
  *    11 vte, 12 vcrit, 13 xbv, 14 grs, 15 hasCharge       (device/diode.go:85-128)
- * M (level 1): 0 typ, 1 beta, 2 vbi, 3 vto (type-normalized), 4 GAMMA,
+ * M (levels 1-3): 0 typ, 1 beta, 2 vbi, 3 vto (type-normalized), 4 GAMMA,
  *    5 PHI, 6 LAMBDA, 7 gdpr, 8 gspr, 9 isd, 10 iss, 11 vcritD, 12 vcritS,
  *    13 czbd, 14 czbdsw, 15 czbs, 16 czbssw, 17 PB, 18 MJ, 19 MJSW, 20 FC,
  *    21..23 fbt[3], 24..26 fsw[3], 27 cgso, 28 cgdo, 29 cgbo, 30 oxideCap,
  *    31 vt, 32 hasJcap, 33 hasGcap                        (device/mosfet.go:218-323)
+ *    34 level (1, 2, 3), 35 NFS, 36 XJ, 37 UCRIT, 38 UEXP, 39 VMAX, 40 NEFF,
+ *    41 DELTA, 42 THETA, 43 ETA, 44 KAPPA, 45 UO, 46 leff, 47 weff, 48 cox,
+ *    49 xd, 50 alpha, 51 cfs, 52 factor, 53 narrow, 54 eta3, 55 sqrtPB,
+ *    56 hasNSUB, 57 LD        (device/mosfet.go:128-324, device/mosfet_levels.go:22-55)
  * Q (Gummel-Poon, area applied): 0 type (+1 NPN, -1 PNP), 1 IS, 2 BF, 3 NF,
  *    4 VAF, 5 IKF, 6 ISE, 7 NE, 8 BR, 9 NR, 10 VAR, 11 IKR, 12 ISC, 13 NC,
  *    14 RB, 15 IRB, 16 RBM, 17 RE, 18 RC, 19 CJE, 20 VJE, 21 MJE, 22 TF,
@@ -101,7 +105,7 @@ typedef struct kh_element {
 #define KH_NPAR_L 3
 #define KH_NPAR_SRC 12
 #define KH_NPAR_D 16
-#define KH_NPAR_M 34
+#define KH_NPAR_M 58
 #define KH_NPAR_Q 52
 
 /* Analysis options: analysis.Options (analysis/op.go:22-62) with the same

@@ -25,7 +25,7 @@ import numpy as np
 # Element kinds (include/kronhip.h)
 R, C, L, V, I, D, M, Q = 1, 2, 3, 4, 5, 6, 7, 8
 KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q"}
-NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 34, Q: 52}
+NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 58, Q: 52}
 
 ELEMENT_DTYPE = np.dtype([("kind", "<i4"), ("node", "<i4", (4,)), ("poff", "<i4")])
 
@@ -49,6 +49,10 @@ EPS_SI = _exact("F('11.7') * F('8.854214871e-12')")
 VT_NOM = _exact("k * T / q")                          # BoltzmanK * TNomK / ChargeQ
 EG_NOM = _exact("F('1.16') - F('7.02e-4') * T * T / (T + 1108)")
 TWO_EPS_SI_Q = _exact("2 * F('11.7') * F('8.854214871e-12') * q")   # 2*epsSi*ChargeQ
+TWO_EPS_SI = _exact("2 * F('11.7') * F('8.854214871e-12')")          # 2*epsSi
+TNOM_R300 = _exact("T / 300")                                         # TNomK / 300
+INV300_MINUS_INVT = _exact("F(1, 300) - 1 / T")                       # 1.0/300 - 1/TNomK
+K_OVER_Q = _exact("k / q")                                            # BoltzmanK / ChargeQ
 EPS_SI_Q = _exact("F('11.7') * F('8.854214871e-12') * q")           # epsSi * ChargeQ
 
 WAVE_NONE, WAVE_PULSE, WAVE_SIN, WAVE_EXP = 0, 1, 2, 3
@@ -353,25 +357,35 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
     mod = _model(c, e)
     mp = mod.params
     level = int(mp.get("level", 1))
-    if level != 1:
-        raise NotImplementedError("only MOSFET level 1 is on this path")
+    if level not in (1, 2, 3):
+        raise ValueError(f"{e.name}: MOSFET level {level} is not implemented")
     typ = -1.0 if mod.type == "pmos" else 1.0
     p = dict(VTO=0.0, KP=2e-5, GAMMA=0.0, PHI=0.6, LAMBDA=0.0, RD=0.0, RS=0.0, CBD=0.0, CBS=0.0, IS=1e-14, PB=0.8,
              CGSO=0.0, CGDO=0.0, CGBO=0.0, RSH=0.0, CJ=0.0, MJ=0.5, CJSW=0.0, MJSW=0.5, JS=0.0, TOX=0.0, NSUB=0.0,
-             NSS=0.0, TPG=1.0, LD=0.0, UO=600.0)
+             NSS=0.0, TPG=1.0, LD=0.0, UO=600.0, NFS=0.0, XJ=0.0, UCRIT=1e4, UEXP=0.0, UTRA=0.0, VMAX=0.0, NEFF=1.0,
+             DELTA=0.0, THETA=0.0, ETA=0.0, KAPPA=0.0, WD=0.0)
     given = set()
+    if level >= 2:  # mos2set, mos3set: MJSW 0.33, TOX 1e-7 (given)
+        p["MJSW"] = 0.33
+        p["TOX"] = 1e-7
+        given.add("tox")
     keys = [("VTO", ("vto", "vt0")), ("KP", ("kp",)), ("GAMMA", ("gamma",)), ("PHI", ("phi",)),
             ("LAMBDA", ("lambda",)), ("RD", ("rd",)), ("RS", ("rs",)), ("CBD", ("cbd",)), ("CBS", ("cbs",)),
             ("IS", ("is",)), ("PB", ("pb",)), ("CGSO", ("cgso",)), ("CGDO", ("cgdo",)), ("CGBO", ("cgbo",)),
             ("RSH", ("rsh",)), ("CJ", ("cj",)), ("MJ", ("mj",)), ("CJSW", ("cjsw",)), ("MJSW", ("mjsw",)),
-            ("JS", ("js",)), ("TOX", ("tox",)), ("NSUB", ("nsub",)), ("NSS", ("nss",)), ("TPG", ("tpg",)),
-            ("LD", ("ld",)), ("UO", ("uo", "u0"))]
+            ("JS", ("js",)), ("TOX", ("tox",)), ("NSUB", ("nsub",)), ("NSS", ("nss",)), ("NFS", ("nfs",)),
+            ("TPG", ("tpg",)), ("XJ", ("xj",)), ("LD", ("ld",)), ("UO", ("uo", "u0")), ("UCRIT", ("ucrit",)),
+            ("UEXP", ("uexp",)), ("UTRA", ("utra",)), ("VMAX", ("vmax",)), ("NEFF", ("neff",)), ("FC", ("fc",)),
+            ("DELTA", ("delta",)), ("THETA", ("theta",)), ("ETA", ("eta",)), ("KAPPA", ("kappa",)), ("WD", ("wd",))]
+    p["FC"] = 0.5
     for name, ks in keys:
         for k in ks:
             if k in mp:
                 p[name] = mp[k]
                 given.add(ks[0])
                 break
+    if level == 3 and "kappa" not in given:
+        p["KAPPA"] = 0.2
     inst = dict(l=100e-6, w=100e-6, ad=0.0, as_=0.0, pd=0.0, ps=0.0, nrd=0.0, nrs=0.0, m=1.0)
     for k, v in e.params.items():
         kk = "as_" if k == "as" else k
@@ -379,7 +393,7 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
             inst[kk] = v
     vt = VT_NOM
     leff = inst["l"] - 2 * p["LD"]
-    weff = inst["w"]
+    weff = inst["w"] - 2 * p["WD"] if level == 3 else inst["w"]
     if not (leff > 0) or not (weff > 0):
         raise ValueError(f"{e.name}: effective channel length or width is not positive")
     cox = 0.0
@@ -390,6 +404,10 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
         if "nsub" in given:
             nsub = p["NSUB"] * 1e6
             ni = NI_SI
+            if level == 3:  # mos3temp: ni scaled from 300 K to TNOM
+                eg = EG_NOM
+                r = TNOM_R300
+                ni *= r * math.sqrt(r) * math.exp(0.5 * eg * INV300_MINUS_INVT / K_OVER_Q)
             if not (nsub > ni):
                 raise ValueError("NSUB below the intrinsic density")
             if "phi" not in given:
@@ -429,15 +447,34 @@ def mosfet_params(c: Circuit, e: Element) -> List[float]:
     czbs = p["CBS"] * mult if "cbs" in given else p["CJ"] * inst["as_"] * mult
     czbdsw = p["CJSW"] * inst["pd"] * mult
     czbssw = p["CJSW"] * inst["ps"] * mult
-    fc = mp.get("fc", 0.5)
+    fc = p["FC"]
     fbt = depl_f(p["PB"], p["MJ"], fc)
     fsw = depl_f(p["PB"], p["MJSW"], fc)
     cgso, cgdo, cgbo = p["CGSO"] * weff * mult, p["CGDO"] * weff * mult, p["CGBO"] * leff * mult
     has_j = czbd + czbdsw + czbs + czbssw > 0
     has_g = oxide_cap > 0 or cgso + cgdo + cgbo > 0
+    # deriveLevel, device/mosfet_levels.go:22-55
+    xd = alpha = cfs = factor = narrow = eta3 = sqrt_pb = 0.0
+    has_nsub = "nsub" in given
+    if has_nsub:
+        alpha = TWO_EPS_SI / (CHARGE_Q * p["NSUB"] * 1e6)
+        xd = math.sqrt(alpha)
+    if level == 2:
+        cfs = CHARGE_Q * p["NFS"] * 1e4 / cox
+        factor = p["DELTA"] * math.pi * EPS_SI / (4 * cox * weff)
+        sqrt_pb = math.sqrt(p["PB"])
+        if p["VMAX"] > 0 and not (p["NEFF"] > 0):
+            raise ValueError(f"{e.name}: NEFF must be positive")
+    elif level == 3:
+        cfs = CHARGE_Q * p["NFS"] * 1e4 / cox
+        narrow = p["DELTA"] * 0.5 * math.pi * EPS_SI / cox
+        eta3 = p["ETA"] * 8.15e-22 / (cox * leff * leff * leff)
     return [typ, beta, vbi, vto, p["GAMMA"], p["PHI"], p["LAMBDA"], gdpr, gspr, isd, iss, vcrit_d, vcrit_s,
             czbd, czbdsw, czbs, czbssw, p["PB"], p["MJ"], p["MJSW"], fc, fbt[0], fbt[1], fbt[2], fsw[0], fsw[1],
-            fsw[2], cgso, cgdo, cgbo, oxide_cap, vt, 1.0 if has_j else 0.0, 1.0 if has_g else 0.0]
+            fsw[2], cgso, cgdo, cgbo, oxide_cap, vt, 1.0 if has_j else 0.0, 1.0 if has_g else 0.0,
+            float(level), p["NFS"], p["XJ"], p["UCRIT"], p["UEXP"], p["VMAX"], p["NEFF"], p["DELTA"], p["THETA"],
+            p["ETA"], p["KAPPA"], p["UO"], leff, weff, cox, xd, alpha, cfs, factor, narrow, eta3, sqrt_pb,
+            1.0 if has_nsub else 0.0, p["LD"]]
 
 
 BJT_DEFAULTS = dict(IS=1e-16, BF=100.0, NF=1.0, VAF=0.0, IKF=0.0, ISE=0.0, NE=1.5, BR=1.0, NR=1.0, VAR=0.0,

@@ -13,6 +13,7 @@
 #include <stdint.h>
 #include <math.h>
 #include "kh_lu.h"
+#include "kh_mos23.h"
 
 namespace kh {
 
@@ -626,7 +627,8 @@ struct MosIds {
 };
 
 KH_HD inline MosIds mos_channel(const Lane& L, const int32_t* e, double vgs, double vds, double vbs) {
-  // channel (device/mosfet.go:464-473) + idsLevel1 (device/mosfet_levels.go:79-113)
+  // channel (device/mosfet.go:464-473) + the level's ids (idsLevel1,
+  // device/mosfet_levels.go:79-113; levels 2 and 3 in kh_mos23.h)
   MosIds r;
   r.md = 1;
   double vg = vgs, vd = vds, vb = vbs;
@@ -637,6 +639,16 @@ KH_HD inline MosIds mos_channel(const Lane& L, const int32_t* e, double vgs, dou
     vb = vbs - vds;
   }
   double PHI = L.P(e, 5), GAMMA = L.P(e, 4), LAMBDA = L.P(e, 6), beta = L.P(e, 1);
+  if (KH_HAS(K_M23)) {  // ids, device/mosfet_levels.go:58-68: levels 2 and 3
+    const int level = (int)L.P(e, 34);
+    if (level == 2 || level == 3) {
+      const MosLevelP lv = mos_level_params(L.par + e[ET_POFF]);
+      const IdsOut o = level == 2 ? idsLevel2(lv, L.P(e, 2), beta, L.P(e, 31), PHI, GAMMA, LAMBDA, vg, vd, vb)
+                                  : idsLevel3(lv, L.P(e, 2), beta, L.P(e, 31), PHI, GAMMA, L.P(e, 57), vg, vd, vb);
+      r.cd = o.cd; r.gm = o.gm; r.gds = o.gds; r.gmbs = o.gmbs; r.von = o.von; r.vdsat = o.vdsat;
+      return r;
+    }
+  }
   double sarg;
   if (vb <= 0) {
     sarg = sqrt(PHI - vb);

@@ -212,6 +212,8 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
   int soff = 0;
   nonlin = false;
   kinds = 0;
+  level_rows.clear();
+  level_vals.clear();
   for (int e = 0; e < nel; e++) {
     const kh_element& E = elems[e];
     int32_t* t = &et[(size_t)e * ET_W];
@@ -282,6 +284,11 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     t[ET_SOFF] = soff;
     soff += ns;
     kinds |= 1u << E.kind;
+    if (E.kind == K_M) {
+      if (params_c0[E.poff + 34] >= 2) kinds |= 1u << K_M23;  // level 2 or 3 currents
+      level_rows.push_back(E.poff + 34);
+      level_vals.push_back(params_c0[E.poff + 34]);
+    }
   }
   nstate = soff;
   n = numNodes - 1 + branches;

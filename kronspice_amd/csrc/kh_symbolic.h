@@ -48,6 +48,11 @@ struct Topology {
   int32_t nslots = 0;
   bool nonlin = false;
   uint32_t kinds = 0;  // bit k set when an element of kind k is present (kh_devices.h)
+  // Parameter rows holding a MOSFET level, and the level circuit 0 has there:
+  // the level is structural (it picks the compiled currents), so every
+  // circuit of the topology must share it.
+  std::vector<int32_t> level_rows;
+  std::vector<double> level_vals;
   std::string error;
   bool build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0);
 };

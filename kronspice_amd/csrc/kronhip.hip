@@ -1064,6 +1064,17 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
     r->B = G.B;
     r->gid = g;
     r->sw_points = sweep_points(p);
+    {  // MOSFET levels: every circuit must have circuit 0's (the compiled currents follow it)
+      const Topology& T = G.topology->t;
+      std::vector<double> lv((size_t)G.B);
+      for (size_t k = 0; k < T.level_rows.size(); k++) {
+        KH_HIP(hipMemcpy(lv.data(), G.d_params + (size_t)T.level_rows[k] * G.B, (size_t)G.B * 8,
+                         hipMemcpyDeviceToHost));
+        for (double v : lv)
+          if (v != T.level_vals[k])
+            return fail(KH_BAD_ARGUMENT, "group " + std::to_string(g) + ": MOSFET levels differ between circuits");
+      }
+    }
     r->dense = use_dense(opt, G.topology->t.n);
     if (r->dense && G.topology->t.n > kMaxDense) return fail(KH_BAD_ARGUMENT, "dense backend above 4096 unknowns");
     kh_status s = r->setup(G.d_params, spec ? G.d_lane_spec : nullptr, G.d_nodeset);

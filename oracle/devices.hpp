@@ -783,13 +783,390 @@ inline void meyer(double vgs, double vgd, double von, double vdsat, double phi, 
   *cgb = 0;
 }
 
-// MOSFET level 1, device/mosfet.go:43-846 and device/mosfet_levels.go:79-113
+constexpr double EPS_SI = 1.035943139907e-10;  // epsSi = 11.7 * 8.854214871e-12, folded exactly
+
+// Forward-mode dual number over (vgs, vds, vbs), device/dual.go:17-106: the
+// MOSFET level 2 and 3 currents are evaluated in it, so gm, gds and gmbs are
+// their exact derivatives. Each operation restates the Go method's arithmetic.
+struct Dual {
+  double v = 0, d[3] = {0, 0, 0};
+};
+inline Dual dcst(double v) { Dual r; r.v = v; return r; }
+inline Dual dvar(double v, int k) { Dual r; r.v = v; r.d[k] = 1; return r; }
+inline Dual operator+(const Dual& a, const Dual& b) {
+  Dual r; r.v = a.v + b.v;
+  for (int k = 0; k < 3; k++) r.d[k] = a.d[k] + b.d[k];
+  return r;
+}
+inline Dual operator-(const Dual& a, const Dual& b) {
+  Dual r; r.v = a.v - b.v;
+  for (int k = 0; k < 3; k++) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+inline Dual addc(Dual a, double c) { a.v += c; return a; }
+inline Dual dneg(const Dual& a) {
+  Dual r; r.v = -a.v;
+  for (int k = 0; k < 3; k++) r.d[k] = -a.d[k];
+  return r;
+}
+inline Dual scale(const Dual& a, double c) {
+  Dual r; r.v = a.v * c;
+  for (int k = 0; k < 3; k++) r.d[k] = a.d[k] * c;
+  return r;
+}
+inline Dual operator*(const Dual& a, const Dual& b) {
+  Dual r; r.v = a.v * b.v;
+  for (int k = 0; k < 3; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+inline Dual operator/(const Dual& a, const Dual& b) {
+  Dual r; double q = a.v / b.v; r.v = q;
+  for (int k = 0; k < 3; k++) r.d[k] = (a.d[k] - q * b.d[k]) / b.v;
+  return r;
+}
+inline Dual chain(const Dual& a, double f, double df) {
+  Dual r; r.v = f;
+  for (int k = 0; k < 3; k++) r.d[k] = df * a.d[k];
+  return r;
+}
+inline Dual dsqrt(const Dual& a) { double s = std::sqrt(a.v); return chain(a, s, 0.5 / s); }
+inline Dual dexp(const Dual& a) { double e = std::exp(a.v); return chain(a, e, e); }
+inline Dual dlog(const Dual& a) { return chain(a, std::log(a.v), 1 / a.v); }
+inline Dual dcos(const Dual& a) { return chain(a, std::cos(a.v), -std::sin(a.v)); }
+inline Dual datan(const Dual& a) { return chain(a, std::atan(a.v), 1 / (1 + a.v * a.v)); }
+inline Dual dabs(const Dual& a) { return a.v < 0 ? dneg(a) : a; }
+inline Dual powc(const Dual& a, double c) { double p = std::pow(a.v, c); return chain(a, p, c * p / a.v); }
+inline Dual dmax(const Dual& a, const Dual& b) { return b.v > a.v ? b : a; }
+inline Dual dmin(const Dual& a, const Dual& b) { return b.v < a.v ? b : a; }
+
+// MOSFET level 2 and 3 parameters and derived constants (the M layout's
+// level block, device/mosfet.go:128-324 and device/mosfet_levels.go:22-55).
+struct MosLevel {
+  int level = 1;
+  double NFS = 0, XJ = 0, UCRIT = 1e4, UEXP = 0, VMAX = 0, NEFF = 1, DELTA = 0, THETA = 0, ETA = 0, KAPPA = 0, UO = 600;
+  double leff = 0, weff = 0, cox = 0, xd = 0, alpha = 0, cfs = 0, factor = 0, narrow = 0, eta3 = 0, sqrtPB = 0;
+  bool hasNSUB = false;
+};
+
+struct IdsOut {
+  double cd = 0, gm = 0, gds = 0, gmbs = 0, von = 0, vdsat = 0;
+};
+inline IdsOut fromDual(const Dual& id, const Dual& von, const Dual& vdsat) {
+  IdsOut o;
+  o.cd = id.v; o.gm = id.d[0]; o.gds = id.d[1]; o.gmbs = id.d[2]; o.von = von.v; o.vdsat = vdsat.v;
+  return o;
+}
+
+// cbrtAbs, device/mosfet_level23.go:263-270
+inline Dual cbrtAbs(Dual a) {
+  a = dabs(a);
+  if (a.v == 0) return Dual();
+  return powc(a, 1.0 / 3);
+}
+
+// level2VelocitySat, device/mosfet_level23.go:195-261: the Baum–Beneking
+// quartic for x = sqrt(vdsat + phi - vbs), Ferrari's closed form; the smallest
+// positive root with |poly| <= 1e-6.
+inline bool level2VelocitySat(const Dual& gammad, const Dual& v1, const Dual& v2, const Dual& invxv, const Dual& sarg3,
+                              Dual* xout) {
+  Dual xv = dcst(1) / invxv;
+  Dual a1 = scale(gammad, 1 / 0.75);
+  Dual b1 = scale(v1 + xv, -2);
+  Dual c1 = scale(gammad * xv, -2);
+  Dual d1 = scale(v1 * (v2 + xv), 2) - v2 * v2 - scale(gammad * sarg3, 4.0 / 3.0);
+  Dual a = dneg(b1);
+  Dual b = a1 * c1 - scale(d1, 4);
+  Dual c = dneg(d1) * (a1 * a1 - scale(b1, 4)) - c1 * c1;
+  Dual r = scale(a * a, -1.0 / 3) + b;
+  Dual s = scale(a * a * a, 2.0 / 27) - scale(a * b, 1.0 / 3) + c;
+  Dual r3 = r * r * r;
+  Dual s2 = s * s;
+  Dual pp = scale(s2, 0.25) + scale(r3, 1.0 / 27);
+  Dual p0 = dabs(pp);
+  Dual p2 = dsqrt(p0);
+  Dual y3;
+  if (pp.v < 0) {
+    Dual ro = dexp(scale(dlog(dsqrt(scale(s2, 0.25) + p0)), 1.0 / 3));
+    Dual fi = datan(scale(p2, -2) / s);
+    y3 = scale(ro * dcos(scale(fi, 1.0 / 3)), 2) - scale(a, 1.0 / 3);
+  } else {
+    y3 = cbrtAbs(scale(s, -0.5) + p2) + cbrtAbs(scale(s, -0.5) - p2) - scale(a, 1.0 / 3);
+  }
+  Dual ra = scale(a1 * a1, 0.25) - b1 + y3;
+  Dual rb = scale(y3 * y3, 0.25) - d1;
+  if (!(ra.v >= 0) || !(rb.v >= 0)) return false;
+  Dual a3 = dsqrt(ra), b3 = dsqrt(rb);
+  const double sig1[4] = {1, -1, 1, -1}, sig2[4] = {1, 1, -1, -1};
+  bool ok = false;
+  Dual x;
+  for (int i = 0; i < 4; i++) {
+    Dual a4 = scale(a1, 0.5) + scale(a3, sig1[i]);
+    Dual b4 = scale(y3, 0.5) + scale(b3, sig2[i]);
+    Dual delta4 = scale(a4 * a4, 0.25) - b4;
+    if (!(delta4.v >= 0)) continue;
+    Dual t = dsqrt(delta4);
+    Dual roots[2] = {scale(a4, -0.5) + t, scale(a4, -0.5) - t};
+    for (const Dual& rt : roots) {
+      if (!(rt.v > 0)) continue;
+      double x2 = rt.v * rt.v;
+      double poly = x2 * x2 + a1.v * x2 * rt.v + b1.v * x2 + c1.v * rt.v + d1.v;
+      if (!(std::fabs(poly) <= 1e-6)) continue;
+      if (!ok || rt.v < x.v) { x = rt; ok = true; }
+    }
+  }
+  *xout = x;
+  return ok;
+}
+
+// idsLevel2, device/mosfet_level23.go:31-193 (SPICE2 level 2, Grove–Frohman).
+inline IdsOut idsLevel2(const MosLevel& lv, double vbi, double beta, double vt, double PHI, double GAMMA,
+                        double LAMBDA, double vgsV, double vdsV, double vbsV) {
+  Dual G = dvar(vgsV, 0), D = dvar(vdsV, 1), B = dvar(vbsV, 2);
+  double phi = PHI;
+  double sphi = std::sqrt(phi);
+  double sphi3 = phi * sphi;
+  Dual phiMinVbs = addc(dneg(B), phi);
+  Dual sarg, dsrgdb;
+  if (B.v <= 0) {
+    sarg = dsqrt(phiMinVbs);
+    dsrgdb = dcst(-0.5) / sarg;
+  } else {
+    sarg = dcst(sphi) / addc(scale(B, 0.5 / phi), 1);
+    dsrgdb = scale(sarg * sarg, -0.5 / sphi3);
+  }
+  Dual barg, dbrgdb;
+  if (B.v - D.v <= 0) {
+    barg = dsqrt(phiMinVbs + D);
+    dbrgdb = dcst(-0.5) / barg;
+  } else {
+    barg = dcst(sphi) / addc(scale(B - D, 0.5 / phi), 1);
+    dbrgdb = scale(barg * barg, -0.5 / sphi3);
+  }
+  double factor = lv.factor;
+  double eta = 1 + factor;
+  Dual vbin = addc(scale(phiMinVbs, factor), vbi);
+  Dual gamasd = dcst(GAMMA), dgddvb = dcst(0);
+  if (GAMMA > 0 || lv.hasNSUB) {
+    if (lv.XJ > 0) {
+      Dual xws = scale(sarg, lv.xd), xwd = scale(barg, lv.xd);
+      Dual args = dsqrt(addc(scale(xws, 2 / lv.XJ), 1));
+      Dual argd = dsqrt(addc(scale(xwd, 2 / lv.XJ), 1));
+      double t = 0.5 * lv.XJ / lv.leff;
+      Dual argss = scale(addc(args, -1), t), argsd = scale(addc(argd, -1), t);
+      gamasd = scale(addc(dneg(argss + argsd), 1), GAMMA);
+      Dual dbargs = scale(scale(dsrgdb, lv.xd) / args, 0.5 / lv.leff);
+      Dual dbargd = scale(scale(dbrgdb, lv.xd) / argd, 0.5 / lv.leff);
+      dgddvb = scale(dbargs + dbargd, -GAMMA);
+    }
+  }
+  Dual vonD = vbin + gamasd * sarg;
+  bool nfs = lv.NFS != 0;
+  Dual xn, argg;
+  if (nfs) {
+    Dual cdonco = addc(dneg(gamasd * dsrgdb + dgddvb * sarg), factor);
+    xn = addc(cdonco, 1 + lv.cfs);
+    Dual vtxn = scale(xn, vt);
+    vonD = vonD + vtxn;
+    argg = dcst(1) / vtxn;
+  } else if (G.v <= vbin.v) {
+    IdsOut o;
+    o.von = vonD.v;
+    return o;  // cutoff
+  }
+  Dual vgst = G - vonD;
+  Dual ufact = dcst(1);
+  double crit = lv.UCRIT * 100 * EPS_SI / lv.cox;
+  if (vgst.v > crit) ufact = powc(dcst(crit) / vgst, lv.UEXP);
+  Dual ueff = scale(ufact, lv.UO * 1e-4);
+  Dual vgsx = G;
+  if (nfs) vgsx = dmax(G, vonD);
+  Dual gammad = scale(gamasd, 1 / eta);
+  Dual vgb = scale(vgsx - vbin, 1 / eta);
+  Dual vdsatD = dcst(0);
+  if (gammad.v > 0) {
+    Dual argv = vgb + phiMinVbs;
+    if (argv.v > 0) {
+      Dual gammd2 = gammad * gammad;
+      Dual arg1 = dsqrt(addc(scale(argv, 4) / gammd2, 1));
+      vdsatD = dmax(vgb + scale(gammd2 * addc(dneg(arg1), 1), 0.5), dcst(0));
+    }
+  } else {
+    vdsatD = dmax(vgb, dcst(0));
+  }
+  Dual sarg3 = sarg * sarg * sarg;
+  if (lv.VMAX > 0) {
+    Dual x;
+    if (level2VelocitySat(gammad, vgb + phiMinVbs, phiMinVbs, scale(ueff, 1 / (lv.VMAX * lv.leff)), sarg3, &x))
+      vdsatD = x * x - phiMinVbs;
+  }
+  Dual xlamda = dcst(LAMBDA);
+  Dual bsarg, bodys;
+  if (D.v != 0) {
+    if (B.v - vdsatD.v <= 0) bsarg = dsqrt(vdsatD + phiMinVbs);
+    else bsarg = dcst(sphi) / addc(scale(B - vdsatD, 0.5 / phi), 1);
+    bodys = bsarg * bsarg * bsarg - sarg3;
+    if (lv.hasNSUB && !(LAMBDA > 0)) {
+      if (lv.VMAX <= 0) {
+        Dual argv = scale(D - vdsatD, 0.25);
+        Dual sargv = dsqrt(addc(argv * argv, 1));
+        xlamda = dsqrt(argv + sargv) * (dcst(lv.xd) / scale(D, lv.leff));
+      } else {
+        double xdv = lv.xd / std::sqrt(lv.NEFF);
+        Dual xlv = dcst(lv.VMAX * xdv * 0.5) / ueff;
+        Dual argv = dmax(D - vdsatD, dcst(0));
+        Dual xls = dsqrt(xlv * xlv + argv);
+        xlamda = (xls - xlv) * (dcst(xdv) / scale(D, lv.leff));
+      }
+    }
+  }
+  double xwb = lv.xd * lv.sqrtPB;
+  double xld = lv.leff - xwb;
+  Dual clfact = addc(dneg(xlamda * D), 1);
+  if (!lv.hasNSUB) xwb = 0.25e-6;
+  Dual xleff = scale(clfact, lv.leff);
+  if (xleff.v < xwb) {
+    Dual deltal = scale(xlamda * D, lv.leff);
+    xleff = dcst(xwb) / addc(scale(addc(deltal, -xld), 1 / xwb), 1);
+    clfact = scale(xleff, 1 / lv.leff);
+  }
+  Dual beta1 = scale(ufact, beta) / clfact;
+  if (D.v <= 1e-10) {
+    Dual g;
+    if (G.v > vonD.v) g = beta1 * (G - vbin - gamasd * sarg);
+    else if (nfs) g = beta1 * (vonD - vbin - gamasd * sarg) * dexp(argg * (G - vonD));
+    IdsOut o;
+    o.gds = g.v; o.von = vonD.v; o.vdsat = vdsatD.v;
+    return o;
+  }
+  Dual id;
+  if (nfs && G.v <= vonD.v) {
+    if (vdsatD.v <= 0) {
+      IdsOut o;
+      o.von = vonD.v; o.vdsat = vdsatD.v;
+      return o;
+    }
+    Dual vdson = dmin(vdsatD, D);
+    Dual body = barg * barg * barg - sarg3;
+    if (D.v > vdsatD.v) body = bodys;
+    Dual cdson = beta1 * ((vonD - vbin - scale(vdson, 0.5 * eta)) * vdson - scale(gamasd * body, 1 / 1.5));
+    id = cdson * dexp(argg * (G - vonD));
+  } else if (D.v <= vdsatD.v) {
+    Dual body = barg * barg * barg - sarg3;
+    id = beta1 * ((G - vbin - scale(D, 0.5 * eta)) * D - scale(gamasd * body, 1 / 1.5));
+  } else {
+    id = beta1 * ((G - vbin - scale(vdsatD, 0.5 * eta)) * vdsatD - scale(gamasd * bodys, 1 / 1.5));
+  }
+  return fromDual(id, vonD, vdsatD);
+}
+
+// idsLevel3, device/mosfet_level23.go:282-378 (SPICE2 level 3, semi-empirical).
+inline IdsOut idsLevel3(const MosLevel& lv, double vbi, double beta, double vt, double PHI, double GAMMA,
+                        double LD, double vgsV, double vdsV, double vbsV) {
+  const double coeff0 = 0.0631353, coeff1 = 0.8013292, coeff2 = -0.01110777;
+  Dual G = dvar(vgsV, 0), D = dvar(vdsV, 1), B = dvar(vbsV, 2);
+  double phi = PHI;
+  double leff = lv.leff, weff = lv.weff;
+  Dual phibs, sqphbs;
+  if (B.v <= 0) {
+    phibs = addc(dneg(B), phi);
+    sqphbs = dsqrt(phibs);
+  } else {
+    sqphbs = dcst(std::sqrt(phi)) / addc(scale(B, 1 / (2 * phi)), 1);
+    phibs = sqphbs * sqphbs;
+  }
+  Dual fshort = dcst(1);
+  if (lv.XJ != 0 && lv.xd != 0) {
+    Dual wponxj = scale(sqphbs, lv.xd / lv.XJ);
+    Dual wconxj = addc(scale(wponxj * wponxj, coeff2) + scale(wponxj, coeff1), coeff0);
+    double djonxj = LD / lv.XJ;
+    Dual arga = addc(wconxj, djonxj);
+    Dual argc = wponxj / addc(wponxj, 1);
+    Dual argb = dsqrt(addc(dneg(argc * argc), 1));
+    fshort = addc(scale(addc(arga * argb, -djonxj), -lv.XJ / leff), 1);
+  }
+  Dual gammas = scale(fshort, GAMMA);
+  Dual fbodys = scale(gammas / sqphbs, 0.25);
+  Dual fbody = addc(fbodys, lv.narrow / weff);
+  Dual qbonco = gammas * sqphbs + scale(phibs, lv.narrow / weff);
+  Dual vth = addc(scale(D, -lv.eta3), vbi) + qbonco;
+  Dual vonD = vth;
+  bool nfs = lv.NFS != 0;
+  Dual xn;
+  if (nfs) {
+    xn = addc(qbonco / scale(phibs, 2), 1 + lv.cfs);
+    vonD = vth + scale(xn, vt);
+  } else if (G.v <= vonD.v) {
+    IdsOut o;
+    o.von = vonD.v;
+    return o;
+  }
+  Dual vgsx = dmax(G, vonD);
+  Dual onfg = addc(scale(vgsx - vth, lv.THETA), 1);
+  Dual fgate = dcst(1) / onfg;
+  Dual us = scale(fgate, lv.UO * 1e-4);
+  Dual onfbdy = dcst(1) / addc(fbody, 1);
+  Dual arga = (vgsx - vth) * onfbdy;
+  Dual vdsatD = arga;
+  Dual vdsc;
+  if (lv.VMAX > 0) {
+    vdsc = dcst(leff * lv.VMAX) / us;
+    Dual argb = dsqrt(arga * arga + vdsc * vdsc);
+    vdsatD = arga + vdsc - argb;
+  }
+  Dual vdsx = dmin(D, vdsatD);
+  Dual betaD = scale(fgate, beta);
+  if (vdsx.v == 0) {
+    Dual g = betaD * (vgsx - vth);
+    if (nfs && G.v < vonD.v) g = g * dexp((G - vonD) / scale(xn, vt));
+    IdsOut o;
+    o.gds = g.v; o.von = vonD.v; o.vdsat = vdsatD.v;
+    return o;
+  }
+  Dual cdo = vgsx - vth - scale(addc(fbody, 1) * vdsx, 0.5);
+  Dual id = betaD * cdo * vdsx;
+  Dual fdrain;
+  if (lv.VMAX > 0) {
+    fdrain = dcst(1) / addc(vdsx / vdsc, 1);
+    id = id * fdrain;
+  }
+  bool clm = false;
+  Dual delxl;
+  if (D.v <= vdsatD.v) {
+    if (lv.VMAX <= 0 && lv.alpha != 0) {
+      Dual r = D / vdsatD;
+      Dual r2 = r * r;
+      delxl = dsqrt(scale(vdsatD, lv.KAPPA * lv.alpha / 8)) * (r2 * r2);
+      clm = true;
+    }
+  } else if (lv.VMAX <= 0) {
+    delxl = dsqrt(scale(D - vdsatD + scale(vdsatD, 1.0 / 8), lv.KAPPA * lv.alpha));
+    clm = true;
+  } else if (lv.alpha != 0) {
+    Dual gdsat = dmax(id * addc(dneg(fdrain), 1) / vdsc, dcst(1e-12));
+    Dual emax = scale(id / gdsat, lv.KAPPA / leff);
+    Dual a = scale(emax, 0.5 * lv.alpha);
+    Dual argb = dsqrt(a * a + scale(D - vdsatD, lv.KAPPA * lv.alpha));
+    delxl = argb - a;
+    clm = true;
+  }
+  if (clm) {
+    if (delxl.v > 0.5 * leff) delxl = addc(dcst(-leff * leff / 4) / delxl, leff);
+    id = id / addc(scale(delxl, -1 / leff), 1);
+  }
+  if (nfs && G.v < vonD.v) id = id * dexp((G - vonD) / scale(xn, vt));
+  return fromDual(id, vonD, vdsatD);
+}
+
+// MOSFET levels 1-3, device/mosfet.go:43-846, device/mosfet_levels.go:79-113 and
+// device/mosfet_level23.go
 struct Mosfet : Device {
   double typ = 1, beta = 0, vbi = 0, vto = 0, GAMMA = 0, PHI = 0.6, LAMBDA = 0, gdpr = 0, gspr = 0;
   double isd = 0, iss = 0, vcritD = 0, vcritS = 0, czbd = 0, czbdsw = 0, czbs = 0, czbssw = 0;
   double PB = 0.8, MJ = 0.5, MJSW = 0.5, FC = 0.5, fbt[3] = {0, 0, 0}, fsw[3] = {0, 0, 0};
   double cgso = 0, cgdo = 0, cgbo = 0, oxideCap = 0, vt = 0;
   bool hasJcap = false, hasGcap = false;
+  MosLevel lv;
+  double LD = 0;
   int g = -1, b = -1, dp = -1, sp = -1;
   double vgs = 0, vds = 0, vbs = 0, vbd = 0, vgd = 0, vgb = 0;
   double mode = 1, cdrain = 0, gm = 0, gds = 0, gmbs = 0, von = 0, vdsat = 0;
@@ -843,9 +1220,16 @@ struct Mosfet : Device {
     *qv = q1 + q2;
     *c = c1 + c2;
   }
-  // idsLevel1, device/mosfet_levels.go:79-113
+  // ids, device/mosfet_levels.go:58-68: the level's drain current
   void ids(double a_vgs, double a_vds, double a_vbs, double* cd, double* ogm, double* ogds, double* ogmbs, double* ovon,
            double* ovdsat) const {
+    if (lv.level == 2 || lv.level == 3) {
+      IdsOut o = lv.level == 2 ? idsLevel2(lv, vbi, beta, vt, PHI, GAMMA, LAMBDA, a_vgs, a_vds, a_vbs)
+                               : idsLevel3(lv, vbi, beta, vt, PHI, GAMMA, LD, a_vgs, a_vds, a_vbs);
+      *cd = o.cd; *ogm = o.gm; *ogds = o.gds; *ogmbs = o.gmbs; *ovon = o.von; *ovdsat = o.vdsat;
+      return;
+    }
+    // idsLevel1, device/mosfet_levels.go:79-113
     double sarg;
     if (a_vbs <= 0) {
       sarg = std::sqrt(PHI - a_vbs);

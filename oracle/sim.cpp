@@ -165,6 +165,12 @@ struct Sim {
           for (int k = 0; k < 3; k++) { q->fbt[k] = p[21 + k]; q->fsw[k] = p[24 + k]; }
           q->cgso = p[27]; q->cgdo = p[28]; q->cgbo = p[29]; q->oxideCap = p[30]; q->vt = p[31];
           q->hasJcap = p[32] != 0; q->hasGcap = p[33] != 0;
+          MosLevel& lv = q->lv;
+          lv.level = (int)p[34]; lv.NFS = p[35]; lv.XJ = p[36]; lv.UCRIT = p[37]; lv.UEXP = p[38]; lv.VMAX = p[39];
+          lv.NEFF = p[40]; lv.DELTA = p[41]; lv.THETA = p[42]; lv.ETA = p[43]; lv.KAPPA = p[44]; lv.UO = p[45];
+          lv.leff = p[46]; lv.weff = p[47]; lv.cox = p[48]; lv.xd = p[49]; lv.alpha = p[50]; lv.cfs = p[51];
+          lv.factor = p[52]; lv.narrow = p[53]; lv.eta3 = p[54]; lv.sqrtPB = p[55]; lv.hasNSUB = p[56] != 0;
+          q->LD = p[57];
           q->acc.von = q->vto;  // derive, device/mosfet.go:321
           d.reset(q);
           break;

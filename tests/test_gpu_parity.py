@@ -309,7 +309,8 @@ def test_nodeset_latch_states_match_oracle():
 
 
 @pytest.mark.parametrize("name", ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc", "bjt_output_dc",
-                                  "bjt_pnp_dc"])
+                                  "bjt_pnp_dc", "mos2_cmos_vtc", "mos2_idvd_dc", "mos2_subvt_dc", "mos3_cmos_vtc",
+                                  "mos3_idvd_dc", "mos3_subvt_dc"])
 def test_dc_sweep_matches_oracle(name):
     """analysis.DC (analysis/dc.go:51-130) through kh_batch_dc: every point
     warm-started from the previous one on the same circuit state; the same
@@ -337,10 +338,11 @@ def test_dc_sweep_matches_oracle(name):
     assert host(r.ledger).max() <= 1e-9
 
 
-AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac"]
+AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac", "mos3_cs_ac"]
 # Decks whose operating point involves no transcendental function of a value
 # that differs between the device library and glibc: bit for bit. The BJT's
-# exponentials differ in the last bit: within 1e-9.
+# exponentials and the level-3 MOSFET's exponentials and powers may differ in
+# the last bit: within 1e-9.
 AC_BITEXACT = {"rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac"}
 
 
@@ -349,8 +351,9 @@ def test_ac_goldens_match_oracle(name):
     """analysis.AC (analysis/ac.go:76-193) through kh_batch_ac: operating point,
     polish, then per frequency the LoadAC stamps, the complex GP solve (CSparse:
     cmplx.Abs pivoting, Go's complex division) and the complex-Tellegen ledger.
-    Every frequency's solution matches the oracle's bit for bit (the BJT deck:
-    within 1e-9, its exponentials being the device library's)."""
+    Every frequency's solution matches the oracle's bit for bit (the BJT and
+    level-3 decks: within 1e-9, their exponentials being the device
+    library's)."""
     from kronspice_amd import netlist
     deck = gr.load_deck(name)
     c = deck.circuit

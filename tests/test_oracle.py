@@ -320,7 +320,8 @@ def test_nodeset_selects_latch_state():
     assert ia["ledger_rel"] <= 1e-12 and ib["ledger_rel"] <= 1e-12
 
 
-DC_DECKS = ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc", "bjt_output_dc", "bjt_pnp_dc"]
+DC_DECKS = ["diode_clipper_dc", "mos1_idvd_dc", "mos1_cmos_vtc", "zener_dc", "bjt_output_dc", "bjt_pnp_dc",
+            "mos2_subvt_dc", "mos3_idvd_dc"]
 
 
 @pytest.mark.parametrize("name", DC_DECKS)
@@ -397,16 +398,17 @@ class _MutatedBackend(OracleBackend):
     mistake (a dropped term or a swapped constant) that the reference's
     goldens must expose."""
 
-    def __init__(self, index, value):
+    def __init__(self, index, value, kind=kc.Q):
         super().__init__(0)
-        self.index, self.value = index, value
+        self.index, self.value, self.kind = index, value, kind
 
     def _mut(self, c, step=0.0, stop=0.0):
         els, p = c.flatten(step, stop)
         p = p.copy()
         for e in els:
-            if e["kind"] == kc.Q:
-                p[e["poff"] + self.index] = self.value(p[e["poff"] + self.index], p[e["poff"]:e["poff"] + kc.NPAR[kc.Q]])
+            if e["kind"] == self.kind:
+                blk = p[e["poff"]:e["poff"] + kc.NPAR[self.kind]]
+                p[e["poff"] + self.index] = self.value(p[e["poff"] + self.index], blk)
         return els, p
 
     def op(self, c):
@@ -443,5 +445,27 @@ def test_bjt_goldens_expose_planted_mistakes(deck, index, value, what):
     try:
         msgs = gr.check(deck, _MutatedBackend(index, value))
     except Exception as ex:  # a mutation may also break the analysis outright
+        msgs = [repr(ex)]
+    assert msgs, f"{deck}: golden does not notice: {what}"
+
+
+@pytest.mark.parametrize("deck,index,value,what", [
+    ("mos2_subvt_dc", 35, lambda v, p: 0.0, "weak inversion (NFS) dropped"),
+    ("mos2_cmos_vtc", 39, lambda v, p: 0.0, "velocity saturation (VMAX) dropped"),
+    ("mos2_idvd_dc", 52, lambda v, p: 0.0, "narrow-channel factor dropped"),
+    ("mos2_subvt_dc", 37, lambda v, p: v * 0.5, "critical field (UCRIT) halved"),
+    ("mos3_subvt_dc", 54, lambda v, p: 0.0, "static feedback (ETA) dropped"),
+    ("mos3_idvd_dc", 42, lambda v, p: 0.0, "mobility modulation (THETA) dropped"),
+    ("mos3_idvd_dc", 44, lambda v, p: 0.2, "KAPPA replaced by its default"),
+    ("mos3_cs_ac", 53, lambda v, p: 0.0, "narrow-channel factor dropped"),
+    ("mos3_inv_tran", 39, lambda v, p: 0.0, "velocity saturation (VMAX) dropped"),
+])
+def test_mos23_goldens_expose_planted_mistakes(deck, index, value, what):
+    """Each golden of MOSFET levels 2 and 3 notices a planted mistake in the
+    level's equations (a dropped term or a wrong constant)."""
+    assert not gr.check(deck, OracleBackend()), "the unaltered oracle must pass"
+    try:
+        msgs = gr.check(deck, _MutatedBackend(index, value, kind=kc.M))
+    except Exception as ex:
         msgs = [repr(ex)]
     assert msgs, f"{deck}: golden does not notice: {what}"
