@@ -407,7 +407,7 @@ def test_ac_rc_lowpass_batch_closed_form():
 
 
 # ---------------------------------------------------------------------------- backends
-@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "mos1_cmos_vtc", "bjt_ce_op", "zener_dc"])
+@pytest.mark.parametrize("name", ["bridge", "diode_fwd", "mos1_op", "mos1_cmos_vtc", "bjt_ce_op", "mos2_subvt_dc"])
 def test_op_goldens_forced_sparse(name):
     """Options.Backend = SparseLU on circuits Auto would give the dense LU:
     the GP path against the oracle's sparse restatement, same status and
