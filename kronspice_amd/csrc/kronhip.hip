@@ -318,6 +318,12 @@ using LuWork = LuWorkT<double>;
 
 int64_t lu_capacity(int64_t max_fill, int n) { return 2 * max_fill + 4 * (int64_t)n + 64; }
 
+// The batched analyses start L and U at the structural fill itself (each of
+// L and U holds at most |L| + |U| + n entries under diagonal pivoting): a
+// circuit's working set stays small (cache and LDS footprint), and the rare
+// circuit whose pivoting fills more waits for the capacity to double.
+int64_t lu_capacity_batch(int64_t max_fill, int n) { return max_fill + n + 16; }
+
 // Pattern pool mirrored on the device.
 struct DevPatterns {
   PatternPool host;
@@ -812,7 +818,7 @@ struct GroupRun {
     for (int64_t c = 0; c < B; c++) wait[c] = 0;
     stats.patterns = (int64_t)pats.host.desc.size();
     KH_HIP(pats.sync());
-    int64_t cap = std::max(work.capL, lu_capacity(pats.host.max_fill, T.n));
+    int64_t cap = std::max(work.capL, lu_capacity_batch(pats.host.max_fill, T.n));
     if (grow) cap = std::max(cap, 2 * work.capL);
     KH_HIP(work.ensure(T.n, B, cap));
     KH_HIP(hipMemcpy(pid.p, hpid.data(), (size_t)B * 4, hipMemcpyHostToDevice));
