@@ -299,6 +299,27 @@ kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const 
                         int32_t* d_status, int32_t* d_accepted, int32_t* d_rejected,
                         double* d_ledger, kh_stats* stats);
 
+/* ---- learned Newton warm start --------------------------------------------
+ * No counterpart in the reference: the small per-topology network that
+ * predicts a circuit's DC node voltages from its value genes; its predictions
+ * are Options.NodeSet guesses (kh_group.d_nodeset), and Newton with its
+ * fallbacks stays the judge. Layers in -> hidden -> hidden -> out with ReLU;
+ * bf16 weights and activations into FP32-accumulating matrix-core products,
+ * FP32 master weights, biases and Adam moments. Inputs d_x [B][n_in] and
+ * targets d_t [B][n_out] are FP32 on the device. */
+typedef struct kh_mlp_s* kh_mlp;
+kh_status kh_mlp_create(int32_t n_in, int32_t n_hidden, int32_t n_out, uint64_t seed, kh_mlp* out);
+kh_status kh_mlp_forward(kh_mlp m, int32_t B, const float* d_x, float* d_y);          /* d_y [B][n_out] */
+kh_status kh_mlp_gradients(kh_mlp m, int32_t B, const float* d_x, const float* d_t,
+                           float* loss);    /* mean squared error and its gradients (held) */
+kh_status kh_mlp_adam_step(kh_mlp m, float lr);                                       /* apply the held gradients */
+/* Parameters (which = 0) or held gradients (1) as one host array in the order
+ * W1 [n_in][hidden], b1, W2 [hidden][hidden], b2, W3 [hidden][n_out], b3;
+ * to_device = 1 writes parameters instead of reading them. */
+kh_status kh_mlp_copy(kh_mlp m, int32_t which, float* host, int64_t count, int32_t to_device);
+const char* kh_mlp_last_error(void);
+void kh_mlp_destroy(kh_mlp m);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,14 @@ def lib():
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_dc.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_ac.argtypes = [P, i32, P, P, P, i32, P, P, P, P, P, P]
+        L.kh_mlp_create.argtypes = [i32, i32, i32, C.c_uint64, P]
+        L.kh_mlp_forward.argtypes = [P, i32, P, P]
+        L.kh_mlp_gradients.argtypes = [P, i32, P, P, P]
+        L.kh_mlp_adam_step.argtypes = [P, C.c_float]
+        L.kh_mlp_copy.argtypes = [P, i32, P, i64, i32]
+        L.kh_mlp_last_error.restype = C.c_char_p
+        L.kh_mlp_destroy.argtypes = [P]
+        L.kh_mlp_destroy.restype = None
         _lib = L
         st = L.kh_init()
         if st != OK:
@@ -433,3 +441,59 @@ def tran(topo: Topology, params, spec: TranSpec, opts: Optional[Options] = None,
     _check(lib().kh_batch_tran(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(lane_spec), _p(x), _p(st),
                                _p(acc), _p(rej), _p(led), C.byref(s)))
     return Result(x, st, accepted=acc, rejected=rej, ledger=led, stats=s.as_dict())
+
+
+class MLP:
+    """The learned warm start's network on the GPU (kh_mlp_*): n_in -> hidden ->
+    hidden -> n_out, ReLU, bf16 matrix-core products with FP32 accumulation,
+    Adam on FP32 master weights."""
+
+    def __init__(self, n_in: int, n_out: int, hidden: int = 256, seed: int = 1):
+        self.n_in, self.n_out, self.hidden = n_in, n_out, hidden
+        h = C.c_void_p()
+        st = lib().kh_mlp_create(n_in, hidden, n_out, seed, C.byref(h))
+        if st != OK:
+            raise KhError(st, lib().kh_mlp_last_error().decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.kh_mlp_destroy(self.h)
+            self.h = None
+
+    def _check(self, st):
+        if st != OK:
+            raise KhError(st, lib().kh_mlp_last_error().decode())
+
+    @property
+    def count(self) -> int:
+        h = self.hidden
+        return self.n_in * h + h + h * h + h + h * self.n_out + self.n_out
+
+    def forward(self, x):
+        """x [B, n_in] float32 device tensor -> [B, n_out]."""
+        torch = _torch()
+        x = x.contiguous()
+        y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
+        self._check(lib().kh_mlp_forward(self.h, x.shape[0], _p(x), _p(y)))
+        return y
+
+    def gradients(self, x, t) -> float:
+        """Mean squared error of the prediction against t [B, n_out]; the
+        gradients are held for adam_step / parameters(grads=True)."""
+        loss = C.c_float()
+        x, t = x.contiguous(), t.contiguous()
+        self._check(lib().kh_mlp_gradients(self.h, x.shape[0], _p(x), _p(t), C.byref(loss)))
+        return loss.value
+
+    def adam_step(self, lr: float):
+        self._check(lib().kh_mlp_adam_step(self.h, lr))
+
+    def parameters(self, grads: bool = False) -> np.ndarray:
+        out = np.zeros(self.count, np.float32)
+        self._check(lib().kh_mlp_copy(self.h, 1 if grads else 0, _p(out), out.size, 0))
+        return out
+
+    def set_parameters(self, values: np.ndarray):
+        v = np.ascontiguousarray(values, np.float32)
+        self._check(lib().kh_mlp_copy(self.h, 0, _p(v), v.size, 1))

@@ -254,6 +254,71 @@ def mosfet_chains(B: int = 16384, patterns: int = 200, seed: int = 3, steps: int
     return [mosfet_chain_batch(p, per[p], seed, steps) for p in range(patterns) if per[p] > 0]
 
 
+# ---------------------------------------------------------------------------- config 4
+AMP_GENES = 60
+AMP_LADDERS = (63, 63, 73)  # sections of the bias, inter-stage and load ladders (200 nodes in all)
+
+
+def bjt_amplifier_circuit(seed: int = 4) -> Tuple[kc.Circuit, List[List[int]]]:
+    """The learned-warm-start topology: a two-stage common-emitter BJT amplifier
+    (BF U(50, 300), IS log-U(1e-16, 1e-14) A, drawn once from the seed) with a
+    resistive ladder from the supply to the first base (bias), a resistive
+    ladder from the first collector to the second base (DC coupling) and a
+    resistive ladder as the second stage's load: 200 nodes. Returns the circuit
+    and, per gene, the resistor elements it scales (genes 0-3: the stages' RC and
+    RE; genes 4-59: groups of consecutive ladder sections)."""
+    one = np.arange(1)
+    bf = [lin_uniform(seed, one, 900 + k, 50.0, 300.0)[0] for k in range(2)]
+    is_ = [log_uniform(seed, one, 910 + k, 1e-16, 1e-14)[0] for k in range(2)]
+    b = kc.Builder("bjt two-stage amplifier")
+    b.vsource("VCC", "vcc", "0", 12.0)
+    groups: List[List[int]] = [[] for _ in range(AMP_GENES)]
+
+    def res(name, a, c, r, gene):
+        b.resistor(name, a, c, r)
+        groups[gene].append(len(b.elems) - 1)
+
+    def ladder(tag, start, end, sections, rs, rp, gene0, ngenes):
+        prev = start
+        for k in range(sections):
+            node = end if (k == sections - 1 and end is not None) else f"{tag}{k + 1}"
+            g = gene0 + (k * ngenes) // sections
+            res(f"R{tag}s{k + 1}", prev, node, rs, g)
+            res(f"R{tag}p{k + 1}", node, "0", rp, g)
+            prev = node
+        return prev
+
+    n1, n2, n3 = AMP_LADDERS
+    ladder("a", "vcc", "b1", n1, 300.0, 3.9e5, 4, 18)
+    res("RC1", "vcc", "c1", 4.7e3, 0)
+    res("RE1", "e1", "0", 1.0e3, 1)
+    b.bjt("Q1", "c1", "b1", "e1", "qa")
+    ladder("b", "c1", "b2", n2, 200.0, 1.0e6, 22, 19)
+    res("RC2", "vcc", "c2", 2.2e3, 2)
+    res("RE2", "e2", "0", 1.5e3, 3)
+    b.bjt("Q2", "c2", "b2", "e2", "qb")
+    ladder("c", "c2", None, n3, 100.0, 4.7e5, 41, 19)
+    for k, name in enumerate(("qa", "qb")):
+        b.model(name, "npn", {"is": is_[k], "bf": bf[k], "vaf": 80.0, "rb": 50.0, "re": 0.5, "rc": 5.0})
+    return b.build(), groups
+
+
+def bjt_amplifiers(B: int = 65536, seed: int = 4, first_circuit: int = 0) -> Tuple[Batch, np.ndarray]:
+    """B draws of the amplifier's 60 genes, each log-uniform over one decade
+    either side of its nominal value (the scale factor of its resistors).
+    Returns the batch (DC OP) and the genes as log10 scale factors [60, B]
+    in [-1, 1] (the network's input)."""
+    c, groups = bjt_amplifier_circuit(seed)
+    els, params = _template(c, B)
+    cs = np.arange(first_circuit, first_circuit + B)
+    genes = np.stack([lin_uniform(seed, cs, f, -1.0, 1.0) for f in range(AMP_GENES)])
+    for g, idx in enumerate(groups):
+        for k in idx:
+            r = params[els[k]["poff"], 0] * 10.0 ** genes[g]
+            _set_resistors(els, params, [k], r[None, :])
+    return Batch("bjt_amplifier_dc_op", c, els, params, c.layout().n), genes
+
+
 def circuit_of(batch: Batch, c: int) -> Tuple[np.ndarray, np.ndarray]:
     """Element table and parameter vector of circuit c of a batch."""
     return batch.elements, batch.circuit_params(c)
