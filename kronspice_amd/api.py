@@ -108,14 +108,6 @@ def lib():
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_dc.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_ac.argtypes = [P, i32, P, P, P, i32, P, P, P, P, P, P]
-        L.kh_mlp_create.argtypes = [i32, i32, i32, C.c_uint64, P]
-        L.kh_mlp_forward.argtypes = [P, i32, P, P]
-        L.kh_mlp_gradients.argtypes = [P, i32, P, P, P]
-        L.kh_mlp_adam_step.argtypes = [P, C.c_float]
-        L.kh_mlp_copy.argtypes = [P, i32, P, i64, i32]
-        L.kh_mlp_last_error.restype = C.c_char_p
-        L.kh_mlp_destroy.argtypes = [P]
-        L.kh_mlp_destroy.restype = None
         _lib = L
         st = L.kh_init()
         if st != OK:
@@ -443,6 +435,27 @@ def tran(topo: Topology, params, spec: TranSpec, opts: Optional[Options] = None,
     return Result(x, st, accepted=acc, rejected=rej, ledger=led, stats=s.as_dict())
 
 
+_mlp_bound = False
+
+
+def _mlp_lib():
+    """The library with the kh_mlp_* signatures bound."""
+    global _mlp_bound
+    L = lib()
+    if not _mlp_bound:
+        P, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        L.kh_mlp_create.argtypes = [i32, i32, i32, C.c_uint64, P]
+        L.kh_mlp_forward.argtypes = [P, i32, P, P]
+        L.kh_mlp_gradients.argtypes = [P, i32, P, P, P]
+        L.kh_mlp_adam_step.argtypes = [P, C.c_float]
+        L.kh_mlp_copy.argtypes = [P, i32, P, i64, i32]
+        L.kh_mlp_last_error.restype = C.c_char_p
+        L.kh_mlp_destroy.argtypes = [P]
+        L.kh_mlp_destroy.restype = None
+        _mlp_bound = True
+    return L
+
+
 class MLP:
     """The learned warm start's network on the GPU (kh_mlp_*): n_in -> hidden ->
     hidden -> n_out, ReLU, bf16 matrix-core products with FP32 accumulation,
@@ -451,9 +464,9 @@ class MLP:
     def __init__(self, n_in: int, n_out: int, hidden: int = 256, seed: int = 1):
         self.n_in, self.n_out, self.hidden = n_in, n_out, hidden
         h = C.c_void_p()
-        st = lib().kh_mlp_create(n_in, hidden, n_out, seed, C.byref(h))
+        st = _mlp_lib().kh_mlp_create(n_in, hidden, n_out, seed, C.byref(h))
         if st != OK:
-            raise KhError(st, lib().kh_mlp_last_error().decode())
+            raise KhError(st, _mlp_lib().kh_mlp_last_error().decode())
         self.h = h
 
     def __del__(self):
@@ -463,7 +476,7 @@ class MLP:
 
     def _check(self, st):
         if st != OK:
-            raise KhError(st, lib().kh_mlp_last_error().decode())
+            raise KhError(st, _mlp_lib().kh_mlp_last_error().decode())
 
     @property
     def count(self) -> int:
@@ -475,7 +488,7 @@ class MLP:
         torch = _torch()
         x = x.contiguous()
         y = torch.empty((x.shape[0], self.n_out), dtype=torch.float32, device=x.device)
-        self._check(lib().kh_mlp_forward(self.h, x.shape[0], _p(x), _p(y)))
+        self._check(_mlp_lib().kh_mlp_forward(self.h, x.shape[0], _p(x), _p(y)))
         return y
 
     def gradients(self, x, t) -> float:
@@ -483,17 +496,17 @@ class MLP:
         gradients are held for adam_step / parameters(grads=True)."""
         loss = C.c_float()
         x, t = x.contiguous(), t.contiguous()
-        self._check(lib().kh_mlp_gradients(self.h, x.shape[0], _p(x), _p(t), C.byref(loss)))
+        self._check(_mlp_lib().kh_mlp_gradients(self.h, x.shape[0], _p(x), _p(t), C.byref(loss)))
         return loss.value
 
     def adam_step(self, lr: float):
-        self._check(lib().kh_mlp_adam_step(self.h, lr))
+        self._check(_mlp_lib().kh_mlp_adam_step(self.h, lr))
 
     def parameters(self, grads: bool = False) -> np.ndarray:
         out = np.zeros(self.count, np.float32)
-        self._check(lib().kh_mlp_copy(self.h, 1 if grads else 0, _p(out), out.size, 0))
+        self._check(_mlp_lib().kh_mlp_copy(self.h, 1 if grads else 0, _p(out), out.size, 0))
         return out
 
     def set_parameters(self, values: np.ndarray):
         v = np.ascontiguousarray(values, np.float32)
-        self._check(lib().kh_mlp_copy(self.h, 0, _p(v), v.size, 1))
+        self._check(_mlp_lib().kh_mlp_copy(self.h, 0, _p(v), v.size, 1))

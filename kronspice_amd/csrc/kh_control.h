@@ -106,7 +106,7 @@ inline namespace KH_CAT(kinds_, KH_KINDS) {
 // words are worked on in place, so they live wherever the circuit's arrays do
 // (device memory, or LDS when the launch stages the circuit).
 struct Ctl {
-  const CtlArgs& A;
+  const CtlArgs A;  // a copy: one indirection less per array access (the object lives in private memory)
   int n;
   int32_t* ci;  // [NCI]
   double* cd;   // [NCD]
@@ -137,29 +137,36 @@ struct Ctl {
 
   // ---- device dispatch -----------------------------------------------------
   KH_HD void load_all() {
-    L.mode = ci[CI_MODE];
-    L.method = ci[CI_METHOD];
-    L.time = cd[CD_TIME];
-    L.H = cd[CD_H];
-    L.gmin = cd[CD_GMIN];
-    L.srcFact = cd[CD_SRCFACT];
-    L.initJct = ci[CI_INITJCT] != 0;
-    L.srcStepping = ci[CI_SRCSTEP] != 0;
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    // A local copy of the device view: its fields (array bases, the load
+    // context, the pattern-growth flag) stay in registers while the elements
+    // stamp, instead of being read and written through this object.
+    Lane lane = L;
+    lane.mode = ci[CI_MODE];
+    lane.method = ci[CI_METHOD];
+    lane.time = cd[CD_TIME];
+    lane.H = cd[CD_H];
+    lane.gmin = cd[CD_GMIN];
+    lane.srcFact = cd[CD_SRCFACT];
+    lane.initJct = ci[CI_INITJCT] != 0;
+    lane.srcStepping = ci[CI_SRCSTEP] != 0;
+    const int nel = A.ckt.nel;
+    const int32_t* et = A.ckt.et;
+    for (int k = 0; k < nel; k++) {
+      const int32_t* e = et + (int64_t)k * ET_W;
       switch (e[ET_KIND]) {
-        case K_R: load_R(L, e); break;
-        case K_C: load_C(L, e); break;
-        case K_L: load_L(L, e); break;
-        case K_V: load_V(L, e); break;
-        case K_I: load_I(L, e); break;
-        case K_D: if (KH_HAS(K_D)) load_D(L, e); break;
-        case K_M: if (KH_HAS(K_M)) load_M(L, e); break;
-        case K_Q: if (KH_HAS(K_Q)) load_Q(L, e); break;
+        case K_R: load_R(lane, e); break;
+        case K_C: load_C(lane, e); break;
+        case K_L: load_L(lane, e); break;
+        case K_V: load_V(lane, e); break;
+        case K_I: load_I(lane, e); break;
+        case K_D: if (KH_HAS(K_D)) load_D(lane, e); break;
+        case K_M: if (KH_HAS(K_M)) load_M(lane, e); break;
+        case K_Q: if (KH_HAS(K_Q)) load_Q(lane, e); break;
       }
     }
+    L = lane;
   }
-  KH_HD OpPt operating(const int32_t* e, const double* x) {
+  KH_HD OpPt operating(const Lane& L, const int32_t* e, const double* x) const {
     switch (e[ET_KIND]) {
       case K_R: {
         OpPt o;
@@ -181,42 +188,54 @@ struct Ctl {
     return o;
   }
   KH_HD void accept(double t, int mode) {  // Sim.accept, analysis/op.go:324-328
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
       switch (e[ET_KIND]) {
-        case K_C: accept_C(L, e, A.x, t, mode); break;
-        case K_L: accept_L(L, e, A.x, t, mode); break;
-        case K_D: if (KH_HAS(K_D)) accept_D(L, e, A.x, t, mode); break;
-        case K_M: if (KH_HAS(K_M)) accept_M(L, e, A.x, t, mode); break;
-        case K_Q: if (KH_HAS(K_Q)) accept_Q(L, e, A.x, t, mode); break;
+        case K_C: accept_C(L, e, x_, t, mode); break;
+        case K_L: accept_L(L, e, x_, t, mode); break;
+        case K_D: if (KH_HAS(K_D)) accept_D(L, e, x_, t, mode); break;
+        case K_M: if (KH_HAS(K_M)) accept_M(L, e, x_, t, mode); break;
+        case K_Q: if (KH_HAS(K_Q)) accept_Q(L, e, x_, t, mode); break;
       }
     }
   }
   KH_HD bool converged() {  // analysis/newton.go:59-82
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
       if (KH_HAS(K_D) && e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
       if (KH_HAS(K_M) && e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
       if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && L.S(e, QS_LIM) != 0) return false;
     }
     for (int i = 0; i < n; i++) {
-      double xv = X(A.x, i), old = X(A.xold, i);
+      double xv = X(x_, i), old = X(A.xold, i);
       double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
       tol += P.relTol * fmax(fabs(xv), fabs(old));
       if (fabs(xv - old) > tol) return false;
     }
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
-      if (KH_HAS(K_D) && e[ET_KIND] == K_D && !conv_D(L, e, A.x, P.relTol, P.absTol)) return false;
-      if (KH_HAS(K_M) && e[ET_KIND] == K_M && !conv_M(L, e, A.x, P.relTol, P.absTol)) return false;
-      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && !conv_Q(L, e, A.x, P.relTol, P.absTol)) return false;
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
+      if (KH_HAS(K_D) && e[ET_KIND] == K_D && !conv_D(L, e, x_, P.relTol, P.absTol)) return false;
+      if (KH_HAS(K_M) && e[ET_KIND] == K_M && !conv_M(L, e, x_, P.relTol, P.absTol)) return false;
+      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && !conv_Q(L, e, x_, P.relTol, P.absTol)) return false;
     }
     return true;
   }
   KH_HD double next_breakpoint(double t) {  // analysis/tran.go:233-243
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
     double nb = kInf;
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
       if (e[ET_KIND] == K_V || e[ET_KIND] == K_I) {
         if ((int)L.P(e, 2) == 0) continue;
         double b = wave_next_breakpoint(L, e, t);
@@ -226,19 +245,23 @@ struct Ctl {
     return nb;
   }
   KH_HD double truncate(double tn, double h, int order) {
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
     TruncCtx tc;
     tc.time = tn; tc.H = h; tc.method = order;
     tc.relTol = P.relTol; tc.absTol = P.absTol; tc.vnTol = P.vnTol; tc.chgTol = P.chgTol; tc.trTol = P.trTol;
     double prop = kInf;
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
       double pk = kInf;
       switch (e[ET_KIND]) {
-        case K_C: pk = trunc_C(L, e, A.x, tc); break;
-        case K_L: pk = trunc_L(L, e, A.x, tc); break;
-        case K_D: if (KH_HAS(K_D)) pk = trunc_D(L, e, A.x, tc); break;
-        case K_M: if (KH_HAS(K_M)) pk = trunc_M(L, e, A.x, tc); break;
-        case K_Q: if (KH_HAS(K_Q)) pk = trunc_Q(L, e, A.x, tc); break;
+        case K_C: pk = trunc_C(L, e, x_, tc); break;
+        case K_L: pk = trunc_L(L, e, x_, tc); break;
+        case K_D: if (KH_HAS(K_D)) pk = trunc_D(L, e, x_, tc); break;
+        case K_M: if (KH_HAS(K_M)) pk = trunc_M(L, e, x_, tc); break;
+        case K_Q: if (KH_HAS(K_Q)) pk = trunc_Q(L, e, x_, tc); break;
         default: continue;
       }
       if (pk < prop) prop = pk;
@@ -248,24 +271,36 @@ struct Ctl {
   // roundoff, analysis/op.go:275-282: AbsProduct (solver/sparse.go:481-490)
   // accumulated over the stamped slots in slot order, then the weighted sum.
   KH_HD double roundoff() {
-    for (int i = 0; i < n; i++) X(A.absbuf, i) = 0;
-    for (int k = 0; k < A.ckt.nslots; k++) {
-      const int s = A.ckt.order ? A.ckt.order[k] : k;
-      int64_t ix = s;
-      if (!A.present[ix]) continue;
-      X(A.absbuf, A.ckt.slot_i[s]) += fabs(A.val[ix]) * fabs(X(A.x, A.ckt.slot_j[s]));
+    double* const ab = A.absbuf;
+    const double* const x = A.x;
+    const double* const val = A.val;
+    const uint8_t* const present = A.present;
+    const int32_t* const si = A.ckt.slot_i;
+    const int32_t* const sj = A.ckt.slot_j;
+    const int32_t* const order = A.ckt.order;
+    const int ns = A.ckt.nslots;
+    for (int i = 0; i < n; i++) ab[i] = 0;
+    for (int k = 0; k < ns; k++) {
+      const int s = order ? order[k] : k;
+      if (!present[s]) continue;
+      ab[si[s]] += fabs(val[s]) * fabs(x[sj[s]]);
     }
+    const double* const rhs = A.rhs;
     double sum = 0;
-    for (int i = 0; i < n; i++) sum += fabs(X(A.x, i)) * (X(A.absbuf, i) + fabs(X(A.rhs, i)));
+    for (int i = 0; i < n; i++) sum += fabs(x[i]) * (ab[i] + fabs(rhs[i]));
     return 64 * 0x1p-52 * sum;
   }
   // Ledger of an operating point: BuildLedger + RelativeResidual
   // (em/energy.go:90-119, output/result.go:100-109).
   KH_HD double ledger_op() {
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
     double inflow = 0, outflow = 0, diss = 0, stor = 0, sup = 0, scale = 0;
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
-      OpPt o = operating(e, A.x);
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
+      OpPt o = operating(L, e, x_);
       double pw = 0;
       int ch = -1;
       int kind = e[ET_KIND];
@@ -298,15 +333,19 @@ struct Ctl {
   }
   // recorder.record's per-point check (analysis/tran.go:291-347).
   KH_HD int record(double t, bool imposed) {
+    const Lane L = this->L;  // register copy of the device view (see load_all)
+    const int nel_ = A.ckt.nel;
+    const int32_t* const et_ = A.ckt.et;
+    double* const x_ = A.x;
     double inflow = 0, outflow = 0, diss = 0, stor = 0, sup = 0, scale = 0;
     // ports of every element, in element order
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    for (int k = 0; k < nel_; k++) {
+      const int32_t* e = et_ + (int64_t)k * ET_W;
       int kind = e[ET_KIND];
       double pw = 0;
       int ch = -1;
       if (kind == K_R || kind == K_V || kind == K_I || kind == K_D || kind == K_M || kind == K_Q) {
-        OpPt o = operating(e, A.x);
+        OpPt o = operating(L, e, x_);
         if (kind == K_R) { if (L.P(e, 0) > 0) { pw = -o.power; ch = 2; } }
         else if (kind == K_V || kind == K_I) { pw = -o.power; ch = 0; }
         else { pw = -(o.power - o.storage); ch = 2; }
@@ -321,7 +360,7 @@ struct Ctl {
     for (int k = 0; k < A.ckt.nel; k++) {  // model-supplied (negative resistors)
       const int32_t* e = el(k);
       if (e[ET_KIND] == K_R && L.P(e, 0) < 0) {
-        OpPt o = operating(e, A.x);
+        OpPt o = operating(L, e, x_);
         double ms = -o.power;
         if (ms != 0) { sup += ms; scale += fabs(ms); }
       }
@@ -330,7 +369,7 @@ struct Ctl {
       const int32_t* e = el(k);
       int kind = e[ET_KIND];
       if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) {
-        OpPt o = operating(e, A.x);
+        OpPt o = operating(L, e, x_);
         stor += o.storage;
         scale += fabs(o.storage);
       }

@@ -172,7 +172,7 @@ int main(int argc, char** argv) {
   kh_stats st;
   for (int w = 0; w < warmup; w++) CHECK_KH(kh_batch_run(ngroups, groups.data(), &opt, sp, &st));
   CHECK_HIP(hipDeviceSynchronize());
-  double lu_s = 0, bytes = 0;
+  double lu_s = 0, bytes = 0, cyc_lu = 0, cyc_ctl = 0;
   int64_t solves = 0, launches = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (int s = 0; s < steps; s++) {
@@ -181,6 +181,8 @@ int main(int argc, char** argv) {
     bytes += (double)st.bytes_algorithmic;
     solves += st.solves;
     launches += st.lu_launches;
+    cyc_lu += st.cycles_lu;
+    cyc_ctl += st.cycles_control;
   }
   CHECK_HIP(hipDeviceSynchronize());
   const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -198,9 +200,10 @@ int main(int argc, char** argv) {
          "\"steps\": %d, "
          "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
-         "\"algorithmic_GBps\": %.4f, \"status_histogram\": %s}\n",
+         "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"status_histogram\": %s}\n",
          (long long)circuits, ngroups, staging, cpw, waves, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
-         (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, h.c_str());
+         (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, cyc_lu / steps, cyc_ctl / steps,
+         h.c_str());
   for (void* p : owned) (void)hipFree(p);
   for (auto t : topo) kh_topology_destroy(t);
   return 0;
