@@ -23,7 +23,7 @@ from . import api
 
 # (staging, circuits per wavefront, resident wavefronts per SIMD) candidates.
 CANDIDATES: Tuple[Tuple[int, int, int], ...] = tuple(
-    (s, c, w) for w in (1, 2, 4) for c in (1, 2, 4, 8, 16, 32) for s in (0, 1) if not (s == 1 and c > 4))
+    (s, c, w) for w in (1, 2, 4) for c in (1, 2, 4, 8, 16, 32) for s in (0, 1) if not (s == 1 and c > 16))
 
 
 @dataclass

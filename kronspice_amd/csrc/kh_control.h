@@ -67,6 +67,10 @@ struct Circ {
   const int32_t* diag_slot; // [n] slot of (i, i)
   int32_t nonlin;           // any nonlinear device
   int32_t dense;            // solved by the dense backend (lu_dense): no sparsity patterns
+  int32_t n_nl, n_rx, n_src;
+  const int32_t* nl;        // [n_nl] nonlinear devices (D, M, Q), element order
+  const int32_t* rx;        // [n_rx] devices with charge or flux (C, L, D, M, Q)
+  const int32_t* src;       // [n_src] sources (V, I)
   const int32_t* order;     // [nslots] slot order of AbsProduct: row-major for the dense
                             // backend (solver/dense.go:63-73), null: slot order
 };
@@ -192,8 +196,9 @@ struct Ctl {
     const int nel_ = A.ckt.nel;
     const int32_t* const et_ = A.ckt.et;
     double* const x_ = A.x;
-    for (int k = 0; k < nel_; k++) {
-      const int32_t* e = et_ + (int64_t)k * ET_W;
+    (void)nel_;
+    for (int q = 0; q < A.ckt.n_rx; q++) {
+      const int32_t* e = et_ + (int64_t)A.ckt.rx[q] * ET_W;
       switch (e[ET_KIND]) {
         case K_C: accept_C(L, e, x_, t, mode); break;
         case K_L: accept_L(L, e, x_, t, mode); break;
@@ -204,27 +209,38 @@ struct Ctl {
     }
   }
   KH_HD bool converged() {  // analysis/newton.go:59-82
+    // Every test is free of side effects, so the result is the conjunction of
+    // the reference's sequence of tests: no device limited, every unknown
+    // within tolerance, every nonlinear device's convergence test. The unknown
+    // test runs without early exits (independent loads overlap); the device
+    // loops visit only the nonlinear devices.
     const Lane L = this->L;  // register copy of the device view (see load_all)
-    const int nel_ = A.ckt.nel;
     const int32_t* const et_ = A.ckt.et;
+    const int32_t* const nl = A.ckt.nl;
+    const int n_nl = A.ckt.n_nl;
     double* const x_ = A.x;
-    for (int k = 0; k < nel_; k++) {
-      const int32_t* e = et_ + (int64_t)k * ET_W;
+    for (int q = 0; q < n_nl; q++) {
+      const int32_t* e = et_ + (int64_t)nl[q] * ET_W;
       if (KH_HAS(K_D) && e[ET_KIND] == K_D && L.S(e, 6) != 0) return false;
       if (KH_HAS(K_M) && e[ET_KIND] == K_M && L.S(e, MS_LIM) != 0) return false;
       if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && L.S(e, QS_LIM) != 0) return false;
     }
+    const double* const xo = A.xold;
+    const int32_t* const isV = A.ckt.isV;
+    const double vnTol = P.vnTol, absTol = P.absTol, relTol = P.relTol;
+    bool ok = true;
     for (int i = 0; i < n; i++) {
-      double xv = X(x_, i), old = X(A.xold, i);
-      double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
-      tol += P.relTol * fmax(fabs(xv), fabs(old));
-      if (fabs(xv - old) > tol) return false;
+      double xv = x_[i], old = xo[i];
+      double tol = isV[i] ? vnTol : absTol;
+      tol += relTol * fmax(fabs(xv), fabs(old));
+      ok = ok && !(fabs(xv - old) > tol);
     }
-    for (int k = 0; k < nel_; k++) {
-      const int32_t* e = et_ + (int64_t)k * ET_W;
-      if (KH_HAS(K_D) && e[ET_KIND] == K_D && !conv_D(L, e, x_, P.relTol, P.absTol)) return false;
-      if (KH_HAS(K_M) && e[ET_KIND] == K_M && !conv_M(L, e, x_, P.relTol, P.absTol)) return false;
-      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && !conv_Q(L, e, x_, P.relTol, P.absTol)) return false;
+    if (!ok) return false;
+    for (int q = 0; q < n_nl; q++) {
+      const int32_t* e = et_ + (int64_t)nl[q] * ET_W;
+      if (KH_HAS(K_D) && e[ET_KIND] == K_D && !conv_D(L, e, x_, relTol, absTol)) return false;
+      if (KH_HAS(K_M) && e[ET_KIND] == K_M && !conv_M(L, e, x_, relTol, absTol)) return false;
+      if (KH_HAS(K_Q) && e[ET_KIND] == K_Q && !conv_Q(L, e, x_, relTol, absTol)) return false;
     }
     return true;
   }
@@ -234,9 +250,10 @@ struct Ctl {
     const int32_t* const et_ = A.ckt.et;
     double* const x_ = A.x;
     double nb = kInf;
-    for (int k = 0; k < nel_; k++) {
-      const int32_t* e = et_ + (int64_t)k * ET_W;
-      if (e[ET_KIND] == K_V || e[ET_KIND] == K_I) {
+    (void)nel_; (void)x_;
+    for (int q = 0; q < A.ckt.n_src; q++) {
+      const int32_t* e = et_ + (int64_t)A.ckt.src[q] * ET_W;
+      {
         if ((int)L.P(e, 2) == 0) continue;
         double b = wave_next_breakpoint(L, e, t);
         if (b < nb) nb = b;
@@ -253,8 +270,9 @@ struct Ctl {
     tc.time = tn; tc.H = h; tc.method = order;
     tc.relTol = P.relTol; tc.absTol = P.absTol; tc.vnTol = P.vnTol; tc.chgTol = P.chgTol; tc.trTol = P.trTol;
     double prop = kInf;
-    for (int k = 0; k < nel_; k++) {
-      const int32_t* e = et_ + (int64_t)k * ET_W;
+    (void)nel_;
+    for (int q = 0; q < A.ckt.n_rx; q++) {
+      const int32_t* e = et_ + (int64_t)A.ckt.rx[q] * ET_W;
       double pk = kInf;
       switch (e[ET_KIND]) {
         case K_C: pk = trunc_C(L, e, x_, tc); break;
@@ -365,14 +383,11 @@ struct Ctl {
         if (ms != 0) { sup += ms; scale += fabs(ms); }
       }
     }
-    for (int k = 0; k < A.ckt.nel; k++) {  // storage of reactive devices
-      const int32_t* e = el(k);
-      int kind = e[ET_KIND];
-      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) {
-        OpPt o = operating(L, e, x_);
-        stor += o.storage;
-        scale += fabs(o.storage);
-      }
+    for (int q = 0; q < A.ckt.n_rx; q++) {  // storage of reactive devices
+      const int32_t* e = et_ + (int64_t)A.ckt.rx[q] * ET_W;
+      OpPt o = operating(L, e, x_);
+      stor += o.storage;
+      scale += fabs(o.storage);
     }
     int err = ST_OK;
     if (scale > cd[CD_PEAK]) cd[CD_PEAK] = scale;
@@ -478,8 +493,8 @@ struct Ctl {
   KH_HD void apply_nodeset() {
     for (int i = 0; i < n; i++) X(A.x, i) = 0;
     for (int i = 0; i < A.ckt.numNodes - 1; i++) X(A.x, i) = A.nodeset[i];
-    for (int k = 0; k < A.ckt.nel; k++) {
-      const int32_t* e = el(k);
+    for (int q = 0; q < A.ckt.n_nl; q++) {
+      const int32_t* e = el(A.ckt.nl[q]);
       switch (e[ET_KIND]) {
         case K_D: if (KH_HAS(K_D)) accept_D(L, e, A.x, 0, MODE_OP); break;
         case K_M: if (KH_HAS(K_M)) accept_M(L, e, A.x, 0, MODE_OP); break;

@@ -68,6 +68,13 @@ __global__ void __launch_bounds__(kBlock, KH_PERSIST_WAVES) k_persist(const Grou
 
 hipError_t KH_CAT(KH_CAT(kh_launch_persist_w, KH_PERSIST_WAVES), KH_CAT(_k, KH_KINDS))(
     unsigned grid, size_t lds_bytes, const Group* G, const int32_t* bgroup, int32_t max_solves, int32_t* counters) {
+  static size_t opted = 64 * 1024;  // dynamic LDS above 64 KiB needs the kernel's opt-in
+  if (lds_bytes > opted) {
+    const hipError_t e = hipFuncSetAttribute((const void*)k_persist, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             (int)lds_bytes);
+    if (e != hipSuccess) return e;
+    opted = lds_bytes;
+  }
   hipLaunchKernelGGL(k_persist, dim3(grid), dim3(kBlock), lds_bytes, 0, G, bgroup, max_solves, counters);
   return hipGetLastError();
 }

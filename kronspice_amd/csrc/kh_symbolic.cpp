@@ -212,6 +212,9 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
   int soff = 0;
   nonlin = false;
   kinds = 0;
+  nl_idx.clear();
+  rx_idx.clear();
+  src_idx.clear();
   level_rows.clear();
   level_vals.clear();
   for (int e = 0; e < nel; e++) {
@@ -284,6 +287,9 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     t[ET_SOFF] = soff;
     soff += ns;
     kinds |= 1u << E.kind;
+    if (E.kind == K_D || E.kind == K_M || E.kind == K_Q) nl_idx.push_back(e);
+    if (E.kind == K_C || E.kind == K_L || E.kind == K_D || E.kind == K_M || E.kind == K_Q) rx_idx.push_back(e);
+    if (E.kind == K_V || E.kind == K_I) src_idx.push_back(e);
     if (E.kind == K_M) {
       if (params_c0[E.poff + 34] >= 2) kinds |= 1u << K_M23;  // level 2 or 3 currents
       level_rows.push_back(E.poff + 34);

@@ -48,6 +48,10 @@ struct Topology {
   int32_t nslots = 0;
   bool nonlin = false;
   uint32_t kinds = 0;  // bit k set when an element of kind k is present (kh_devices.h)
+  // Element indices, in element order, of the nonlinear devices (D, M, Q), the
+  // devices with stored charge or flux (C, L, D, M, Q) and the sources (V, I):
+  // the loops that only concern them skip the rest.
+  std::vector<int32_t> nl_idx, rx_idx, src_idx;
   // Parameter rows holding a MOSFET level, and the level circuit 0 has there:
   // the level is structural (it picks the compiled currents), so every
   // circuit of the topology must share it.

@@ -357,6 +357,7 @@ inline unsigned grid_for(int64_t B) { return (unsigned)((B + kBlock - 1) / kBloc
 struct kh_topology_s {
   Topology t;
   DBuf et, isV, slot_i, slot_j, diag, rowmajor;  // rowmajor: slots sorted by (row, column)
+  DBuf lists;  // nonlinear, reactive and source element indices, concatenated
   std::mutex mu;  // batch entry points on one topology run one at a time
 };
 
@@ -631,6 +632,11 @@ kh_status kh_topology_create(int32_t num_nodes, const kh_element* elems, int32_t
   });
   KH_HIP(t->rowmajor.alloc(rm.size() * 4));
   KH_HIP(hipMemcpy(t->rowmajor.p, rm.data(), rm.size() * 4, hipMemcpyHostToDevice));
+  std::vector<int32_t> li(T.nl_idx);
+  li.insert(li.end(), T.rx_idx.begin(), T.rx_idx.end());
+  li.insert(li.end(), T.src_idx.begin(), T.src_idx.end());
+  KH_HIP(t->lists.alloc(std::max<size_t>(li.size(), 1) * 4));
+  if (!li.empty()) KH_HIP(hipMemcpy(t->lists.p, li.data(), li.size() * 4, hipMemcpyHostToDevice));
   *out = t.release();
   return KH_OK;
 }
@@ -741,6 +747,12 @@ struct GroupRun {
     a.ckt.nonlin = T.nonlin ? 1 : 0;
     a.ckt.dense = dense ? 1 : 0;
     a.ckt.order = dense ? topo->rowmajor.as<int32_t>() : nullptr;
+    a.ckt.n_nl = (int32_t)T.nl_idx.size();
+    a.ckt.n_rx = (int32_t)T.rx_idx.size();
+    a.ckt.n_src = (int32_t)T.src_idx.size();
+    a.ckt.nl = topo->lists.as<int32_t>();
+    a.ckt.rx = a.ckt.nl + a.ckt.n_nl;
+    a.ckt.src = a.ckt.rx + a.ckt.n_rx;
     a.p = p;
     a.B = B;
     a.par = par.as<double>();
@@ -890,11 +902,11 @@ struct Runner {
   static constexpr float kLaunchTargetMs = 25.0f;
   int32_t cpb = 1;
   // LDS staging (k_persist): a group is staged when cpb circuits' working
-  // sets fit in kLdsLimit bytes (the per-workgroup LDS available without
+  // sets fit in lds_limit bytes (the per-workgroup LDS available without
   // opting in to more).
   bool lds_staging = false;
   int32_t waves = 1;  // k_persist variant: resident wavefronts per SIMD (1, 2 or 4)
-  static constexpr int64_t kLdsLimit = 64 * 1024;
+  int64_t lds_limit = 64 * 1024;  // raised to the device's opt-in maximum (160 KiB on gfx950)
   size_t lds_launch = 0;
 
   ~Runner() {
@@ -941,7 +953,7 @@ struct Runner {
             // one wavefront in different LDS banks.
             int64_t b = round8(lds_need(hg[g]));
             if (((b / 8) & 1) == 0) b += 8;
-            if (b * hg[g].cpb <= kLdsLimit) {
+            if (b * hg[g].cpb <= lds_limit) {
               hg[g].lds_bytes = (int32_t)b;
               lds_launch = std::max(lds_launch, (size_t)(b * hg[g].cpb));
             }
@@ -1061,6 +1073,12 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
   Runner R;
   R.cpb = cpb;
   R.lds_staging = opt && opt->staging != 0;
+  if (R.lds_staging) {
+    int dev = 0, optin = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
+      R.lds_limit = std::max<int64_t>(R.lds_limit, optin);
+  }
   R.waves = (opt && (opt->waves_per_simd == 2 || opt->waves_per_simd == 4)) ? opt->waves_per_simd : 1;
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];

@@ -299,7 +299,9 @@ def bjt_amplifier_circuit(seed: int = 4) -> Tuple[kc.Circuit, List[List[int]]]:
     b.bjt("Q2", "c2", "b2", "e2", "qb")
     ladder("c", "c2", None, n3, 100.0, 4.7e5, 41, 19)
     for k, name in enumerate(("qa", "qb")):
-        b.model(name, "npn", {"is": is_[k], "bf": bf[k], "vaf": 80.0, "rb": 50.0, "re": 0.5, "rc": 5.0})
+        # no series resistances: every unknown but the supply's branch current is a
+        # node voltage, so a NodeSet guess determines the whole starting point
+        b.model(name, "npn", {"is": is_[k], "bf": bf[k], "vaf": 80.0})
     return b.build(), groups
 
 

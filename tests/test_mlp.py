@@ -2,6 +2,8 @@
 the network's bf16 matrix-core forward pass and gradients against the numpy
 oracle (oracle/mlp.py, which rounds the same operands to bf16), training, and
 what the trained network gains over the cold start."""
+import json
+
 import numpy as np
 import pytest
 
@@ -81,6 +83,7 @@ def test_warm_start_saves_newton_iterations():
     hist = ws.fit(data, steps=1500)
     assert np.mean(hist[-50:]) < 0.1 * hist[0]
     rep = WS.evaluate(ws, 4096)
-    assert rep["warm_status_ok"] >= rep["cold_status_ok"]
-    assert rep["iters_saved_fraction"] > 0.2, rep
-    assert rep["max_rel_diff_polished"] <= 1e-9, rep
+    msg = json.dumps(rep)
+    assert rep["warm_status_ok"] >= rep["cold_status_ok"], msg
+    assert rep["iters_saved_fraction"] > 0.2, msg
+    assert rep["max_rel_diff_polished"] <= 1e-9, msg
