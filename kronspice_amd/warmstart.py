@@ -122,15 +122,16 @@ def evaluate(ws: WarmStart, B: int, seed: int = 4, first: int = 1 << 20) -> dict
 
 def main(argv: Optional[list] = None) -> int:
     ap = argparse.ArgumentParser(description="train and evaluate the learned warm start")
-    ap.add_argument("--train", type=int, default=32768)
+    ap.add_argument("--train", type=int, default=65536)
     ap.add_argument("--eval", type=int, default=65536)
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=8000)
+    ap.add_argument("--lr", type=float, default=3e-3)
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     data = make_data(a.train)
     t1 = time.perf_counter()
     ws = WarmStart(data.genes.shape[1], data.volts.shape[1])
-    hist = ws.fit(data, steps=a.steps)
+    hist = ws.fit(data, steps=a.steps, lr=a.lr)
     t2 = time.perf_counter()
     rep = evaluate(ws, a.eval)
     rep.update(train_circuits=a.train, data_seconds=t1 - t0, train_seconds=t2 - t1, train_steps=a.steps,
