@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--tune-fraction", type=float, default=0.05,
                     help="fraction of each circuit's interval the autotuner's probe simulates")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 form")
+    ap.add_argument("--no-warm-start", action="store_true",
+                    help="skip the learned warm start's measurement (two-stage BJT amplifier DC OP)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); circuits shard across GPUs with no "
@@ -154,6 +156,27 @@ def main():
                "sample": f"first {m} circuits of the batch, FP64 C++ restatement of the reference's Go path, "
                          f"{threads} worker threads, one task per circuit ({tcpu:.2f} s)"}
 
+    warm = None
+    if not args.no_warm_start:
+        # The learned Newton warm start (the metric's learned form) on its own
+        # workload: a 200-node two-stage BJT amplifier, DC operating points of
+        # fresh gene draws, cold against network-seeded Newton.
+        from kronspice_amd import warmstart as WS
+        tw = time.perf_counter()
+        wdata = WS.make_data(65536)
+        ws = WS.WarmStart(wdata.genes.shape[1], wdata.volts.shape[1])
+        hist = ws.fit(wdata, steps=8000, lr=3e-3)
+        trained = time.perf_counter() - tw
+        rep = WS.evaluate(ws, 65536)
+        warm = {"workload": "two-stage BJT amplifier, 200 nodes, 60 genes, DC OP (polished)",
+                "train_circuits": 65536, "train_seconds": trained, "eval_circuits": 65536,
+                "cold_iters_mean": rep["cold_iters_mean"], "warm_iters_mean": rep["warm_iters_mean"],
+                "iters_saved_fraction": rep["iters_saved_fraction"],
+                "cold_circuits_per_s": 65536 / rep["cold_seconds"],
+                "warm_circuits_per_s": 65536 / (rep["predict_seconds"] + rep["warm_seconds"]),
+                "status_ok": rep["warm_status_ok"], "max_rel_diff_polished": rep["max_rel_diff_polished"],
+                "loss_last": float(np.mean(hist[-50:]))}
+
     achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
     per_solve = measured_traffic_per_solve(args.staging, max(args.cpw, 1), max(args.waves, 1))
     traffic = per_solve * solves / max(lu_n, 1) if per_solve is not None else None
@@ -192,7 +215,8 @@ def main():
                   "chosen_vs_plain": (plain_s / (ms_per_step / 1e3)) if plain_s else None,
                   "autotune": ({"probe": f"whole batch, first {args.tune_fraction:g} of each interval",
                                 "seconds": {f"staging={a},cpw={c},waves={w}": t for a, c, w, t in tuned.timings}}
-                               if tuned else None)},
+                               if tuned else None),
+                  "learned_warm_start": warm},
         "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
                    "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "staging": args.staging,
                    "solves_per_step": solves / args.steps,
