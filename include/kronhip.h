@@ -134,6 +134,10 @@ typedef struct kh_options {
                                2 or 4 -> fewer registers (spilled), more circuits in flight */
   int32_t backend;          /* Options.Backend (solver/matrix.go:49-93): 0 auto (dense LU below
                                30 unknowns, sparse GP LU from 30), 1 dense, 2 sparse */
+  int32_t precision;        /* sparse LU arithmetic: 0 -> FP64 (the reference's); 1 -> FP32
+                               factors with FP64 iterative refinement, pivot sequence certified
+                               equal to FP64's, every uncertified system redone in FP64
+                               (no reference counterpart: the reference factors in FP64 only) */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -164,6 +168,8 @@ typedef struct kh_stats {
   int64_t control_launches;
   double  cycles_lu;         /* summed over circuits: shader clock cycles spent in factor+solve */
   double  cycles_control;    /* ... and in the rest of the analysis (stamping, Newton, LTE, ledger) */
+  int64_t refine_passes;     /* precision 1: FP32 solves of the refinement, summed */
+  int64_t fp64_redos;        /* precision 1: systems the mixed path could not certify (redone in FP64) */
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/
@@ -200,6 +206,15 @@ kh_status kh_lu_batch_create(int32_t n, int64_t nnz, const int32_t* ri, const in
                              int32_t B, kh_lu_batch* out);
 kh_status kh_lu_batch_solve(kh_lu_batch h, const double* d_vals, const double* d_b,
                             double* d_x, int32_t* d_status, int32_t* d_prow, void* stream);
+/* Mixed precision form of kh_lu_batch_solve (no reference counterpart: the
+ * reference factors in FP64 only). Each system is factored in FP32 with the
+ * same reach and pivot rule, its pivot sequence certified equal to the FP64
+ * factorization's, and solved with FP64 iterative refinement; a system that
+ * cannot be certified is redone exactly as kh_lu_batch_solve does.
+ * d_passes (optional, [B]): FP32 solves of the refinement, 0 where the system
+ * was redone in FP64. */
+kh_status kh_lu_batch_solve_mixed(kh_lu_batch h, const double* d_vals, const double* d_b, double* d_x,
+                                  int32_t* d_status, int32_t* d_prow, int32_t* d_passes, void* stream);
 kh_status kh_lu_batch_info(kh_lu_batch h, int32_t* q, int64_t* nnzL, int64_t* nnzU,
                            int64_t* capL, int64_t* capU);
 void kh_lu_batch_destroy(kh_lu_batch h);

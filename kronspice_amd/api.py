@@ -45,7 +45,9 @@ class Options(C.Structure):
                 ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
                 ("staging", C.c_int32), ("waves_per_simd", C.c_int32),
                 # Options.Backend: 0 auto (dense below 30 unknowns), 1 dense, 2 sparse
-                ("backend", C.c_int32)]
+                ("backend", C.c_int32),
+                # sparse LU arithmetic: 0 FP64, 1 FP32 factors + FP64 refinement
+                ("precision", C.c_int32)]
 
 
 class TranSpec(C.Structure):
@@ -60,7 +62,8 @@ class Stats(C.Structure):
                 ("rejected_steps", C.c_int64), ("analyses", C.c_int64), ("patterns", C.c_int64),
                 ("ticks", C.c_int64), ("bytes_algorithmic", C.c_int64), ("seconds", C.c_double),
                 ("lu_seconds", C.c_double), ("control_seconds", C.c_double), ("lu_launches", C.c_int64),
-                ("control_launches", C.c_int64), ("cycles_lu", C.c_double), ("cycles_control", C.c_double)]
+                ("control_launches", C.c_int64), ("cycles_lu", C.c_double), ("cycles_control", C.c_double),
+                ("refine_passes", C.c_int64), ("fp64_redos", C.c_int64)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}
@@ -96,6 +99,7 @@ def lib():
         L.kh_sys_destroy.restype = None
         L.kh_lu_batch_create.argtypes = [i32, i64, P, P, i32, P]
         L.kh_lu_batch_solve.argtypes = [P, P, P, P, P, P, P]
+        L.kh_lu_batch_solve_mixed.argtypes = [P, P, P, P, P, P, P, P]
         L.kh_lu_batch_info.argtypes = [P, P, P, P, P, P]
         L.kh_lu_batch_destroy.argtypes = [P]
         L.kh_lu_batch_destroy.restype = None
@@ -213,6 +217,12 @@ class LUBatch:
         """vals [nnz, B], b and x [n, B] float64 device tensors; status [B] int32."""
         _check(lib().kh_lu_batch_solve(self.h, _p(vals), _p(b), _p(x), _p(status), _p(prow),
                                        C.c_void_p(stream) if stream else None))
+
+    def solve_mixed(self, vals, b, x, status=None, prow=None, passes=None, stream=None):
+        """FP32 factors + FP64 refinement; passes [B] int32 receives the FP32
+        solves per system (0 where the system was redone in FP64)."""
+        _check(lib().kh_lu_batch_solve_mixed(self.h, _p(vals), _p(b), _p(x), _p(status), _p(prow), _p(passes),
+                                             C.c_void_p(stream) if stream else None))
 
     def info(self):
         q = np.zeros(self.n, np.int32)

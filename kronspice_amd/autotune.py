@@ -2,14 +2,19 @@
 
 The persistent kernel can run each circuit in place in device memory or with
 its working set staged in LDS, with one or several circuits per wavefront,
-and compiled for 1, 2 or 4 resident wavefronts per SIMD (register budget).
+and compiled for 1, 2 or 4 resident wavefronts per SIMD (register budget);
+its sparse LU runs in FP64 or as FP32 factors with FP64 refinement.
 Which is fastest depends on the circuit size (LDS footprint), the number of
 circuits and the divergence between them, so the choice is
 measured: each candidate runs a probe — the whole batch (so the number of
 circuits in flight, which decides how well latencies overlap, is the real
 one) over a short stretch of the analysis — and the fastest candidate is the
-configuration for the full run. Results do not depend on the choice — every configuration runs
-the same device code on the same values — only time does.
+configuration for the full run. The launch configuration is tuned in FP64,
+then each other LU precision is probed at the chosen configuration. Results
+do not depend on the launch configuration — every configuration runs the
+same device code on the same values — only time does; the mixed precision
+LU agrees with the FP64 path to round-off (its pivot rows are certified
+equal to FP64's) rather than bit for bit.
 """
 from __future__ import annotations
 
@@ -31,38 +36,47 @@ class Choice:
     staging: int
     circuits_per_wave: int
     waves_per_simd: int
+    precision: int
     seconds: float
-    timings: List[Tuple[int, int, int, float]]
+    timings: List[Tuple[int, int, int, int, float]]  # (staging, cpw, waves, precision, seconds)
 
     def options(self, base: Optional[api.Options] = None) -> api.Options:
         o = api.Options() if base is None else base
         o.staging = self.staging
         o.circuits_per_wave = self.circuits_per_wave
         o.waves_per_simd = self.waves_per_simd
+        o.precision = self.precision
         return o
 
 
 def tune(run: Callable[[api.Options], object], candidates: Sequence[Tuple[int, int, int]] = CANDIDATES,
-         base: Optional[api.Options] = None) -> Choice:
+         base: Optional[api.Options] = None, precisions: Sequence[int] = (0, 1)) -> Choice:
     """run(options) performs the analysis on the sample; the fastest candidate
     wins. Candidates the library refuses (KhError) are skipped."""
     import torch
-    timings = []
-    for staging, cpw, waves in candidates:
+
+    def probe(staging, cpw, waves, prec):
         o = api.Options() if base is None else api.Options.from_buffer_copy(base)
-        o.staging, o.circuits_per_wave, o.waves_per_simd = staging, cpw, waves
+        o.staging, o.circuits_per_wave, o.waves_per_simd, o.precision = staging, cpw, waves, prec
         try:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             run(o)
             torch.cuda.synchronize()
-            timings.append((staging, cpw, waves, time.perf_counter() - t0))
+            timings.append((staging, cpw, waves, prec, time.perf_counter() - t0))
         except api.KhError:
-            continue
+            pass
+
+    timings: List[Tuple[int, int, int, int, float]] = []
+    for staging, cpw, waves in candidates:
+        probe(staging, cpw, waves, precisions[0])
     if not timings:
         raise RuntimeError("no launch configuration ran")
-    s, c, w, t = min(timings, key=lambda r: r[3])
-    return Choice(s, c, w, t, timings)
+    s, c, w, _, _ = min(timings, key=lambda r: r[4])
+    for prec in precisions[1:]:
+        probe(s, c, w, prec)
+    s, c, w, pr, t = min(timings, key=lambda r: r[4])
+    return Choice(s, c, w, pr, t, timings)
 
 
 def short_lane_spec(lane_spec: np.ndarray, fraction: float) -> np.ndarray:

@@ -28,7 +28,7 @@ enum { WAIT_NONE = 0, WAIT_PATTERN = 1, WAIT_CAPACITY = 2 };
 enum {
   CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
   CD_SS_F, CD_PT_H, CD_TR_T, CD_TR_H, CD_TR_TN, CD_PEAK, CD_WORST, CD_MAXREL, CD_LEDGER, CD_MINSTEP_SEEN,
-  CD_MAXSTEP_SEEN, CD_CYC_LU, CD_CYC_CTL, NCD
+  CD_MAXSTEP_SEEN, CD_CYC_LU, CD_CYC_CTL, CD_REFINE, CD_REDO, NCD
 };
 
 enum { PH_INIT = 0, PH_RUN = 1, PH_DONE = 2 };
@@ -48,6 +48,7 @@ enum {
 struct CtlParams {
   double relTol, absTol, vnTol, gmin, ledgerTol;
   int32_t itl1, itl4, noGmin, noSource, noPT, skipEnergy, analysis;
+  int32_t precision;  // sparse LU: 0 FP64, 1 FP32 factors + FP64 refinement (kh_lu.h)
   // transient
   double step, stop, start, maxStep, chgTol, trTol, minStep;
   int64_t maxSteps;

@@ -184,7 +184,22 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
       if (a.ckt.dense) {
         st = lu_dense<double, true>(la, 0, a.ckt.slot_i, a.ckt.slot_j);
       } else {
-        st = lu_factor<double, true>(la, 0);
+        st = LU_UNSURE;
+        if (a.p.precision) {
+          // FP32 factors + FP64 refinement; a system it cannot certify is
+          // redone in FP64 below, so results follow the FP64 path.
+          st = lu_factor_mixed<true>(la, 0);
+          if (st == LU_OK) {
+            int passes = 0;
+            st = lu_solve_mixed<true>(la, 0, &passes);
+            if (st == LU_OK) ctl.cd[CD_REFINE] += (double)passes;
+          }
+          if (st == LU_UNSURE) ctl.cd[CD_REDO] += 1.0;
+        }
+        if (st == LU_UNSURE) {
+          st = lu_factor<double, true>(la, 0);
+          if (st == LU_OK) st = lu_solve<double, true>(la, 0);
+        }
         if (st == LU_CAPACITY) {
           // L/U outgrew the group's capacity (pivoting away from the diagonal
           // fills more than the structural estimate): the host enlarges it and
@@ -192,7 +207,6 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
           ctl.ci[CI_WAIT] = WAIT_CAPACITY;
           break;
         }
-        if (st == LU_OK) st = lu_solve<double, true>(la, 0);
       }
       lus[0] = st;
       solves++;

@@ -24,7 +24,10 @@
 
 namespace kh {
 
-enum LuStatus { LU_OK = 0, LU_SINGULAR = 1, LU_NONFINITE = 2, LU_CAPACITY = 14 };
+// LU_UNSURE: the mixed-precision factorization or refinement could not
+// certify its result (see lu_factor_mixed); the caller redoes the system in
+// FP64. It never leaves the library.
+enum LuStatus { LU_OK = 0, LU_SINGULAR = 1, LU_NONFINITE = 2, LU_CAPACITY = 14, LU_UNSURE = 16 };
 
 // Complex values (CSparse, solver/sparse.go:495-531) with Go's complex128
 // arithmetic: componentwise + and -, (ac - bd) + (ad + bc)i products, Smith's
@@ -353,6 +356,301 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
     if (!lu_finite(v)) return LU_NONFINITE;
     a.x[(int64_t)q[k] * B + c] = v;
   }
+  return LU_OK;
+}
+
+// ---- Mixed precision: FP32 factors, FP64 iterative refinement ----------------
+//
+// lu_factor_mixed factors the same permuted matrix as lu_factor with the same
+// reach lists and the same pivot rule, but holds the working column, L, U and
+// the pivots in FP32 (half the bytes of the factors). lu_solve_mixed solves
+// with those factors and refines the solution with FP64 residuals
+// r = b - A·y computed from the FP64 stamped values, until the correction is
+// at round-off. The FP64 path stays the reference: every case the mixed path
+// cannot certify returns LU_UNSURE and is redone by lu_factor + lu_solve.
+//
+// Certified pivot sequence. Every FP32 column entry x_r carries a first-order
+// bound e_r on its distance from the exact value of the same operations:
+// conversion of A(r,k) to FP32 (u·|a|, u = 2^-24), and for each update
+// x_i -= l·x_j the propagated |l|·e_j + δl·|x_j| plus the rounding of the
+// product and the difference, where δl is the stored error bound of the FP32
+// L entry (it follows from the bounds of its numerator and of the pivot at
+// the column that produced it). The FP64 factorization's values lie well
+// inside these bounds (its own rounding is 2^29 times smaller), so with
+// intervals x_r ± kMixSafety·e_r a column's pivot is accepted only when the
+// FP64 rule evaluated anywhere inside the intervals picks the same row: the
+// argmax must win by more than both intervals, the diagonal preference
+// (|x_k| >= 1e-3·best, solver/sparse.go:274-277) must hold or fail for every
+// value in them, and the largest candidate must clear the singularity
+// threshold tiny = colMax·n·1e-14, computed from the FP64 values as the
+// reference does. The pivot rows of an accepted mixed factorization therefore
+// equal the FP64 factorization's, which the parity tests check row for row.
+//
+// Storage inside the FP64 workspace: the FP32 column values in the first n
+// floats of xw and their error bounds in the next n; the FP32 L entries in the
+// first capL floats of lx and their error bounds in the next capL; U and the
+// pivots in the first halves of ux and udiag. The refinement uses w (permuted
+// right-hand side, FP64), xw (the solution y, FP64), x (the residual, FP64,
+// until the solution is written), xi and pstack (FP32 right-hand side and
+// correction). Entry i of a lane is at i*B + c as everywhere else.
+constexpr float kMixU = 5.9604645e-8f;  // 2^-24, FP32 unit round-off
+constexpr float kMixSafety = 4.0f;      // margin on the first-order error bounds
+constexpr int kMixMaxIter = 8;
+
+template <bool Unit = false>
+KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
+  const int n = a.n;
+  const int32_t pidc = a.pid[c];
+  const PatDesc pd = a.pats[pidc];
+  const int32_t* ap = a.pool + pd.ap;
+  const int32_t* arow = a.pool + pd.arow;
+  const int32_t* aslot = a.pool + pd.aslot;
+  float* xf = reinterpret_cast<float*>(a.xw);
+  float* xe = xf + (int64_t)n * B;
+  float* lxf = reinterpret_cast<float*>(a.lx);
+  float* lxe = lxf + a.capL * B;
+  float* uxf = reinterpret_cast<float*>(a.ux);
+  float* udf = reinterpret_cast<float*>(a.udiag);
+  double scale = 0.0;
+  for (int s = 0; s < a.nslots; s++) {
+    const double av = fabs(a.val[(int64_t)s * B + c]);
+    if (!kh_finite(av)) return LU_UNSURE;  // the FP64 path reports ErrNonFinite
+    if (av > scale) scale = av;
+    if (av != 0 && (av > 1e30 || av < 1e-30)) return LU_UNSURE;  // outside FP32's comfortable range
+  }
+  if (scale == 0) return LU_UNSURE;  // the FP64 path reports ErrSingular
+  bool reuse = a.cpid[c] == pidc;
+  a.cpid[c] = -1;
+  for (int i = 0; i < n; i++) a.pinv[(int64_t)i * B + c] = -1;
+  if (!reuse)
+    for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
+  int64_t nl = 0, nu = 0, nr = 0;
+  for (int k = 0; k < n; k++) {
+    a.lp[(int64_t)k * B + c] = (int32_t)nl;
+    a.up[(int64_t)k * B + c] = (int32_t)nu;
+    int64_t r0 = nr, r1;
+    bool diag_in_reach;
+    if (reuse) {
+      r1 = a.rp[(int64_t)(k + 1) * B + c];
+      diag_in_reach = a.dk[(int64_t)k * B + c] != 0;
+    } else {
+      int top = lu_reach<double, Unit>(a, c, k, ap, arow);
+      r1 = r0 + (n - top);
+      if (r1 > a.capR) return LU_CAPACITY;
+      for (int p = top; p < n; p++) a.rch[(r0 + p - top) * B + c] = a.xi[(int64_t)p * B + c];
+      diag_in_reach = a.mark[(int64_t)k * B + c] == k + 1;
+      a.rp[(int64_t)(k + 1) * B + c] = (int32_t)r1;
+      a.dk[(int64_t)k * B + c] = diag_in_reach ? 1 : 0;
+    }
+    nr = r1;
+    for (int64_t p = r0; p < r1; p++) {
+      const int64_t ir = (int64_t)a.rch[p * B + c] * B + c;
+      xf[ir] = 0.0f;
+      xe[ir] = 0.0f;
+    }
+    double colMax = 0.0;
+    for (int p = ap[k]; p < ap[k + 1]; p++) {
+      const double v = a.val[(int64_t)aslot[p] * B + c];
+      const int64_t ir = (int64_t)arow[p] * B + c;
+      const float vf = (float)v;
+      xf[ir] = vf;
+      xe[ir] = kMixU * fabsf(vf);
+      const double av = fabs(v);
+      if (av > colMax) colMax = av;
+    }
+    const double tiny = colMax * (double)n * 1e-14;
+    for (int64_t p = r0; p < r1; p++) {
+      const int32_t r = a.rch[p * B + c];
+      const int32_t j = a.pinv[(int64_t)r * B + c];
+      if (j < 0) continue;
+      const float xj = xf[(int64_t)r * B + c], ej = xe[(int64_t)r * B + c];
+      if (xj == 0.0f && ej == 0.0f) continue;
+      const float axj = fabsf(xj);
+      const int32_t q1 = a.lp[(int64_t)(j + 1) * B + c];
+      for (int32_t q = a.lp[(int64_t)j * B + c]; q < q1; q++) {
+        const int64_t ix = (int64_t)a.li[(int64_t)q * B + c] * B + c;
+        const float l = lxf[(int64_t)q * B + c];
+        const float prod = l * xj;
+        const float xn = xf[ix] - prod;
+        xe[ix] = xe[ix] + fabsf(l) * ej + lxe[(int64_t)q * B + c] * axj + kMixU * (fabsf(prod) + fabsf(xn));
+        xf[ix] = xn;
+      }
+    }
+    int32_t piv = -1, rhi = -1;
+    float best = -1.0f, maxhi = -1.0f, maxhi2 = -1.0f, maxlo = -1.0f;
+    for (int64_t p = r0; p < r1; p++) {
+      const int32_t r = a.rch[p * B + c];
+      const int64_t ir = (int64_t)r * B + c;
+      const int32_t pr = a.pinv[ir];
+      if (pr >= 0) {
+        if (nu >= a.capU) return LU_CAPACITY;
+        a.ui[nu * B + c] = pr;
+        uxf[nu * B + c] = xf[ir];
+        nu++;
+        continue;
+      }
+      const float av = fabsf(xf[ir]), err = kMixSafety * xe[ir];
+      if (av > best || (av == best && r < piv)) {
+        best = av;
+        piv = r;
+      }
+      const float hi = av + err, lo = av - err;
+      if (hi > maxhi) {
+        maxhi2 = maxhi;
+        maxhi = hi;
+        rhi = r;
+      } else if (hi > maxhi2) {
+        maxhi2 = hi;
+      }
+      if (lo > maxlo) maxlo = lo;
+    }
+    if (piv < 0 || !((double)maxlo > tiny)) return LU_UNSURE;
+    bool diag = false;
+    if (a.pinv[(int64_t)k * B + c] < 0 && diag_in_reach) {
+      const float ak = fabsf(xf[(int64_t)k * B + c]), ek = kMixSafety * xe[(int64_t)k * B + c];
+      const float lok = ak - ek, hik = ak + ek;
+      if ((double)lok >= 1e-3 * (double)maxhi && (double)lok > tiny) {
+        diag = true;
+      } else if (!((double)hik < 1e-3 * (double)maxlo || (double)hik <= tiny)) {
+        return LU_UNSURE;
+      }
+    }
+    if (diag) {
+      piv = k;
+    } else {
+      const float lop = best - kMixSafety * xe[(int64_t)piv * B + c];
+      const float other = rhi == piv ? maxhi2 : maxhi;
+      if (!(lop > other)) return LU_UNSURE;
+    }
+    if (reuse && piv != a.prow[(int64_t)k * B + c]) {
+      reuse = false;
+      for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
+    }
+    const float pv = xf[(int64_t)piv * B + c], apv = fabsf(pv), epv = xe[(int64_t)piv * B + c];
+    udf[(int64_t)k * B + c] = pv;
+    a.pinv[(int64_t)piv * B + c] = k;
+    a.prow[(int64_t)k * B + c] = piv;
+    for (int64_t p = r0; p < r1; p++) {
+      const int32_t r = a.rch[p * B + c];
+      const int64_t ir = (int64_t)r * B + c;
+      if (a.pinv[ir] < 0) {
+        if (nl >= a.capL) return LU_CAPACITY;
+        const float l = xf[ir] / pv, al = fabsf(l);
+        a.li[nl * B + c] = r;
+        lxf[nl * B + c] = l;
+        lxe[nl * B + c] = (xe[ir] + al * epv) / apv + kMixU * al;
+        nl++;
+      }
+    }
+  }
+  a.lp[(int64_t)n * B + c] = (int32_t)nl;
+  a.up[(int64_t)n * B + c] = (int32_t)nu;
+  a.rp[c] = 0;
+  a.cpid[c] = pidc;
+  return LU_OK;
+}
+
+// Solve with the FP32 factors of lu_factor_mixed and refine in FP64. Each pass
+// forms the FP64 residual of the current solution y against the FP64 stamped
+// values (the first pass starts from y = 0, r = b), rounds it to FP32, solves
+// for the correction with the FP32 factors and adds it to y in FP64. The
+// corrections shrink geometrically; with rho the ratio of the last two, what
+// remains after a pass is about rho times its correction. It stops when that
+// is within 2^-46 (1.4e-14) of every component plus 2^-60 of the largest
+// component (for components at zero), or when the corrections have reached
+// the FP64 residual's own noise (no longer halving, below 1e-12 of the
+// solution). Divergence, a non-finite value or kMixMaxIter passes return
+// LU_UNSURE. *iters receives the number of FP32 solves.
+template <bool Unit = false>
+KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters) {
+  const int64_t B = Unit ? 1 : a.B;
+  if (Unit) c = 0;
+  const int n = a.n;
+  const PatDesc pd = a.pats[a.pid[c]];
+  const int32_t* q = a.pool + pd.q;
+  const int32_t* qinv = a.pool + pd.qinv;
+  const int32_t* ap = a.pool + pd.ap;
+  const int32_t* arow = a.pool + pd.arow;
+  const int32_t* aslot = a.pool + pd.aslot;
+  const float* lxf = reinterpret_cast<const float*>(a.lx);
+  const float* uxf = reinterpret_cast<const float*>(a.ux);
+  const float* udf = reinterpret_cast<const float*>(a.udiag);
+  double* w = a.w;
+  double* y = a.xw;
+  double* r = a.x;
+  float* t = reinterpret_cast<float*>(a.xi);
+  float* z = reinterpret_cast<float*>(a.pstack);
+  for (int i = 0; i < n; i++) w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
+  double prev = __builtin_huge_val();
+  int it = 0;
+  for (;;) {
+    if (it == 0) {
+      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = (float)w[(int64_t)i * B + c];
+    } else {
+      for (int i = 0; i < n; i++) r[(int64_t)i * B + c] = w[(int64_t)i * B + c];
+      for (int k = 0; k < n; k++) {
+        const double yk = y[(int64_t)k * B + c];
+        if (yk == 0) continue;
+        for (int p = ap[k]; p < ap[k + 1]; p++) {
+          const int64_t ir = (int64_t)arow[p] * B + c;
+          const double prod = a.val[(int64_t)aslot[p] * B + c] * yk;
+          r[ir] = r[ir] - prod;
+        }
+      }
+      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = (float)r[(int64_t)i * B + c];
+    }
+    for (int k = 0; k < n; k++) {
+      const float zk = t[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
+      z[(int64_t)k * B + c] = zk;
+      if (zk == 0.0f) continue;
+      const int32_t p1 = a.lp[(int64_t)(k + 1) * B + c];
+      for (int32_t p = a.lp[(int64_t)k * B + c]; p < p1; p++) {
+        const int64_t ix = (int64_t)a.li[(int64_t)p * B + c] * B + c;
+        t[ix] = t[ix] - lxf[(int64_t)p * B + c] * zk;
+      }
+    }
+    for (int k = n - 1; k >= 0; k--) {
+      const float zk = z[(int64_t)k * B + c] / udf[(int64_t)k * B + c];
+      z[(int64_t)k * B + c] = zk;
+      if (zk == 0.0f) continue;
+      const int32_t p1 = a.up[(int64_t)(k + 1) * B + c];
+      for (int32_t p = a.up[(int64_t)k * B + c]; p < p1; p++) {
+        const int64_t ix = (int64_t)a.ui[(int64_t)p * B + c] * B + c;
+        z[ix] = z[ix] - uxf[(int64_t)p * B + c] * zk;
+      }
+    }
+    double ymax = 0.0, dmax = 0.0;
+    for (int k = 0; k < n; k++) {
+      const double dz = (double)z[(int64_t)k * B + c];
+      const double yk = (it == 0 ? 0.0 : y[(int64_t)k * B + c]) + dz;
+      if (!kh_finite(yk)) return LU_UNSURE;
+      y[(int64_t)k * B + c] = yk;
+      if (fabs(yk) > ymax) ymax = fabs(yk);
+      if (fabs(dz) > dmax) dmax = fabs(dz);
+    }
+    it++;
+    const double rel = ymax > 0 ? dmax / ymax : 0.0;
+    if (it >= 2) {
+      // rho: contraction of the corrections; what remains after this pass is
+      // about rho times this pass's correction.
+      const double rho = fmin(1.0, 2.0 * rel / prev);
+      bool tight = true;
+      for (int k = 0; k < n && tight; k++) {
+        const double yk = y[(int64_t)k * B + c];
+        const double dz = (double)z[(int64_t)k * B + c];
+        tight = rho * fabs(dz) <= 0x1p-46 * fabs(yk) + 0x1p-60 * ymax;
+      }
+      if (tight) break;
+      if (rel <= 1e-12 && rel > 0.5 * prev) break;  // at the residual's round-off
+      if (rel > prev) return LU_UNSURE;             // diverging
+    }
+    prev = it == 1 ? 1.0 : rel;
+    if (it >= kMixMaxIter) return LU_UNSURE;
+  }
+  for (int k = 0; k < n; k++) a.x[(int64_t)q[k] * B + c] = y[(int64_t)k * B + c];
+  *iters = it;
   return LU_OK;
 }
 

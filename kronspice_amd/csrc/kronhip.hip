@@ -63,6 +63,24 @@ __global__ void __launch_bounds__(kBlock) k_lu(LuArgs a, const int32_t* need, co
   status[c] = st;
 }
 
+// Mixed precision factor + solve of every lane (kh_lu_batch_solve_mixed):
+// FP32 factors and FP64 refinement; a lane the mixed path cannot certify is
+// redone in FP64 exactly as k_lu does (passes[c] = 0 then).
+__global__ void __launch_bounds__(kBlock) k_lu_mixed(LuArgs a, int32_t* status, int32_t* passes) {
+  int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= a.B) return;
+  int it = 0;
+  int st = lu_factor_mixed(a, c);
+  if (st == LU_OK) st = lu_solve_mixed(a, c, &it);
+  if (st == LU_UNSURE) {
+    it = 0;
+    st = lu_factor(a, c);
+    if (st == LU_OK) st = lu_solve(a, c);
+  }
+  status[c] = st;
+  if (passes) passes[c] = it;
+}
+
 // FNV-1a over the present slot ids of each waiting circuit (contiguous
 // per-circuit storage).
 __global__ void __launch_bounds__(kBlock) k_hash(const uint8_t* present, int32_t nslots, int64_t B,
@@ -591,6 +609,29 @@ kh_status kh_lu_batch_solve(kh_lu_batch h, const double* d_vals, const double* d
   return KH_OK;
 }
 
+kh_status kh_lu_batch_solve_mixed(kh_lu_batch h, const double* d_vals, const double* d_b, double* d_x,
+                                  int32_t* d_status, int32_t* d_prow, int32_t* d_passes, void* stream) {
+  if (!h || !d_vals || !d_b || !d_x) return fail(KH_BAD_ARGUMENT, "null argument");
+  LuArgs a = {};
+  a.n = h->n;
+  a.nslots = (int32_t)h->nnz;
+  a.B = h->B;
+  a.pats = h->pats.desc.as<PatDesc>();
+  a.pool = h->pats.pool.as<int32_t>();
+  a.pid = h->pid.as<int32_t>();
+  a.val = d_vals;
+  a.rhs = d_b;
+  a.x = d_x;
+  h->work.fill(a);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  int32_t* status = d_status ? d_status : h->status.as<int32_t>();
+  hipLaunchKernelGGL(k_lu_mixed, dim3(grid_for(h->B)), dim3(kBlock), 0, st, a, status, d_passes);
+  KH_HIP(hipGetLastError());
+  if (d_prow)
+    KH_HIP(hipMemcpyAsync(d_prow, h->work.prow.p, (size_t)h->n * h->B * 4, hipMemcpyDeviceToDevice, st));
+  return KH_OK;
+}
+
 kh_status kh_lu_batch_info(kh_lu_batch h, int32_t* q, int64_t* nnzL, int64_t* nnzU, int64_t* capL, int64_t* capU) {
   if (!h) return fail(KH_BAD_ARGUMENT, "null handle");
   if (q) std::memcpy(q, h->pat.q.data(), (size_t)h->n * 4);
@@ -877,6 +918,8 @@ struct GroupRun {
     for (int64_t c = 0; c < B; c++) {
       stats.cycles_lu += dv[c * NCD + CD_CYC_LU];
       stats.cycles_control += dv[c * NCD + CD_CYC_CTL];
+      stats.refine_passes += (int64_t)dv[c * NCD + CD_REFINE];
+      stats.fp64_redos += (int64_t)dv[c * NCD + CD_REDO];
     }
     // Algorithmic bytes per executed solve: matrix values and right-hand side
     // read once, solution written once, 8*(nnzA + 2n), plus the device
@@ -884,7 +927,16 @@ struct GroupRun {
     // patterns only grow).
     int64_t nnz = 0;
     for (const auto& d : pats.host.desc) nnz = std::max<int64_t>(nnz, d.nnz);
-    stats.bytes_algorithmic = stats.solves * 8 * (nnz + 2 * (int64_t)n + topo->t.nparams);
+    // Mixed precision (FP32 factors): 4*(nnzA + 2n) per certified solve plus
+    // 8*nnzA + 16n (FP64 residual) per further refinement pass; a system
+    // redone in FP64 counts as an FP64 solve.
+    const int64_t P = topo->t.nparams, n2 = 2 * (int64_t)n;
+    const int64_t mixed = stats.solves - stats.fp64_redos;
+    if (stats.refine_passes > 0)
+      stats.bytes_algorithmic = mixed * (4 * (nnz + n2) + 8 * P) + (stats.refine_passes - mixed) * (8 * nnz + 8 * n2) +
+                                stats.fp64_redos * 8 * (nnz + n2 + P);
+    else
+      stats.bytes_algorithmic = stats.solves * 8 * (nnz + n2 + P);
     return KH_OK;
   }
 };
@@ -1003,6 +1055,8 @@ struct Runner {
       stats.bytes_algorithmic += g->stats.bytes_algorithmic;
       stats.cycles_lu += g->stats.cycles_lu;
       stats.cycles_control += g->stats.cycles_control;
+      stats.refine_passes += g->stats.refine_passes;
+      stats.fp64_redos += g->stats.fp64_redos;
     }
   }
 };
@@ -1021,6 +1075,7 @@ CtlParams resolve(const kh_options* o, int analysis) {
   p.noPT = o ? o->no_pseudo_transient : 0;
   p.skipEnergy = o ? o->skip_energy_check : 0;
   p.analysis = analysis;
+  p.precision = o ? o->precision : 0;
   return p;
 }
 

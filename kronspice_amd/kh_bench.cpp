@@ -1,7 +1,7 @@
 // kh_bench — runs a batch file through kh_batch_run from plain C++ (no
 // Python, no PyTorch in the process), for timing and for profilers.
 //
-//   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave] [waves per SIMD]
+//   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave] [waves per SIMD] [precision]
 //   kh_bench --copy <MiB>    one device-to-device copy kernel of that size:
 //                            the reference point for hardware byte counters
 //
@@ -87,6 +87,7 @@ int main(int argc, char** argv) {
   const int staging = argc > 4 ? atoi(argv[4]) : 0;
   const int cpw = argc > 5 ? atoi(argv[5]) : 0;
   const int waves = argc > 6 ? atoi(argv[6]) : 0;
+  const int precision = argc > 7 ? atoi(argv[7]) : 0;
   FILE* f = fopen(argv[1], "rb");
   if (!f) return die("cannot open the batch file");
   char magic[4];
@@ -162,6 +163,7 @@ int main(int argc, char** argv) {
   opt.staging = staging;
   opt.circuits_per_wave = cpw;
   opt.waves_per_simd = waves;
+  opt.precision = precision;
   kh_tran_spec spec;
   memset(&spec, 0, sizeof(spec));
   spec.step = spec3[0];
@@ -173,7 +175,7 @@ int main(int argc, char** argv) {
   for (int w = 0; w < warmup; w++) CHECK_KH(kh_batch_run(ngroups, groups.data(), &opt, sp, &st));
   CHECK_HIP(hipDeviceSynchronize());
   double lu_s = 0, bytes = 0, cyc_lu = 0, cyc_ctl = 0;
-  int64_t solves = 0, launches = 0;
+  int64_t solves = 0, launches = 0, passes = 0, redos = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (int s = 0; s < steps; s++) {
     CHECK_KH(kh_batch_run(ngroups, groups.data(), &opt, sp, &st));
@@ -183,6 +185,8 @@ int main(int argc, char** argv) {
     launches += st.lu_launches;
     cyc_lu += st.cycles_lu;
     cyc_ctl += st.cycles_control;
+    passes += st.refine_passes;
+    redos += st.fp64_redos;
   }
   CHECK_HIP(hipDeviceSynchronize());
   const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -197,11 +201,11 @@ int main(int argc, char** argv) {
   for (auto& kv : hist) h += (h.size() > 1 ? ", \"" : "\"") + std::to_string(kv.first) + "\": " + std::to_string(kv.second);
   h += "}";
   printf("{\"circuits\": %lld, \"groups\": %d, \"staging\": %d, \"circuits_per_wave\": %d, \"waves_per_simd\": %d, "
-         "\"steps\": %d, "
+         "\"precision\": %d, \"refine_passes_per_step\": %.1f, \"fp64_redos_per_step\": %.1f, \"steps\": %d, "
          "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
          "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"status_histogram\": %s}\n",
-         (long long)circuits, ngroups, staging, cpw, waves, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
+         (long long)circuits, ngroups, staging, cpw, waves, precision, (double)passes / steps, (double)redos / steps, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, cyc_lu / steps, cyc_ctl / steps,
          h.c_str());
   for (void* p : owned) (void)hipFree(p);
