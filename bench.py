@@ -2,30 +2,31 @@
 
 Default workload (one "step" = one batched analysis over the whole batch,
 inputs already resident in HBM):
-  mos_chains_uneven — 16384 uneven MOSFET-chain netlists over 200 distinct
-  sparsity patterns (20-2000 nodes, log-uniform; chains of 1-50 common-source
-  level-1 stages with KP log-U(2e-5,2e-4), VTO U(0.4,1.0) V, W/L U(1,100),
-  bias dividers, RC loads and RC ladders), all in one uneven batch: the DC
-  operating point, then a 2-period transient with max step stop/500
-  (trapezoidal, LTE step control, Newton to reltol 1e-3, abstol 1e-12).
-  --workload rectifier: 4096 half-wave rectifier + 32-section RC ladder
-  circuits of one topology (4 source periods, max step stop/2000).
+  rectifier_rc_ladder_tran — 4096 half-wave rectifier + 32-section RC ladder
+  circuits of one topology (sine amplitude U(1,20) V, f log-U(50 Hz, 1 MHz);
+  diode IS log-U(1e-15,1e-12) A, N U(1,2); R log-U(10,10k) ohm,
+  C log-U(1n,10u) F), each run for 4 periods of its own source with max step
+  stop/2000: the DC operating point, then trapezoidal integration with LTE
+  step control, Newton to reltol 1e-3, abstol 1e-12 (analysis.Tran).
+  --workload mos_chains: 16384 uneven MOSFET-chain netlists over 200 sparsity
+  patterns (20-2000 nodes), DC OP then 2 periods at max step stop/500, in one
+  uneven batch (minutes per batch on the current kernel: run it on its own).
 
 Forms: the plain FP64 GPU path (in place in HBM, one circuit per wavefront,
 FP64 LU) runs the batch once; the chosen form is what the autotuner
 (kronspice_amd/autotune.py) picks on a short probe of the whole batch —
 LDS staging, circuits per wavefront, register budget, then FP64 LU against
 FP32 factors + FP64 refinement — and it is what `value` times. The learned
-Newton warm start (a per-topology network, so it belongs to one topology) is
-measured on its own workload, the 200-node two-stage BJT amplifier's DC
-operating points, in `forms.learned_warm_start`.
+Newton warm start (a per-topology network) is measured on its own workload,
+the 200-node two-stage BJT amplifier's DC operating points, in
+`forms.learned_warm_start`.
 
 `value` counts the circuits solved per second: the batch size times the
-fraction of a parity sample (one circuit of every pattern) whose status is
-OK and whose final state is within 1e-9 of the FP64 oracle's, over the
-chosen form's time per batch. The same sample, run by the FP64 C++
-restatement of the reference (oracle/, one task per circuit on the host's
-cores), gives `cpu_baseline`.
+fraction of the parity sample (the whole batch for one topology, one circuit
+of every pattern for the uneven workload) whose status is OK and whose final
+state is within 1e-9 of the FP64 oracle's, over the chosen form's time per
+batch. The same sample, run by the FP64 C++ restatement of the reference
+(oracle/, one task per circuit on the host's cores), gives `cpu_baseline`.
 
 `roofline` is for the dominant kernel, k_persist (the persistent
 factor+solve+stamp kernel; the run's other kernels are layout changes at its
@@ -135,12 +136,12 @@ def make_work(name, batch):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["mos_chains", "rectifier"], default="mos_chains")
+    ap.add_argument("--workload", choices=["rectifier", "mos_chains"], default="rectifier")
     ap.add_argument("--batch", type=int, default=0, help="circuits (0: the workload's size)")
     ap.add_argument("--sample-per-group", type=int, default=0,
-                    help="parity / CPU-baseline sample per group (0: 1 per pattern, 128 for one topology)")
+                    help="parity / CPU-baseline sample per group (0: 1 per pattern, the whole batch for one topology)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpw", type=int, default=0, help="circuits per wavefront (0: autotuned)")
     ap.add_argument("--staging", type=int, default=-1, help="0 in place, 1 LDS staged (-1: autotuned)")
@@ -204,7 +205,7 @@ def main():
 
     # Parity on the sample against the FP64 oracle (whose time on the host's
     # cores is the CPU baseline).
-    per = args.sample_per_group or (1 if len(work.batches) > 1 else 128)
+    per = args.sample_per_group or (1 if len(work.batches) > 1 else work.B)
     sample = work.sample(per)
     threads = min(HOST_THREADS, os.cpu_count() or 1)
     ref, tcpu = work.oracle(sample, threads)
@@ -227,7 +228,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = {"value": len(ref) / tcpu, "unit": "circuits/s", "cores": threads, "kind": "port",
-               "sample": f"{len(ref)} circuits ({per} per topology group, every group) of the same batch, FP64 C++ "
+               "sample": f"{len(ref)} circuits ({min(per, work.B)} per topology group, every group) of the same batch, FP64 C++ "
                          f"restatement of the reference's Go path, {threads} worker threads, one task per circuit "
                          f"({tcpu:.2f} s)"}
 

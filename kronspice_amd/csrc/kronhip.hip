@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -992,6 +993,11 @@ struct Runner {
       ~EvGuard() { for (int k = 0; k < 2; k++) (void)hipEventDestroy(e[k]); }
     } guard{ev};
     auto t0 = std::chrono::steady_clock::now();
+    // KH_PROGRESS=<seconds>: a line on stderr at that interval while the
+    // batch runs (long batches under a watchdog).
+    const char* pe = std::getenv("KH_PROGRESS");
+    const double progress_s = pe ? std::atof(pe) : 0.0;
+    double next_report = progress_s;
     bool dirty = true;
     for (;;) {
       if (dirty) {  // group descriptors: LU workspace pointers change when capacity grows
@@ -1038,6 +1044,14 @@ struct Runner {
         }
       }
       if (running == 0) break;
+      if (progress_s > 0) {
+        const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (el >= next_report) {
+          fprintf(stderr, "kh_batch_run: %.0f s, %lld launches, %lld circuits running\n", el,
+                  (long long)stats.ticks, (long long)running);
+          next_report = el + progress_s;
+        }
+      }
     }
     KH_HIP(hipDeviceSynchronize());
     stats.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
