@@ -1257,20 +1257,16 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   GroupRun& G = *keep[0];
   const Topology& T = t->t;
   const int n = T.n;
-  // Frequencies in chunks of at most kAcLanes systems.
+  // Frequencies in chunks of at most kAcLanes systems, fewer when the
+  // factors of that many systems would not fit in device memory.
   constexpr int64_t kAcLanes = 1 << 18;
-  const int32_t fchunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(nf, kAcLanes / B));
-  const int64_t BLmax = (int64_t)fchunk * B;
   DBuf dfreq, val, rhs, x, present, lst, lrr, pid, done, points, maxrr;
   KH_HIP(dfreq.alloc((size_t)nf * 8));
   KH_HIP(hipMemcpy(dfreq.p, freqs, (size_t)nf * 8, hipMemcpyHostToDevice));
-  KH_HIP(val.alloc((size_t)T.nslots * BLmax * sizeof(kcplx)));
-  KH_HIP(rhs.alloc((size_t)n * BLmax * sizeof(kcplx)));
-  KH_HIP(x.alloc((size_t)n * BLmax * sizeof(kcplx)));
+  KH_HIP(val.alloc((size_t)T.nslots * B * sizeof(kcplx)));
+  KH_HIP(rhs.alloc((size_t)n * B * sizeof(kcplx)));
   KH_HIP(present.alloc((size_t)B * T.nslots));
-  KH_HIP(lst.alloc((size_t)BLmax * 4));
-  KH_HIP(lrr.alloc((size_t)BLmax * 8));
-  KH_HIP(pid.alloc((size_t)BLmax * 4));
+  KH_HIP(lst.alloc((size_t)B * 4));
   KH_HIP(done.alloc((size_t)B * 4));
   KH_HIP(points.alloc((size_t)B * 4));
   KH_HIP(maxrr.alloc((size_t)B * 8));
@@ -1289,49 +1285,85 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   a.val = val.as<kcplx>();
   a.rhs = rhs.as<kcplx>();
   a.lstatus = lst.as<int32_t>();
-  a.lrr = lrr.as<double>();
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  for (auto& e : ev) KH_HIP(hipEventCreate(&e));
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() { for (int k = 0; k < 2; k++) (void)hipEventDestroy(e[k]); }
+  } guard{ev};
   auto t0 = std::chrono::steady_clock::now();
+  // The AC pattern of each circuit (LoadAC skips zero channel conductances,
+  // so it depends on the operating point): the first frequency is stamped once
+  // for every circuit to learn it; one ordering per distinct pattern.
+  a.f0 = 0;
+  a.BL = B;
+  a.present = present.as<uint8_t>();
+  hipLaunchKernelGGL(k_ac_stamp, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, a);
+  KH_HIP(hipGetLastError());
+  {
+    std::vector<uint8_t> pr((size_t)B * T.nslots);
+    KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
+    std::vector<int32_t> wst(B);
+    KH_HIP(hipMemcpy2D(wst.data(), 4, G.ci.as<int32_t>() + CI_STATUS, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
+    bool any_ok = false;
+    for (int64_t c = 0; c < B; c++) any_ok = any_ok || wst[c] == ST_OK;
+    if (!any_ok) nf = 0;  // no operating point succeeded: statuses are the OP's
+    for (int64_t c = 0; c < B && any_ok && !G.dense; c++) {  // the dense backend has no pattern
+      if (wst[c] != ST_OK) continue;
+      std::vector<int32_t> slots;
+      for (int s2 = 0; s2 < T.nslots; s2++)
+        if (pr[(size_t)c * T.nslots + s2]) slots.push_back(s2);
+      std::string key = slots_key(slots);
+      auto f = pats.host.by_key.find(key);
+      if (f != pats.host.by_key.end()) {
+        cpid[c] = f->second;
+      } else {
+        cpid[c] = pats.host.add(analyze_pattern(n, T.slot_i, T.slot_j, slots), key);
+      }
+    }
+    KH_HIP(pats.sync());
+  }
+  const int64_t cap = G.dense ? (int64_t)n * n : lu_capacity(pats.host.max_fill, n);
+  // Device bytes per lane: values, right-hand side, solution, status words and
+  // the LU workspace (LuWorkT::ensure / ensure_dense).
+  const int64_t lane_bytes =
+      ((int64_t)T.nslots + 2 * (int64_t)n) * (int64_t)sizeof(kcplx) + 16 +
+      (G.dense ? cap * (int64_t)sizeof(kcplx) + 2 * (int64_t)n * (int64_t)sizeof(kcplx) + 4 * (int64_t)n + 4
+               : 3 * (int64_t)n * (int64_t)sizeof(kcplx) + 2 * cap * ((int64_t)sizeof(kcplx) + 4) +
+                     (2 * cap + n) * 4 + 8 * ((int64_t)n + 1) + 24 * (int64_t)n + 8);
+  size_t free_b = 0, total_b = 0;
+  KH_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t fit = std::max<int64_t>(1, (int64_t)(0.6 * (double)free_b) / (lane_bytes * B));
+  int64_t max_lanes = kAcLanes;
+  if (const char* e = std::getenv("KH_AC_CHUNK_LANES")) max_lanes = std::max<int64_t>(1, std::atoll(e));  // tests
+  const int32_t fchunk =
+      (int32_t)std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::max(nf, 1), max_lanes / B, fit}));
+  const int64_t BLmax = (int64_t)fchunk * B;
+  KH_HIP(val.alloc((size_t)T.nslots * BLmax * sizeof(kcplx)));
+  KH_HIP(rhs.alloc((size_t)n * BLmax * sizeof(kcplx)));
+  KH_HIP(x.alloc((size_t)n * BLmax * sizeof(kcplx)));
+  KH_HIP(lst.alloc((size_t)BLmax * 4));
+  KH_HIP(lrr.alloc((size_t)BLmax * 8));
+  KH_HIP(pid.alloc((size_t)BLmax * 4));
+  a.val = val.as<kcplx>();
+  a.rhs = rhs.as<kcplx>();
+  a.lstatus = lst.as<int32_t>();
+  a.lrr = lrr.as<double>();
+  a.present = nullptr;
+  if (nf > 0) {
+    if (G.dense) KH_HIP(work.ensure_dense(n, BLmax));
+    else KH_HIP(work.ensure(n, BLmax, cap));
+    std::vector<int32_t> lp((size_t)BLmax);
+    for (int64_t l = 0; l < BLmax; l++) lp[l] = std::max(cpid[l % B], 0);
+    KH_HIP(hipMemcpy(pid.p, lp.data(), lp.size() * 4, hipMemcpyHostToDevice));
+  }
   for (int32_t f0 = 0; f0 < nf; f0 += fchunk) {
     const int32_t nfc = std::min(fchunk, nf - f0);
     a.f0 = f0;
     a.BL = (int64_t)nfc * B;
-    a.present = f0 == 0 ? present.as<uint8_t>() : nullptr;
     const unsigned grid = (unsigned)((a.BL + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_ac_stamp, dim3(grid), dim3(kBlock), 0, 0, a);
     KH_HIP(hipGetLastError());
-    if (f0 == 0) {
-      // The AC pattern of each circuit (LoadAC skips zero channel conductances,
-      // so it depends on the operating point): one ordering per distinct pattern.
-      std::vector<uint8_t> pr((size_t)B * T.nslots);
-      KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
-      std::vector<int32_t> wst(B);
-      KH_HIP(hipMemcpy2D(wst.data(), 4, G.ci.as<int32_t>() + CI_STATUS, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
-      bool any_ok = false;
-      for (int64_t c = 0; c < B; c++) any_ok = any_ok || wst[c] == ST_OK;
-      if (!any_ok) break;  // no operating point succeeded
-      for (int64_t c = 0; c < B && !G.dense; c++) {  // the dense backend has no pattern
-        if (wst[c] != ST_OK) continue;
-        std::vector<int32_t> slots;
-        for (int s2 = 0; s2 < T.nslots; s2++)
-          if (pr[(size_t)c * T.nslots + s2]) slots.push_back(s2);
-        std::string key = slots_key(slots);
-        auto f = pats.host.by_key.find(key);
-        if (f != pats.host.by_key.end()) {
-          cpid[c] = f->second;
-        } else {
-          cpid[c] = pats.host.add(analyze_pattern(n, T.slot_i, T.slot_j, slots), key);
-        }
-      }
-      KH_HIP(pats.sync());
-      if (G.dense) KH_HIP(work.ensure_dense(n, BLmax));
-      else KH_HIP(work.ensure(n, BLmax, lu_capacity(pats.host.max_fill, n)));
-      std::vector<int32_t> lp((size_t)BLmax);
-      for (int64_t l = 0; l < BLmax; l++) lp[l] = std::max(cpid[l % B], 0);
-      KH_HIP(hipMemcpy(pid.p, lp.data(), lp.size() * 4, hipMemcpyHostToDevice));
-      KH_HIP(hipMemset(done.p, 0, (size_t)B * 4));
-      KH_HIP(hipMemset(points.p, 0, (size_t)B * 4));
-      KH_HIP(hipMemset(maxrr.p, 0, (size_t)B * 8));
-    }
     LuArgsT<kcplx> lu = {};
     lu.n = n;
     lu.nslots = T.nslots;
@@ -1344,12 +1376,26 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     lu.x = x.as<kcplx>();
     work.fill(lu);
     if (!G.dense) KH_HIP(work.invalidate());
+    KH_HIP(hipEventRecord(ev[0], 0));
     hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac);
     KH_HIP(hipGetLastError());
+    KH_HIP(hipEventRecord(ev[1], 0));
     hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),
                        d_status, points.as<int32_t>(), maxrr.as<double>());
     KH_HIP(hipGetLastError());
-    if (stats) stats->solves += a.BL;
+    KH_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    KH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    if (stats) {
+      // k_ac_solve: complex values and right-hand side read once, solution
+      // written once, 16*(nnzA + 2n) bytes per system.
+      int64_t nnz = G.dense ? (int64_t)n * n : 0;
+      for (const auto& d : pats.host.desc) nnz = std::max<int64_t>(nnz, d.nnz);
+      stats->solves += a.BL;
+      stats->lu_seconds += ms * 1e-3;
+      stats->lu_launches++;
+      stats->bytes_algorithmic += a.BL * 16 * (nnz + 2 * (int64_t)n);
+    }
   }
   KH_HIP(hipDeviceSynchronize());
   if (d_points) KH_HIP(hipMemcpy(d_points, points.p, (size_t)B * 4, hipMemcpyDeviceToDevice));

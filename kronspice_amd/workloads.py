@@ -254,6 +254,49 @@ def mosfet_chains(B: int = 16384, patterns: int = 200, seed: int = 3, steps: int
     return [mosfet_chain_batch(p, per[p], seed, steps) for p in range(patterns) if per[p] > 0]
 
 
+# ---------------------------------------------------------------------------- config 5
+def rlc_grid(w: int = 64, h: int = 64, draws: int = 16, seed: int = 5, source_at_far_corner: bool = False) -> Batch:
+    """Complex AC sweep workload: a w x h grid mesh whose every edge is a
+    series R-L branch (R log-U(0.1,10) ohm, L log-U(1 nH,1 uH); an internal
+    node between them) and every node a capacitor to ground (C log-U(1 pF,
+    1 nF)); a unit AC current source into the corner node (0,0); a 1 Mohm
+    resistor from that node to ground gives the mesh its DC path (capacitors
+    are not DC paths, circuit/circuit.go:717-719). `draws` value draws of the
+    one topology; the transfer impedance is v at the opposite corner.
+    source_at_far_corner drives the opposite corner instead (the same network
+    and values: the reciprocity check)."""
+    b = kc.Builder(f"rlc_grid_{w}x{h}")
+    node = lambda x, y: f"g{x}_{y}"
+    src = node(w - 1, h - 1) if source_at_far_corner else node(0, 0)
+    b.isource("IAC", "0", src, 0.0, ac=1.0)
+    b.resistor("RDC", node(0, 0), "0", 1e6)
+    ridx, lidx, cidx = [], [], []
+    for y in range(h):
+        for x in range(w):
+            for (x2, y2, tag) in ((x + 1, y, "h"), (x, y + 1, "v")):
+                if x2 >= w or y2 >= h:
+                    continue
+                mid = f"m{tag}{x}_{y}"
+                b.resistor(f"R{tag}{x}_{y}", node(x, y), mid, 1.0); ridx.append(len(b.elems) - 1)
+                b.inductor(f"L{tag}{x}_{y}", mid, node(x2, y2), 1e-8); lidx.append(len(b.elems) - 1)
+            b.capacitor(f"C{x}_{y}", node(x, y), "0", 1e-11); cidx.append(len(b.elems) - 1)
+    c = b.build()
+    els, params = _template(c, draws)
+    cs = np.arange(draws)
+    _set_resistors(els, params, ridx, np.stack([log_uniform(seed, cs, 3 * k, 0.1, 10.0) for k in range(len(ridx))]))
+    for k, e in enumerate(lidx):
+        params[els[e]["poff"]] = log_uniform(seed, cs, 3 * k + 1, 1e-9, 1e-6)
+    for k, e in enumerate(cidx):
+        params[els[e]["poff"]] = log_uniform(seed, cs, 3 * k + 2, 1e-12, 1e-9)
+    _refresh(c, els, params)
+    return Batch(f"rlc_grid_{w}x{h}_ac", c, els, params, c.layout().n)
+
+
+def rlc_grid_frequencies(nf: int = 1024) -> np.ndarray:
+    """nf log-spaced frequencies from 1 kHz to 10 GHz."""
+    return np.logspace(3.0, 10.0, nf)
+
+
 # ---------------------------------------------------------------------------- config 4
 AMP_GENES = 60
 AMP_LADDERS = (63, 63, 73)  # sections of the bias, inter-stage and load ladders (200 nodes in all)
