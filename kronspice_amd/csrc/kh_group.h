@@ -188,10 +188,10 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
         if (a.p.precision) {
           // FP32 factors + FP64 refinement; a system it cannot certify is
           // redone in FP64 below, so results follow the FP64 path.
-          st = lu_factor_mixed<true>(la, 0);
+          st = lu_factor_mixed<double, true>(la, 0);
           if (st == LU_OK) {
             int passes = 0;
-            st = lu_solve_mixed<true>(la, 0, &passes);
+            st = lu_solve_mixed<double, true>(la, 0, &passes);
             if (st == LU_OK) ctl.cd[CD_REFINE] += (double)passes;
           }
           if (st == LU_UNSURE) ctl.cd[CD_REDO] += 1.0;

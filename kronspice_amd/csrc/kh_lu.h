@@ -363,42 +363,113 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
 //
 // lu_factor_mixed factors the same permuted matrix as lu_factor with the same
 // reach lists and the same pivot rule, but holds the working column, L, U and
-// the pivots in FP32 (half the bytes of the factors). lu_solve_mixed solves
-// with those factors and refines the solution with FP64 residuals
-// r = b - A·y computed from the FP64 stamped values, until the correction is
-// at round-off. The FP64 path stays the reference: every case the mixed path
-// cannot certify returns LU_UNSURE and is redone by lu_factor + lu_solve.
+// the pivots in single precision (FP32 for real systems, complex64 for AC
+// systems: half the bytes of the factors). lu_solve_mixed solves with those
+// factors and refines the solution with double-precision residuals
+// r = b - A·y computed from the stamped FP64 / complex128 values, until the
+// correction is at round-off. The double-precision path stays the reference:
+// every case the mixed path cannot certify returns LU_UNSURE and is redone by
+// lu_factor + lu_solve.
 //
-// Certified pivot sequence. Every FP32 column entry x_r carries a first-order
-// bound e_r on its distance from the exact value of the same operations:
-// conversion of A(r,k) to FP32 (u·|a|, u = 2^-24), and for each update
+// Certified pivot sequence. Every single-precision column entry x_r carries a
+// first-order bound e_r on its distance from the exact value of the same
+// operations: conversion of A(r,k) (u·|a|, u = 2^-24), and for each update
 // x_i -= l·x_j the propagated |l|·e_j + δl·|x_j| plus the rounding of the
-// product and the difference, where δl is the stored error bound of the FP32
-// L entry (it follows from the bounds of its numerator and of the pivot at
-// the column that produced it). The FP64 factorization's values lie well
-// inside these bounds (its own rounding is 2^29 times smaller), so with
-// intervals x_r ± kMixSafety·e_r a column's pivot is accepted only when the
-// FP64 rule evaluated anywhere inside the intervals picks the same row: the
-// argmax must win by more than both intervals, the diagonal preference
-// (|x_k| >= 1e-3·best, solver/sparse.go:274-277) must hold or fail for every
-// value in them, and the largest candidate must clear the singularity
-// threshold tiny = colMax·n·1e-14, computed from the FP64 values as the
+// product and the difference, where δl is the stored error bound of the
+// single-precision L entry (it follows from the bounds of its numerator and of
+// the pivot at the column that produced it); complex products, quotients and
+// magnitudes carry their larger rounding constants. The double-precision
+// factorization's values lie well inside these bounds (its own rounding is
+// 2^29 times smaller), so with intervals |x_r| ± kMixSafety·e_r a column's
+// pivot is accepted only when the reference rule evaluated anywhere inside
+// the intervals picks the same row: the argmax must win by more than both
+// intervals, the diagonal preference (|x_k| >= 1e-3·best,
+// solver/sparse.go:274-277) must hold or fail for every value in them, and
+// the largest candidate must clear the singularity threshold
+// tiny = colMax·n·1e-14, computed from the double-precision values as the
 // reference does. The pivot rows of an accepted mixed factorization therefore
-// equal the FP64 factorization's, which the parity tests check row for row.
+// equal the double-precision factorization's, which the parity tests check
+// row for row.
 //
-// Storage inside the FP64 workspace: the FP32 column values in the first n
-// floats of xw and their error bounds in the next n; the FP32 L entries in the
-// first capL floats of lx and their error bounds in the next capL; U and the
-// pivots in the first halves of ux and udiag. The refinement uses w (permuted
-// right-hand side, FP64), xw (the solution y, FP64), x (the residual, FP64,
-// until the solution is written), xi and pstack (FP32 right-hand side and
-// correction). Entry i of a lane is at i*B + c as everywhere else.
-constexpr float kMixU = 5.9604645e-8f;  // 2^-24, FP32 unit round-off
+// Storage inside the double-precision workspace (entry i of a lane at i*B + c
+// as everywhere else): the single-precision column values at the start of xw
+// and their float error bounds after them; the single-precision L entries at
+// the start of lx and their error bounds after them; U and the pivots at the
+// start of ux and udiag. The refinement uses w (permuted right-hand side), xw
+// (the solution y), x (the residual, until the solution is written) in double
+// precision, and two single-precision vectors: the right-hand side and the
+// correction (xi and pstack for real systems; the free tails of lx and udiag
+// for complex ones).
+constexpr float kMixU = 5.9604645e-8f;  // 2^-24, single-precision unit round-off
 constexpr float kMixSafety = 4.0f;      // margin on the first-order error bounds
 constexpr int kMixMaxIter = 8;
 
-template <bool Unit = false>
-KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
+struct fcplx {
+  float re, im;
+};
+KH_HD inline fcplx operator*(fcplx a, fcplx b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+KH_HD inline fcplx operator-(fcplx a, fcplx b) { return {a.re - b.re, a.im - b.im}; }
+
+// Single-precision partner of each value type, with the rounding constants
+// (in units of u) of the operations the mixed factorization uses.
+template <typename T> struct Mix;
+template <> struct Mix<double> {
+  using Lo = float;
+  static constexpr float kMul = 1.0f, kDiv = 1.0f, kMag = 0.0f;
+  KH_HD static float lo(double v) { return (float)v; }
+  KH_HD static double hi(float v) { return (double)v; }
+  KH_HD static float conv_err(double v, float f) { return (float)fabs(v - (double)f); }
+  KH_HD static float mag(float v) { return fabsf(v); }
+  KH_HD static float mag_err(float, float) { return 0.0f; }
+  KH_HD static float div(float a, float b) { return a / b; }
+  KH_HD static bool zero(float v) { return v == 0.0f; }
+  KH_HD static double nrm(double v) { return fabs(v); }
+  KH_HD static bool in_range(double v) {
+    const double av = fabs(v);
+    return av == 0 || (av <= 1e30 && av >= 1e-30);
+  }
+};
+template <> struct Mix<kcplx> {
+  using Lo = fcplx;
+  // complex product 2√2 u, Smith's quotient ~6u, scaled magnitude ~3u
+  static constexpr float kMul = 3.0f, kDiv = 8.0f, kMag = 3.0f;
+  KH_HD static fcplx lo(kcplx v) { return {(float)v.re, (float)v.im}; }
+  KH_HD static kcplx hi(fcplx v) { return {(double)v.re, (double)v.im}; }
+  KH_HD static float conv_err(kcplx v, fcplx f) {
+    return (float)(fabs(v.re - (double)f.re) + fabs(v.im - (double)f.im));
+  }
+  // the scaled magnitude is exact for a purely real or imaginary value
+  KH_HD static float mag_err(fcplx v, float av) {
+    return (v.re != 0.0f && v.im != 0.0f) ? kMag * kMixU * av : 0.0f;
+  }
+  KH_HD static float mag(fcplx v) {
+    float p = fabsf(v.re), q = fabsf(v.im);
+    if (p < q) {
+      const float t = p;
+      p = q;
+      q = t;
+    }
+    if (p == 0.0f) return 0.0f;
+    q = q / p;
+    return p * sqrtf(1.0f + q * q);
+  }
+  KH_HD static fcplx div(fcplx n, fcplx m) {
+    if (fabsf(m.re) >= fabsf(m.im)) {
+      const float ratio = m.im / m.re, denom = m.re + ratio * m.im;
+      return {(n.re + n.im * ratio) / denom, (n.im - n.re * ratio) / denom};
+    }
+    const float ratio = m.re / m.im, denom = m.im + ratio * m.re;
+    return {(n.re * ratio + n.im) / denom, (n.im * ratio - n.re) / denom};
+  }
+  KH_HD static bool zero(fcplx v) { return v.re == 0.0f && v.im == 0.0f; }
+  KH_HD static double nrm(kcplx v) { return fmax(fabs(v.re), fabs(v.im)); }
+  KH_HD static bool in_range(kcplx v) { return Mix<double>::in_range(v.re) && Mix<double>::in_range(v.im); }
+};
+
+template <typename T, bool Unit = false>
+KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
+  using M = Mix<T>;
+  using Lo = typename M::Lo;
   const int64_t B = Unit ? 1 : a.B;
   if (Unit) c = 0;
   const int n = a.n;
@@ -407,20 +478,21 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
   const int32_t* ap = a.pool + pd.ap;
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
-  float* xf = reinterpret_cast<float*>(a.xw);
-  float* xe = xf + (int64_t)n * B;
-  float* lxf = reinterpret_cast<float*>(a.lx);
-  float* lxe = lxf + a.capL * B;
-  float* uxf = reinterpret_cast<float*>(a.ux);
-  float* udf = reinterpret_cast<float*>(a.udiag);
+  Lo* xf = reinterpret_cast<Lo*>(a.xw);
+  float* xe = reinterpret_cast<float*>(xf + (int64_t)n * B);
+  Lo* lxf = reinterpret_cast<Lo*>(a.lx);
+  float* lxe = reinterpret_cast<float*>(lxf + a.capL * B);
+  Lo* uxf = reinterpret_cast<Lo*>(a.ux);
+  Lo* udf = reinterpret_cast<Lo*>(a.udiag);
   double scale = 0.0;
   for (int s = 0; s < a.nslots; s++) {
-    const double av = fabs(a.val[(int64_t)s * B + c]);
-    if (!kh_finite(av)) return LU_UNSURE;  // the FP64 path reports ErrNonFinite
+    const T v = a.val[(int64_t)s * B + c];
+    if (!lu_finite(v)) return LU_UNSURE;  // the double-precision path reports ErrNonFinite
+    const double av = lu_mag(v);
     if (av > scale) scale = av;
-    if (av != 0 && (av > 1e30 || av < 1e-30)) return LU_UNSURE;  // outside FP32's comfortable range
+    if (!M::in_range(v)) return LU_UNSURE;  // outside single precision's comfortable range
   }
-  if (scale == 0) return LU_UNSURE;  // the FP64 path reports ErrSingular
+  if (scale == 0) return LU_UNSURE;  // the double-precision path reports ErrSingular
   bool reuse = a.cpid[c] == pidc;
   a.cpid[c] = -1;
   for (int i = 0; i < n; i++) a.pinv[(int64_t)i * B + c] = -1;
@@ -436,7 +508,7 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
       r1 = a.rp[(int64_t)(k + 1) * B + c];
       diag_in_reach = a.dk[(int64_t)k * B + c] != 0;
     } else {
-      int top = lu_reach<double, Unit>(a, c, k, ap, arow);
+      int top = lu_reach<T, Unit>(a, c, k, ap, arow);
       r1 = r0 + (n - top);
       if (r1 > a.capR) return LU_CAPACITY;
       for (int p = top; p < n; p++) a.rch[(r0 + p - top) * B + c] = a.xi[(int64_t)p * B + c];
@@ -447,17 +519,17 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
     nr = r1;
     for (int64_t p = r0; p < r1; p++) {
       const int64_t ir = (int64_t)a.rch[p * B + c] * B + c;
-      xf[ir] = 0.0f;
+      xf[ir] = M::lo(lu_zero_value<T>());
       xe[ir] = 0.0f;
     }
     double colMax = 0.0;
     for (int p = ap[k]; p < ap[k + 1]; p++) {
-      const double v = a.val[(int64_t)aslot[p] * B + c];
+      const T v = a.val[(int64_t)aslot[p] * B + c];
       const int64_t ir = (int64_t)arow[p] * B + c;
-      const float vf = (float)v;
+      const Lo vf = M::lo(v);
       xf[ir] = vf;
-      xe[ir] = kMixU * fabsf(vf);
-      const double av = fabs(v);
+      xe[ir] = M::conv_err(v, vf);
+      const double av = lu_mag(v);
       if (av > colMax) colMax = av;
     }
     const double tiny = colMax * (double)n * 1e-14;
@@ -465,16 +537,18 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
       const int32_t r = a.rch[p * B + c];
       const int32_t j = a.pinv[(int64_t)r * B + c];
       if (j < 0) continue;
-      const float xj = xf[(int64_t)r * B + c], ej = xe[(int64_t)r * B + c];
-      if (xj == 0.0f && ej == 0.0f) continue;
-      const float axj = fabsf(xj);
+      const Lo xj = xf[(int64_t)r * B + c];
+      const float ej = xe[(int64_t)r * B + c];
+      if (M::zero(xj) && ej == 0.0f) continue;
+      const float axj = M::mag(xj);
       const int32_t q1 = a.lp[(int64_t)(j + 1) * B + c];
       for (int32_t q = a.lp[(int64_t)j * B + c]; q < q1; q++) {
         const int64_t ix = (int64_t)a.li[(int64_t)q * B + c] * B + c;
-        const float l = lxf[(int64_t)q * B + c];
-        const float prod = l * xj;
-        const float xn = xf[ix] - prod;
-        xe[ix] = xe[ix] + fabsf(l) * ej + lxe[(int64_t)q * B + c] * axj + kMixU * (fabsf(prod) + fabsf(xn));
+        const Lo l = lxf[(int64_t)q * B + c];
+        const float al = M::mag(l);
+        const Lo prod = l * xj;
+        const Lo xn = xf[ix] - prod;
+        xe[ix] = xe[ix] + al * ej + lxe[(int64_t)q * B + c] * axj + kMixU * (M::kMul * al * axj + M::mag(xn));
         xf[ix] = xn;
       }
     }
@@ -491,7 +565,7 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
         nu++;
         continue;
       }
-      const float av = fabsf(xf[ir]), err = kMixSafety * xe[ir];
+      const float av = M::mag(xf[ir]), err = kMixSafety * (xe[ir] + M::mag_err(xf[ir], av));
       if (av > best || (av == best && r < piv)) {
         best = av;
         piv = r;
@@ -509,7 +583,8 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
     if (piv < 0 || !((double)maxlo > tiny)) return LU_UNSURE;
     bool diag = false;
     if (a.pinv[(int64_t)k * B + c] < 0 && diag_in_reach) {
-      const float ak = fabsf(xf[(int64_t)k * B + c]), ek = kMixSafety * xe[(int64_t)k * B + c];
+      const float ak = M::mag(xf[(int64_t)k * B + c]);
+      const float ek = kMixSafety * (xe[(int64_t)k * B + c] + M::mag_err(xf[(int64_t)k * B + c], ak));
       const float lok = ak - ek, hik = ak + ek;
       if ((double)lok >= 1e-3 * (double)maxhi && (double)lok > tiny) {
         diag = true;
@@ -520,15 +595,29 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
     if (diag) {
       piv = k;
     } else {
-      const float lop = best - kMixSafety * xe[(int64_t)piv * B + c];
-      const float other = rhi == piv ? maxhi2 : maxhi;
-      if (!(lop > other)) return LU_UNSURE;
+      // The argmax: every other candidate's interval lies below the winner's,
+      // or both values are exact and equal and the winner has the lower row
+      // (the reference's tie-break, solver/sparse.go:262-268).
+      const float wp = kMixSafety * (xe[(int64_t)piv * B + c] + M::mag_err(xf[(int64_t)piv * B + c], best));
+      const float lop = best - wp;
+      if (!(lop > (rhi == piv ? maxhi2 : maxhi))) {
+        for (int64_t p = r0; p < r1; p++) {
+          const int32_t r = a.rch[p * B + c];
+          const int64_t ir = (int64_t)r * B + c;
+          if (r == piv || a.pinv[ir] >= 0) continue;
+          const float av = M::mag(xf[ir]), wr = kMixSafety * (xe[ir] + M::mag_err(xf[ir], av));
+          if (lop > av + wr) continue;
+          if (wp == 0.0f && wr == 0.0f && av == best && piv < r) continue;
+          return LU_UNSURE;
+        }
+      }
     }
     if (reuse && piv != a.prow[(int64_t)k * B + c]) {
       reuse = false;
       for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
     }
-    const float pv = xf[(int64_t)piv * B + c], apv = fabsf(pv), epv = xe[(int64_t)piv * B + c];
+    const Lo pv = xf[(int64_t)piv * B + c];
+    const float apv = M::mag(pv), epv = xe[(int64_t)piv * B + c];
     udf[(int64_t)k * B + c] = pv;
     a.pinv[(int64_t)piv * B + c] = k;
     a.prow[(int64_t)k * B + c] = piv;
@@ -537,10 +626,11 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
       const int64_t ir = (int64_t)r * B + c;
       if (a.pinv[ir] < 0) {
         if (nl >= a.capL) return LU_CAPACITY;
-        const float l = xf[ir] / pv, al = fabsf(l);
+        const Lo l = M::div(xf[ir], pv);
+        const float al = M::mag(l);
         a.li[nl * B + c] = r;
         lxf[nl * B + c] = l;
-        lxe[nl * B + c] = (xe[ir] + al * epv) / apv + kMixU * al;
+        lxe[nl * B + c] = (xe[ir] + al * epv) / apv + M::kDiv * kMixU * al;
         nl++;
       }
     }
@@ -552,19 +642,22 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<double>& a, int64_t c) {
   return LU_OK;
 }
 
-// Solve with the FP32 factors of lu_factor_mixed and refine in FP64. Each pass
-// forms the FP64 residual of the current solution y against the FP64 stamped
-// values (the first pass starts from y = 0, r = b), rounds it to FP32, solves
-// for the correction with the FP32 factors and adds it to y in FP64. The
-// corrections shrink geometrically; with rho the ratio of the last two, what
-// remains after a pass is about rho times its correction. It stops when that
-// is within 2^-46 (1.4e-14) of every component plus 2^-60 of the largest
-// component (for components at zero), or when the corrections have reached
-// the FP64 residual's own noise (no longer halving, below 1e-12 of the
-// solution). Divergence, a non-finite value or kMixMaxIter passes return
-// LU_UNSURE. *iters receives the number of FP32 solves.
-template <bool Unit = false>
-KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters) {
+// Solve with the single-precision factors of lu_factor_mixed and refine in
+// double precision. Each pass forms the residual of the current solution y
+// against the stamped double-precision values (the first pass starts from
+// y = 0, r = b), rounds it to single precision, solves for the correction with
+// the single-precision factors and adds it to y. The corrections shrink
+// geometrically; with rho the ratio of the last two, what remains after a
+// pass is about rho times its correction. It stops when that is within 2^-46
+// (1.4e-14) of every component plus 2^-60 of the largest component (for
+// components at zero), or when the corrections have reached the residual's
+// own round-off (no longer halving, below 1e-12 of the solution). Divergence,
+// a non-finite value or kMixMaxIter passes return LU_UNSURE. *iters receives
+// the number of single-precision solves.
+template <typename T, bool Unit = false>
+KH_HD inline int lu_solve_mixed(const LuArgsT<T>& a, int64_t c, int* iters) {
+  using M = Mix<T>;
+  using Lo = typename M::Lo;
   const int64_t B = Unit ? 1 : a.B;
   if (Unit) c = 0;
   const int n = a.n;
@@ -574,37 +667,46 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters)
   const int32_t* ap = a.pool + pd.ap;
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
-  const float* lxf = reinterpret_cast<const float*>(a.lx);
-  const float* uxf = reinterpret_cast<const float*>(a.ux);
-  const float* udf = reinterpret_cast<const float*>(a.udiag);
-  double* w = a.w;
-  double* y = a.xw;
-  double* r = a.x;
-  float* t = reinterpret_cast<float*>(a.xi);
-  float* z = reinterpret_cast<float*>(a.pstack);
+  const Lo* lxf = reinterpret_cast<const Lo*>(a.lx);
+  const Lo* uxf = reinterpret_cast<const Lo*>(a.ux);
+  const Lo* udf = reinterpret_cast<const Lo*>(a.udiag);
+  T* w = a.w;
+  T* y = a.xw;
+  T* r = a.x;
+  Lo* t;
+  Lo* z;
+  if constexpr (sizeof(Lo) == 4) {
+    t = reinterpret_cast<Lo*>(a.xi);
+    z = reinterpret_cast<Lo*>(a.pstack);
+  } else {
+    // lx holds 2*capL*B single-precision values' worth of L and bounds in its
+    // first 12*capL*B bytes; capL >= 4n leaves room for n more values.
+    t = reinterpret_cast<Lo*>(reinterpret_cast<char*>(a.lx) + 12 * a.capL * B);
+    z = reinterpret_cast<Lo*>(a.udiag) + (int64_t)n * B;
+  }
   for (int i = 0; i < n; i++) w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
   double prev = __builtin_huge_val();
   int it = 0;
   for (;;) {
     if (it == 0) {
-      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = (float)w[(int64_t)i * B + c];
+      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = M::lo(w[(int64_t)i * B + c]);
     } else {
       for (int i = 0; i < n; i++) r[(int64_t)i * B + c] = w[(int64_t)i * B + c];
       for (int k = 0; k < n; k++) {
-        const double yk = y[(int64_t)k * B + c];
-        if (yk == 0) continue;
+        const T yk = y[(int64_t)k * B + c];
+        if (lu_zero(yk)) continue;
         for (int p = ap[k]; p < ap[k + 1]; p++) {
           const int64_t ir = (int64_t)arow[p] * B + c;
-          const double prod = a.val[(int64_t)aslot[p] * B + c] * yk;
+          const T prod = a.val[(int64_t)aslot[p] * B + c] * yk;
           r[ir] = r[ir] - prod;
         }
       }
-      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = (float)r[(int64_t)i * B + c];
+      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = M::lo(r[(int64_t)i * B + c]);
     }
     for (int k = 0; k < n; k++) {
-      const float zk = t[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
+      const Lo zk = t[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
       z[(int64_t)k * B + c] = zk;
-      if (zk == 0.0f) continue;
+      if (M::zero(zk)) continue;
       const int32_t p1 = a.lp[(int64_t)(k + 1) * B + c];
       for (int32_t p = a.lp[(int64_t)k * B + c]; p < p1; p++) {
         const int64_t ix = (int64_t)a.li[(int64_t)p * B + c] * B + c;
@@ -612,9 +714,9 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters)
       }
     }
     for (int k = n - 1; k >= 0; k--) {
-      const float zk = z[(int64_t)k * B + c] / udf[(int64_t)k * B + c];
+      const Lo zk = M::div(z[(int64_t)k * B + c], udf[(int64_t)k * B + c]);
       z[(int64_t)k * B + c] = zk;
-      if (zk == 0.0f) continue;
+      if (M::zero(zk)) continue;
       const int32_t p1 = a.up[(int64_t)(k + 1) * B + c];
       for (int32_t p = a.up[(int64_t)k * B + c]; p < p1; p++) {
         const int64_t ix = (int64_t)a.ui[(int64_t)p * B + c] * B + c;
@@ -623,12 +725,12 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters)
     }
     double ymax = 0.0, dmax = 0.0;
     for (int k = 0; k < n; k++) {
-      const double dz = (double)z[(int64_t)k * B + c];
-      const double yk = (it == 0 ? 0.0 : y[(int64_t)k * B + c]) + dz;
-      if (!kh_finite(yk)) return LU_UNSURE;
+      const T dz = M::hi(z[(int64_t)k * B + c]);
+      const T yk = it == 0 ? dz : y[(int64_t)k * B + c] + dz;
+      if (!lu_finite(yk)) return LU_UNSURE;
       y[(int64_t)k * B + c] = yk;
-      if (fabs(yk) > ymax) ymax = fabs(yk);
-      if (fabs(dz) > dmax) dmax = fabs(dz);
+      ymax = fmax(ymax, M::nrm(yk));
+      dmax = fmax(dmax, M::nrm(dz));
     }
     it++;
     const double rel = ymax > 0 ? dmax / ymax : 0.0;
@@ -638,9 +740,9 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<double>& a, int64_t c, int* iters)
       const double rho = fmin(1.0, 2.0 * rel / prev);
       bool tight = true;
       for (int k = 0; k < n && tight; k++) {
-        const double yk = y[(int64_t)k * B + c];
-        const double dz = (double)z[(int64_t)k * B + c];
-        tight = rho * fabs(dz) <= 0x1p-46 * fabs(yk) + 0x1p-60 * ymax;
+        const double yk = M::nrm(y[(int64_t)k * B + c]);
+        const double dz = M::nrm(M::hi(z[(int64_t)k * B + c]));
+        tight = rho * dz <= 0x1p-46 * yk + 0x1p-60 * ymax;
       }
       if (tight) break;
       if (rel <= 1e-12 && rel > 0.5 * prev) break;  // at the residual's round-off

@@ -182,16 +182,30 @@ __global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
 // Factors and solves every stamped lane (complex GP LU), evaluates its
 // ledger (analysis/ac.go:148-190) and writes its solution to the caller's
 // [nf][n][B] complex array.
+// precision 1: complex64 factors with complex128 refinement (kh_lu.h), the
+// complex128 path for every system it cannot certify; mixc[0] counts the
+// complex64 solves of the refinement, mixc[1] the systems redone.
 __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu, double ledger_tol,
-                                                     int32_t skip_energy, double* xout) {
+                                                     int32_t skip_energy, double* xout, int32_t precision,
+                                                     unsigned long long* mixc) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= a.BL || a.lstatus[l] < 0) return;
   int st;
   if (a.ckt.dense) {
     st = lu_dense(lu, l, a.ckt.slot_i, a.ckt.slot_j);
   } else {
-    st = lu_factor(lu, l);
-    if (st == LU_OK) st = lu_solve(lu, l);
+    st = LU_UNSURE;
+    if (precision) {
+      int passes = 0;
+      st = lu_factor_mixed(lu, l);
+      if (st == LU_OK) st = lu_solve_mixed(lu, l, &passes);
+      if (st == LU_OK) atomicAdd(&mixc[0], (unsigned long long)passes);
+      if (st == LU_UNSURE) atomicAdd(&mixc[1], 1ull);
+    }
+    if (st == LU_UNSURE) {
+      st = lu_factor(lu, l);
+      if (st == LU_OK) st = lu_solve(lu, l);
+    }
   }
   if (st != LU_OK) {
     a.lstatus[l] = st;
@@ -1270,6 +1284,8 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   KH_HIP(done.alloc((size_t)B * 4));
   KH_HIP(points.alloc((size_t)B * 4));
   KH_HIP(maxrr.alloc((size_t)B * 8));
+  DBuf mixc;
+  KH_HIP(mixc.alloc(16));
   DevPatterns pats;
   LuWorkT<kcplx> work;
   std::vector<int32_t> cpid(B, -1);
@@ -1377,7 +1393,8 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     work.fill(lu);
     if (!G.dense) KH_HIP(work.invalidate());
     KH_HIP(hipEventRecord(ev[0], 0));
-    hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac);
+    hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac,
+                       p.precision, mixc.as<unsigned long long>());
     KH_HIP(hipGetLastError());
     KH_HIP(hipEventRecord(ev[1], 0));
     hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),
@@ -1398,6 +1415,12 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     }
   }
   KH_HIP(hipDeviceSynchronize());
+  if (stats && p.precision) {
+    unsigned long long mc[2] = {0, 0};
+    KH_HIP(hipMemcpy(mc, mixc.p, 16, hipMemcpyDeviceToHost));
+    stats->refine_passes += (int64_t)mc[0];
+    stats->fp64_redos += (int64_t)mc[1];
+  }
   if (d_points) KH_HIP(hipMemcpy(d_points, points.p, (size_t)B * 4, hipMemcpyDeviceToDevice));
   if (d_ledger) KH_HIP(hipMemcpy(d_ledger, maxrr.p, (size_t)B * 8, hipMemcpyDeviceToDevice));
   if (stats) stats->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -64,3 +64,22 @@ def test_rlc_grid_chunked_sweep_identical():
         del os.environ["KH_AC_CHUNK_LANES"]
     assert (st == 0).all() and (pts == len(f)).all()
     assert np.array_equal(x0, x1)
+
+
+def test_rlc_grid_mixed_precision_matches_complex128():
+    """precision 1 on the AC sweep: complex64 factors + complex128 refinement,
+    complex128 redo of whatever the mixed path cannot certify. Every system
+    agrees with the complex128 sweep to 1e-12 of its largest unknown."""
+    b = W.rlc_grid(8, 8, draws=4)
+    f = W.rlc_grid_frequencies(32)
+    t = api.Topology(b.circuit.num_nodes, b.elements, b.circuit_params(0))
+    r0 = api.ac(t, b.params, f)
+    r1 = api.ac(t, b.params, f, opts=api.Options(precision=1))
+    x0, x1 = r0.x.cpu().numpy(), r1.x.cpu().numpy()
+    assert (r1.status.cpu().numpy() == 0).all()
+    den = np.max(np.abs(x0), axis=1)  # [nf, B]
+    rel = np.max(np.abs(x1 - x0), axis=1) / np.maximum(den, 1e-300)
+    s = r1.stats
+    print(f"AC mixed: max rel {rel.max():.2e}, refine passes {s['refine_passes']}, redone {s['fp64_redos']}")
+    assert rel.max() <= 1e-12
+    assert s["refine_passes"] > 0
