@@ -183,11 +183,13 @@ __global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
 // ledger (analysis/ac.go:148-190) and writes its solution to the caller's
 // [nf][n][B] complex array.
 // precision 1: complex64 factors with complex128 refinement (kh_lu.h), the
-// complex128 path for every system it cannot certify; mixc[0] counts the
-// complex64 solves of the refinement, mixc[1] the systems redone.
+// complex128 path for every system it cannot certify; mixl[l] receives the
+// lane's complex64 solves, -1 where it was redone (per-lane words summed on
+// the host: a 64-bit atomicAdd from divergent lanes here corrupted the
+// results of lanes 32-63 of a wavefront on gfx950).
 __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu, double ledger_tol,
                                                      int32_t skip_energy, double* xout, int32_t precision,
-                                                     unsigned long long* mixc) {
+                                                     int32_t* mixl) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= a.BL || a.lstatus[l] < 0) return;
   int st;
@@ -199,8 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
       int passes = 0;
       st = lu_factor_mixed(lu, l);
       if (st == LU_OK) st = lu_solve_mixed(lu, l, &passes);
-      if (st == LU_OK) atomicAdd(&mixc[0], (unsigned long long)passes);
-      if (st == LU_UNSURE) atomicAdd(&mixc[1], 1ull);
+      mixl[l] = st == LU_OK ? passes : (st == LU_UNSURE ? -1 : 0);
     }
     if (st == LU_UNSURE) {
       st = lu_factor(lu, l);
@@ -1284,8 +1285,7 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   KH_HIP(done.alloc((size_t)B * 4));
   KH_HIP(points.alloc((size_t)B * 4));
   KH_HIP(maxrr.alloc((size_t)B * 8));
-  DBuf mixc;
-  KH_HIP(mixc.alloc(16));
+  DBuf mixl;
   DevPatterns pats;
   LuWorkT<kcplx> work;
   std::vector<int32_t> cpid(B, -1);
@@ -1361,6 +1361,8 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   KH_HIP(lst.alloc((size_t)BLmax * 4));
   KH_HIP(lrr.alloc((size_t)BLmax * 8));
   KH_HIP(pid.alloc((size_t)BLmax * 4));
+  KH_HIP(mixl.alloc((size_t)BLmax * 4));
+  std::vector<int32_t> hmix(p.precision ? (size_t)BLmax : 0);
   a.val = val.as<kcplx>();
   a.rhs = rhs.as<kcplx>();
   a.lstatus = lst.as<int32_t>();
@@ -1394,7 +1396,7 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     if (!G.dense) KH_HIP(work.invalidate());
     KH_HIP(hipEventRecord(ev[0], 0));
     hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac,
-                       p.precision, mixc.as<unsigned long long>());
+                       p.precision, mixl.as<int32_t>());
     KH_HIP(hipGetLastError());
     KH_HIP(hipEventRecord(ev[1], 0));
     hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),
@@ -1412,15 +1414,16 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
       stats->lu_seconds += ms * 1e-3;
       stats->lu_launches++;
       stats->bytes_algorithmic += a.BL * 16 * (nnz + 2 * (int64_t)n);
+      if (p.precision) {
+        KH_HIP(hipMemcpy(hmix.data(), mixl.p, (size_t)a.BL * 4, hipMemcpyDeviceToHost));
+        for (int64_t l = 0; l < a.BL; l++) {
+          if (hmix[l] > 0) stats->refine_passes += hmix[l];
+          if (hmix[l] < 0) stats->fp64_redos++;
+        }
+      }
     }
   }
   KH_HIP(hipDeviceSynchronize());
-  if (stats && p.precision) {
-    unsigned long long mc[2] = {0, 0};
-    KH_HIP(hipMemcpy(mc, mixc.p, 16, hipMemcpyDeviceToHost));
-    stats->refine_passes += (int64_t)mc[0];
-    stats->fp64_redos += (int64_t)mc[1];
-  }
   if (d_points) KH_HIP(hipMemcpy(d_points, points.p, (size_t)B * 4, hipMemcpyDeviceToDevice));
   if (d_ledger) KH_HIP(hipMemcpy(d_ledger, maxrr.p, (size_t)B * 8, hipMemcpyDeviceToDevice));
   if (stats) stats->seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
