@@ -170,6 +170,9 @@ typedef struct kh_stats {
   double  cycles_control;    /* ... and in the rest of the analysis (stamping, Newton, LTE, ledger) */
   int64_t refine_passes;     /* precision 1: FP32 solves of the refinement, summed */
   int64_t fp64_redos;        /* precision 1: systems the mixed path could not certify (redone in FP64) */
+  /* parts of cycles_control, summed over circuits: stamping (Sim.solve's Clear + Load), Newton's
+     convergence test, LTE step proposals, accepting a time point, its energy-ledger record */
+  double  cycles_stamp, cycles_converge, cycles_lte, cycles_accept, cycles_ledger;
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/

@@ -63,7 +63,9 @@ class Stats(C.Structure):
                 ("ticks", C.c_int64), ("bytes_algorithmic", C.c_int64), ("seconds", C.c_double),
                 ("lu_seconds", C.c_double), ("control_seconds", C.c_double), ("lu_launches", C.c_int64),
                 ("control_launches", C.c_int64), ("cycles_lu", C.c_double), ("cycles_control", C.c_double),
-                ("refine_passes", C.c_int64), ("fp64_redos", C.c_int64)]
+                ("refine_passes", C.c_int64), ("fp64_redos", C.c_int64),
+                ("cycles_stamp", C.c_double), ("cycles_converge", C.c_double), ("cycles_lte", C.c_double),
+                ("cycles_accept", C.c_double), ("cycles_ledger", C.c_double)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

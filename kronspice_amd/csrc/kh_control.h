@@ -28,7 +28,8 @@ enum { WAIT_NONE = 0, WAIT_PATTERN = 1, WAIT_CAPACITY = 2 };
 enum {
   CD_TIME = 0, CD_H, CD_GMIN, CD_DIAG_GMIN, CD_PTG, CD_SRCFACT, CD_GM_G, CD_GM_FACTOR, CD_SS_DONE, CD_SS_STEP,
   CD_SS_F, CD_PT_H, CD_TR_T, CD_TR_H, CD_TR_TN, CD_PEAK, CD_WORST, CD_MAXREL, CD_LEDGER, CD_MINSTEP_SEEN,
-  CD_MAXSTEP_SEEN, CD_CYC_LU, CD_CYC_CTL, CD_REFINE, CD_REDO, NCD
+  CD_MAXSTEP_SEEN, CD_CYC_LU, CD_CYC_CTL, CD_REFINE, CD_REDO, CD_CYC_STAMP, CD_CYC_CONV, CD_CYC_LTE,
+  CD_CYC_LEDGER, CD_CYC_ACCEPT, NCD
 };
 
 enum { PH_INIT = 0, PH_RUN = 1, PH_DONE = 2 };
@@ -105,6 +106,12 @@ KH_HD inline bool retryable(int e) { return e == ST_NOCONV || e == ST_NONFINITE 
 
 // The state machine's device dispatch depends on the element kinds compiled
 // in (KH_KINDS), so each kind set gets its own namespace.
+// Shader clock for the per-phase cycle counters (kh_stats); 0 where the
+// including unit provides none.
+#ifndef KH_CLOCK
+#define KH_CLOCK() ((uint64_t)0)
+#endif
+
 inline namespace KH_CAT(kinds_, KH_KINDS) {
 
 // One circuit's state machine over its own (unit-stride) arrays; the control
@@ -404,6 +411,7 @@ struct Ctl {
   // ---- solve request (Sim.solve, analysis/op.go:289-322): stamp now, factor
   // in the next LU launch.
   KH_HD void request_solve() {
+    const uint64_t t0 = KH_CLOCK();
     for (int i = 0; i < n; i++) {
       X(A.xold, i) = X(A.x, i);
       X(A.rhs, i) = 0;
@@ -425,6 +433,7 @@ struct Ctl {
     }
     ci[CI_NEED_SOLVE] = 1;
     if (L.grew && !A.ckt.dense) ci[CI_WAIT] = WAIT_PATTERN;  // the dense LU has no pattern
+    cd[CD_CYC_STAMP] += (double)(KH_CLOCK() - t0);
   }
   // Slot of the diagonal entry (i, i): the device slot when a device stamps
   // it, else a slot reserved after the device slots.
@@ -460,7 +469,10 @@ struct Ctl {
       else request_solve();
       return;
     }
-    if (converged()) { newton_done(ST_OK); return; }
+    const uint64_t t0 = KH_CLOCK();
+    const bool conv = converged();
+    cd[CD_CYC_CONV] += (double)(KH_CLOCK() - t0);
+    if (conv) { newton_done(ST_OK); return; }
     if (ci[CI_NT_IT] >= ci[CI_NT_MAX]) newton_done(ST_NOCONV);
     else request_solve();
   }
@@ -826,7 +838,9 @@ struct Ctl {
       tran_loop_top();
       return;
     }
+    uint64_t t0 = KH_CLOCK();
     double proposal = truncate(tn, h, order);
+    cd[CD_CYC_LTE] += (double)(KH_CLOCK() - t0);
     if (proposal < 0.9 * h) {
       h = fmax(proposal, h / 8);
       cd[CD_TR_H] = h;
@@ -834,8 +848,12 @@ struct Ctl {
       tran_loop_top();
       return;
     }
+    t0 = KH_CLOCK();
     accept(tn, MODE_TRAN);
+    const uint64_t t1 = KH_CLOCK();
     int le = record(tn, false);
+    cd[CD_CYC_ACCEPT] += (double)(t1 - t0);
+    cd[CD_CYC_LEDGER] += (double)(KH_CLOCK() - t1);
     if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
     for (int i = 0; i < n; i++) X(A.xacc, i) = X(A.x, i);
     ci[CI_ACCEPTED]++;

@@ -936,6 +936,11 @@ struct GroupRun {
       stats.cycles_control += dv[c * NCD + CD_CYC_CTL];
       stats.refine_passes += (int64_t)dv[c * NCD + CD_REFINE];
       stats.fp64_redos += (int64_t)dv[c * NCD + CD_REDO];
+      stats.cycles_stamp += dv[c * NCD + CD_CYC_STAMP];
+      stats.cycles_converge += dv[c * NCD + CD_CYC_CONV];
+      stats.cycles_lte += dv[c * NCD + CD_CYC_LTE];
+      stats.cycles_accept += dv[c * NCD + CD_CYC_ACCEPT];
+      stats.cycles_ledger += dv[c * NCD + CD_CYC_LEDGER];
     }
     // Algorithmic bytes per executed solve: matrix values and right-hand side
     // read once, solution written once, 8*(nnzA + 2n), plus the device
@@ -1086,6 +1091,11 @@ struct Runner {
       stats.cycles_control += g->stats.cycles_control;
       stats.refine_passes += g->stats.refine_passes;
       stats.fp64_redos += g->stats.fp64_redos;
+      stats.cycles_stamp += g->stats.cycles_stamp;
+      stats.cycles_converge += g->stats.cycles_converge;
+      stats.cycles_lte += g->stats.cycles_lte;
+      stats.cycles_accept += g->stats.cycles_accept;
+      stats.cycles_ledger += g->stats.cycles_ledger;
     }
   }
 };

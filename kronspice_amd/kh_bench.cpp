@@ -174,7 +174,7 @@ int main(int argc, char** argv) {
   kh_stats st;
   for (int w = 0; w < warmup; w++) CHECK_KH(kh_batch_run(ngroups, groups.data(), &opt, sp, &st));
   CHECK_HIP(hipDeviceSynchronize());
-  double lu_s = 0, bytes = 0, cyc_lu = 0, cyc_ctl = 0;
+  double lu_s = 0, bytes = 0, cyc_lu = 0, cyc_ctl = 0, cyc_part[5] = {0, 0, 0, 0, 0};
   int64_t solves = 0, launches = 0, passes = 0, redos = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (int s = 0; s < steps; s++) {
@@ -187,6 +187,11 @@ int main(int argc, char** argv) {
     cyc_ctl += st.cycles_control;
     passes += st.refine_passes;
     redos += st.fp64_redos;
+    cyc_part[0] += st.cycles_stamp;
+    cyc_part[1] += st.cycles_converge;
+    cyc_part[2] += st.cycles_lte;
+    cyc_part[3] += st.cycles_accept;
+    cyc_part[4] += st.cycles_ledger;
   }
   CHECK_HIP(hipDeviceSynchronize());
   const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -204,9 +209,12 @@ int main(int argc, char** argv) {
          "\"precision\": %d, \"refine_passes_per_step\": %.1f, \"fp64_redos_per_step\": %.1f, \"steps\": %d, "
          "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
-         "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"status_histogram\": %s}\n",
+         "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"cycles_stamp\": %.6g, "
+         "\"cycles_converge\": %.6g, \"cycles_lte\": %.6g, \"cycles_accept\": %.6g, \"cycles_ledger\": %.6g, "
+         "\"status_histogram\": %s}\n",
          (long long)circuits, ngroups, staging, cpw, waves, precision, (double)passes / steps, (double)redos / steps, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, cyc_lu / steps, cyc_ctl / steps,
+         cyc_part[0] / steps, cyc_part[1] / steps, cyc_part[2] / steps, cyc_part[3] / steps, cyc_part[4] / steps,
          h.c_str());
   for (void* p : owned) (void)hipFree(p);
   for (auto t : topo) kh_topology_destroy(t);
