@@ -391,15 +391,17 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
 // equal the double-precision factorization's, which the parity tests check
 // row for row.
 //
-// Storage inside the double-precision workspace (entry i of a lane at i*B + c
-// as everywhere else): the single-precision column values at the start of xw
-// and their float error bounds after them; the single-precision L entries at
-// the start of lx and their error bounds after them; U and the pivots at the
-// start of ux and udiag. The refinement uses w (permuted right-hand side), xw
-// (the solution y), x (the residual, until the solution is written) in double
-// precision, and two single-precision vectors: the right-hand side and the
-// correction (xi and pstack for real systems; the free tails of lx and udiag
-// for complex ones).
+// Storage inside the double-precision workspace: each single-precision value
+// lives in the first half of its own entry's double-precision slot (entry i of
+// lane c at i*B + c, as everywhere else), and a value's error bound in the
+// slot's second half, so no lane's single-precision data ever overlaps
+// another lane's entries: the column values and bounds in xw, the L entries
+// and bounds in lx, U in ux, the pivots in the first halves of udiag. The
+// refinement keeps w (permuted right-hand side), xw (the solution y) and x
+// (the residual, until the solution is written) in double precision; the
+// single-precision right-hand side of each pass is the first half of x's
+// slots (formed from the residual entry by entry) and the single-precision
+// correction the second half of udiag's.
 constexpr float kMixU = 5.9604645e-8f;  // 2^-24, single-precision unit round-off
 constexpr float kMixSafety = 4.0f;      // margin on the first-order error bounds
 constexpr int kMixMaxIter = 8;
@@ -466,6 +468,21 @@ template <> struct Mix<kcplx> {
   KH_HD static bool in_range(kcplx v) { return Mix<double>::in_range(v.re) && Mix<double>::in_range(v.im); }
 };
 
+// The single-precision value held in the first half of slot p of a T array,
+// and the float held in its second half.
+template <typename T>
+KH_HD inline typename Mix<T>::Lo& mix_lo(T* base, int64_t p) {
+  return *reinterpret_cast<typename Mix<T>::Lo*>(base + p);
+}
+template <typename T>
+KH_HD inline float& mix_err(T* base, int64_t p) {
+  return *reinterpret_cast<float*>(reinterpret_cast<char*>(base + p) + sizeof(typename Mix<T>::Lo));
+}
+template <typename T>
+KH_HD inline typename Mix<T>::Lo& mix_hi_lo(T* base, int64_t p) {  // a Lo in the slot's second half
+  return *reinterpret_cast<typename Mix<T>::Lo*>(reinterpret_cast<char*>(base + p) + sizeof(typename Mix<T>::Lo));
+}
+
 template <typename T, bool Unit = false>
 KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
   using M = Mix<T>;
@@ -478,12 +495,10 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
   const int32_t* ap = a.pool + pd.ap;
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
-  Lo* xf = reinterpret_cast<Lo*>(a.xw);
-  float* xe = reinterpret_cast<float*>(xf + (int64_t)n * B);
-  Lo* lxf = reinterpret_cast<Lo*>(a.lx);
-  float* lxe = reinterpret_cast<float*>(lxf + a.capL * B);
-  Lo* uxf = reinterpret_cast<Lo*>(a.ux);
-  Lo* udf = reinterpret_cast<Lo*>(a.udiag);
+  T* const xw = a.xw;
+  T* const lx = a.lx;
+  T* const ux = a.ux;
+  T* const ud = a.udiag;
   double scale = 0.0;
   for (int s = 0; s < a.nslots; s++) {
     const T v = a.val[(int64_t)s * B + c];
@@ -519,16 +534,16 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
     nr = r1;
     for (int64_t p = r0; p < r1; p++) {
       const int64_t ir = (int64_t)a.rch[p * B + c] * B + c;
-      xf[ir] = M::lo(lu_zero_value<T>());
-      xe[ir] = 0.0f;
+      mix_lo(xw, ir) = M::lo(lu_zero_value<T>());
+      mix_err(xw, ir) = 0.0f;
     }
     double colMax = 0.0;
     for (int p = ap[k]; p < ap[k + 1]; p++) {
       const T v = a.val[(int64_t)aslot[p] * B + c];
       const int64_t ir = (int64_t)arow[p] * B + c;
       const Lo vf = M::lo(v);
-      xf[ir] = vf;
-      xe[ir] = M::conv_err(v, vf);
+      mix_lo(xw, ir) = vf;
+      mix_err(xw, ir) = M::conv_err(v, vf);
       const double av = lu_mag(v);
       if (av > colMax) colMax = av;
     }
@@ -537,19 +552,19 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
       const int32_t r = a.rch[p * B + c];
       const int32_t j = a.pinv[(int64_t)r * B + c];
       if (j < 0) continue;
-      const Lo xj = xf[(int64_t)r * B + c];
-      const float ej = xe[(int64_t)r * B + c];
+      const Lo xj = mix_lo(xw, (int64_t)r * B + c);
+      const float ej = mix_err(xw, (int64_t)r * B + c);
       if (M::zero(xj) && ej == 0.0f) continue;
       const float axj = M::mag(xj);
       const int32_t q1 = a.lp[(int64_t)(j + 1) * B + c];
       for (int32_t q = a.lp[(int64_t)j * B + c]; q < q1; q++) {
         const int64_t ix = (int64_t)a.li[(int64_t)q * B + c] * B + c;
-        const Lo l = lxf[(int64_t)q * B + c];
+        const Lo l = mix_lo(lx, (int64_t)q * B + c);
         const float al = M::mag(l);
         const Lo prod = l * xj;
-        const Lo xn = xf[ix] - prod;
-        xe[ix] = xe[ix] + al * ej + lxe[(int64_t)q * B + c] * axj + kMixU * (M::kMul * al * axj + M::mag(xn));
-        xf[ix] = xn;
+        const Lo xn = mix_lo(xw, ix) - prod;
+        mix_err(xw, ix) = mix_err(xw, ix) + al * ej + mix_err(lx, (int64_t)q * B + c) * axj + kMixU * (M::kMul * al * axj + M::mag(xn));
+        mix_lo(xw, ix) = xn;
       }
     }
     int32_t piv = -1, rhi = -1;
@@ -561,11 +576,11 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
       if (pr >= 0) {
         if (nu >= a.capU) return LU_CAPACITY;
         a.ui[nu * B + c] = pr;
-        uxf[nu * B + c] = xf[ir];
+        mix_lo(ux, nu * B + c) = mix_lo(xw, ir);
         nu++;
         continue;
       }
-      const float av = M::mag(xf[ir]), err = kMixSafety * (xe[ir] + M::mag_err(xf[ir], av));
+      const float av = M::mag(mix_lo(xw, ir)), err = kMixSafety * (mix_err(xw, ir) + M::mag_err(mix_lo(xw, ir), av));
       if (av > best || (av == best && r < piv)) {
         best = av;
         piv = r;
@@ -583,8 +598,8 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
     if (piv < 0 || !((double)maxlo > tiny)) return LU_UNSURE;
     bool diag = false;
     if (a.pinv[(int64_t)k * B + c] < 0 && diag_in_reach) {
-      const float ak = M::mag(xf[(int64_t)k * B + c]);
-      const float ek = kMixSafety * (xe[(int64_t)k * B + c] + M::mag_err(xf[(int64_t)k * B + c], ak));
+      const float ak = M::mag(mix_lo(xw, (int64_t)k * B + c));
+      const float ek = kMixSafety * (mix_err(xw, (int64_t)k * B + c) + M::mag_err(mix_lo(xw, (int64_t)k * B + c), ak));
       const float lok = ak - ek, hik = ak + ek;
       if ((double)lok >= 1e-3 * (double)maxhi && (double)lok > tiny) {
         diag = true;
@@ -598,14 +613,14 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
       // The argmax: every other candidate's interval lies below the winner's,
       // or both values are exact and equal and the winner has the lower row
       // (the reference's tie-break, solver/sparse.go:262-268).
-      const float wp = kMixSafety * (xe[(int64_t)piv * B + c] + M::mag_err(xf[(int64_t)piv * B + c], best));
+      const float wp = kMixSafety * (mix_err(xw, (int64_t)piv * B + c) + M::mag_err(mix_lo(xw, (int64_t)piv * B + c), best));
       const float lop = best - wp;
       if (!(lop > (rhi == piv ? maxhi2 : maxhi))) {
         for (int64_t p = r0; p < r1; p++) {
           const int32_t r = a.rch[p * B + c];
           const int64_t ir = (int64_t)r * B + c;
           if (r == piv || a.pinv[ir] >= 0) continue;
-          const float av = M::mag(xf[ir]), wr = kMixSafety * (xe[ir] + M::mag_err(xf[ir], av));
+          const float av = M::mag(mix_lo(xw, ir)), wr = kMixSafety * (mix_err(xw, ir) + M::mag_err(mix_lo(xw, ir), av));
           if (lop > av + wr) continue;
           if (wp == 0.0f && wr == 0.0f && av == best && piv < r) continue;
           return LU_UNSURE;
@@ -616,9 +631,9 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
       reuse = false;
       for (int i = 0; i < n; i++) a.mark[(int64_t)i * B + c] = 0;
     }
-    const Lo pv = xf[(int64_t)piv * B + c];
-    const float apv = M::mag(pv), epv = xe[(int64_t)piv * B + c];
-    udf[(int64_t)k * B + c] = pv;
+    const Lo pv = mix_lo(xw, (int64_t)piv * B + c);
+    const float apv = M::mag(pv), epv = mix_err(xw, (int64_t)piv * B + c);
+    mix_lo(ud, (int64_t)k * B + c) = pv;
     a.pinv[(int64_t)piv * B + c] = k;
     a.prow[(int64_t)k * B + c] = piv;
     for (int64_t p = r0; p < r1; p++) {
@@ -626,11 +641,11 @@ KH_HD inline int lu_factor_mixed(const LuArgsT<T>& a, int64_t c) {
       const int64_t ir = (int64_t)r * B + c;
       if (a.pinv[ir] < 0) {
         if (nl >= a.capL) return LU_CAPACITY;
-        const Lo l = M::div(xf[ir], pv);
+        const Lo l = M::div(mix_lo(xw, ir), pv);
         const float al = M::mag(l);
         a.li[nl * B + c] = r;
-        lxf[nl * B + c] = l;
-        lxe[nl * B + c] = (xe[ir] + al * epv) / apv + M::kDiv * kMixU * al;
+        mix_lo(lx, nl * B + c) = l;
+        mix_err(lx, nl * B + c) = (mix_err(xw, ir) + al * epv) / apv + M::kDiv * kMixU * al;
         nl++;
       }
     }
@@ -667,29 +682,18 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<T>& a, int64_t c, int* iters) {
   const int32_t* ap = a.pool + pd.ap;
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
-  const Lo* lxf = reinterpret_cast<const Lo*>(a.lx);
-  const Lo* uxf = reinterpret_cast<const Lo*>(a.ux);
-  const Lo* udf = reinterpret_cast<const Lo*>(a.udiag);
+  T* const lx = a.lx;
+  T* const ux = a.ux;
+  T* const ud = a.udiag;
   T* w = a.w;
   T* y = a.xw;
   T* r = a.x;
-  Lo* t;
-  Lo* z;
-  if constexpr (sizeof(Lo) == 4) {
-    t = reinterpret_cast<Lo*>(a.xi);
-    z = reinterpret_cast<Lo*>(a.pstack);
-  } else {
-    // lx holds 2*capL*B single-precision values' worth of L and bounds in its
-    // first 12*capL*B bytes; capL >= 4n leaves room for n more values.
-    t = reinterpret_cast<Lo*>(reinterpret_cast<char*>(a.lx) + 12 * a.capL * B);
-    z = reinterpret_cast<Lo*>(a.udiag) + (int64_t)n * B;
-  }
   for (int i = 0; i < n; i++) w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
   double prev = __builtin_huge_val();
   int it = 0;
   for (;;) {
     if (it == 0) {
-      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = M::lo(w[(int64_t)i * B + c]);
+      for (int i = 0; i < n; i++) mix_lo(r, (int64_t)i * B + c) = M::lo(w[(int64_t)i * B + c]);
     } else {
       for (int i = 0; i < n; i++) r[(int64_t)i * B + c] = w[(int64_t)i * B + c];
       for (int k = 0; k < n; k++) {
@@ -701,31 +705,31 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<T>& a, int64_t c, int* iters) {
           r[ir] = r[ir] - prod;
         }
       }
-      for (int i = 0; i < n; i++) t[(int64_t)i * B + c] = M::lo(r[(int64_t)i * B + c]);
+      for (int i = 0; i < n; i++) mix_lo(r, (int64_t)i * B + c) = M::lo(r[(int64_t)i * B + c]);
     }
     for (int k = 0; k < n; k++) {
-      const Lo zk = t[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
-      z[(int64_t)k * B + c] = zk;
+      const Lo zk = mix_lo(r, (int64_t)a.prow[(int64_t)k * B + c] * B + c);
+      mix_hi_lo(ud, (int64_t)k * B + c) = zk;
       if (M::zero(zk)) continue;
       const int32_t p1 = a.lp[(int64_t)(k + 1) * B + c];
       for (int32_t p = a.lp[(int64_t)k * B + c]; p < p1; p++) {
         const int64_t ix = (int64_t)a.li[(int64_t)p * B + c] * B + c;
-        t[ix] = t[ix] - lxf[(int64_t)p * B + c] * zk;
+        mix_lo(r, ix) = mix_lo(r, ix) - mix_lo(lx, (int64_t)p * B + c) * zk;
       }
     }
     for (int k = n - 1; k >= 0; k--) {
-      const Lo zk = M::div(z[(int64_t)k * B + c], udf[(int64_t)k * B + c]);
-      z[(int64_t)k * B + c] = zk;
+      const Lo zk = M::div(mix_hi_lo(ud, (int64_t)k * B + c), mix_lo(ud, (int64_t)k * B + c));
+      mix_hi_lo(ud, (int64_t)k * B + c) = zk;
       if (M::zero(zk)) continue;
       const int32_t p1 = a.up[(int64_t)(k + 1) * B + c];
       for (int32_t p = a.up[(int64_t)k * B + c]; p < p1; p++) {
         const int64_t ix = (int64_t)a.ui[(int64_t)p * B + c] * B + c;
-        z[ix] = z[ix] - uxf[(int64_t)p * B + c] * zk;
+        mix_hi_lo(ud, ix) = mix_hi_lo(ud, ix) - mix_lo(ux, (int64_t)p * B + c) * zk;
       }
     }
     double ymax = 0.0, dmax = 0.0;
     for (int k = 0; k < n; k++) {
-      const T dz = M::hi(z[(int64_t)k * B + c]);
+      const T dz = M::hi(mix_hi_lo(ud, (int64_t)k * B + c));
       const T yk = it == 0 ? dz : y[(int64_t)k * B + c] + dz;
       if (!lu_finite(yk)) return LU_UNSURE;
       y[(int64_t)k * B + c] = yk;
@@ -741,7 +745,7 @@ KH_HD inline int lu_solve_mixed(const LuArgsT<T>& a, int64_t c, int* iters) {
       bool tight = true;
       for (int k = 0; k < n && tight; k++) {
         const double yk = M::nrm(y[(int64_t)k * B + c]);
-        const double dz = M::nrm(M::hi(z[(int64_t)k * B + c]));
+        const double dz = M::nrm(M::hi(mix_hi_lo(ud, (int64_t)k * B + c)));
         tight = rho * dz <= 0x1p-46 * yk + 0x1p-60 * ymax;
       }
       if (tight) break;

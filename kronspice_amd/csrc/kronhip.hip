@@ -184,9 +184,7 @@ __global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
 // [nf][n][B] complex array.
 // precision 1: complex64 factors with complex128 refinement (kh_lu.h), the
 // complex128 path for every system it cannot certify; mixl[l] receives the
-// lane's complex64 solves, -1 where it was redone (per-lane words summed on
-// the host: a 64-bit atomicAdd from divergent lanes here corrupted the
-// results of lanes 32-63 of a wavefront on gfx950).
+// lane's complex64 solves, -1 where it was redone (summed on the host).
 __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu, double ledger_tol,
                                                      int32_t skip_energy, double* xout, int32_t precision,
                                                      int32_t* mixl) {
