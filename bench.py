@@ -149,20 +149,30 @@ def bench_ac(args, log):
     d_params = api.to_device(b.params)
     torch.cuda.synchronize()
     systems = b.B * len(freqs)
-    r = None
-    for _ in range(args.warmup):
-        r = api.ac(t, d_params, freqs)
-    torch.cuda.synchronize()
-    lu_s = bytes_alg = launches = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        r = api.ac(t, d_params, freqs)
-        lu_s += r.stats["lu_seconds"]
-        bytes_alg += r.stats["bytes_algorithmic"]
-        launches += r.stats["lu_launches"]
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
-    log(f"rlc grid {G}x{G}: n={b.n}, {systems} systems in {dt:.3f} s")
+
+    def timed(prec):
+        opts = api.Options(precision=prec)
+        r = None
+        for _ in range(args.warmup):
+            r = api.ac(t, d_params, freqs, opts=opts)
+        torch.cuda.synchronize()
+        tot = {"lu_seconds": 0.0, "bytes_algorithmic": 0.0, "lu_launches": 0.0, "refine_passes": 0.0, "fp64_redos": 0.0}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            r = api.ac(t, d_params, freqs, opts=opts)
+            for k in tot:
+                tot[k] += r.stats[k]
+        torch.cuda.synchronize()
+        return r, (time.perf_counter() - t0) / args.steps, tot
+
+    # Plain form: complex128 LU; the other form: complex64 factors with
+    # complex128 refinement. The faster one is the chosen form.
+    forms = {p: timed(p) for p in (0, 1)}
+    prec = min(forms, key=lambda p: forms[p][1])
+    r, dt, tot = forms[prec]
+    lu_s, bytes_alg, launches = tot["lu_seconds"], tot["bytes_algorithmic"], tot["lu_launches"]
+    log(f"rlc grid {G}x{G}: n={b.n}, {systems} systems: complex128 {forms[0][1]:.3f} s, "
+        f"complex64+refinement {forms[1][1]:.3f} s")
     x = r.x.cpu().numpy()
     st = r.status.cpu().numpy()
     # Parity and CPU baseline: the oracle's sweep of every draw at 16 of the frequencies.
@@ -189,10 +199,13 @@ def bench_ac(args, log):
     out = {
         "metric": METRIC, "value": systems * solved / dt, "unit": "complex systems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded counter-based generator)",
+        "scaling": "weak", "vs_baseline": None, "dtype": "c128" if prec == 0 else "c64 LU + c128 refinement",
+        "data": "synthetic (seeded counter-based generator)",
         "config": {"workload": f"rlc_grid_ac: {G}x{G} RLC mesh (series R-L per edge, C per node), unit AC current "
                                f"at one corner, 16 draws x 1024 frequencies 1 kHz-10 GHz",
-                   "unknowns": b.n, "systems": systems, "form": "complex128 GP LU, one lane per system",
+                   "unknowns": b.n, "systems": systems,
+                   "form": f"chosen: {'complex128 GP LU' if prec == 0 else 'complex64 GP LU + complex128 refinement'}, "
+                           "one lane per system",
                    "parallelism": "lane per (draw, frequency) system, 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ac_solve",
@@ -203,6 +216,11 @@ def bench_ac(args, log):
                                    f"operating point included) ({tcpu:.2f} s)"},
         "parity": {"sample": len(ok), "solved_fraction": solved, "bit_identical_fraction": float(np.mean(exact)),
                    "max_rel_err": float(max(rels))},
+        "forms": {"plain_c128_gpu": {"systems_per_s": systems / forms[0][1], "seconds": forms[0][1]},
+                  "c64_refined": {"systems_per_s": systems / forms[1][1], "seconds": forms[1][1],
+                                  "refine_passes": forms[1][2]["refine_passes"] / args.steps,
+                                  "c128_redos": forms[1][2]["fp64_redos"] / args.steps},
+                  "chosen_vs_plain": forms[0][1] / dt},
     }
     print(json.dumps(out))
 
