@@ -114,6 +114,24 @@ KH_HD inline bool retryable(int e) { return e == ST_NOCONV || e == ST_NONFINITE 
 
 inline namespace KH_CAT(kinds_, KH_KINDS) {
 
+// Loops whose iterations are independent issue the loads of kChunk
+// iterations before using any of them: one lane's program is a chain of
+// dependent memory latencies, and a loop that waits for each iteration's
+// loads pays one latency per iteration. Arithmetic order is unchanged.
+constexpr int kChunk = 8;
+
+KH_HD inline void copy_n(double* __restrict__ d, const double* __restrict__ s, int n) {
+  int i = 0;
+  for (; i + kChunk <= n; i += kChunk) {
+    double v[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) v[u] = s[i + u];
+#pragma unroll
+    for (int u = 0; u < kChunk; u++) d[i + u] = v[u];
+  }
+  for (; i < n; i++) d[i] = s[i];
+}
+
 // One circuit's state machine over its own (unit-stride) arrays; the control
 // words are worked on in place, so they live wherever the circuit's arrays do
 // (device memory, or LDS when the launch stages the circuit).
@@ -237,7 +255,24 @@ struct Ctl {
     const int32_t* const isV = A.ckt.isV;
     const double vnTol = P.vnTol, absTol = P.absTol, relTol = P.relTol;
     bool ok = true;
-    for (int i = 0; i < n; i++) {
+    int i = 0;
+    for (; i + kChunk <= n; i += kChunk) {
+      double xv[kChunk], old[kChunk];
+      int32_t v[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) {
+        xv[u] = x_[i + u];
+        old[u] = xo[i + u];
+        v[u] = isV[i + u];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) {
+        double tol = v[u] ? vnTol : absTol;
+        tol += relTol * fmax(fabs(xv[u]), fabs(old[u]));
+        ok = ok && !(fabs(xv[u] - old[u]) > tol);
+      }
+    }
+    for (; i < n; i++) {
       double xv = x_[i], old = xo[i];
       double tol = isV[i] ? vnTol : absTol;
       tol += relTol * fmax(fabs(xv), fabs(old));
@@ -306,14 +341,37 @@ struct Ctl {
     const int32_t* const order = A.ckt.order;
     const int ns = A.ckt.nslots;
     for (int i = 0; i < n; i++) ab[i] = 0;
-    for (int k = 0; k < ns; k++) {
+    int k = 0;
+    for (; k + kChunk <= ns; k += kChunk) {
+      int32_t sl[kChunk], row[kChunk];
+      double term[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) sl[u] = order ? order[k + u] : k + u;
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) {
+        row[u] = present[sl[u]] ? si[sl[u]] : -1;
+        term[u] = fabs(val[sl[u]]) * fabs(x[sj[sl[u]]]);
+      }
+#pragma unroll
+      for (int u = 0; u < kChunk; u++)  // accumulated in slot order
+        if (row[u] >= 0) ab[row[u]] += term[u];
+    }
+    for (; k < ns; k++) {
       const int s = order ? order[k] : k;
       if (!present[s]) continue;
       ab[si[s]] += fabs(val[s]) * fabs(x[sj[s]]);
     }
     const double* const rhs = A.rhs;
     double sum = 0;
-    for (int i = 0; i < n; i++) sum += fabs(x[i]) * (ab[i] + fabs(rhs[i]));
+    int i = 0;
+    for (; i + kChunk <= n; i += kChunk) {
+      double term[kChunk];
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) term[u] = fabs(x[i + u]) * (ab[i + u] + fabs(rhs[i + u]));
+#pragma unroll
+      for (int u = 0; u < kChunk; u++) sum += term[u];
+    }
+    for (; i < n; i++) sum += fabs(x[i]) * (ab[i] + fabs(rhs[i]));
     return 64 * 0x1p-52 * sum;
   }
   // Ledger of an operating point: BuildLedger + RelativeResidual
@@ -412,10 +470,8 @@ struct Ctl {
   // in the next LU launch.
   KH_HD void request_solve() {
     const uint64_t t0 = KH_CLOCK();
-    for (int i = 0; i < n; i++) {
-      X(A.xold, i) = X(A.x, i);
-      X(A.rhs, i) = 0;
-    }
+    copy_n(A.xold, A.x, n);
+    for (int i = 0; i < n; i++) X(A.rhs, i) = 0;
     for (int s = 0; s < A.ckt.nslots; s++) A.val[s] = 0;
     load_all();
     double ptG = cd[CD_PTG];
@@ -855,7 +911,7 @@ struct Ctl {
     cd[CD_CYC_ACCEPT] += (double)(t1 - t0);
     cd[CD_CYC_LEDGER] += (double)(KH_CLOCK() - t1);
     if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
-    for (int i = 0; i < n; i++) X(A.xacc, i) = X(A.x, i);
+    copy_n(A.xacc, A.x, n);
     ci[CI_ACCEPTED]++;
     cd[CD_MINSTEP_SEEN] = fmin(cd[CD_MINSTEP_SEEN], h);
     cd[CD_MAXSTEP_SEEN] = fmax(cd[CD_MAXSTEP_SEEN], h);

@@ -137,6 +137,8 @@ using LuArgs = LuArgsT<double>;
 
 KH_HD inline bool kh_finite(double v) { return !isnan(v) && !isinf(v); }
 
+constexpr int kLuChunk = 8;  // independent loads issued together (see lu_solve)
+
 // reach, solver/sparse.go:301-352: non-recursive DFS from every row of
 // A(:,k) through the graph of L; returns top, the reach is xi[top..n).
 // mark[r] == k + 1 means "visited while processing column k" (the reference's
@@ -205,11 +207,25 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
   const int32_t* arow = a.pool + pd.arow;
   const int32_t* aslot = a.pool + pd.aslot;
   double scale = 0.0;
-  for (int s = 0; s < a.nslots; s++) {
-    T v = a.val[(int64_t)s * B + c];
-    if (!lu_finite(v)) return LU_NONFINITE;
-    double av = lu_mag(v);
-    if (av > scale) scale = av;
+  {
+    int s = 0;
+    for (; s + kLuChunk <= a.nslots; s += kLuChunk) {
+      T v[kLuChunk];
+#pragma unroll
+      for (int u = 0; u < kLuChunk; u++) v[u] = a.val[(int64_t)(s + u) * B + c];
+#pragma unroll
+      for (int u = 0; u < kLuChunk; u++) {
+        if (!lu_finite(v[u])) return LU_NONFINITE;
+        const double av = lu_mag(v[u]);
+        if (av > scale) scale = av;
+      }
+    }
+    for (; s < a.nslots; s++) {
+      T v = a.val[(int64_t)s * B + c];
+      if (!lu_finite(v)) return LU_NONFINITE;
+      double av = lu_mag(v);
+      if (av > scale) scale = av;
+    }
   }
   if (scale == 0) {
     a.err_step[c] = -1;
@@ -239,13 +255,38 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
       a.dk[(int64_t)k * B + c] = diag_in_reach ? 1 : 0;
     }
     nr = r1;
-    for (int64_t p = r0; p < r1; p++) a.xw[(int64_t)a.rch[p * B + c] * B + c] = lu_zero_value<T>();
+    {
+      int64_t p = r0;
+      for (; p + kLuChunk <= r1; p += kLuChunk) {
+        int32_t rr[kLuChunk];
+#pragma unroll
+        for (int u = 0; u < kLuChunk; u++) rr[u] = a.rch[(p + u) * B + c];
+#pragma unroll
+        for (int u = 0; u < kLuChunk; u++) a.xw[(int64_t)rr[u] * B + c] = lu_zero_value<T>();
+      }
+      for (; p < r1; p++) a.xw[(int64_t)a.rch[p * B + c] * B + c] = lu_zero_value<T>();
+    }
     double colMax = 0.0;
-    for (int p = ap[k]; p < ap[k + 1]; p++) {
-      T v = a.val[(int64_t)aslot[p] * B + c];
-      a.xw[(int64_t)arow[p] * B + c] = v;
-      double av = lu_mag(v);
-      if (av > colMax) colMax = av;
+    {
+      int p = ap[k];
+      const int pe = ap[k + 1];
+      for (; p + kLuChunk <= pe; p += kLuChunk) {
+        T v[kLuChunk];
+#pragma unroll
+        for (int u = 0; u < kLuChunk; u++) v[u] = a.val[(int64_t)aslot[p + u] * B + c];
+#pragma unroll
+        for (int u = 0; u < kLuChunk; u++) {
+          a.xw[(int64_t)arow[p + u] * B + c] = v[u];
+          const double av = lu_mag(v[u]);
+          if (av > colMax) colMax = av;
+        }
+      }
+      for (; p < pe; p++) {
+        T v = a.val[(int64_t)aslot[p] * B + c];
+        a.xw[(int64_t)arow[p] * B + c] = v;
+        double av = lu_mag(v);
+        if (av > colMax) colMax = av;
+      }
     }
     double tiny = colMax * (double)n * 1e-14;
     // x = L \ A(:,k) in reach order.
@@ -327,7 +368,23 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
   const PatDesc pd = a.pats[a.pid[c]];
   const int32_t* q = a.pool + pd.q;
   const int32_t* qinv = a.pool + pd.qinv;
-  for (int i = 0; i < n; i++) a.w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
+  {
+    // The loads of kLuChunk entries are issued before any is used (one lane's
+    // program is a chain of memory latencies); the same values move.
+    int i = 0;
+    for (; i + kLuChunk <= n; i += kLuChunk) {
+      int32_t qi[kLuChunk];
+      T v[kLuChunk];
+#pragma unroll
+      for (int u = 0; u < kLuChunk; u++) {
+        qi[u] = qinv[i + u];
+        v[u] = a.rhs[(int64_t)(i + u) * B + c];
+      }
+#pragma unroll
+      for (int u = 0; u < kLuChunk; u++) a.w[(int64_t)qi[u] * B + c] = v[u];
+    }
+    for (; i < n; i++) a.w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
+  }
   T* y = a.xw;
   for (int k = 0; k < n; k++) {
     T yk = a.w[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
@@ -351,7 +408,22 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
       y[ix] = y[ix] - prod;
     }
   }
-  for (int k = 0; k < n; k++) {
+  int k = 0;
+  for (; k + kLuChunk <= n; k += kLuChunk) {
+    int32_t qk[kLuChunk];
+    T v[kLuChunk];
+#pragma unroll
+    for (int u = 0; u < kLuChunk; u++) {
+      qk[u] = q[k + u];
+      v[u] = y[(int64_t)(k + u) * B + c];
+    }
+#pragma unroll
+    for (int u = 0; u < kLuChunk; u++) {
+      if (!lu_finite(v[u])) return LU_NONFINITE;  // entries before it are written, as the reference does
+      a.x[(int64_t)qk[u] * B + c] = v[u];
+    }
+  }
+  for (; k < n; k++) {
     T v = y[(int64_t)k * B + c];
     if (!lu_finite(v)) return LU_NONFINITE;
     a.x[(int64_t)q[k] * B + c] = v;
