@@ -126,15 +126,16 @@ def main(argv: Optional[list] = None) -> int:
     ap.add_argument("--eval", type=int, default=65536)
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--batch", type=int, default=1024, help="training mini-batch")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     data = make_data(a.train)
     t1 = time.perf_counter()
     ws = WarmStart(data.genes.shape[1], data.volts.shape[1])
-    hist = ws.fit(data, steps=a.steps, lr=a.lr)
+    hist = ws.fit(data, steps=a.steps, batch=a.batch, lr=a.lr)
     t2 = time.perf_counter()
     rep = evaluate(ws, a.eval)
-    rep.update(train_circuits=a.train, data_seconds=t1 - t0, train_seconds=t2 - t1, train_steps=a.steps,
+    rep.update(train_circuits=a.train, data_seconds=t1 - t0, train_seconds=t2 - t1, train_steps=a.steps, batch=a.batch,
                loss_first=hist[0], loss_last=float(np.mean(hist[-50:])))
     print(json.dumps(rep))
     return 0
