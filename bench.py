@@ -336,13 +336,15 @@ def main():
         # cold against network-seeded Newton.
         from kronspice_amd import warmstart as WS
         tw = time.perf_counter()
-        wdata = WS.make_data(65536)
+        wdata = WS.make_data(200000)  # 200k solver-generated operating points
         ws = WS.WarmStart(wdata.genes.shape[1], wdata.volts.shape[1])
-        hist = ws.fit(wdata, steps=8000, lr=3e-3)
+        hist = ws.fit(wdata, steps=8000, batch=4096, lr=3e-3)
+        del wdata
         trained = time.perf_counter() - tw
         rep = WS.evaluate(ws, 65536)
         warm = {"workload": "two-stage BJT amplifier, 200 nodes, 60 genes, DC OP (polished)",
-                "train_circuits": 65536, "train_seconds": trained, "eval_circuits": 65536,
+                "train_circuits": 200000, "train_seconds": trained, "eval_circuits": 65536,
+                "guess_err_volts_median": rep["guess_err_volts_median"],
                 "cold_iters_mean": rep["cold_iters_mean"], "warm_iters_mean": rep["warm_iters_mean"],
                 "iters_saved_fraction": rep["iters_saved_fraction"],
                 "cold_circuits_per_s": 65536 / rep["cold_seconds"],
