@@ -496,3 +496,51 @@ def test_mos_chain_transient_sensitivity_to_one_ulp():
         e1, y1, _ = oracle.op_polished(b.circuit.num_nodes, b.elements, np.ascontiguousarray(p1[:, c]))
         assert e0 == 0 and e1 == 0
         assert np.max(np.abs(y1 - y0)) / np.max(np.abs(y0)) <= 1e-12
+
+
+def test_rlc_grid_ac_reciprocity_and_dc_limit():
+    """The oracle's complex sweep on the RLC grid mesh (the AC workload's
+    shape) obeys physics it does not encode: the network is reciprocal, so
+    the transfer impedance with source and probe swapped is the same; at a
+    frequency low enough that every inductor is a short and every capacitor
+    open, the mesh is resistors only and the far corner's voltage is the
+    source corner's voltage less the drop along the R-only network, which a
+    real nodal solve of the resistive mesh gives independently."""
+    w = h = 5
+    a = W.rlc_grid(w, h, draws=2)
+    b = W.rlc_grid(w, h, draws=2, source_at_far_corner=True)
+    f = np.array([1e3, 1e6, 1e8])
+    for d in range(2):
+        ea, xa, _ = oracle.ac(a.circuit.num_nodes, a.elements, a.circuit_params(d), f)
+        eb, xb, _ = oracle.ac(b.circuit.num_nodes, b.elements, b.circuit_params(d), f)
+        assert ea == 0 and eb == 0
+        za = xa[:, a.circuit.node(f"g{w - 1}_{h - 1}") - 1]
+        zb = xb[:, b.circuit.node("g0_0") - 1]
+        assert np.max(np.abs(za - zb) / np.abs(za)) <= 1e-10
+    # Near DC: node voltages of the resistive mesh (R per edge, 1 Mohm to
+    # ground at the source corner, unit current in) from a real nodal solve.
+    ea, xa, _ = oracle.ac(a.circuit.num_nodes, a.elements, a.circuit_params(0), np.array([1e-6]))
+    assert ea == 0
+    els, p = a.elements, a.circuit_params(0)
+    nn = w * h
+    G = np.zeros((nn, nn))
+    # edge resistances in element order (each edge is R then L: g(x,y) -> mid -> g(x2,y2))
+    idx = 0
+    rvals = [p[els[k]["poff"]] for k, e in enumerate(a.circuit.elements) if e.kind == kc.R and e.name != "RDC"]
+    for y in range(h):
+        for x in range(w):
+            for (x2, y2) in ((x + 1, y), (x, y + 1)):
+                if x2 >= w or y2 >= h:
+                    continue
+                g = 1.0 / rvals[idx]
+                idx += 1
+                i, j = y * w + x, y2 * w + x2
+                G[i, i] += g; G[j, j] += g; G[i, j] -= g; G[j, i] -= g
+    G[0, 0] += 1e-6
+    rhs = np.zeros(nn)
+    rhs[0] = 1.0
+    v = np.linalg.solve(G, rhs)
+    for y in range(h):
+        for x in range(w):
+            got = xa[0, a.circuit.node(f"g{x}_{y}") - 1]
+            assert abs(got - v[y * w + x]) <= 1e-6 * abs(v[0]), (x, y)
