@@ -63,3 +63,38 @@ def test_rectifier_batch_runs_in_oracle():
     x, st, acc, rej, led = oracle.batch_tran(b.circuit.num_nodes, b.elements, b.params, b.n, spec,
                                              lane_spec=b.lane_spec)
     assert (st == 0).all() and (acc >= 2000).all()
+
+
+def test_ctypes_structures_match_the_header(tmp_path):
+    """The Python mirror's structures (api.Options, Stats, TranSpec, Group,
+    DcSpec; the oracle's KhOptions, KhTranSpec) have the C header's sizes and
+    field offsets, so neither side reads past or short of the other's."""
+    import subprocess
+    src = tmp_path / "sz.c"
+    fields = {
+        "kh_options": [f for f, _ in api.Options._fields_],
+        "kh_stats": [f for f, _ in api.Stats._fields_],
+        "kh_tran_spec": [f for f, _ in api.TranSpec._fields_],
+        "kh_group": [f for f, _ in api.Group._fields_],
+        "kh_dc_spec": [f for f, _ in api.DcSpec._fields_],
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{ROOT}/include/kronhip.h"', "int main(void) {"]
+    for st, fs in fields.items():
+        lines.append(f'  printf("{st} %zu\\n", sizeof({st}));')
+        for f in fs:
+            base = f.split("[")[0]
+            lines.append(f'  printf("{st}.{base} %zu\\n", offsetof({st}, {base}));')
+    lines.append("  return 0;\n}")
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "sz"
+    r = subprocess.run(["gcc", "-std=c11", str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
+    py = {"kh_options": api.Options, "kh_stats": api.Stats, "kh_tran_spec": api.TranSpec, "kh_group": api.Group,
+          "kh_dc_spec": api.DcSpec}
+    for st, cls in py.items():
+        assert int(got[st]) == ctypes.sizeof(cls), st
+        for f, _ in cls._fields_:
+            assert int(got[f"{st}.{f}"]) == getattr(cls, f).offset, (st, f)
+    assert ctypes.sizeof(oracle.KhOptions) == int(got["kh_options"])
+    assert ctypes.sizeof(oracle.KhTranSpec) == int(got["kh_tran_spec"])
