@@ -259,6 +259,33 @@ typedef struct kh_group {
 kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt,
                        const kh_tran_spec* spec, kh_stats* stats);
 
+/* ---- a whole generation (the population evaluation's batched caller) ------
+ * The GA evaluates every candidate of a generation with analysis.OP or Tran on
+ * a pool of goroutines, results stored by candidate index (evo/engine.go:519-612,
+ * evo/fitness.go:98-190). kh_generation_run takes the generation as it comes —
+ * candidates of any mix of topologies, host arrays — groups the candidates
+ * that share a topology (node count, element table, parameter count, MOSFET
+ * levels and whether a per-circuit time scale is given), runs every group in
+ * one uneven batch (kh_batch_run) and writes each candidate's results at its
+ * own index, so selection stays deterministic whatever the grouping. */
+typedef struct kh_candidate {
+  int32_t num_nodes;
+  int32_t n_elems;
+  int32_t nparams;
+  int32_t reserved;
+  const kh_element* elems;    /* host [n_elems] */
+  const double* params;       /* host [nparams] */
+  const double* lane_spec;    /* transient, optional host [4]: step, stop, max step, min step */
+} kh_candidate;
+/* MNA unknowns of a candidate's circuit (the length of its x_out array), or a
+ * negative status when its element table is invalid. */
+int32_t kh_candidate_size(const kh_candidate* c);
+/* x_out (optional): per candidate a host array of kh_candidate_size doubles
+ * (or null to skip it); status, iters, accepted, rejected: optional host [n]. */
+kh_status kh_generation_run(int32_t n, const kh_candidate* cands, const kh_options* opt, const kh_tran_spec* spec,
+                            double* const* x_out, int32_t* status, int32_t* iters, int32_t* accepted,
+                            int32_t* rejected, kh_stats* stats);
+
 /* DC sweep request: analysis.DC with one or two swept sources
  * (analysis/dc.go:22-28 DCSweep, :51-130 DC). Sweep 0 is the inner loop;
  * its points are start + k*step, k = 0 .. floor((stop-start)/step + 1e-9)

@@ -110,6 +110,9 @@ def lib():
         L.kh_topology_destroy.argtypes = [P]
         L.kh_topology_destroy.restype = None
         L.kh_batch_run.argtypes = [i32, P, P, P, P]
+        L.kh_candidate_size.argtypes = [P]
+        L.kh_candidate_size.restype = i32
+        L.kh_generation_run.argtypes = [i32, P, P, P, P, P, P, P, P, P]
         L.kh_batch_op.argtypes = [P, i32, P, P, P, P, P, P]
         L.kh_batch_tran.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
         L.kh_batch_dc.argtypes = [P, i32, P, P, P, P, P, P, P, P, P, P]
@@ -424,6 +427,44 @@ def run_groups(topos, params_list, spec: Optional[TranSpec] = None, opts: Option
     for r in results:
         r.stats = stats
     return results, stats
+
+
+class Candidate(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("n_elems", C.c_int32), ("nparams", C.c_int32), ("reserved", C.c_int32),
+                ("elems", C.c_void_p), ("params", C.c_void_p), ("lane_spec", C.c_void_p)]
+
+
+def run_generation(candidates, spec: Optional[TranSpec] = None, opts: Optional[Options] = None):
+    """A whole generation through kh_generation_run: candidates is a list of
+    (num_nodes, elements, params [P], lane_spec [4] or None) of any mix of
+    topologies. Returns (x list of [n_i] arrays, status, iters, accepted,
+    rejected, stats), everything at the candidate's own index."""
+    L = lib()
+    n = len(candidates)
+    cs = (Candidate * n)()
+    keep = []
+    sizes = []
+    for i, (nn, els, prm, lane) in enumerate(candidates):
+        els = np.ascontiguousarray(els)
+        prm = np.ascontiguousarray(prm, np.float64)
+        lane = None if lane is None else np.ascontiguousarray(lane, np.float64)
+        keep += [els, prm, lane]
+        cs[i].num_nodes, cs[i].n_elems, cs[i].nparams = nn, len(els), len(prm)
+        cs[i].elems = els.ctypes.data
+        cs[i].params = prm.ctypes.data
+        cs[i].lane_spec = lane.ctypes.data if lane is not None else None
+        k = L.kh_candidate_size(C.byref(cs[i]))
+        if k < 0:
+            raise KhError(-k, f"candidate {i}: invalid circuit")
+        sizes.append(k)
+    xs = [np.zeros(k) for k in sizes]
+    ptrs = (C.c_void_p * n)(*[x.ctypes.data for x in xs])
+    st, it, acc, rej = (np.zeros(n, np.int32) for _ in range(4))
+    o = Options() if opts is None else opts
+    s = Stats()
+    _check(L.kh_generation_run(n, cs, C.byref(o), C.byref(spec) if spec is not None else None, ptrs, _p(st), _p(it),
+                               _p(acc), _p(rej), C.byref(s)))
+    return xs, st, it, acc, rej, s.as_dict()
 
 
 def tran(topo: Topology, params, spec: TranSpec, opts: Optional[Options] = None, lane_spec=None) -> Result:

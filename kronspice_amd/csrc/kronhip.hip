@@ -17,6 +17,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1225,6 +1226,134 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
     if (s != KH_OK) return s;
   }
   return batch_run(ngroups, groups, opt, p, spec, nullptr, stats);
+}
+
+int32_t kh_candidate_size(const kh_candidate* c) {
+  if (!c || !c->elems || c->n_elems <= 0 || !c->params || c->num_nodes < 1) return -KH_BAD_ARGUMENT;
+  Topology T;
+  if (!T.build(c->num_nodes, c->elems, c->n_elems, c->nparams, c->params)) return -KH_UNSUPPORTED;
+  return T.n;
+}
+
+kh_status kh_generation_run(int32_t n, const kh_candidate* cands, const kh_options* opt, const kh_tran_spec* spec,
+                            double* const* x_out, int32_t* status, int32_t* iters, int32_t* accepted,
+                            int32_t* rejected, kh_stats* stats) {
+  if (n <= 0 || !cands) return fail(KH_BAD_ARGUMENT, "no candidates");
+  // Group by topology: node count, parameter count, element table, whether a
+  // time scale is given, then the MOSFET levels (structural: they pick the
+  // compiled currents), which the first candidate's built topology locates.
+  std::unordered_map<std::string, std::vector<int32_t>> by_key;
+  std::vector<std::string> order;  // groups in order of first appearance
+  for (int32_t i = 0; i < n; i++) {
+    const kh_candidate& c = cands[i];
+    if (!c.elems || c.n_elems <= 0 || !c.params || c.num_nodes < 1 || c.nparams < 0)
+      return fail(KH_BAD_ARGUMENT, "candidate " + std::to_string(i) + ": bad arrays");
+    std::string key(reinterpret_cast<const char*>(c.elems), (size_t)c.n_elems * sizeof(kh_element));
+    const int32_t head[4] = {c.num_nodes, c.n_elems, c.nparams, (spec && c.lane_spec) ? 1 : 0};
+    key.append(reinterpret_cast<const char*>(head), sizeof(head));
+    auto f = by_key.find(key);
+    if (f == by_key.end()) {
+      order.push_back(key);
+      by_key[key].push_back(i);
+    } else {
+      f->second.push_back(i);
+    }
+  }
+  std::vector<std::vector<int32_t>> members;
+  for (const auto& key : order) {
+    const std::vector<int32_t>& idx = by_key[key];
+    const kh_candidate& c0 = cands[idx[0]];
+    Topology T;
+    if (!T.build(c0.num_nodes, c0.elems, c0.n_elems, c0.nparams, c0.params))
+      return fail(KH_UNSUPPORTED, "candidate " + std::to_string(idx[0]) + ": " + T.error);
+    std::map<std::vector<double>, size_t> by_level;
+    std::vector<std::vector<int32_t>> split;
+    for (int32_t i : idx) {
+      std::vector<double> lv;
+      for (int32_t r : T.level_rows) lv.push_back(cands[i].params[r]);
+      auto f = by_level.find(lv);
+      if (f == by_level.end()) {
+        by_level[lv] = split.size();
+        split.push_back({i});
+      } else {
+        split[f->second].push_back(i);
+      }
+    }
+    for (auto& m : split) members.push_back(std::move(m));
+  }
+  // One topology and one set of device arrays per group.
+  struct G {
+    kh_topology t = nullptr;
+    int32_t np = 0, nx = 0;
+    DBuf par, lane, x, st, it, acc, rej;
+    ~G() { if (t) kh_topology_destroy(t); }
+  };
+  std::vector<std::unique_ptr<G>> gs;
+  std::vector<kh_group> groups;
+  for (const auto& m : members) {
+    const kh_candidate& c0 = cands[m[0]];
+    auto g = std::make_unique<G>();
+    kh_status s = kh_topology_create(c0.num_nodes, c0.elems, c0.n_elems, c0.nparams, c0.params, &g->t);
+    if (s != KH_OK) return s;
+    const int64_t B = (int64_t)m.size(), P = c0.nparams;
+    g->np = (int32_t)P;
+    g->nx = g->t->t.n;
+    std::vector<double> hp((size_t)std::max<int64_t>(P, 1) * B);
+    for (int64_t j = 0; j < B; j++)
+      for (int64_t k = 0; k < P; k++) hp[(size_t)(k * B + j)] = cands[m[j]].params[k];
+    KH_HIP(g->par.alloc(hp.size() * 8));
+    KH_HIP(hipMemcpy(g->par.p, hp.data(), hp.size() * 8, hipMemcpyHostToDevice));
+    const bool has_lane = spec && c0.lane_spec;
+    if (has_lane) {
+      std::vector<double> hl((size_t)4 * B);
+      for (int64_t j = 0; j < B; j++)
+        for (int k = 0; k < 4; k++) hl[(size_t)(k * B + j)] = cands[m[j]].lane_spec[k];
+      KH_HIP(g->lane.alloc(hl.size() * 8));
+      KH_HIP(hipMemcpy(g->lane.p, hl.data(), hl.size() * 8, hipMemcpyHostToDevice));
+    }
+    KH_HIP(g->x.alloc((size_t)g->nx * B * 8));
+    KH_HIP(g->st.alloc((size_t)B * 4));
+    KH_HIP(g->it.alloc((size_t)B * 4));
+    KH_HIP(g->acc.alloc((size_t)B * 4));
+    KH_HIP(g->rej.alloc((size_t)B * 4));
+    kh_group k = {};
+    k.topology = g->t;
+    k.B = (int32_t)B;
+    k.d_params = g->par.as<double>();
+    k.d_lane_spec = has_lane ? g->lane.as<double>() : nullptr;
+    k.d_x = g->x.as<double>();
+    k.d_status = g->st.as<int32_t>();
+    k.d_iters = g->it.as<int32_t>();
+    k.d_accepted = g->acc.as<int32_t>();
+    k.d_rejected = g->rej.as<int32_t>();
+    groups.push_back(k);
+    gs.push_back(std::move(g));
+  }
+  kh_status s = kh_batch_run((int32_t)groups.size(), groups.data(), opt, spec, stats);
+  if (s != KH_OK) return s;
+  // Results back to each candidate's own index.
+  for (size_t gi = 0; gi < gs.size(); gi++) {
+    const G& g = *gs[gi];
+    const std::vector<int32_t>& m = members[gi];
+    const int64_t B = (int64_t)m.size();
+    std::vector<int32_t> hs(B), hi(B), ha(B), hr(B);
+    std::vector<double> hx((size_t)g.nx * B);
+    KH_HIP(hipMemcpy(hs.data(), g.st.p, (size_t)B * 4, hipMemcpyDeviceToHost));
+    KH_HIP(hipMemcpy(hi.data(), g.it.p, (size_t)B * 4, hipMemcpyDeviceToHost));
+    KH_HIP(hipMemcpy(ha.data(), g.acc.p, (size_t)B * 4, hipMemcpyDeviceToHost));
+    KH_HIP(hipMemcpy(hr.data(), g.rej.p, (size_t)B * 4, hipMemcpyDeviceToHost));
+    if (x_out) KH_HIP(hipMemcpy(hx.data(), g.x.p, hx.size() * 8, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < B; j++) {
+      const int32_t i = m[j];
+      if (status) status[i] = hs[j];
+      if (iters) iters[i] = hi[j];
+      if (accepted) accepted[i] = ha[j];
+      if (rejected) rejected[i] = hr[j];
+      if (x_out && x_out[i])
+        for (int32_t k = 0; k < g.nx; k++) x_out[i][k] = hx[(size_t)k * B + j];
+    }
+  }
+  return KH_OK;
 }
 
 kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,

@@ -77,6 +77,7 @@ def test_ctypes_structures_match_the_header(tmp_path):
         "kh_tran_spec": [f for f, _ in api.TranSpec._fields_],
         "kh_group": [f for f, _ in api.Group._fields_],
         "kh_dc_spec": [f for f, _ in api.DcSpec._fields_],
+        "kh_candidate": [f for f, _ in api.Candidate._fields_],
     }
     lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{ROOT}/include/kronhip.h"', "int main(void) {"]
     for st, fs in fields.items():
@@ -91,7 +92,7 @@ def test_ctypes_structures_match_the_header(tmp_path):
     assert r.returncode == 0, r.stderr
     got = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines())
     py = {"kh_options": api.Options, "kh_stats": api.Stats, "kh_tran_spec": api.TranSpec, "kh_group": api.Group,
-          "kh_dc_spec": api.DcSpec}
+          "kh_dc_spec": api.DcSpec, "kh_candidate": api.Candidate}
     for st, cls in py.items():
         assert int(got[st]) == ctypes.sizeof(cls), st
         for f, _ in cls._fields_:
