@@ -127,15 +127,16 @@ def main(argv: Optional[list] = None) -> int:
     ap.add_argument("--steps", type=int, default=8000)
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--batch", type=int, default=1024, help="training mini-batch")
+    ap.add_argument("--hidden", type=int, default=256, help="hidden width (a multiple of 64)")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
     data = make_data(a.train)
     t1 = time.perf_counter()
-    ws = WarmStart(data.genes.shape[1], data.volts.shape[1])
+    ws = WarmStart(data.genes.shape[1], data.volts.shape[1], hidden=a.hidden)
     hist = ws.fit(data, steps=a.steps, batch=a.batch, lr=a.lr)
     t2 = time.perf_counter()
     rep = evaluate(ws, a.eval)
-    rep.update(train_circuits=a.train, data_seconds=t1 - t0, train_seconds=t2 - t1, train_steps=a.steps, batch=a.batch,
+    rep.update(train_circuits=a.train, data_seconds=t1 - t0, train_seconds=t2 - t1, train_steps=a.steps, batch=a.batch, hidden=a.hidden,
                loss_first=hist[0], loss_last=float(np.mean(hist[-50:])))
     print(json.dumps(rep))
     return 0
