@@ -161,12 +161,34 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   l.rch += c * l.capR; l.rp += c * (n + 1); l.dk += c * n; l.cpid += c;
 }
 
+// The LU arguments of a circuit staged in LDS, typed as LDS pointers (the
+// pattern pool stays in device memory).
+KH_HD inline LuArgsT<double, AsLds> lds_lu(const LuArgs& l) {
+  using A = LuArgsT<double, AsLds>;
+  A r;
+  r.n = l.n; r.nslots = l.nslots; r.B = l.B; r.capL = l.capL; r.capU = l.capU; r.capR = l.capR;
+  r.pats = l.pats; r.pool = l.pool;
+  r.pid = (A::P<const int32_t>)l.pid;
+  r.val = (A::P<const double>)l.val; r.rhs = (A::P<const double>)l.rhs;
+  r.x = (A::P<double>)l.x; r.xw = (A::P<double>)l.xw; r.w = (A::P<double>)l.w;
+  r.udiag = (A::P<double>)l.udiag; r.lx = (A::P<double>)l.lx; r.ux = (A::P<double>)l.ux;
+  r.li = (A::P<int32_t>)l.li; r.ui = (A::P<int32_t>)l.ui; r.lp = (A::P<int32_t>)l.lp; r.up = (A::P<int32_t>)l.up;
+  r.pinv = (A::P<int32_t>)l.pinv; r.prow = (A::P<int32_t>)l.prow; r.xi = (A::P<int32_t>)l.xi;
+  r.pstack = (A::P<int32_t>)l.pstack; r.mark = (A::P<int32_t>)l.mark; r.err_step = (A::P<int32_t>)l.err_step;
+  r.rch = (A::P<int32_t>)l.rch; r.rp = (A::P<int32_t>)l.rp; r.dk = (A::P<int32_t>)l.dk;
+  r.cpid = (A::P<int32_t>)l.cpid;
+  return r;
+}
+
 inline namespace KH_CAT(kinds_, KH_KINDS) {
 
 // One circuit's persistent loop: factor + solve the pending system, advance
 // the analysis to its next solve (stamping it), until the analysis ends, the
 // stamps need a new ordering or the factors more capacity, or max_solves
 // solves were done.
+// Lds: the group's arrays are staged in LDS (the view of lds_view); the
+// FP64 sparse LU then addresses them as LDS.
+template <bool Lds = false>
 KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bool* running, bool* waiting) {
   CtlArgs a;
   LuArgs la;
@@ -197,8 +219,14 @@ KH_HD inline void run_circuit(const Group& g, int64_t cc, int32_t max_solves, bo
           if (st == LU_UNSURE) ctl.cd[CD_REDO] += 1.0;
         }
         if (st == LU_UNSURE) {
-          st = lu_factor<double, true>(la, 0);
-          if (st == LU_OK) st = lu_solve<double, true>(la, 0);
+          if constexpr (Lds) {
+            const LuArgsT<double, AsLds> ll = lds_lu(la);
+            st = lu_factor<double, true>(ll, 0);
+            if (st == LU_OK) st = lu_solve<double, true>(ll, 0);
+          } else {
+            st = lu_factor<double, true>(la, 0);
+            if (st == LU_OK) st = lu_solve<double, true>(la, 0);
+          }
         }
         if (st == LU_CAPACITY) {
           // L/U outgrew the group's capacity (pivoting away from the diagonal

@@ -83,6 +83,22 @@ template <typename T> KH_HD inline T lu_zero_value();
 template <> KH_HD inline double lu_zero_value<double>() { return 0.0; }
 template <> KH_HD inline kcplx lu_zero_value<kcplx>() { return {0.0, 0.0}; }
 
+// Address-space policy of a lane's arrays: generic pointers, or (device
+// compilation only) pointers into LDS, for a circuit whose working set is
+// staged there, so its accesses compile to LDS instructions instead of flat
+// ones (a flat access waits on the vector-memory counter, which also covers
+// every store still in flight to HBM).
+struct AsGen {
+  template <class T> using P = T*;
+};
+struct AsLds {
+#if defined(__HIP_DEVICE_COMPILE__)
+  template <class T> using P = __attribute__((address_space(3))) T*;
+#else
+  template <class T> using P = T*;
+#endif
+};
+
 // Permuted column structure of one pattern (solver/sparse.go:154-187):
 // column k holds rows arow[ap[k]..ap[k+1]) (permuted ids, ascending) with
 // values at slot aslot[...] of the lane's value array.
@@ -98,40 +114,41 @@ struct PatDesc {
 
 // Everything a lane needs to factor and solve. Pointers are device memory.
 // T = double (real systems) or kcplx (AC systems).
-template <typename T>
+template <typename T, typename AS = AsGen>
 struct LuArgsT {
+  template <class U> using P = typename AS::template P<U>;
   int32_t n;          // unknowns
   int32_t nslots;     // value slots per lane
   int64_t B;          // lanes in the batch (SoA stride)
   int64_t capL, capU; // L and U entry capacity per lane
   const PatDesc* pats;
   const int32_t* pool;
-  const int32_t* pid;       // [B] pattern id per lane
-  const T* val;             // [nslots][B]
-  const T* rhs;             // [n][B]
-  T* x;                     // [n][B] solution (written on success)
+  P<const int32_t> pid;       // [B] pattern id per lane
+  P<const T> val;             // [nslots][B]
+  P<const T> rhs;             // [n][B]
+  P<T> x;                     // [n][B] solution (written on success)
   // workspace
-  T* xw;                    // [n][B] dense working column
-  T* w;                     // [n][B] permuted right-hand side
-  T* udiag;                 // [n][B]
-  T* lx;                    // [capL][B]
-  T* ux;                    // [capU][B]
-  int32_t* li;              // [capL][B]
-  int32_t* ui;              // [capU][B]
-  int32_t* lp;              // [n+1][B]
-  int32_t* up;              // [n+1][B]
-  int32_t* pinv;            // [n][B]
-  int32_t* prow;            // [n][B]
-  int32_t* xi;              // [n][B]
-  int32_t* pstack;          // [n][B]
-  int32_t* mark;            // [n][B]
-  int32_t* err_step;        // [B] pivot step of a singular failure
+  P<T> xw;                    // [n][B] dense working column
+  P<T> w;                     // [n][B] permuted right-hand side
+  P<T> udiag;                 // [n][B]
+  P<T> lx;                    // [capL][B]
+  P<T> ux;                    // [capU][B]
+  P<int32_t> li;              // [capL][B]
+  P<int32_t> ui;              // [capU][B]
+  P<int32_t> lp;              // [n+1][B]
+  P<int32_t> up;              // [n+1][B]
+  P<int32_t> pinv;            // [n][B]
+  P<int32_t> prow;            // [n][B]
+  P<int32_t> xi;              // [n][B]
+  P<int32_t> pstack;          // [n][B]
+  P<int32_t> mark;            // [n][B]
+  P<int32_t> err_step;        // [B] pivot step of a singular failure
   // reach cache of the previous factorization (see lu_factor)
   int64_t capR;             // entries of rch per lane (>= |L| + |U| + n)
-  int32_t* rch;             // [capR][B] reach lists of all columns, concatenated
-  int32_t* rp;              // [n+1][B] start of column k's reach in rch
-  int32_t* dk;              // [n][B] 1 if row k is in the reach of column k
-  int32_t* cpid;            // [B] pattern id the cache belongs to, -1: none
+  P<int32_t> rch;             // [capR][B] reach lists of all columns, concatenated
+  P<int32_t> rp;              // [n+1][B] start of column k's reach in rch
+  P<int32_t> dk;              // [n][B] 1 if row k is in the reach of column k
+  P<int32_t> cpid;            // [B] pattern id the cache belongs to, -1: none
 };
 using LuArgs = LuArgsT<double>;
 
@@ -143,8 +160,8 @@ constexpr int kLuChunk = 8;  // independent loads issued together (see lu_solve)
 // A(:,k) through the graph of L; returns top, the reach is xi[top..n).
 // mark[r] == k + 1 means "visited while processing column k" (the reference's
 // generation counter, restarted per factorization).
-template <typename T, bool Unit = false>
-KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
+template <typename T, bool Unit = false, typename AS = AsGen>
+KH_HD inline int lu_reach(const LuArgsT<T, AS>& a, int64_t c, int k, const int32_t* ap, const int32_t* arow) {
   const int64_t B = Unit ? 1 : a.B;
   if (Unit) c = 0;
   const int n = a.n;
@@ -196,8 +213,8 @@ KH_HD inline int lu_reach(const LuArgsT<T>& a, int64_t c, int k, const int32_t* 
 // The pivot rule is still evaluated at every column exactly as the reference
 // does; at the first column whose pivot differs, the search resumes. Results
 // are identical to re-running the search every time.
-template <typename T, bool Unit = false>
-KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
+template <typename T, bool Unit = false, typename AS = AsGen>
+KH_HD inline int lu_factor(const LuArgsT<T, AS>& a, int64_t c) {
   const int64_t B = Unit ? 1 : a.B;
   if (Unit) c = 0;
   const int n = a.n;
@@ -246,7 +263,7 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
       r1 = a.rp[(int64_t)(k + 1) * B + c];
       diag_in_reach = a.dk[(int64_t)k * B + c] != 0;
     } else {
-      int top = lu_reach<T, Unit>(a, c, k, ap, arow);
+      int top = lu_reach<T, Unit, AS>(a, c, k, ap, arow);
       r1 = r0 + (n - top);
       if (r1 > a.capR) return LU_CAPACITY;
       for (int p = top; p < n; p++) a.rch[(r0 + p - top) * B + c] = a.xi[(int64_t)p * B + c];
@@ -360,8 +377,8 @@ KH_HD inline int lu_factor(const LuArgsT<T>& a, int64_t c) {
 // solve, solver/sparse.go:355-401 (the factorization is held in the lane's
 // workspace). x is written in original unknown order, stopping at the first
 // non-finite entry as the reference does.
-template <typename T, bool Unit = false>
-KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
+template <typename T, bool Unit = false, typename AS = AsGen>
+KH_HD inline int lu_solve(const LuArgsT<T, AS>& a, int64_t c) {
   const int64_t B = Unit ? 1 : a.B;
   if (Unit) c = 0;
   const int n = a.n;
@@ -385,7 +402,7 @@ KH_HD inline int lu_solve(const LuArgsT<T>& a, int64_t c) {
     }
     for (; i < n; i++) a.w[(int64_t)qinv[i] * B + c] = a.rhs[(int64_t)i * B + c];
   }
-  T* y = a.xw;
+  const auto y = a.xw;
   for (int k = 0; k < n; k++) {
     T yk = a.w[(int64_t)a.prow[(int64_t)k * B + c] * B + c];
     y[(int64_t)k * B + c] = yk;

@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(kBlock, KH_PERSIST_WAVES) k_persist(const Grou
     __syncthreads();
     if ((int)threadIdx.x < nc) {
       Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes, c0 + threadIdx.x);
-      run_circuit(v, 0, max_solves, &running, &waiting);
+      run_circuit<true>(v, 0, max_solves, &running, &waiting);
     }
     __syncthreads();
     for (int t = 0; t < nc; t++) {
