@@ -348,7 +348,7 @@ kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const 
  * No counterpart in the reference: the small per-topology network that
  * predicts a circuit's DC node voltages from its value genes; its predictions
  * are Options.NodeSet guesses (kh_group.d_nodeset), and Newton with its
- * fallbacks stays the judge. Layers in -> hidden -> hidden -> out with ReLU;
+ * fallbacks decides the answer. Layers in -> hidden -> hidden -> out with ReLU;
  * bf16 weights and activations into FP32-accumulating matrix-core products,
  * FP32 master weights, biases and Adam moments. Inputs d_x [B][n_in] and
  * targets d_t [B][n_out] are FP32 on the device. */
