@@ -851,8 +851,12 @@ struct GroupRun {
     KH_HIP(hipGetLastError());
     std::vector<int32_t> wait(B), hpid(B);
     std::vector<uint64_t> hh(B);
-    int32_t* wcol = ci.as<int32_t>() + CI_WAIT;
-    KH_HIP(hipMemcpy2D(wait.data(), 4, wcol, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
+    // The control words come back whole (a strided copy of one word per
+    // circuit costs milliseconds); the wait words are cleared in that copy
+    // and it goes back whole.
+    std::vector<int32_t> hci((size_t)NCI * B);
+    KH_HIP(hipMemcpy(hci.data(), ci.p, hci.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < B; c++) wait[c] = hci[(size_t)c * NCI + CI_WAIT];
     KH_HIP(hipMemcpy(hh.data(), hash.p, (size_t)B * 8, hipMemcpyDeviceToHost));
     KH_HIP(hipMemcpy(hpid.data(), pid.p, (size_t)B * 4, hipMemcpyDeviceToHost));
     std::unordered_map<uint64_t, int32_t> by_hash;
@@ -883,14 +887,14 @@ struct GroupRun {
       by_hash[hh[c]] = id;
       hpid[c] = id;
     }
-    for (int64_t c = 0; c < B; c++) wait[c] = 0;
+    for (int64_t c = 0; c < B; c++) hci[(size_t)c * NCI + CI_WAIT] = 0;
     stats.patterns = (int64_t)pats.host.desc.size();
     KH_HIP(pats.sync());
     int64_t cap = std::max(work.capL, lu_capacity_batch(pats.host.max_fill, T.n));
     if (grow) cap = std::max(cap, 2 * work.capL);
     KH_HIP(work.ensure(T.n, B, cap));
     KH_HIP(hipMemcpy(pid.p, hpid.data(), (size_t)B * 4, hipMemcpyHostToDevice));
-    KH_HIP(hipMemcpy2D(wcol, (size_t)NCI * 4, wait.data(), 4, 4, B, hipMemcpyHostToDevice));
+    KH_HIP(hipMemcpy(ci.p, hci.data(), hci.size() * 4, hipMemcpyHostToDevice));
     return KH_OK;
   }
 
@@ -1456,8 +1460,9 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   {
     std::vector<uint8_t> pr((size_t)B * T.nslots);
     KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
-    std::vector<int32_t> wst(B);
-    KH_HIP(hipMemcpy2D(wst.data(), 4, G.ci.as<int32_t>() + CI_STATUS, (size_t)NCI * 4, 4, B, hipMemcpyDeviceToHost));
+    std::vector<int32_t> wst(B), hci((size_t)NCI * B);
+    KH_HIP(hipMemcpy(hci.data(), G.ci.p, hci.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t c = 0; c < B; c++) wst[c] = hci[(size_t)c * NCI + CI_STATUS];
     bool any_ok = false;
     for (int64_t c = 0; c < B; c++) any_ok = any_ok || wst[c] == ST_OK;
     if (!any_ok) nf = 0;  // no operating point succeeded: statuses are the OP's
