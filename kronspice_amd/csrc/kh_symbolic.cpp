@@ -241,6 +241,8 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         t[ET_TERM] = unk(E.node[0]);
         t[ET_TERM + 1] = unk(E.node[1]);
         t[ET_TERM + 2] = t[ET_TERM];
+        layout_rows.push_back(E.poff + 14);
+        layout_vals.push_back(params_c0[E.poff + 14] > 0);
         if (params_c0[E.poff + 14] > 0) {
           int j = alloc();
           internal.push_back(j);
@@ -255,6 +257,10 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         int d = unk(E.node[0]), s = unk(E.node[1]);
         t[ET_TERM] = d; t[ET_TERM + 1] = s; t[ET_TERM + 2] = unk(E.node[2]); t[ET_TERM + 3] = unk(E.node[3]);
         int dp = d, sp = s;
+        for (int r : {7, 8}) {
+          layout_rows.push_back(E.poff + r);
+          layout_vals.push_back(params_c0[E.poff + r] > 0);
+        }
         if (params_c0[E.poff + 7] > 0) { dp = alloc(); internal.push_back(dp); }
         if (params_c0[E.poff + 8] > 0) { sp = alloc(); internal.push_back(sp); }
         t[ET_TERM + 4] = dp; t[ET_TERM + 5] = sp;
@@ -267,6 +273,10 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         int c = unk(E.node[0]), em = unk(E.node[1]), b = unk(E.node[2]);
         t[ET_TERM] = c; t[ET_TERM + 1] = em; t[ET_TERM + 2] = b; t[ET_TERM + 3] = unk(E.node[3]);
         int cp = c, bp = b, ep = em;
+        for (int r : {18, 14, 17}) {
+          layout_rows.push_back(E.poff + r);
+          layout_vals.push_back(params_c0[E.poff + r] > 0);
+        }
         if (params_c0[E.poff + 18] > 0) { cp = alloc(); internal.push_back(cp); }
         if (params_c0[E.poff + 14] > 0) { bp = alloc(); internal.push_back(bp); }
         if (params_c0[E.poff + 17] > 0) { ep = alloc(); internal.push_back(ep); }

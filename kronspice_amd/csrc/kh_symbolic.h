@@ -57,6 +57,12 @@ struct Topology {
   // circuit of the topology must share it.
   std::vector<int32_t> level_rows;
   std::vector<double> level_vals;
+  // Parameter rows whose sign decides an internal node (diode RS, MOSFET RD
+  // and RS, BJT RC, RB and RE as conductances), and whether circuit 0 has that
+  // node: the unknown layout follows circuit 0, so every circuit of the
+  // topology must agree.
+  std::vector<int32_t> layout_rows;
+  std::vector<uint8_t> layout_vals;
   std::string error;
   bool build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0);
 };

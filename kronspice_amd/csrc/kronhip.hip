@@ -1185,7 +1185,8 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
     r->B = G.B;
     r->gid = g;
     r->sw_points = sweep_points(p);
-    {  // MOSFET levels: every circuit must have circuit 0's (the compiled currents follow it)
+    {  // MOSFET levels and internal nodes: every circuit must have circuit 0's
+       // (the compiled currents and the unknown layout follow it)
       const Topology& T = G.topology->t;
       std::vector<double> lv((size_t)G.B);
       for (size_t k = 0; k < T.level_rows.size(); k++) {
@@ -1194,6 +1195,15 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
         for (double v : lv)
           if (v != T.level_vals[k])
             return fail(KH_BAD_ARGUMENT, "group " + std::to_string(g) + ": MOSFET levels differ between circuits");
+      }
+      for (size_t k = 0; k < T.layout_rows.size(); k++) {
+        KH_HIP(hipMemcpy(lv.data(), G.d_params + (size_t)T.layout_rows[k] * G.B, (size_t)G.B * 8,
+                         hipMemcpyDeviceToHost));
+        for (double v : lv)
+          if ((v > 0) != (T.layout_vals[k] != 0))
+            return fail(KH_BAD_ARGUMENT, "group " + std::to_string(g) +
+                                             ": series resistances that decide internal nodes differ between circuits"
+                                             " (parameter row " + std::to_string(T.layout_rows[k]) + ")");
       }
     }
     r->dense = use_dense(opt, G.topology->t.n);
@@ -1245,7 +1255,9 @@ kh_status kh_generation_run(int32_t n, const kh_candidate* cands, const kh_optio
   if (n <= 0 || !cands) return fail(KH_BAD_ARGUMENT, "no candidates");
   // Group by topology: node count, parameter count, element table, whether a
   // time scale is given, then the MOSFET levels (structural: they pick the
-  // compiled currents), which the first candidate's built topology locates.
+  // compiled currents) and the series resistances that create internal nodes
+  // (structural: they decide the unknown layout), which the first candidate's
+  // built topology locates.
   std::unordered_map<std::string, std::vector<int32_t>> by_key;
   std::vector<std::string> order;  // groups in order of first appearance
   for (int32_t i = 0; i < n; i++) {
@@ -1275,6 +1287,7 @@ kh_status kh_generation_run(int32_t n, const kh_candidate* cands, const kh_optio
     for (int32_t i : idx) {
       std::vector<double> lv;
       for (int32_t r : T.level_rows) lv.push_back(cands[i].params[r]);
+      for (int32_t r : T.layout_rows) lv.push_back(cands[i].params[r] > 0 ? 1.0 : 0.0);
       auto f = by_level.find(lv);
       if (f == by_level.end()) {
         by_level[lv] = split.size();
@@ -1449,36 +1462,53 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
     ~EvGuard() { for (int k = 0; k < 2; k++) (void)hipEventDestroy(e[k]); }
   } guard{ev};
   auto t0 = std::chrono::steady_clock::now();
-  // The AC pattern of each circuit (LoadAC skips zero channel conductances,
-  // so it depends on the operating point): the first frequency is stamped once
-  // for every circuit to learn it; one ordering per distinct pattern.
-  a.f0 = 0;
+  // The AC pattern of each circuit (LoadAC skips zero admittances, so it
+  // depends on the operating point and on whether omega is 0): the first
+  // frequency is stamped once for every circuit to learn it, one ordering per
+  // distinct pattern. The reference's CSparse keeps every entry it has seen
+  // and re-orders when one appears (solver/sparse.go:114-143), so a sweep
+  // that starts at 0 Hz has the 0 Hz pattern there and, from the first
+  // nonzero frequency on, the union with that frequency's stamps (a jwC term
+  // is zero only at omega = 0, so every nonzero frequency stamps the same
+  // entries): that frequency is stamped too, into the same marks.
+  int32_t fsplit = 0;  // frequencies below fsplit use cpid0, the rest cpid1
+  if (nf > 1 && freqs[0] == 0) {
+    fsplit = 1;
+    while (fsplit < nf && freqs[fsplit] == 0) fsplit++;
+    if (fsplit == nf) fsplit = 0;  // every frequency is 0 Hz: one pattern
+  }
+  std::vector<int32_t> cpid1(B, -1);
   a.BL = B;
   a.present = present.as<uint8_t>();
-  hipLaunchKernelGGL(k_ac_stamp, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, a);
-  KH_HIP(hipGetLastError());
   {
-    std::vector<uint8_t> pr((size_t)B * T.nslots);
-    KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
     std::vector<int32_t> wst(B), hci((size_t)NCI * B);
     KH_HIP(hipMemcpy(hci.data(), G.ci.p, hci.size() * 4, hipMemcpyDeviceToHost));
     for (int64_t c = 0; c < B; c++) wst[c] = hci[(size_t)c * NCI + CI_STATUS];
     bool any_ok = false;
     for (int64_t c = 0; c < B; c++) any_ok = any_ok || wst[c] == ST_OK;
     if (!any_ok) nf = 0;  // no operating point succeeded: statuses are the OP's
-    for (int64_t c = 0; c < B && any_ok && !G.dense; c++) {  // the dense backend has no pattern
-      if (wst[c] != ST_OK) continue;
-      std::vector<int32_t> slots;
-      for (int s2 = 0; s2 < T.nslots; s2++)
-        if (pr[(size_t)c * T.nslots + s2]) slots.push_back(s2);
-      std::string key = slots_key(slots);
-      auto f = pats.host.by_key.find(key);
-      if (f != pats.host.by_key.end()) {
-        cpid[c] = f->second;
-      } else {
-        cpid[c] = pats.host.add(analyze_pattern(n, T.slot_i, T.slot_j, slots), key);
+    std::vector<uint8_t> pr((size_t)B * T.nslots);
+    for (int probe = 0; probe < (fsplit > 0 ? 2 : 1); probe++) {
+      a.f0 = probe == 0 ? 0 : fsplit;
+      hipLaunchKernelGGL(k_ac_stamp, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, a);
+      KH_HIP(hipGetLastError());
+      KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
+      std::vector<int32_t>& ids = probe == 0 ? cpid : cpid1;
+      for (int64_t c = 0; c < B && any_ok && !G.dense; c++) {  // the dense backend has no pattern
+        if (wst[c] != ST_OK) continue;
+        std::vector<int32_t> slots;
+        for (int s2 = 0; s2 < T.nslots; s2++)
+          if (pr[(size_t)c * T.nslots + s2]) slots.push_back(s2);
+        std::string key = slots_key(slots);
+        auto f = pats.host.by_key.find(key);
+        if (f != pats.host.by_key.end()) {
+          ids[c] = f->second;
+        } else {
+          ids[c] = pats.host.add(analyze_pattern(n, T.slot_i, T.slot_j, slots), key);
+        }
       }
     }
+    if (fsplit == 0) cpid1 = cpid;
     KH_HIP(pats.sync());
   }
   const int64_t cap = G.dense ? (int64_t)n * n : lu_capacity(pats.host.max_fill, n);
@@ -1513,14 +1543,17 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   if (nf > 0) {
     if (G.dense) KH_HIP(work.ensure_dense(n, BLmax));
     else KH_HIP(work.ensure(n, BLmax, cap));
-    std::vector<int32_t> lp((size_t)BLmax);
-    for (int64_t l = 0; l < BLmax; l++) lp[l] = std::max(cpid[l % B], 0);
-    KH_HIP(hipMemcpy(pid.p, lp.data(), lp.size() * 4, hipMemcpyHostToDevice));
   }
+  std::vector<int32_t> lp((size_t)BLmax), lp_dev;  // pattern id per lane (lp_dev: what the device holds)
   for (int32_t f0 = 0; f0 < nf; f0 += fchunk) {
     const int32_t nfc = std::min(fchunk, nf - f0);
     a.f0 = f0;
     a.BL = (int64_t)nfc * B;
+    for (int64_t l = 0; l < a.BL; l++) lp[l] = std::max((f0 + l / B < fsplit ? cpid : cpid1)[l % B], 0);
+    if (lp != lp_dev) {
+      KH_HIP(hipMemcpy(pid.p, lp.data(), (size_t)a.BL * 4, hipMemcpyHostToDevice));
+      lp_dev = lp;
+    }
     const unsigned grid = (unsigned)((a.BL + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_ac_stamp, dim3(grid), dim3(kBlock), 0, 0, a);
     KH_HIP(hipGetLastError());

@@ -80,6 +80,10 @@ def lib():
         _lib.oracle_size.argtypes = [C.c_int32, P, C.c_int32, P]
         _lib.oracle_op.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
         _lib.oracle_op_polished.argtypes = [C.c_int32, P, C.c_int32, P, P, P, P, P]
+        _lib.oracle_dc_method.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, C.c_int32, P]
+        _lib.oracle_plant_stepping.argtypes = [C.c_int32, C.c_double]
+        _lib.oracle_lte_step.restype = C.c_double
+        _lib.oracle_lte_step.argtypes = [C.c_int32, C.c_double, C.c_double, P, P, C.c_int32] + [C.c_double] * 7
         _lib.oracle_tran.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
         _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
         _lib.oracle_dc.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P, P, P]
@@ -192,6 +196,48 @@ def op(num_nodes: int, els: np.ndarray, params: np.ndarray, opts: Optional[KhOpt
     d = {f[0]: getattr(info, f[0]) for f in OpInfo._fields_}
     d.update(q=q, prow=prow, current=cur)
     return e, x, d
+
+
+DC_METHODS = {"newton": 0, "gmin": 1, "source": 2, "ptran": 3}
+
+
+def dc_method(num_nodes, els, params, method: str, opts=None, trace: bool = False):
+    """One DC method alone on a fresh circuit (analysis/newton_test.go:74-104):
+    "newton" = dcSolve, "gmin" / "source" / "ptran" = gminStepping,
+    sourceStepping, pseudoTransient (analysis/newton.go:95-311).
+    Returns (status, x, Newton iterations[, trace]); the trace is an array of
+    (method, parameter, status) rows, one per Newton call: method 0 plain,
+    1 gmin (the diagonal conductance), 2 source (the source factor),
+    3 pseudo-transient (the step h)."""
+    n = size(num_nodes, els, params)
+    x = np.zeros(n)
+    it = C.c_int32()
+    o = opts if opts is not None else KhOptions()
+    cap = 3 * 20000
+    tr = np.zeros(cap) if trace else None
+    tl = C.c_int32()
+    e = lib().oracle_dc_method(num_nodes, _p(els), len(els), _p(params), C.byref(o), DC_METHODS[method], _p(x),
+                               C.byref(it), _p(tr), cap, C.byref(tl))
+    if not trace:
+        return e, x, it.value
+    assert tl.value <= cap
+    return e, x, it.value, tr[: tl.value].reshape(-1, 3)
+
+
+def plant_stepping(which: str, value: float):
+    """Planted mistake (tests only): alters one stepping constant of the
+    oracle — "gmin_cap" (10), "source_growth" (1.5), "pt_growth" (2); value 0
+    restores the reference's."""
+    lib().oracle_plant_stepping({"gmin_cap": 1, "source_growth": 2, "pt_growth": 3}[which], value)
+
+
+def lte_step(method: int, time: float, h: float, ts, qs, hist_n: int, qn: float, i_n: float, i_prev: float,
+             abstol: float = 1e-12, reltol: float = 1e-3, chgtol: float = 1e-14, trtol: float = 7.0) -> float:
+    """lteStep (device/lte.go:50-85) on an explicit history (most recent first)."""
+    ts = np.ascontiguousarray(ts, np.float64)
+    qs = np.ascontiguousarray(qs, np.float64)
+    return lib().oracle_lte_step(method, time, h, _p(ts), _p(qs), hist_n, qn, i_n, i_prev, abstol, reltol,
+                                 chgtol, trtol)
 
 
 def op_polished(num_nodes, els, params, opts=None, guess=None):

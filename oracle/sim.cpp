@@ -120,8 +120,19 @@ enum { E_OK = 0, E_SINGULAR = 1, E_NONFINITE = 2, E_BADSIZE = 3, E_NOCONV = 4, E
 
 static bool retryable(int e) { return e == E_NOCONV || e == E_NONFINITE || e == E_SINGULAR; }  // newton.go:86-89
 
+// Planted mistakes (tests only): stepping constants of analysis/newton.go that
+// a test may alter to show its checks notice the change. 0 = the reference's.
+static double g_plant_gmin_cap = 0;     // factor cap of gminStepping (10, newton.go:153)
+static double g_plant_src_growth = 0;   // step growth of sourceStepping (1.5, newton.go:205)
+static double g_plant_pt_growth = 0;    // step growth of pseudoTransient (2, newton.go:309)
+
 struct Sim {
   int numNodes = 1;
+  // Optional trace of the continuation: per Newton call (method, parameter,
+  // status), method 0 plain, 1 gmin (diagGmin), 2 source (SrcFact),
+  // 3 pseudo-transient (step h).
+  std::vector<double>* trace = nullptr;
+  double ptH = 0;
   int backendSel = 0;  // Options.Backend the Sim was built with (the AC matrix resolves it too)
   std::vector<std::unique_ptr<Device>> devs;
   std::vector<int> reactive, nonlin, breakers;
@@ -245,6 +256,16 @@ struct Sim {
 
   // newton + converged, analysis/newton.go:36-82
   int newton(int maxIter, const Options& o, int* iters) {
+    if (!trace) return newton_(maxIter, o, iters);
+    const double kind = diagGmin > 0 ? 1 : (lc.sourceStepping ? 2 : (ptG > 0 ? 3 : 0));
+    const double par = kind == 1 ? diagGmin : (kind == 2 ? lc.srcFact : (kind == 3 ? ptH : 0));
+    const int e = newton_(maxIter, o, iters);
+    trace->push_back(kind);
+    trace->push_back(par);
+    trace->push_back(e);
+    return e;
+  }
+  int newton_(int maxIter, const Options& o, int* iters) {
     lc.gmin = o.gmin;
     for (int it = 0; it < maxIter; it++) {
       int e = solve();
@@ -307,7 +328,7 @@ struct Sim {
         save = x;
         if (g == 0) { ret = E_OK; goto done; }
         ok = true;
-        factor = std::min(factor * std::sqrt(factor), 10.0);
+        factor = std::min(factor * std::sqrt(factor), g_plant_gmin_cap > 0 ? g_plant_gmin_cap : 10.0);
         if (g / factor < o.gmin) g = 0; else g /= factor;
         continue;
       } else if (!retryable(err)) {
@@ -349,7 +370,7 @@ struct Sim {
         if (f == 1) { ret = E_OK; goto out; }
         save = x;
         done = f;
-        step = std::min(step * 1.5, 0.5);
+        step = std::min(step * (g_plant_src_growth > 0 ? g_plant_src_growth : 1.5), 0.5);
         continue;
       } else if (!retryable(err)) {
         ret = err;
@@ -377,6 +398,7 @@ struct Sim {
     int ret = E_NOCONV;
     for (int k = 0; k < 2000; k++) {
       ptG = 1 / h;
+      ptH = h;
       int err = newton(o.itl1, o, iters);
       if (err != E_OK) {
         if (!retryable(err)) { ret = err; goto out; }
@@ -397,7 +419,7 @@ struct Sim {
         ret = newton(o.itl1, o, iters);
         goto out;
       }
-      h = std::min(h * 2, 1e12);
+      h = std::min(h * (g_plant_pt_growth > 0 ? g_plant_pt_growth : 2.0), 1e12);
     }
     ret = E_NOCONV;
   out:
@@ -867,6 +889,70 @@ int oracle_op_polished(int32_t num_nodes, const kh_element* el, int32_t nel, con
   if (x_out) std::copy(s.x.begin(), s.x.end(), x_out);
   if (iters_out) *iters_out = iters;
   return e;
+}
+
+// One DC method alone on a freshly built circuit, as
+// analysis/newton_test.go:74-104 (TestSteppingAgrees) runs them: 0 dcSolve
+// (plain Newton, then the fallbacks the options allow), 1 gminStepping,
+// 2 sourceStepping, 3 pseudoTransient (analysis/newton.go:95-311). The
+// circuit starts in ModeOP with InitJct off, as a new Sim does.
+// trace (optional, cap doubles): the continuation trace of Sim::trace;
+// *trace_len receives its length (which may exceed cap: then it is cut).
+int oracle_dc_method(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params,
+                     const kh_options* ko, int32_t method, double* x_out, int32_t* iters_out, double* trace,
+                     int32_t cap, int32_t* trace_len) {
+  Sim s;
+  std::vector<double> tr;
+  if (trace) s.trace = &tr;
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
+  if (e != E_OK) return e;
+  Options o;
+  o.from(ko);
+  s.lc.mode = ModeOP;
+  s.lc.time = 0;
+  int iters = 0;
+  switch (method) {
+    case 0: e = s.dcSolve(o, &iters); break;
+    case 1: e = s.gminStepping(o, &iters); break;
+    case 2: e = s.sourceStepping(o, &iters); break;
+    case 3: e = s.pseudoTransient(o, &iters); break;
+    default: return E_BADSPEC;
+  }
+  if (x_out) std::copy(s.x.begin(), s.x.end(), x_out);
+  if (iters_out) *iters_out = iters;
+  if (trace) {
+    for (size_t k = 0; k < tr.size() && (int32_t)k < cap; k++) trace[k] = tr[k];
+    if (trace_len) *trace_len = (int32_t)tr.size();
+  }
+  return e;
+}
+
+// Sets a planted mistake (tests only): which 1 gmin factor cap, 2 source-step
+// growth, 3 pseudo-transient growth; value 0 restores the reference's constant.
+void oracle_plant_stepping(int32_t which, double value) {
+  if (which == 1) g_plant_gmin_cap = value;
+  if (which == 2) g_plant_src_growth = value;
+  if (which == 3) g_plant_pt_growth = value;
+}
+
+// lteStep (device/lte.go:50-85) on an explicit history: candidate point
+// (time, qn), accepted points ts[0..2], qs[0..2] (most recent first), hist_n
+// valid; method 1 backward Euler, 2 trapezoidal.
+double oracle_lte_step(int32_t method, double time, double H, const double* ts, const double* qs, int32_t hist_n,
+                       double qn, double in, double in_prev, double abs_tol, double rel_tol, double chg_tol,
+                       double tr_tol) {
+  History h;
+  for (int k = 0; k < 3; k++) { h.t[k] = ts[k]; h.q[k] = qs[k]; }
+  h.n = hist_n;
+  TruncContext tc;
+  tc.time = time;
+  tc.H = H;
+  tc.method = method;
+  tc.relTol = rel_tol;
+  tc.absTol = abs_tol;
+  tc.chgTol = chg_tol;
+  tc.trTol = tr_tol;
+  return lteStep(h, qn, in, in_prev, abs_tol, tc);
 }
 
 typedef struct oracle_tran_info {

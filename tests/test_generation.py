@@ -65,3 +65,51 @@ def test_generation_tran_equals_per_topology_batches():
             if bb is b:
                 assert st[k] == s0[c] and acc[k] == a0[c] and rej[k] == r0[c]
                 assert np.array_equal(xs[k], x0[:, c])
+
+
+def diode_clamp(rs, v, seed):
+    """One element table whose diode has series resistance rs: RS > 0 gives
+    the diode an internal node (device/diode.go), RS = 0 does not."""
+    from kronspice_amd import circuit as kc
+    c = (kc.Builder("clamp").vsource("V1", "a", "0", v).resistor("R1", "a", "b", 1e3 + 37 * seed)
+         .diode("D1", "b", "0", "dm").resistor("R2", "b", "0", 5e3).model("dm", "d", {"is": 1e-14, "rs": rs})
+         .build())
+    els, p = c.flatten()
+    return c.num_nodes, els, p, None
+
+
+def test_generation_separates_internal_node_layouts():
+    """Candidates with one element table but RS = 0 and RS > 0 diodes have
+    different unknown layouts (3 and 4 unknowns): the generation groups them
+    apart, each candidate's x has its own length, and each agrees with its
+    own batch and with the oracle."""
+    cands = [diode_clamp(0.0 if k % 3 else 2.5, 2.0 + 0.25 * k, k) for k in range(24)]
+    sizes = [api.lib().kh_candidate_size(api.C.byref(_cand(c))) for c in cands]
+    assert set(sizes) == {3, 4}
+    xs, st, it, _, _, stats = api.run_generation(cands)
+    assert (st == 0).all()
+    for k, (nn, els, p, _) in enumerate(cands):
+        assert len(xs[k]) == sizes[k]
+        e, xo, info = oracle.op(nn, els, p)
+        assert e == 0 and info["iters"] == it[k]
+        assert np.max(np.abs(xs[k] - xo)) <= 1e-9 * np.max(np.abs(xo))
+
+
+def test_batch_rejects_mixed_internal_node_layouts():
+    nn, els, p0, _ = diode_clamp(2.5, 3.0, 0)
+    _, _, p1, _ = diode_clamp(0.0, 3.0, 1)
+    t = api.Topology(nn, els, p0)
+    with pytest.raises(api.KhError) as e:
+        api.op(t, np.stack([p0, p1], axis=1))
+    assert e.value.status == api.BAD_ARGUMENT
+
+
+def _cand(c):
+    nn, els, prm, _ = c
+    k = api.Candidate()
+    k.num_nodes, k.n_elems, k.nparams = nn, len(els), len(prm)
+    k.elems = np.ascontiguousarray(els).ctypes.data
+    k.params = np.ascontiguousarray(prm, np.float64).ctypes.data
+    k.lane_spec = None
+    _cand.keep = (els, prm)
+    return k

@@ -375,6 +375,31 @@ def test_ac_goldens_match_oracle(name):
     assert host(r.ledger).max() <= 1e-9
 
 
+@pytest.mark.parametrize("backend", [0, 2])
+def test_ac_bjt_sweep_from_0hz_grows_pattern(backend):
+    """A linear sweep that starts at 0 Hz (ACSpec lin, analysis/ac.go:32-73):
+    the BJT's base-collector admittance jwC_bx (RB > 0, XCJC < 1) is zero at
+    0 Hz, so LoadAC skips it there (device/bjt.go:721-734) and the reference's
+    CSparse adds those entries at the first nonzero frequency, re-ordering the
+    grown pattern (solver/sparse.go:114-143). Every frequency, forced sparse
+    or under Auto, within 1e-9 of the oracle, which grows its pattern the same
+    way."""
+    deck = gr.load_deck("bjt_ce_ac")
+    c = deck.circuit
+    els, p = c.flatten()
+    freqs = np.linspace(0.0, 2e8, 9)
+    o = oracle.options(backend=backend)
+    e0, xo, _ = oracle.ac(c.num_nodes, els, p, freqs, opts=o)
+    assert e0 == 0
+    t = api.Topology(c.num_nodes, els, p)
+    B = 8
+    r = api.ac(t, np.repeat(p[:, None], B, axis=1), freqs, opts=api.Options(backend=backend))
+    x, st, pts = host(r.x), host(r.status), host(r.points)
+    assert (st == 0).all() and (pts == len(freqs)).all()
+    for b in range(B):
+        assert np.max(np.abs(x[:, :, b] - xo)) <= 1e-9 * np.max(np.abs(xo)), b
+
+
 def test_ac_rc_lowpass_batch_closed_form():
     """Many RC low-pass circuits with random R and C: v(out)/v(in) must be the
     textbook 1/(1 + j w R C) at every frequency (to 1e-12), and every circuit
