@@ -135,7 +135,13 @@ KH_HD inline void copy_n(double* __restrict__ d, const double* __restrict__ s, i
 // One circuit's state machine over its own (unit-stride) arrays; the control
 // words are worked on in place, so they live wherever the circuit's arrays do
 // (device memory, or LDS when the launch stages the circuit).
-struct Ctl {
+//
+// D is the executing form (CRTP): every loop over unknowns, elements or value
+// slots is a member that D may replace — Ctl runs them on one lane, the
+// wavefront form (kh_wave.h) spreads them over the lanes of a wavefront that
+// all run the same circuit's state machine in step.
+template <class D>
+struct CtlT {
   const CtlArgs A;  // a copy: one indirection less per array access (the object lives in private memory)
   int n;
   int32_t* ci;  // [NCI]
@@ -143,7 +149,9 @@ struct Ctl {
   Lane L;
   CtlParams P;  // options and transient spec as seen by this circuit
 
-  KH_HD Ctl(const CtlArgs& a) : A(a), n(a.ckt.n), ci(a.ci), cd(a.cd) {
+  KH_HD D& self() { return *static_cast<D*>(this); }
+
+  KH_HD CtlT(const CtlArgs& a) : A(a), n(a.ckt.n), ci(a.ci), cd(a.cd) {
     L.n = n;
     L.et = a.ckt.et; L.par = a.par; L.st = a.st; L.val = a.val; L.present = a.present; L.rhs = a.rhs;
     L.X = a.xold;
@@ -411,7 +419,7 @@ struct Ctl {
       }
     }
     double res = inflow + sup - outflow - diss - stor;
-    double ro = roundoff();
+    double ro = self().roundoff();
     if (scale == 0 || fabs(res) <= fmax(1e-21, ro)) return 0;
     return fabs(res / scale);
   }
@@ -458,7 +466,7 @@ struct Ctl {
     int err = ST_OK;
     if (scale > cd[CD_PEAK]) cd[CD_PEAK] = scale;
     double res = fabs(inflow + sup - outflow - diss - stor);
-    if (!imposed && res > fmax(1e-21, roundoff())) {
+    if (!imposed && res > fmax(1e-21, self().roundoff())) {
       if (res > cd[CD_WORST]) cd[CD_WORST] = res;
       if (res / cd[CD_PEAK] > 1e-3) err = ST_LEDGER;
     }
@@ -470,10 +478,18 @@ struct Ctl {
   // in the next LU launch.
   KH_HD void request_solve() {
     const uint64_t t0 = KH_CLOCK();
-    copy_n(A.xold, A.x, n);
-    for (int i = 0; i < n; i++) X(A.rhs, i) = 0;
-    for (int s = 0; s < A.ckt.nslots; s++) A.val[s] = 0;
-    load_all();
+    self().vcopy(A.xold, A.x);
+    self().zero_system();
+    self().load_all();
+    self().diag_terms();
+    ci[CI_NEED_SOLVE] = 1;
+    if (L.grew && !A.ckt.dense) ci[CI_WAIT] = WAIT_PATTERN;  // the dense LU has no pattern
+    cd[CD_CYC_STAMP] += (double)(KH_CLOCK() - t0);
+  }
+  // Sim.solve's terms after the devices' (analysis/op.go:300-318): the
+  // pseudo-transient diagonal +-1/h with its history, the gmin-stepping
+  // diagonal.
+  KH_HD void diag_terms() {
     double ptG = cd[CD_PTG];
     if (ptG > 0) {
       for (int i = 0; i < n; i++) {
@@ -487,9 +503,42 @@ struct Ctl {
       for (int i = 0; i < n; i++)
         if (A.ckt.isV[i]) add_diag(diag_slot(i), g);
     }
-    ci[CI_NEED_SOLVE] = 1;
-    if (L.grew && !A.ckt.dense) ci[CI_WAIT] = WAIT_PATTERN;  // the dense LU has no pattern
-    cd[CD_CYC_STAMP] += (double)(KH_CLOCK() - t0);
+  }
+  KH_HD void zero_system() {
+    for (int i = 0; i < n; i++) X(A.rhs, i) = 0;
+    for (int s = 0; s < A.ckt.nslots; s++) A.val[s] = 0;
+  }
+  // Vector helpers over the n unknowns.
+  KH_HD void vzero(double* v) { for (int i = 0; i < n; i++) v[i] = 0; }
+  KH_HD void vcopy(double* d, const double* s) { copy_n(d, s, n); }
+  // pseudoTransient's steady-state test (analysis/newton.go:291-300).
+  KH_HD bool pt_moved() {
+    for (int i = 0; i < n; i++) {
+      double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
+      double xv = X(A.x, i);
+      if (fabs(xv - X(A.ptprev, i)) > P.relTol * fabs(xv) + tol) return true;
+    }
+    return false;
+  }
+  // polish's stopping test (analysis/op.go:262-267).
+  KH_HD bool polish_done() {
+    bool done = true;
+    for (int i = 0; i < n; i++) {
+      double xv = X(A.x, i);
+      if (fabs(xv - X(A.xold, i)) > 1e-12 * fmax(1.0, fabs(xv))) done = false;
+    }
+    return done;
+  }
+  // SetInitialCondition of every C and L (analysis/tran.go:100-108).
+  KH_HD void uic_init() {
+    for (int k = 0; k < A.ckt.nel; k++) {
+      const int32_t* e = el(k);
+      if (e[ET_KIND] == K_C) { L.S(e, 3) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; L.S(e, 4) = 0; }
+      if (e[ET_KIND] == K_L) { L.S(e, 0) = 0; L.S(e, 1) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; }
+    }
+  }
+  KH_HD void sweep_store(int point) {
+    for (int i = 0; i < n; i++) A.sw_x[(int64_t)point * n + i] = X(A.x, i);
   }
   // Slot of the diagonal entry (i, i): the device slot when a device stamps
   // it, else a slot reserved after the device slots.
@@ -526,7 +575,7 @@ struct Ctl {
       return;
     }
     const uint64_t t0 = KH_CLOCK();
-    const bool conv = converged();
+    const bool conv = self().converged();
     cd[CD_CYC_CONV] += (double)(KH_CLOCK() - t0);
     if (conv) { newton_done(ST_OK); return; }
     if (ci[CI_NT_IT] >= ci[CI_NT_MAX]) newton_done(ST_NOCONV);
@@ -551,7 +600,7 @@ struct Ctl {
     ci[CI_DC_RET] = ret;
     ci[CI_INITJCT] = ci[CI_WARM] ? 0 : 1;
     if (!ci[CI_WARM] && A.nodeset) {
-      apply_nodeset();
+      self().apply_nodeset();
       ci[CI_INITJCT] = 0;
     }
     start_newton(P.itl1, CALLER_DC_PLAIN);
@@ -560,7 +609,7 @@ struct Ctl {
   // unknown k-1, the rest 0), then every nonlinear device seeds its limiting
   // state from it.
   KH_HD void apply_nodeset() {
-    for (int i = 0; i < n; i++) X(A.x, i) = 0;
+    self().vzero(A.x);
     for (int i = 0; i < A.ckt.numNodes - 1; i++) X(A.x, i) = A.nodeset[i];
     for (int q = 0; q < A.ckt.n_nl; q++) {
       const int32_t* e = el(A.ckt.nl[q]);
@@ -599,7 +648,8 @@ struct Ctl {
 
   // gminStepping, analysis/newton.go:134-176
   KH_HD void gmin_begin() {
-    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.xsave, i) = 0; }
+    self().vzero(A.x);
+    self().vzero(A.xsave);
     ci[CI_INITJCT] = 1;
     cd[CD_GM_G] = 1e-2;
     cd[CD_GM_FACTOR] = 10.0;
@@ -611,7 +661,7 @@ struct Ctl {
   KH_HD void gmin_done(int r) {
     double g = cd[CD_GM_G], factor = cd[CD_GM_FACTOR];
     if (r == ST_OK) {
-      for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+      self().vcopy(A.xsave, A.x);
       if (g == 0) { gmin_end(ST_OK); return; }
       ci[CI_GM_OK] = 1;
       factor = fmin(factor * sqrt(factor), 10.0);
@@ -625,7 +675,7 @@ struct Ctl {
     }
     if (!retryable(r)) { gmin_end(r); return; }
     if (!ci[CI_GM_OK]) { gmin_end(ST_NOCONV); return; }
-    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xsave, i);
+    self().vcopy(A.x, A.xsave);
     double prev = g * factor;
     if (g == 0) prev = P.gmin;
     factor = sqrt(factor);
@@ -644,7 +694,8 @@ struct Ctl {
 
   // sourceStepping, analysis/newton.go:180-214
   KH_HD void src_begin() {
-    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.xsave, i) = 0; }
+    self().vzero(A.x);
+    self().vzero(A.xsave);
     ci[CI_SRCSTEP] = 1;
     cd[CD_SS_DONE] = 0;
     cd[CD_SS_STEP] = 0.1;
@@ -653,7 +704,7 @@ struct Ctl {
   }
   KH_HD void src0_done(int r) {
     if (r != ST_OK) { src_end(r); return; }
-    for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+    self().vcopy(A.xsave, A.x);
     ci[CI_SS_K] = 0;
     src_next();
   }
@@ -667,7 +718,7 @@ struct Ctl {
     double f = cd[CD_SS_F];
     if (r == ST_OK) {
       if (f == 1) { src_end(ST_OK); return; }
-      for (int i = 0; i < n; i++) X(A.xsave, i) = X(A.x, i);
+      self().vcopy(A.xsave, A.x);
       cd[CD_SS_DONE] = f;
       cd[CD_SS_STEP] = fmin(cd[CD_SS_STEP] * 1.5, 0.5);
       if (++ci[CI_SS_K] >= 1000) { src_end(ST_NOCONV); return; }
@@ -675,7 +726,7 @@ struct Ctl {
       return;
     }
     if (!retryable(r)) { src_end(r); return; }
-    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xsave, i);
+    self().vcopy(A.x, A.xsave);
     cd[CD_SS_STEP] /= 2;
     if (cd[CD_SS_STEP] < 1e-3) { src_end(ST_NOCONV); return; }
     if (++ci[CI_SS_K] >= 1000) { src_end(ST_NOCONV); return; }
@@ -690,7 +741,8 @@ struct Ctl {
 
   // pseudoTransient, analysis/newton.go:260-311
   KH_HD void pt_begin() {
-    for (int i = 0; i < n; i++) { X(A.x, i) = 0; X(A.ptprev, i) = 0; }
+    self().vzero(A.x);
+    self().vzero(A.ptprev);
     ci[CI_INITJCT] = 1;
     cd[CD_PT_H] = 1e-6;
     ci[CI_PT_QUIET] = 0;
@@ -701,7 +753,7 @@ struct Ctl {
   KH_HD void pt_done(int r) {
     if (r != ST_OK) {
       if (!retryable(r)) { pt_end(r); return; }
-      for (int i = 0; i < n; i++) X(A.x, i) = X(A.ptprev, i);
+      self().vcopy(A.x, A.ptprev);
       cd[CD_PT_H] /= 4;
       if (cd[CD_PT_H] < 1e-15) { pt_end(ST_NOCONV); return; }
       if (++ci[CI_PT_K] >= 2000) { pt_end(ST_NOCONV); return; }
@@ -709,13 +761,8 @@ struct Ctl {
       start_newton(P.itl1, CALLER_PT);
       return;
     }
-    bool moved = false;
-    for (int i = 0; i < n; i++) {
-      double tol = A.ckt.isV[i] ? P.vnTol : P.absTol;
-      double xv = X(A.x, i);
-      if (fabs(xv - X(A.ptprev, i)) > P.relTol * fabs(xv) + tol) { moved = true; break; }
-    }
-    for (int i = 0; i < n; i++) X(A.ptprev, i) = X(A.x, i);
+    const bool moved = self().pt_moved();
+    self().vcopy(A.ptprev, A.x);
     if (!moved) ci[CI_PT_QUIET]++;
     else ci[CI_PT_QUIET] = 0;
     if (ci[CI_PT_QUIET] >= 3 && cd[CD_PT_H] >= 1e6) {
@@ -743,8 +790,8 @@ struct Ctl {
   KH_HD void op_after_dc(int r) {
     if (P.analysis == AN_DC) { sweep_point_done(r); return; }
     if (r != ST_OK) { finish(r); return; }
-    accept(0, MODE_OP);
-    double rr = ledger_op();
+    self().accept(0, MODE_OP);
+    double rr = self().ledger_op();
     cd[CD_LEDGER] = rr;
     if (!P.skipEnergy && rr > P.ledgerTol) { finish(ST_LEDGER); return; }
     if (P.analysis == AN_OP_POLISH && A.ckt.nonlin) {
@@ -757,14 +804,10 @@ struct Ctl {
   }
   KH_HD void polish_step(int e) {
     if (e != ST_OK) { finish(e); return; }
-    bool done = true;
-    for (int i = 0; i < n; i++) {
-      double xv = X(A.x, i);
-      if (fabs(xv - X(A.xold, i)) > 1e-12 * fmax(1.0, fabs(xv))) done = false;
-    }
+    const bool done = self().polish_done();
     ci[CI_PL_K]++;
     if (done || ci[CI_PL_K] >= 2) {
-      accept(0, MODE_OP);
+      self().accept(0, MODE_OP);
       finish(ST_OK);
       return;
     }
@@ -803,12 +846,12 @@ struct Ctl {
     const int point = ci[CI_SW_J] * P.swCount[0] + ci[CI_SW_I];
     A.sw_it[point] = ci[CI_ITERS] - ci[CI_SW_IT0];
     if (r != ST_OK) { finish(r); return; }
-    accept(0, MODE_OP);
-    double rr = ledger_op();
+    self().accept(0, MODE_OP);
+    double rr = self().ledger_op();
     cd[CD_LEDGER] = rr;
     if (rr > cd[CD_MAXREL]) cd[CD_MAXREL] = rr;
     if (!P.skipEnergy && rr > P.ledgerTol) { finish(ST_LEDGER); return; }
-    for (int i = 0; i < n; i++) A.sw_x[(int64_t)point * n + i] = X(A.x, i);
+    self().sweep_store(point);
     ci[CI_SW_DONE]++;
     if (++ci[CI_SW_I] == P.swCount[0]) {
       ci[CI_SW_I] = 0;
@@ -820,11 +863,7 @@ struct Ctl {
   // ---- transient, analysis/tran.go:95-231 --------------------------------------
   KH_HD void tran_begin() {
     if (P.uic) {
-      for (int k = 0; k < A.ckt.nel; k++) {  // SetInitialCondition
-        const int32_t* e = el(k);
-        if (e[ET_KIND] == K_C) { L.S(e, 3) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; L.S(e, 4) = 0; }
-        if (e[ET_KIND] == K_L) { L.S(e, 0) = 0; L.S(e, 1) = L.P(e, 2) != 0 ? L.P(e, 1) : 0.0; }
-      }
+      self().uic_init();
       ci[CI_MODE] = MODE_UICINIT;
       cd[CD_TIME] = 0;
       ci[CI_INITJCT] = 1;
@@ -837,14 +876,14 @@ struct Ctl {
   }
   KH_HD void uic_done(int r) {
     if (r != ST_OK) { finish(r); return; }
-    accept(0, MODE_UICINIT);
-    record(0, true);
+    self().accept(0, MODE_UICINIT);
+    self().record(0, true);
     tran_init_loop();
   }
   KH_HD void tran_after_dc(int r) {
     if (r != ST_OK) { finish(r); return; }
-    accept(0, MODE_TRANOP);
-    int le = record(0, false);
+    self().accept(0, MODE_TRANOP);
+    int le = self().record(0, false);
     if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
     tran_init_loop();
   }
@@ -852,7 +891,7 @@ struct Ctl {
     cd[CD_TR_T] = 0;
     cd[CD_TR_H] = fmin(P.maxStep, fmin(P.step, P.stop / 100)) / 10;
     ci[CI_TR_ORDER] = BE;
-    for (int i = 0; i < n; i++) X(A.xacc, i) = X(A.x, i);
+    self().vcopy(A.xacc, A.x);
     cd[CD_MINSTEP_SEEN] = kInf;
     cd[CD_MAXSTEP_SEEN] = 0;
     tran_loop_top();
@@ -862,7 +901,7 @@ struct Ctl {
     double eps = 1e-12 * P.stop;
     if (!(t < P.stop - eps)) { tran_finish(); return; }
     if (ci[CI_ACCEPTED] >= P.maxSteps) { finish(ST_MAXSTEPS); return; }
-    double nb = next_breakpoint(t);
+    double nb = self().next_breakpoint(t);
     double tn = t + h;
     int hit = 0;
     if (tn >= nb - eps) { tn = nb; hit = 1; }
@@ -879,7 +918,7 @@ struct Ctl {
   }
   KH_HD bool reject(double hNew) {  // tran.go:136-143; true if the step became too small
     ci[CI_REJECTED]++;
-    for (int i = 0; i < n; i++) X(A.x, i) = X(A.xacc, i);
+    self().vcopy(A.x, A.xacc);
     return hNew < P.minStep;
   }
   KH_HD void tran_newton_done(int r) {
@@ -895,7 +934,7 @@ struct Ctl {
       return;
     }
     uint64_t t0 = KH_CLOCK();
-    double proposal = truncate(tn, h, order);
+    double proposal = self().truncate(tn, h, order);
     cd[CD_CYC_LTE] += (double)(KH_CLOCK() - t0);
     if (proposal < 0.9 * h) {
       h = fmax(proposal, h / 8);
@@ -905,13 +944,13 @@ struct Ctl {
       return;
     }
     t0 = KH_CLOCK();
-    accept(tn, MODE_TRAN);
+    self().accept(tn, MODE_TRAN);
     const uint64_t t1 = KH_CLOCK();
-    int le = record(tn, false);
+    int le = self().record(tn, false);
     cd[CD_CYC_ACCEPT] += (double)(t1 - t0);
     cd[CD_CYC_LEDGER] += (double)(KH_CLOCK() - t1);
     if (le != ST_OK && !P.skipEnergy) { finish(le); return; }
-    copy_n(A.xacc, A.x, n);
+    self().vcopy(A.xacc, A.x);
     ci[CI_ACCEPTED]++;
     cd[CD_MINSTEP_SEEN] = fmin(cd[CD_MINSTEP_SEEN], h);
     cd[CD_MAXSTEP_SEEN] = fmax(cd[CD_MAXSTEP_SEEN], h);
@@ -921,7 +960,7 @@ struct Ctl {
     ci[CI_TR_ORDER] = P.method;
     if (ci[CI_TR_HIT]) {
       ci[CI_TR_ORDER] = BE;
-      double nb2 = next_breakpoint(tn);
+      double nb2 = self().next_breakpoint(tn);
       next = fmin(next, 0.1 * fmin(P.maxStep, nb2 - tn));
     }
     cd[CD_TR_H] = fmax(next, P.minStep);
@@ -952,6 +991,11 @@ struct Ctl {
     if (e == ST_CAPACITY) { finish(ST_CAPACITY); return; }
     after_solve(e);
   }
+};
+
+// The state machine on one lane: every loop runs serially.
+struct Ctl : CtlT<Ctl> {
+  KH_HD Ctl(const CtlArgs& a) : CtlT<Ctl>(a) {}
 };
 
 }  // inline namespace kinds_<KH_KINDS>

@@ -435,10 +435,11 @@ struct Sim {
   }
 
   // polish, analysis/op.go:251-271
-  int polish() {
+  int polish(int* iters = nullptr) {
     if (nonlin.empty()) return E_OK;
     for (int k = 0; k < 2; k++) {
       int e = solve();
+      if (iters) (*iters)++;
       if (e != E_OK) return e;
       bool done = true;
       for (int i = 0; i < n; i++)
@@ -874,7 +875,8 @@ int oracle_op(int32_t num_nodes, const kh_element* el, int32_t nel, const double
   return e;
 }
 
-// OP then polish (analysis/op.go:243-271): the operating point exact to round-off.
+// OP then polish (analysis/op.go:243-271): the operating point exact to
+// round-off. iters_out counts every solve, the polish's included.
 int oracle_op_polished(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params,
                        const kh_options* ko, const double* guess, double* x_out, int32_t* iters_out) {
   Sim s;
@@ -885,9 +887,9 @@ int oracle_op_polished(int32_t num_nodes, const kh_element* el, int32_t nel, con
   int iters = 0;
   double lr;
   e = s.op(o, &iters, &lr, guess);
-  if (e == E_OK) e = s.polish();
+  if (e == E_OK) e = s.polish(&iters);
   if (x_out) std::copy(s.x.begin(), s.x.end(), x_out);
-  if (iters_out) *iters_out = iters;
+  if (iters_out) *iters_out = iters;  // Newton iterations and polish solves: every solve of the circuit
   return e;
 }
 
