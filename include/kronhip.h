@@ -138,6 +138,11 @@ typedef struct kh_options {
                                factors with FP64 iterative refinement, pivot sequence certified
                                equal to FP64's, every uncertified system redone in FP64
                                (no reference counterpart: the reference factors in FP64 only) */
+  int32_t circuit_wave;     /* batched analyses: 0 -> each circuit runs on one lane (k_persist,
+                               circuits_per_wave / staging / waves_per_simd apply); 1 -> each
+                               circuit runs on a whole wavefront, its stamping, tests and LU
+                               spread over the lanes, circuits taken largest first from a queue
+                               (k_wave; FP64 LU: precision is not applied) */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */

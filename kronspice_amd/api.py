@@ -47,7 +47,7 @@ class Options(C.Structure):
                 # Options.Backend: 0 auto (dense below 30 unknowns), 1 dense, 2 sparse
                 ("backend", C.c_int32),
                 # sparse LU arithmetic: 0 FP64, 1 FP32 factors + FP64 refinement
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("circuit_wave", C.c_int32)]
 
 
 class TranSpec(C.Structure):

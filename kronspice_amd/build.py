@@ -18,8 +18,9 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
-# (source, extra flags, object): the persistent kernel once per register
-# budget (resident wavefronts per SIMD) and element-kind set, each in its own
+# (source, extra flags, object): the lane-per-circuit persistent kernel once
+# per register budget (resident wavefronts per SIMD) and element-kind set, the
+# wavefront-per-circuit kernel once per element-kind set, each in its own
 # translation unit.
 PERSIST_WAVES = (1, 2, 4)
 PERSIST_KINDS = (126, 190, 318, 1022)  # diodes, level-1 MOSFETs, BJTs, everything (kh_persist.h)
@@ -28,7 +29,8 @@ UNITS = [
     ("kh_symbolic.cpp", [], "kh_symbolic.o"),
     ("kh_mlp.hip", [], "kh_mlp.o"),
 ] + [("kh_persist.hip", [f"-DKH_PERSIST_WAVES={w}", f"-DKH_KINDS={k}"], f"kh_persist_w{w}_k{k}.o")
-     for w in PERSIST_WAVES for k in PERSIST_KINDS]
+     for w in PERSIST_WAVES for k in PERSIST_KINDS] + [
+    ("kh_wave.hip", [f"-DKH_KINDS={k}"], f"kh_wave_k{k}.o") for k in PERSIST_KINDS]
 BENCH = os.path.join(HERE, "kh_bench")
 OBJ = os.path.join(HERE, "_obj")
 

@@ -75,6 +75,21 @@ struct Circ {
   const int32_t* src;       // [n_src] sources (V, I)
   const int32_t* order;     // [nslots] slot order of AbsProduct: row-major for the dense
                             // backend (solver/dense.go:63-73), null: slot order
+  // The wavefront form (kh_wave.h): elements stamp in chunks of consecutive
+  // elements; chunk c holds elements [ch_el[c], ch_el[c+1]) and touches the
+  // targets tgt[ch_t[c] .. ch_t[c+1]) (a value slot, or nslots + row), target t
+  // by the elements tel[tel_ptr[t] .. tel_ptr[t+1]) in element order; element k
+  // records at rec_off[k] of its chunk's record. ro_ptr / ro_slot: the slots of
+  // each row in AbsProduct's accumulation order.
+  int32_t nchunks, rec_cap;
+  const int32_t* ch_el;
+  const int32_t* ch_t;
+  const int32_t* tgt;
+  const int32_t* tel_ptr;
+  const int32_t* tel;
+  const int32_t* rec_off;
+  const int32_t* ro_ptr;
+  const int32_t* ro_slot;
 };
 
 struct CtlArgs {
@@ -156,6 +171,11 @@ struct CtlT {
     L.et = a.ckt.et; L.par = a.par; L.st = a.st; L.val = a.val; L.present = a.present; L.rhs = a.rhs;
     L.X = a.xold;
     L.grew = false;
+    L.rslot = nullptr;
+    L.rval = nullptr;
+    L.rn = 0;
+    L.rcap = 0;
+    L.nslots = a.ckt.nslots;
     P = a.p;
     if (a.lane_spec) {  // per-circuit time scale: step, stop, max step, min step
       P.step = a.lane_spec[0];

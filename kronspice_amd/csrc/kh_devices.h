@@ -66,6 +66,14 @@ struct Lane {
   double* rhs;           // [n]
   const double* X;       // [n] linearization point (xOld)
   bool grew;
+  // Record mode (rslot non-null): every add and addRHS is written to the
+  // element's record, in call order, as (target, value) with target = slot,
+  // or nslots + row for the right-hand side, instead of being accumulated;
+  // the wavefront form sums the records of all elements per target in element
+  // order (kh_wave.h). rn counts the calls, rcap bounds the record.
+  int32_t* rslot;
+  double* rval;
+  int32_t rn, rcap, nslots;
   // LoadContext (device/device.go:127-148)
   int mode, method;
   double time, H, gmin, srcFact;
@@ -78,6 +86,11 @@ struct Lane {
   KH_HD void add(const int32_t* e, int ti, int tj, double v) {
     int s = e[ET_SLOT + ti * KH_NT + tj];
     if (s < 0) return;
+    if (rslot) {
+      if (rn < rcap) { rslot[rn] = s; rval[rn] = v; }
+      rn++;
+      return;
+    }
     int64_t ix = s;
     val[ix] = val[ix] + v;
     if (!present[ix]) {
@@ -86,7 +99,13 @@ struct Lane {
     }
   }
   KH_HD void addRHS(int i, double v) {
-    if (i >= 0 && i < n) rhs[i] += v;
+    if (i < 0 || i >= n) return;
+    if (rslot) {
+      if (rn < rcap) { rslot[rn] = nslots + i; rval[rn] = v; }
+      rn++;
+      return;
+    }
+    rhs[i] += v;
   }
   KH_HD double volt(const int32_t* e, const double* x) const {  // Base.voltage, device/device.go:310-319
     double v = 0;
@@ -96,6 +115,26 @@ struct Lane {
     return v;
   }
 };
+
+// The most add and addRHS calls one Load of an element of this kind makes
+// (load_R ... load_Q below: R 4; C 4 + 2; L 5 + 1; V 4 + 1; I 2; D 8 + 2;
+// M: channel 12 + 2, two bulk junctions 2 x (4 + 2), five capacitances
+// 5 x (4 + 2), two series resistances 2 x 4; Q: the transport branch 8 + 2,
+// three branches 3 x (4 + 2), the charge branches 8 + 2 and 3 x (4 + 2), three
+// series resistances 3 x 4): the size of its record in the wavefront form.
+KH_HD inline int record_capacity(int kind) {
+  switch (kind) {
+    case K_R: return 4;
+    case K_C: return 6;
+    case K_L: return 6;
+    case K_V: return 5;
+    case K_I: return 2;
+    case K_D: return 10;
+    case K_M: return 64;
+    case K_Q: return 68;
+  }
+  return 0;
+}
 
 struct OpPt {  // device.OperatingPoint
   double voltage, current, power, storage;

@@ -356,4 +356,61 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
   return true;
 }
 
+WavePlan build_wave_plan(const Topology& T, int rec_limit) {
+  WavePlan w;
+  const int nel = T.nel, ns = T.nslots;
+  w.rec_off.assign(nel, 0);
+  w.ch_el.push_back(0);
+  int cur = 0, cnt = 0;
+  for (int k = 0; k < nel; k++) {
+    const int cap = record_capacity(T.et[(size_t)k * ET_W + ET_KIND]);
+    if (cnt == 64 || (cnt > 0 && cur + cap > rec_limit)) {
+      w.ch_el.push_back(k);
+      w.rec_cap = std::max(w.rec_cap, cur);
+      cur = 0;
+      cnt = 0;
+    }
+    w.rec_off[k] = cur;
+    cur += cap;
+    cnt++;
+  }
+  w.ch_el.push_back(nel);
+  w.rec_cap = std::max(w.rec_cap, cur);
+  const int nch = (int)w.ch_el.size() - 1;
+  w.ch_t.push_back(0);
+  w.tel_ptr.push_back(0);
+  for (int c = 0; c < nch; c++) {
+    std::map<int32_t, std::vector<int32_t>> touch;  // target -> elements, in element order
+    for (int k = w.ch_el[c]; k < w.ch_el[c + 1]; k++) {
+      const int32_t* e = &T.et[(size_t)k * ET_W];
+      std::set<int32_t> mine;
+      for (int q = 0; q < KH_NT * KH_NT; q++)
+        if (e[ET_SLOT + q] >= 0) mine.insert(e[ET_SLOT + q]);
+      for (int t = 0; t < KH_NT; t++)
+        if (e[ET_TERM + t] >= 0 && e[ET_TERM + t] < T.n) mine.insert(ns + e[ET_TERM + t]);
+      for (int32_t tg : mine) touch[tg].push_back(k);
+    }
+    for (auto& kv : touch) {
+      w.tgt.push_back(kv.first);
+      for (int32_t k : kv.second) w.tel.push_back(k);
+      w.tel_ptr.push_back((int32_t)w.tel.size());
+    }
+    w.ch_t.push_back((int32_t)w.tgt.size());
+  }
+  // AbsProduct rows: slot order, and row-major (column order within a row).
+  std::vector<std::vector<int32_t>> rows(T.n);
+  for (int s2 = 0; s2 < ns; s2++) rows[T.slot_i[s2]].push_back(s2);
+  w.ro_ptr_sparse.push_back(0);
+  w.ro_ptr_dense.push_back(0);
+  for (int i = 0; i < T.n; i++) {
+    for (int32_t s2 : rows[i]) w.ro_slot_sparse.push_back(s2);
+    w.ro_ptr_sparse.push_back((int32_t)w.ro_slot_sparse.size());
+    std::vector<int32_t> r = rows[i];
+    std::sort(r.begin(), r.end(), [&](int32_t a, int32_t b) { return T.slot_j[a] < T.slot_j[b]; });
+    for (int32_t s2 : r) w.ro_slot_dense.push_back(s2);
+    w.ro_ptr_dense.push_back((int32_t)w.ro_slot_dense.size());
+  }
+  return w;
+}
+
 }  // namespace kh

@@ -67,4 +67,20 @@ struct Topology {
   bool build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0);
 };
 
+// Stamping plan of the wavefront form (kh_wave.h, the fields of Circ that
+// name it): consecutive elements in chunks of at most 64 whose records
+// (record_capacity entries each) fit rec_limit entries; per chunk the targets
+// its elements can touch — every slot of an element's slot table and every
+// terminal's right-hand-side row (nslots + row) — each with the chunk's
+// elements that can touch it, in element order. ro_*: the slots of each row
+// in the order AbsProduct accumulates them, slot order for the sparse
+// backend (solver/sparse.go:481-490) and row-major for the dense one
+// (solver/dense.go:63-73).
+struct WavePlan {
+  std::vector<int32_t> ch_el, ch_t, tgt, tel_ptr, tel, rec_off;
+  std::vector<int32_t> ro_ptr_sparse, ro_slot_sparse, ro_ptr_dense, ro_slot_dense;
+  int32_t rec_cap = 0;  // the largest chunk's record entries
+};
+WavePlan build_wave_plan(const Topology& T, int rec_limit);
+
 }  // namespace kh

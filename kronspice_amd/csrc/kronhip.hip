@@ -30,6 +30,7 @@
 #include "kh_group.h"
 #include "kh_persist.h"
 #include "kh_symbolic.h"
+#include "kh_wave.h"
 
 using namespace kh;
 
@@ -391,6 +392,11 @@ struct kh_topology_s {
   Topology t;
   DBuf et, isV, slot_i, slot_j, diag, rowmajor;  // rowmajor: slots sorted by (row, column)
   DBuf lists;  // nonlinear, reactive and source element indices, concatenated
+  // the wavefront form's stamping plan and AbsProduct rows (WavePlan), concatenated:
+  // ch_el, ch_t, tgt, tel_ptr, tel, rec_off, ro_ptr / ro_slot sparse, ro_ptr / ro_slot dense
+  DBuf wave;
+  int64_t wave_off[10] = {};
+  int32_t nchunks = 0, rec_cap = 0;
   std::mutex mu;  // batch entry points on one topology run one at a time
 };
 
@@ -693,6 +699,21 @@ kh_status kh_topology_create(int32_t num_nodes, const kh_element* elems, int32_t
   li.insert(li.end(), T.src_idx.begin(), T.src_idx.end());
   KH_HIP(t->lists.alloc(std::max<size_t>(li.size(), 1) * 4));
   if (!li.empty()) KH_HIP(hipMemcpy(t->lists.p, li.data(), li.size() * 4, hipMemcpyHostToDevice));
+  {
+    const WavePlan w = build_wave_plan(T, kWaveRecordLimit);
+    const std::vector<int32_t>* parts[10] = {&w.ch_el,        &w.ch_t,          &w.tgt,          &w.tel_ptr,
+                                             &w.tel,          &w.rec_off,       &w.ro_ptr_sparse, &w.ro_slot_sparse,
+                                             &w.ro_ptr_dense, &w.ro_slot_dense};
+    std::vector<int32_t> all;
+    for (int k = 0; k < 10; k++) {
+      t->wave_off[k] = (int64_t)all.size();
+      all.insert(all.end(), parts[k]->begin(), parts[k]->end());
+    }
+    t->nchunks = (int32_t)w.ch_el.size() - 1;
+    t->rec_cap = w.rec_cap;
+    KH_HIP(t->wave.alloc(std::max<size_t>(all.size(), 1) * 4));
+    if (!all.empty()) KH_HIP(hipMemcpy(t->wave.p, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+  }
   *out = t.release();
   return KH_OK;
 }
@@ -716,6 +737,25 @@ KH_PERSIST_KINDS(2)
 KH_PERSIST_KINDS(4)
 #undef KH_PERSIST_KINDS
 #undef KH_PERSIST_VARIANT
+
+#define KH_WAVE_VARIANT(K)                                                                                 \
+  hipError_t kh_launch_wave_k##K(unsigned, size_t, const Group*, const int32_t*, const int32_t*, int32_t, int32_t*, \
+                                 int32_t, int32_t*);
+KH_WAVE_VARIANT(126)
+KH_WAVE_VARIANT(190)
+KH_WAVE_VARIANT(318)
+KH_WAVE_VARIANT(1022)
+#undef KH_WAVE_VARIANT
+
+hipError_t kh_launch_wave(unsigned kinds, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* gw,
+                          const int32_t* cw, int32_t nwork, int32_t* next, int32_t max_solves, int32_t* counters) {
+  using L = hipError_t (*)(unsigned, size_t, const Group*, const int32_t*, const int32_t*, int32_t, int32_t*, int32_t,
+                           int32_t*);
+  static const L table[4] = {kh_launch_wave_k126, kh_launch_wave_k190, kh_launch_wave_k318, kh_launch_wave_k1022};
+  const unsigned v = persist_kinds(kinds);
+  const int kv = v == kKindsD ? 0 : v == kKindsM ? 1 : v == kKindsQ ? 2 : 3;
+  return table[kv](grid, lds_bytes, G, gw, cw, nwork, next, max_solves, counters);
+}
 
 hipError_t kh_launch_persist(int waves, unsigned kinds, unsigned grid, size_t lds_bytes, const Group* G,
                              const int32_t* bgroup, int32_t max_solves, int32_t* counters) {
@@ -806,6 +846,16 @@ struct GroupRun {
     a.ckt.n_nl = (int32_t)T.nl_idx.size();
     a.ckt.n_rx = (int32_t)T.rx_idx.size();
     a.ckt.n_src = (int32_t)T.src_idx.size();
+    {
+      const int32_t* w = topo->wave.as<int32_t>();
+      const int64_t* o = topo->wave_off;
+      a.ckt.nchunks = topo->nchunks;
+      a.ckt.rec_cap = topo->rec_cap;
+      a.ckt.ch_el = w + o[0]; a.ckt.ch_t = w + o[1]; a.ckt.tgt = w + o[2]; a.ckt.tel_ptr = w + o[3];
+      a.ckt.tel = w + o[4]; a.ckt.rec_off = w + o[5];
+      a.ckt.ro_ptr = w + (dense ? o[8] : o[6]);
+      a.ckt.ro_slot = w + (dense ? o[9] : o[7]);
+    }
     a.ckt.nl = topo->lists.as<int32_t>();
     a.ckt.rx = a.ckt.nl + a.ckt.n_nl;
     a.ckt.src = a.ckt.rx + a.ckt.n_rx;
@@ -984,6 +1034,10 @@ struct Runner {
   int32_t waves = 1;  // k_persist variant: resident wavefronts per SIMD (1, 2 or 4)
   int64_t lds_limit = 64 * 1024;  // raised to the device's opt-in maximum (160 KiB on gfx950)
   size_t lds_launch = 0;
+  // k_wave (kh_options.circuit_wave): one wavefront per circuit, circuits
+  // taken from a work list, largest first.
+  bool wave = false;
+  DBuf dgw, dcw, dnext;
 
   ~Runner() {
     if (hcount) (void)hipHostFree(hcount);
@@ -995,6 +1049,43 @@ struct Runner {
     for (auto* gr : groups) kinds |= gr->topo->t.kinds;
     std::vector<int32_t> bg;
     std::vector<Group> hg(ng);
+    unsigned wave_grid = 0;
+    int32_t nwork = 0;
+    if (wave) {
+      // Work list: every circuit, the largest topologies first (the longest
+      // serial chains start first; the small ones fill in behind them).
+      std::vector<std::pair<int32_t, int32_t>> order;
+      for (size_t g = 0; g < ng; g++) order.push_back({groups[g]->topo->t.n, (int32_t)g});
+      std::stable_sort(order.begin(), order.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+      std::vector<int32_t> gw, cw;
+      for (const auto& o : order)
+        for (int64_t c = 0; c < groups[o.second]->B; c++) {
+          gw.push_back(o.second);
+          cw.push_back((int32_t)c);
+        }
+      nwork = (int32_t)gw.size();
+      KH_HIP(dgw.alloc(gw.size() * 4));
+      KH_HIP(dcw.alloc(cw.size() * 4));
+      KH_HIP(dnext.alloc(8));
+      KH_HIP(hipMemcpy(dgw.p, gw.data(), gw.size() * 4, hipMemcpyHostToDevice));
+      KH_HIP(hipMemcpy(dcw.p, cw.data(), cw.size() * 4, hipMemcpyHostToDevice));
+      int64_t lds = 0;
+      for (size_t g = 0; g < ng; g++)
+        lds = std::max(lds, wave_lds_bytes(groups[g]->topo->t.n, groups[g]->topo->rec_cap));
+      lds = round8(lds);
+      int dev = 0, optin = 0, ncu = 0, per_cu = 0;
+      KH_HIP(hipGetDevice(&dev));
+      KH_HIP(hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev));
+      KH_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      KH_HIP(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+      if (lds > optin)
+        return fail(KH_CAPACITY, "circuit_wave: a circuit of this size needs " + std::to_string(lds) +
+                                     " bytes of LDS per wavefront, above the device's " + std::to_string(optin));
+      lds_launch = (size_t)lds;
+      const int64_t resident = (int64_t)std::max(ncu, 1) *
+                               std::min<int64_t>(16, std::max<int64_t>(1, (per_cu > 0 ? per_cu : optin) / lds));
+      wave_grid = (unsigned)std::min<int64_t>(nwork, resident);
+    }
     for (size_t g = 0; g < ng; g++) {
       hg[g].block0 = (int64_t)bg.size();
       hg[g].gid = (int32_t)g;
@@ -1024,12 +1115,12 @@ struct Runner {
     bool dirty = true;
     for (;;) {
       if (dirty) {  // group descriptors: LU workspace pointers change when capacity grows
-        lds_launch = 0;
+        if (!wave) lds_launch = 0;
         for (size_t g = 0; g < ng; g++) {
           hg[g].ctl = groups[g]->ctl_args(p);
           hg[g].lu = groups[g]->lu_args();
           hg[g].lds_bytes = 0;
-          if (lds_staging) {
+          if (lds_staging && !wave) {
             // An odd number of 8-byte words per circuit puts the circuits of
             // one wavefront in different LDS banks.
             int64_t b = round8(lds_need(hg[g]));
@@ -1044,9 +1135,14 @@ struct Runner {
         dirty = false;
       }
       KH_HIP(hipMemsetAsync(counters.p, 0, ng * 8, 0));
+      if (wave) KH_HIP(hipMemsetAsync(dnext.p, 0, 8, 0));
       KH_HIP(hipEventRecord(ev[0], 0));
-      KH_HIP(kh_launch_persist(waves, kinds, (unsigned)bg.size(), lds_launch, dgroups.as<Group>(),
-                               dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>()));
+      if (wave)
+        KH_HIP(kh_launch_wave(kinds, wave_grid, lds_launch, dgroups.as<Group>(), dgw.as<int32_t>(), dcw.as<int32_t>(),
+                              nwork, dnext.as<int32_t>(), max_solves, counters.as<int32_t>()));
+      else
+        KH_HIP(kh_launch_persist(waves, kinds, (unsigned)bg.size(), lds_launch, dgroups.as<Group>(),
+                                 dbgroup.as<int32_t>(), max_solves, counters.as<int32_t>()));
       KH_HIP(hipEventRecord(ev[1], 0));
       KH_HIP(hipMemcpyAsync(hcount, counters.p, ng * 8, hipMemcpyDeviceToHost, 0));
       KH_HIP(hipStreamSynchronize(0));
@@ -1177,6 +1273,7 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
       R.lds_limit = std::max<int64_t>(R.lds_limit, optin);
   }
   R.waves = (opt && (opt->waves_per_simd == 2 || opt->waves_per_simd == 4)) ? opt->waves_per_simd : 1;
+  R.wave = opt && opt->circuit_wave != 0;
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];
     if (!G.topology || G.B <= 0 || !G.d_params) return fail(KH_BAD_ARGUMENT, "bad group " + std::to_string(g));

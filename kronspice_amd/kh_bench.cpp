@@ -2,6 +2,7 @@
 // Python, no PyTorch in the process), for timing and for profilers.
 //
 //   kh_bench <batch file> [steps] [warmup] [staging] [circuits per wave] [waves per SIMD] [precision]
+//            [circuit per wavefront: 1 = k_wave] [backend]
 //   kh_bench --copy <MiB>    one device-to-device copy kernel of that size:
 //                            the reference point for hardware byte counters
 //
@@ -88,6 +89,8 @@ int main(int argc, char** argv) {
   const int cpw = argc > 5 ? atoi(argv[5]) : 0;
   const int waves = argc > 6 ? atoi(argv[6]) : 0;
   const int precision = argc > 7 ? atoi(argv[7]) : 0;
+  const int circuit_wave = argc > 8 ? atoi(argv[8]) : 0;
+  const int backend = argc > 9 ? atoi(argv[9]) : 0;
   FILE* f = fopen(argv[1], "rb");
   if (!f) return die("cannot open the batch file");
   char magic[4];
@@ -164,6 +167,8 @@ int main(int argc, char** argv) {
   opt.circuits_per_wave = cpw;
   opt.waves_per_simd = waves;
   opt.precision = precision;
+  opt.circuit_wave = circuit_wave;
+  opt.backend = backend;
   kh_tran_spec spec;
   memset(&spec, 0, sizeof(spec));
   spec.step = spec3[0];
@@ -206,13 +211,13 @@ int main(int argc, char** argv) {
   for (auto& kv : hist) h += (h.size() > 1 ? ", \"" : "\"") + std::to_string(kv.first) + "\": " + std::to_string(kv.second);
   h += "}";
   printf("{\"circuits\": %lld, \"groups\": %d, \"staging\": %d, \"circuits_per_wave\": %d, \"waves_per_simd\": %d, "
-         "\"precision\": %d, \"refine_passes_per_step\": %.1f, \"fp64_redos_per_step\": %.1f, \"steps\": %d, "
+         "\"precision\": %d, \"circuit_wave\": %d, \"backend\": %d, \"refine_passes_per_step\": %.1f, \"fp64_redos_per_step\": %.1f, \"steps\": %d, "
          "\"seconds_per_step\": %.6f, \"circuits_per_s\": %.3f, "
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
          "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"cycles_stamp\": %.6g, "
          "\"cycles_converge\": %.6g, \"cycles_lte\": %.6g, \"cycles_accept\": %.6g, \"cycles_ledger\": %.6g, "
          "\"status_histogram\": %s}\n",
-         (long long)circuits, ngroups, staging, cpw, waves, precision, (double)passes / steps, (double)redos / steps, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
+         (long long)circuits, ngroups, staging, cpw, waves, precision, circuit_wave, backend, (double)passes / steps, (double)redos / steps, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, cyc_lu / steps, cyc_ctl / steps,
          cyc_part[0] / steps, cyc_part[1] / steps, cyc_part[2] / steps, cyc_part[3] / steps, cyc_part[4] / steps,
          h.c_str());
