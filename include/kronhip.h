@@ -142,7 +142,9 @@ typedef struct kh_options {
                                circuits_per_wave / staging / waves_per_simd apply); 1 -> each
                                circuit runs on a whole wavefront, its stamping, tests and LU
                                spread over the lanes, circuits taken largest first from a queue
-                               (k_wave; FP64 LU: precision is not applied) */
+                               (k_wave; FP64 LU: precision is not applied). Switched off in this
+                               build: 1 returns KH_UNSUPPORTED (its first device run did not
+                               finish); the CPU emulation of its device code checks it */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -201,6 +203,24 @@ kh_status kh_sys_absproduct(kh_sys s, const double* x, double* y);   /* Matrix.A
 kh_status kh_sys_nnz(kh_sys s, int64_t* a, int64_t* l, int64_t* u);  /* Sparse.NNZ */
 kh_status kh_sys_pivots(kh_sys s, int32_t* q, int32_t* prow);        /* ordering q and pivot rows of the held factorization */
 void kh_sys_destroy(kh_sys s);
+
+/* ---- complex per-system seam: solver.CMatrix (solver/matrix.go:39-46) --------
+ * CSparse (solver/sparse.go:495-531) behind the CMatrix interface, which
+ * analysis.AC obtains from SolverFactory.CMatrix (analysis/op.go:67-78):
+ * complex128 stamps accumulated in first-Add order, Gilbert–Peierls LU with
+ * pivoting on cmplx.Abs magnitudes on the GPU. Complex values are (re, im)
+ * double pairs: v [2*count], b and x [2*n]. AbsProduct (solver/sparse.go:
+ * 519-527) writes y [n] = sum_j |A_ij|·|x_j| with cmplx.Abs magnitudes. */
+typedef struct kh_csys_s* kh_csys;
+kh_status kh_csys_create(int32_t n, kh_csys* out);                   /* solver.NewCSparse */
+kh_status kh_csys_clear(kh_csys s);                                  /* CMatrix.Clear */
+kh_status kh_csys_add_batch(kh_csys s, const int32_t* i, const int32_t* j,
+                            const double* v, int64_t count);         /* CMatrix.Add x count */
+kh_status kh_csys_solve(kh_csys s, const double* b, double* x);      /* CMatrix.Solve */
+kh_status kh_csys_absproduct(kh_csys s, const double* x, double* y); /* CMatrix.AbsProduct */
+kh_status kh_csys_nnz(kh_csys s, int64_t* a, int64_t* l, int64_t* u);  /* CSparse.NNZ */
+kh_status kh_csys_pivots(kh_csys s, int32_t* q, int32_t* prow);      /* ordering q and pivot rows */
+void kh_csys_destroy(kh_csys s);
 
 /* ---- batched numeric LU on one pattern ---------------------------------------
  * B systems share the entry list (ri[k], cj[k]), k < nnz, in first-Add order;

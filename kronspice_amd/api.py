@@ -99,6 +99,15 @@ def lib():
         L.kh_sys_pivots.argtypes = [P, P, P]
         L.kh_sys_destroy.argtypes = [P]
         L.kh_sys_destroy.restype = None
+        for name in ("kh_csys_clear", "kh_csys_destroy"):
+            getattr(L, name).argtypes = [P]
+        L.kh_csys_destroy.restype = None
+        L.kh_csys_create.argtypes = [i32, P]
+        L.kh_csys_add_batch.argtypes = [P, P, P, P, i64]
+        L.kh_csys_solve.argtypes = [P, P, P]
+        L.kh_csys_absproduct.argtypes = [P, P, P]
+        L.kh_csys_nnz.argtypes = [P, P, P, P]
+        L.kh_csys_pivots.argtypes = [P, P, P]
         L.kh_lu_batch_create.argtypes = [i32, i64, P, P, i32, P]
         L.kh_lu_batch_solve.argtypes = [P, P, P, P, P, P, P]
         L.kh_lu_batch_solve_mixed.argtypes = [P, P, P, P, P, P, P, P]
@@ -199,6 +208,60 @@ class Sparse:
     def pivots(self):
         q = np.zeros(self.n, np.int32); prow = np.zeros(self.n, np.int32)
         _check(lib().kh_sys_pivots(self.h, _p(q), _p(prow)))
+        return q, prow
+
+
+class CSparse:
+    """solver.CMatrix on the GPU (CSparse, solver/sparse.go:495-531): complex
+    stamps staged on the host, one kh_csys_add_batch per Solve."""
+
+    def __init__(self, n: int):
+        self.n = n
+        self.h = C.c_void_p()
+        _check(lib().kh_csys_create(n, C.byref(self.h)))
+        self._i, self._j, self._v = [], [], []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_csys_destroy(self.h)
+            self.h = None
+
+    def clear(self):
+        self._flush()
+        _check(lib().kh_csys_clear(self.h))
+
+    def add(self, i: int, j: int, v: complex):
+        self._i.append(i); self._j.append(j); self._v.append(complex(v))
+
+    def _flush(self):
+        if self._i:
+            i = np.asarray(self._i, np.int32); j = np.asarray(self._j, np.int32)
+            v = np.asarray(self._v, np.complex128)
+            _check(lib().kh_csys_add_batch(self.h, _p(i), _p(j), _p(v), len(i)))
+            self._i, self._j, self._v = [], [], []
+
+    def solve(self, b) -> np.ndarray:
+        self._flush()
+        b = np.ascontiguousarray(b, np.complex128)
+        x = np.zeros(self.n, np.complex128)
+        _check(lib().kh_csys_solve(self.h, _p(b), _p(x)))
+        return x
+
+    def abs_product(self, x) -> np.ndarray:
+        self._flush()
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros(self.n)
+        _check(lib().kh_csys_absproduct(self.h, _p(x), _p(y)))
+        return y
+
+    def nnz(self):
+        a, l, u = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().kh_csys_nnz(self.h, C.byref(a), C.byref(l), C.byref(u)))
+        return a.value, l.value, u.value
+
+    def pivots(self):
+        q = np.zeros(self.n, np.int32); prow = np.zeros(self.n, np.int32)
+        _check(lib().kh_csys_pivots(self.h, _p(q), _p(prow)))
         return q, prow
 
 

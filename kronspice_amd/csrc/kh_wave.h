@@ -37,11 +37,11 @@ constexpr int kWaveLanes = 64;
 constexpr int kWaveRecordLimit = 1024;
 
 // LDS bytes a wavefront needs for a circuit of n unknowns whose stamping
-// chunks need rec_cap record entries: control words, records, reduction
-// scratch, and the LU's working column, permuted right-hand side, pivot map,
-// pivot rows and column pointers.
+// chunks need rec_cap record entries: records, reduction scratch, and the
+// LU's working column, permuted right-hand side, pivot map, pivot rows and
+// column pointers.
 KH_HD inline int64_t wave_lds_bytes(int64_t n, int64_t rec_cap) {
-  return 512 + rec_cap * 12 + 64 * 4 + 64 * 8 * 6 + 16 * n + 12 * n + 8 * (n + 1) + 64;
+  return rec_cap * 12 + 64 * 4 + 64 * 8 * 6 + 16 * n + 12 * n + 8 * (n + 1) + 64;
 }
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -56,6 +56,8 @@ __device__ inline int wlane() { return (int)threadIdx.x; }
 __device__ inline void lds_fence() {
 #if defined(__HIP_DEVICE_COMPILE__)
   asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#else
+  __syncthreads();  // host emulation of a wavefront (lanes as threads): a barrier
 #endif
 }
 // Global and LDS writes of every lane visible to every lane of the workgroup
@@ -96,15 +98,41 @@ __device__ inline int64_t wsum_i(int64_t v) {
   return v;
 }
 
-// The LU working set of one circuit held in LDS.
-struct WLu {
-  LdsP<double> x;      // [n] working column; the solve's y
-  LdsP<double> w;      // [n] permuted right-hand side
+// The LU working set of one system held in LDS (T = double, or kcplx for
+// the complex seam).
+template <class T>
+struct WLuT {
+  LdsP<T> x;           // [n] working column; the solve's y
+  LdsP<T> w;           // [n] permuted right-hand side
   LdsP<int32_t> pinv;  // [n]
   LdsP<int32_t> prow;  // [n]
   LdsP<int32_t> lp;    // [n+1]
   LdsP<int32_t> up;    // [n+1]
 };
+using WLu = WLuT<double>;
+
+// Carves W out of LDS at p: returns the bytes used (each array 16-aligned).
+template <class T>
+__device__ inline int64_t wlu_carve(char* p, int n, WLuT<T>& W) {
+  int64_t off = 0;
+  auto take = [&](int64_t bytes) {
+    char* q = p + off;
+    off += (bytes + 15) & ~(int64_t)15;
+    return q;
+  };
+  W.x = (LdsP<T>)(T*)take((int64_t)n * sizeof(T));
+  W.w = (LdsP<T>)(T*)take((int64_t)n * sizeof(T));
+  W.pinv = (LdsP<int32_t>)(int32_t*)take((int64_t)n * 4);
+  W.prow = (LdsP<int32_t>)(int32_t*)take((int64_t)n * 4);
+  W.lp = (LdsP<int32_t>)(int32_t*)take((int64_t)(n + 1) * 4);
+  W.up = (LdsP<int32_t>)(int32_t*)take((int64_t)(n + 1) * 4);
+  return off;
+}
+// LDS bytes of wlu_carve.
+KH_HD inline int64_t wlu_lds_bytes(int64_t n, int64_t tsize) {
+  auto r = [](int64_t b) { return (b + 15) & ~(int64_t)15; };
+  return 2 * r(n * tsize) + 2 * r(n * 4) + 2 * r((n + 1) * 4);
+}
 
 // Wavefront argmax over (|x|, row) with the reference's rule: the larger
 // magnitude wins, equal magnitudes go to the lower row (solver/sparse.go:
@@ -122,7 +150,9 @@ __device__ inline void warg(double& best, int& piv) {
 
 // reach, solver/sparse.go:301-352, run by every lane with identical values
 // (the same reads and writes): returns top, the reach is xi[top..n).
-__device__ inline int wlu_reach(const LuArgs& a, const WLu& W, int k, const int32_t* ap, const int32_t* arow) {
+template <class T>
+__device__ inline int wlu_reach(const LuArgsT<T>& a, const WLuT<T>& W, int k, const int32_t* ap,
+                                const int32_t* arow) {
   const int n = a.n;
   const int32_t gen = k + 1;
   int top = n;
@@ -164,7 +194,8 @@ __device__ inline int wlu_reach(const LuArgs& a, const WLu& W, int k, const int3
 // factor, solver/sparse.go:199-293, on one circuit's unit-stride arrays with
 // the working set of W; the same reach reuse as lu_factor (kh_lu.h). Returns
 // the same status on every lane.
-__device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
+template <class T>
+__device__ inline int wlu_factor(const LuArgsT<T>& a, const WLuT<T>& W) {
   const int lane = wlane();
   const int n = a.n;
   const int32_t pidc = a.pid[0];
@@ -175,9 +206,9 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
   double scale = 0.0;
   bool bad = false;
   for (int s = lane; s < a.nslots; s += kWaveLanes) {
-    const double v = a.val[s];
+    const T v = a.val[s];
     if (!lu_finite(v)) bad = true;
-    const double av = fabs(v);
+    const double av = lu_mag(v);
     if (av > scale) scale = av;
   }
   if (wany(bad)) return LU_NONFINITE;
@@ -207,8 +238,12 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
       r1 = a.rp[k + 1];
       diag_in_reach = a.dk[k] != 0;
     } else {
+      // The depth-first search is serial: lane 0 runs it, the others wait.
       lds_fence();
-      const int top = wlu_reach(a, W, k, ap, arow);
+      int top = 0;
+      if (lane == 0) top = wlu_reach(a, W, k, ap, arow);
+      top = __shfl(top, 0);
+      wsync();
       r1 = r0 + (n - top);
       if (r1 > a.capR) return LU_CAPACITY;
       for (int p = top + lane; p < n; p += kWaveLanes) a.rch[r0 + p - top] = a.xi[p];
@@ -221,13 +256,13 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
     }
     nr = r1;
     const int m = r1 - r0;
-    for (int p = lane; p < m; p += kWaveLanes) W.x[a.rch[r0 + p]] = 0.0;
+    for (int p = lane; p < m; p += kWaveLanes) W.x[a.rch[r0 + p]] = lu_zero_value<T>();
     lds_fence();
     double colMax = 0.0;
     for (int p = ap[k] + lane; p < ap[k + 1]; p += kWaveLanes) {
-      const double v = a.val[aslot[p]];
+      const T v = a.val[aslot[p]];
       W.x[arow[p]] = v;
-      const double av = fabs(v);
+      const double av = lu_mag(v);
       if (av > colMax) colMax = av;
     }
     colMax = wmax(colMax);
@@ -249,12 +284,12 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
       for (int t = 0; t < cnt; t++) {
         const int jt = __shfl(j, t);
         if (jt < 0) continue;
-        const double xj = W.x[__shfl(r, t)];
-        if (xj == 0) continue;
+        const T xj = W.x[__shfl(r, t)];
+        if (lu_zero(xj)) continue;
         const int qa = __shfl(q0, t), ql = __shfl(len, t);
         for (int q = lane; q < ql; q += kWaveLanes) {
           const int i = a.li[qa + q];
-          const double prod = a.lx[qa + q] * xj;
+          const T prod = a.lx[qa + q] * xj;
           W.x[i] = W.x[i] - prod;
         }
         lds_fence();
@@ -280,7 +315,7 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
       }
       nu += cu;
       if (p < m && pr < 0) {
-        const double av = fabs(W.x[r]);
+        const double av = lu_mag(W.x[r]);
         if (av > best || (av == best && r < piv)) {
           best = av;
           piv = r;
@@ -294,7 +329,7 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
     }
     // KLU diagonal preference (solver/sparse.go:274-277).
     if (W.pinv[k] < 0 && diag_in_reach) {
-      const double ak = fabs(W.x[k]);
+      const double ak = lu_mag(W.x[k]);
       if (ak >= 1e-3 * best && ak > tiny) piv = k;
     }
     if (reuse && piv != W.prow[k]) {
@@ -303,7 +338,7 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
       for (int i = lane; i < n; i += kWaveLanes) a.mark[i] = 0;
       wsync();
     }
-    const double pv = W.x[piv];
+    const T pv = W.x[piv];
     lds_fence();
     if (lane == 0) {
       a.udiag[k] = pv;
@@ -324,7 +359,7 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
       const int at = nl + wrank(isL);
       if (isL) {
         a.li[at] = r;
-        a.lx[at] = W.x[r] / pv;
+        a.lx[at] = lu_div(W.x[r], pv);
       }
       nl += cl;
     }
@@ -351,7 +386,8 @@ __device__ inline int wlu_factor(const LuArgs& a, const WLu& W) {
 // solve, solver/sparse.go:355-401, with the factors wlu_factor left in W and
 // in the circuit's arrays. x is written in original unknown order up to the
 // first non-finite entry, as the reference does.
-__device__ inline int wlu_solve(const LuArgs& a, const WLu& W) {
+template <class T>
+__device__ inline int wlu_solve(const LuArgsT<T>& a, const WLuT<T>& W) {
   const int lane = wlane();
   const int n = a.n;
   const PatDesc pd = a.pats[a.pid[0]];
@@ -364,14 +400,14 @@ __device__ inline int wlu_solve(const LuArgs& a, const WLu& W) {
   {
     int qa = W.lp[0], ql = W.lp[1] - qa;
     int ni = 0;
-    double nv = 0;
+    T nv = lu_zero_value<T>();
     if (lane < ql) {
       ni = a.li[qa + lane];
       nv = a.lx[qa + lane];
     }
     for (int k = 0; k < n; k++) {
       const int ci = ni, cqa = qa, cql = ql;
-      const double cv = nv;
+      const T cv = nv;
       if (k + 1 < n) {
         qa = W.lp[k + 1];
         ql = W.lp[k + 2] - qa;
@@ -380,16 +416,16 @@ __device__ inline int wlu_solve(const LuArgs& a, const WLu& W) {
           nv = a.lx[qa + lane];
         }
       }
-      const double yk = W.w[W.prow[k]];
+      const T yk = W.w[W.prow[k]];
       if (lane == 0) W.x[k] = yk;
-      if (yk != 0) {
+      if (!lu_zero(yk)) {
         if (lane < cql) {
-          const double prod = cv * yk;
+          const T prod = cv * yk;
           W.w[ci] = W.w[ci] - prod;
         }
         for (int p = kWaveLanes + lane; p < cql; p += kWaveLanes) {
           const int i = a.li[cqa + p];
-          const double prod = a.lx[cqa + p] * yk;
+          const T prod = a.lx[cqa + p] * yk;
           W.w[i] = W.w[i] - prod;
         }
       }
@@ -400,14 +436,14 @@ __device__ inline int wlu_solve(const LuArgs& a, const WLu& W) {
   {
     int qa = W.up[n - 1], ql = W.up[n] - qa;
     int ni = 0;
-    double nv = 0, nd = a.udiag[n - 1];
+    T nv = lu_zero_value<T>(), nd = a.udiag[n - 1];
     if (lane < ql) {
       ni = a.ui[qa + lane];
       nv = a.ux[qa + lane];
     }
     for (int k = n - 1; k >= 0; k--) {
       const int ci = ni, cqa = qa, cql = ql;
-      const double cv = nv, cd = nd;
+      const T cv = nv, cd = nd;
       if (k > 0) {
         qa = W.up[k - 1];
         ql = W.up[k] - qa;
@@ -417,17 +453,17 @@ __device__ inline int wlu_solve(const LuArgs& a, const WLu& W) {
           nv = a.ux[qa + lane];
         }
       }
-      const double zk = W.x[k] / cd;
+      const T zk = lu_div(W.x[k], cd);
       lds_fence();
       if (lane == 0) W.x[k] = zk;
-      if (zk != 0) {
+      if (!lu_zero(zk)) {
         if (lane < cql) {
-          const double prod = cv * zk;
+          const T prod = cv * zk;
           W.x[ci] = W.x[ci] - prod;
         }
         for (int p = kWaveLanes + lane; p < cql; p += kWaveLanes) {
           const int i = a.ui[cqa + p];
-          const double prod = a.ux[cqa + p] * zk;
+          const T prod = a.ux[cqa + p] * zk;
           W.x[i] = W.x[i] - prod;
         }
       }
@@ -875,8 +911,6 @@ __device__ inline void run_circuit_wave(const Group& g, int64_t cc, int32_t max_
   const int n = a.ckt.n;
   // LDS layout (wave_lds_bytes)
   char* p = lds;
-  int32_t* lci = (int32_t*)p; p += 128;
-  double* lcd = (double*)p; p += 384;
   WArea R;
   const int rc = a.ckt.rec_cap;
   R.rslot = (LdsP<int32_t>)(int32_t*)p; p += (int64_t)rc * 4;
@@ -891,13 +925,16 @@ __device__ inline void run_circuit_wave(const Group& g, int64_t cc, int32_t max_
   W.prow = (LdsP<int32_t>)(int32_t*)p; p += (int64_t)n * 4;
   W.lp = (LdsP<int32_t>)(int32_t*)p; p += (int64_t)(n + 1) * 4;
   W.up = (LdsP<int32_t>)(int32_t*)p;
+  // Every lane keeps its own copy of the control words and advances it
+  // identically; lane 0 writes them back.
   int32_t* gci = a.ci;
   double* gcd = a.cd;
-  for (int i = lane; i < NCI; i += kWaveLanes) lci[i] = gci[i];
-  for (int i = lane; i < NCD; i += kWaveLanes) lcd[i] = gcd[i];
-  lds_fence();
-  a.ci = lci;
-  a.cd = lcd;
+  int32_t pci[NCI];
+  double pcd[NCD];
+  for (int i = 0; i < NCI; i++) pci[i] = gci[i];
+  for (int i = 0; i < NCD; i++) pcd[i] = gcd[i];
+  a.ci = pci;
+  a.cd = pcd;
   WCtl ctl(a, R);
   int32_t* lus = const_cast<int32_t*>(a.lu_status);
   int solves = 0;
@@ -911,9 +948,11 @@ __device__ inline void run_circuit_wave(const Group& g, int64_t cc, int32_t max_
       if (ctl.stamp_overflow) {
         st = LU_CAPACITY;
       } else if (a.ckt.dense) {
-        // the dense backend (n < 30 under Auto): every lane runs the serial
-        // factorization on the circuit's own workspace with identical values
-        st = lu_dense<double, true>(la, 0, a.ckt.slot_i, a.ckt.slot_j);
+        // the dense backend (n < 30 under Auto): lane 0 runs the serial
+        // factorization on the circuit's own workspace
+        st = 0;
+        if (lane == 0) st = lu_dense<double, true>(la, 0, a.ckt.slot_i, a.ckt.slot_j);
+        st = __shfl(st, 0);
         wsync();
       } else {
         st = wlu_factor(la, W);
@@ -922,7 +961,6 @@ __device__ inline void run_circuit_wave(const Group& g, int64_t cc, int32_t max_
           // L/U outgrew the group's capacity: the host enlarges it and the
           // factorization is redone in the next launch.
           ctl.ci[CI_WAIT] = WAIT_CAPACITY;
-          lds_fence();
           break;
         }
       }
@@ -935,18 +973,16 @@ __device__ inline void run_circuit_wave(const Group& g, int64_t cc, int32_t max_
     }
     const uint64_t t1 = KH_CLOCK();
     ctl.step();
-    lds_fence();
     cyc_ctl += KH_CLOCK() - t1;
   }
-  if (lane == 0) {
-    ctl.cd[CD_CYC_LU] += (double)cyc_lu;
-    ctl.cd[CD_CYC_CTL] += (double)cyc_ctl;
-  }
-  lds_fence();
+  ctl.cd[CD_CYC_LU] += (double)cyc_lu;
+  ctl.cd[CD_CYC_CTL] += (double)cyc_ctl;
   *running = ctl.ci[CI_PHASE] != PH_DONE;
   *waiting = ctl.ci[CI_WAIT] != 0;
-  for (int i = lane; i < NCI; i += kWaveLanes) gci[i] = lci[i];
-  for (int i = lane; i < NCD; i += kWaveLanes) gcd[i] = lcd[i];
+  if (lane == 0) {
+    for (int i = 0; i < NCI; i++) gci[i] = pci[i];
+    for (int i = 0; i < NCD; i++) gcd[i] = pcd[i];
+  }
   wsync();
 }
 

@@ -75,6 +75,7 @@ def lib():
         _lib.oracle_sparse_solve.argtypes = [P, P, P, C.c_int32]
         _lib.oracle_sparse_info.argtypes = [P, C.c_int32, P, P, P, P, P, P]
         _lib.oracle_sparse_absproduct.argtypes = [P, P, P]
+        _lib.oracle_csparse_absproduct.argtypes = [P, P, P]
         _lib.oracle_min_degree.argtypes = [C.c_int32, P, P, C.c_int64, P]
         _lib.oracle_dense_solve.argtypes = [C.c_int32, P, P, P, C.c_int64, P, P]
         _lib.oracle_size.argtypes = [C.c_int32, P, C.c_int32, P]
@@ -147,9 +148,13 @@ class Sparse:
         return dict(nnz_a=a.value, nnz_l=l.value, nnz_u=u.value, q=q, prow=prow, err_step=es.value)
 
     def absproduct(self, x) -> np.ndarray:
-        x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.zeros(self.n)
-        lib().oracle_sparse_absproduct(self.h, _p(x), _p(y))
+        if self.cplx:
+            x = np.ascontiguousarray(x, dtype=np.complex128)
+            lib().oracle_csparse_absproduct(self.h, _p(x), _p(y))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64)
+            lib().oracle_sparse_absproduct(self.h, _p(x), _p(y))
         return y
 
 

@@ -1073,6 +1073,10 @@ void oracle_sparse_info(void* h, int32_t cplx_, int64_t* a, int64_t* l, int64_t*
 void oracle_sparse_absproduct(void* h, const double* x, double* y) {
   static_cast<SparseLU<double>*>(h)->absProduct(x, y);
 }
+// CSparse.AbsProduct (solver/sparse.go:519-527); x is n (re, im) pairs.
+void oracle_csparse_absproduct(void* h, const double* x, double* y) {
+  static_cast<SparseLU<cplx>*>(h)->absProduct(reinterpret_cast<const cplx*>(x), y);
+}
 void oracle_min_degree(int32_t n, const int32_t* ri, const int32_t* ci, int64_t nnz, int32_t* order) {
   std::vector<int32_t> r(ri, ri + nnz), c(ci, ci + nnz);
   std::vector<int> o = minimumDegree(n, r, c);
