@@ -204,6 +204,16 @@ kh_status kh_sys_nnz(kh_sys s, int64_t* a, int64_t* l, int64_t* u);  /* Sparse.N
 kh_status kh_sys_pivots(kh_sys s, int32_t* q, int32_t* prow);        /* ordering q and pivot rows of the held factorization */
 void kh_sys_destroy(kh_sys s);
 
+/* kh_sys_solve without device allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc). Same result as
+ * kh_sys_solve, bit for bit, from the same factorization; the system's
+ * pattern, values, right-hand side, solution and LU workspace live in a
+ * 32 MiB static device array per GPU, which the process holds until it ends.
+ * A system whose workspace exceeds it is refused with KH_BADSIZE. Calls on
+ * one GPU share the array: calls that overlap on one GPU race. */
+kh_status kh_sys_solve_static(kh_sys s, const double* b, double* x);
+
 /* ---- complex per-system seam: solver.CMatrix (solver/matrix.go:39-46) --------
  * CSparse (solver/sparse.go:495-531) behind the CMatrix interface, which
  * analysis.AC obtains from SolverFactory.CMatrix (analysis/op.go:67-78):

@@ -94,6 +94,7 @@ def lib():
         L.kh_sys_clear.argtypes = [P]
         L.kh_sys_add_batch.argtypes = [P, P, P, P, i64]
         L.kh_sys_solve.argtypes = [P, P, P]
+        L.kh_sys_solve_static.argtypes = [P, P, P]
         L.kh_sys_absproduct.argtypes = [P, P, P]
         L.kh_sys_nnz.argtypes = [P, P, P, P]
         L.kh_sys_pivots.argtypes = [P, P, P]
@@ -191,6 +192,16 @@ class Sparse:
         b = np.ascontiguousarray(b, np.float64)
         x = np.zeros(self.n)
         _check(lib().kh_sys_solve(self.h, _p(b), _p(x)))
+        return x
+
+    def solve_static(self, b) -> np.ndarray:
+        """solve without device allocation: the library's static device array
+        holds the system (32 MiB per GPU; larger systems raise KhError BADSIZE;
+        overlapping calls on one GPU race)."""
+        self._flush()
+        b = np.ascontiguousarray(b, np.float64)
+        x = np.zeros(self.n)
+        _check(lib().kh_sys_solve_static(self.h, _p(b), _p(x)))
         return x
 
     def abs_product(self, x) -> np.ndarray:

@@ -431,9 +431,10 @@ struct SysT {
   std::unordered_map<int64_t, int32_t> slot;
   std::vector<int32_t> ri, ci;
   std::vector<T> val;
-  bool dirty = true;
+  bool dirty = true;         // the pattern changed since the last analysis
   bool valid = false;
-  uint64_t generation = 0;  // counts pattern changes (the static scratch's owner tag)
+  uint64_t generation = 0;  // counts pattern analyses (the static scratch's owner tag)
+  uint64_t dev_generation = 0;  // the analysis the device buffers were sized for (0: none)
   Pattern pat;
   DevPatterns pats;
   LuWorkT<T> work;
@@ -521,12 +522,13 @@ void sys_analyze(SysT<T>* s) {
   s->pats.up_desc = s->pats.up_pool = 0;
   s->pats.host.add(s->pat, slots_key(slots));
   s->generation++;
+  s->dirty = false;
 }
 
 template <typename T>
 kh_status sys_prepare(SysT<T>* s) {
-  if (s->dirty) {
-    sys_analyze(s);
+  sys_analyze(s);
+  if (s->dev_generation != s->generation) {
     KH_HIP(s->pats.sync());
     KH_HIP(s->work.ensure(s->n, 1, lu_capacity(s->pat.fill_estimate, s->n)));
     KH_HIP(s->work.invalidate());  // pattern id 0 is reused for the new pattern
@@ -536,7 +538,7 @@ kh_status sys_prepare(SysT<T>* s) {
     KH_HIP(s->dx.alloc((size_t)s->n * sizeof(T)));
     KH_HIP(s->pid.alloc(4));
     KH_HIP(s->status.alloc(4));
-    s->dirty = false;
+    s->dev_generation = s->generation;
   }
   return KH_OK;
 }
@@ -642,6 +644,85 @@ kh_status sys_pivots(const SysT<T>* s, int32_t* q, int32_t* prow) {
   return KH_OK;
 }
 
+// ---- the allocation-free seam: kh_sys_solve_static --------------------------------------
+// One system's pattern, values, solution and LU workspace in a file-scope
+// device array (per device: each GPU holds its own copy, at its own address,
+// until the process ends). Calls on one GPU share it and race if they
+// overlap; the array keeps the last call's contents, so every call writes
+// what the kernels read before they write it (the pattern when another
+// system or pattern used the array last, the reach-cache tag).
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+struct ScratchDev {
+  char* ptr = nullptr;          // the device address of `scratch` on this device
+  const void* owner = nullptr;  // the system whose pattern the array holds
+  uint64_t generation = 0;
+};
+std::mutex g_scratch_mu;
+ScratchDev g_scratch[kMaxDevices];
+
+// The current device's copy of `scratch` (hipGetSymbolAddress once per device).
+kh_status scratch_dev(ScratchDev** out) {
+  int dev = 0;
+  KH_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(KH_NO_DEVICE, "device index out of range");
+  ScratchDev& d = g_scratch[dev];
+  if (!d.ptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return fail(KH_HIP_ERROR, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    d.ptr = static_cast<char*>(p);
+  }
+  *out = &d;
+  return KH_OK;
+}
+
+// Byte layout of one system in the scratch array.
+struct ScratchLayout {
+  size_t desc, pool, pid, cpid, val, rhs, x, status, xw, w, udiag, lx, ux, li, ui, lp, up, pinv, prow, xi, pstack,
+      mark, err, rch, rp, dk, total;
+  int64_t capL = 0, capR = 0;
+  void plan(int n, int64_t nslots, size_t pool_words, int64_t cap) {
+    capL = cap;
+    capR = 2 * cap + n;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+      const size_t at = off;
+      off += (bytes + 255) & ~(size_t)255;
+      return at;
+    };
+    desc = take(sizeof(PatDesc));
+    pool = take(pool_words * 4);
+    pid = take(4);
+    cpid = take(4);
+    status = take(4);
+    err = take(4);
+    val = take((size_t)nslots * 8);
+    rhs = take((size_t)n * 8);
+    x = take((size_t)n * 8);
+    xw = take((size_t)n * 8);
+    w = take((size_t)n * 8);
+    udiag = take((size_t)n * 8);
+    lx = take((size_t)cap * 8);
+    ux = take((size_t)cap * 8);
+    li = take((size_t)cap * 4);
+    ui = take((size_t)cap * 4);
+    lp = take((size_t)(n + 1) * 4);
+    up = take((size_t)(n + 1) * 4);
+    pinv = take((size_t)n * 4);
+    prow = take((size_t)n * 4);
+    xi = take((size_t)n * 4);
+    pstack = take((size_t)n * 4);
+    mark = take((size_t)n * 4);
+    rch = take((size_t)capR * 4);
+    rp = take((size_t)(n + 1) * 4);
+    dk = take((size_t)n * 4);
+    total = off;
+  }
+};
+
 }  // namespace
 
 extern "C" {
@@ -672,6 +753,73 @@ kh_status kh_sys_solve(kh_sys s, const double* b, double* x) {
   return sys_solve<double>(s, b, x);
 }
 
+kh_status kh_sys_solve_static(kh_sys s, const double* b, double* x) {
+  if (!s || !b || !x) return fail(KH_BAD_ARGUMENT, "null argument");
+  if (s->val.empty()) return fail(KH_SINGULAR, "empty matrix");
+  sys_analyze(s);  // host only: the device buffers of kh_sys_solve stay as they are
+  const std::vector<int32_t>& pool = s->pats.host.pool;
+  ScratchLayout L;
+  L.plan(s->n, (int64_t)s->val.size(), pool.size(), lu_capacity(s->pat.fill_estimate, s->n));
+  if (L.total > kScratchBytes)
+    return fail(KH_BADSIZE, "kh_sys_solve_static: the system needs " + std::to_string(L.total) +
+                                " bytes of workspace, above the static " + std::to_string(kScratchBytes));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  ScratchDev* d = nullptr;
+  kh_status st = scratch_dev(&d);
+  if (st != KH_OK) return st;
+  char* base = d->ptr;
+  if (d->owner != s || d->generation != s->generation) {
+    // Another system (or pattern) used the array last: its pattern, the
+    // pattern id and an empty reach cache.
+    const int32_t zero = 0, none = -1;
+    KH_HIP(hipMemcpy(base + L.desc, s->pats.host.desc.data(), sizeof(PatDesc), hipMemcpyHostToDevice));
+    KH_HIP(hipMemcpy(base + L.pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
+    KH_HIP(hipMemcpy(base + L.pid, &zero, 4, hipMemcpyHostToDevice));
+    KH_HIP(hipMemcpy(base + L.cpid, &none, 4, hipMemcpyHostToDevice));
+    d->owner = s;
+    d->generation = s->generation;
+  }
+  KH_HIP(hipMemcpy(base + L.val, s->val.data(), s->val.size() * 8, hipMemcpyHostToDevice));
+  KH_HIP(hipMemcpy(base + L.rhs, b, (size_t)s->n * 8, hipMemcpyHostToDevice));
+  LuArgs a = {};
+  a.n = s->n;
+  a.nslots = (int32_t)s->val.size();
+  a.B = 1;
+  a.capL = a.capU = L.capL;
+  a.capR = L.capR;
+  a.pats = (const PatDesc*)(base + L.desc);
+  a.pool = (const int32_t*)(base + L.pool);
+  a.pid = (const int32_t*)(base + L.pid);
+  a.val = (const double*)(base + L.val);
+  a.rhs = (const double*)(base + L.rhs);
+  a.x = (double*)(base + L.x);
+  a.xw = (double*)(base + L.xw);
+  a.w = (double*)(base + L.w);
+  a.udiag = (double*)(base + L.udiag);
+  a.lx = (double*)(base + L.lx);
+  a.ux = (double*)(base + L.ux);
+  a.li = (int32_t*)(base + L.li);
+  a.ui = (int32_t*)(base + L.ui);
+  a.lp = (int32_t*)(base + L.lp);
+  a.up = (int32_t*)(base + L.up);
+  a.pinv = (int32_t*)(base + L.pinv);
+  a.prow = (int32_t*)(base + L.prow);
+  a.xi = (int32_t*)(base + L.xi);
+  a.pstack = (int32_t*)(base + L.pstack);
+  a.mark = (int32_t*)(base + L.mark);
+  a.err_step = (int32_t*)(base + L.err);
+  a.rch = (int32_t*)(base + L.rch);
+  a.rp = (int32_t*)(base + L.rp);
+  a.dk = (int32_t*)(base + L.dk);
+  a.cpid = (int32_t*)(base + L.cpid);
+  KH_HIP(launch_sys(a, (int32_t*)(base + L.status), 0));
+  int32_t hs = 0;
+  KH_HIP(hipMemcpy(&hs, base + L.status, 4, hipMemcpyDeviceToHost));
+  if (hs == LU_OK || hs == LU_NONFINITE) KH_HIP(hipMemcpy(x, base + L.x, (size_t)s->n * 8, hipMemcpyDeviceToHost));
+  if (hs == LU_CAPACITY) d->owner = nullptr;  // the reach cache may be partial
+  return lu_status_error(hs);
+}
+
 kh_status kh_sys_absproduct(kh_sys s, const double* x, double* y) {
   if (!s || !x || !y) return fail(KH_BAD_ARGUMENT, "null argument");
   sys_absproduct<double>(s, x, y);
@@ -688,7 +836,13 @@ kh_status kh_sys_pivots(kh_sys s, int32_t* q, int32_t* prow) {
   return sys_pivots<double>(s, q, prow);
 }
 
-void kh_sys_destroy(kh_sys s) { delete s; }
+void kh_sys_destroy(kh_sys s) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  for (auto& d : g_scratch)
+    if (d.owner == s) d.owner = nullptr;
+  delete s;
+}
 
 // ---- complex seam: CSparse behind CMatrix (solver/matrix.go:39-46) -----------------------
 kh_status kh_csys_create(int32_t n, kh_csys* out) {
