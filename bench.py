@@ -264,14 +264,20 @@ def main():
     work = make_work(args.workload, args.batch)
     log(f"{work.name}: {work.B} circuits in {len(work.batches)} groups")
 
-    # Plain FP64 GPU form: in place, one circuit per wavefront, FP64 LU.
+    # Plain FP64 GPU form: in place, one circuit per wavefront, FP64 LU; the
+    # same untimed warmup runs and timed steps as the chosen form.
     plain_s = plain_stats = None
     if not args.no_plain:
-        t0 = time.perf_counter()
-        _, plain_stats = work.run(api.Options(staging=0, circuits_per_wave=1, waves_per_simd=1, precision=0))
+        plain = api.Options(staging=0, circuits_per_wave=1, waves_per_simd=1, precision=0)
+        for _ in range(args.warmup):
+            work.run(plain)
         torch.cuda.synchronize()
-        plain_s = time.perf_counter() - t0
-        log(f"plain FP64 form: {plain_s:.2f} s")
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            _, plain_stats = work.run(plain)
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t0) / args.steps
+        log(f"plain FP64 form: {plain_s:.2f} s per batch ({args.warmup} warmup, {args.steps} timed)")
 
     # Chosen form: the autotuner's pick on a short probe of the whole batch.
     tuned = None
