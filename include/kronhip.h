@@ -127,8 +127,10 @@ typedef struct kh_options {
   int32_t circuits_per_wave; /* batched analyses: circuits sharing one wavefront
                                (1..64; 0 -> 1) */
   int32_t staging;          /* batched analyses: 0 -> circuits are worked on in place in
-                               device memory; 1 -> each circuit's working set is held in
-                               LDS during a launch when it fits (64 KiB per workgroup) */
+                               device memory; 1 -> during a launch each circuit holds in LDS
+                               its whole working set when it fits its share of a compute
+                               unit's LDS (the share that keeps the launch resident), else
+                               its most used arrays up to that share */
   int32_t waves_per_simd;   /* batched analyses: register budget of the persistent kernel,
                                as resident wavefronts per SIMD: 0 or 1 -> unconstrained,
                                2 or 4 -> fewer registers (spilled), more circuits in flight */
@@ -180,6 +182,9 @@ typedef struct kh_stats {
   /* parts of cycles_control, summed over circuits: stamping (Sim.solve's Clear + Load), Newton's
      convergence test, LTE step proposals, accepting a time point, its energy-ledger record */
   double  cycles_stamp, cycles_converge, cycles_lte, cycles_accept, cycles_ledger;
+  /* staging 1: LDS bytes per circuit of the staged arrays (largest over groups, last launch), and the
+     groups of the last launch that stage only part of their arrays (the rest stays in device memory) */
+  int64_t staged_bytes, staged_partial;
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/

@@ -65,7 +65,8 @@ class Stats(C.Structure):
                 ("control_launches", C.c_int64), ("cycles_lu", C.c_double), ("cycles_control", C.c_double),
                 ("refine_passes", C.c_int64), ("fp64_redos", C.c_int64),
                 ("cycles_stamp", C.c_double), ("cycles_converge", C.c_double), ("cycles_lte", C.c_double),
-                ("cycles_accept", C.c_double), ("cycles_ledger", C.c_double)]
+                ("cycles_accept", C.c_double), ("cycles_ledger", C.c_double),
+                ("staged_bytes", C.c_int64), ("staged_partial", C.c_int64)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -24,100 +24,172 @@ struct Group {
   int32_t gid;
   int32_t cpb;     // circuits per workgroup
   int32_t npar, nstate;
-  int32_t lds_bytes;  // > 0: each circuit's working set is staged in this much LDS
-  int32_t pad;
+  int32_t lds_bytes;  // > 0: each circuit's staged arrays take this much LDS
+  int32_t stage_all;  // every array is staged (the LU then addresses LDS directly)
+  uint64_t stage_mask;  // bit i: the i-th array of for_each_array is staged
 };
 
 // Visits the pointer field of every per-circuit array of a group (null until
 // the array is allocated; the LU workspace is allocated with the first pattern):
-// f(field, bytes of one circuit's slice, copy in, copy out). Circuit c's slice
+// f(field, bytes of one circuit's slice, copy in, copy out, rank). Circuit c's slice
 // starts at field + c * bytes. "Copy in" marks what a circuit reads before
 // writing within one launch (its state between launches), "copy out" what it
 // changes that outlives the launch; the rest is scratch of one factorization
 // or one convergence test. Pattern ids and per-circuit transient
 // specifications are read-only; parameters change only where a DC sweep
 // overrides a source's value. The DC sweep's per-point outputs are not here:
-// they are written once per point, straight to device memory.
+// they are written once per point, straight to device memory. The rank
+// orders the arrays for partial staging by how often the state machine and
+// the factorization touch them: 0 control words, 1 the unknowns and
+// right-hand side, 2 the factorization's vectors and column pointers, 3 the
+// stamped values, 4 the factors, 5 the reach lists, 6 device state, 7
+// parameters, 8 vectors one analysis step uses once, 9 the depth-first
+// search's stacks (used only when a reach list is rebuilt).
 template <typename G, typename F>
 KH_HD inline void for_each_array(G& g, F&& f) {
   const int64_t n = g.ctl.ckt.n, ns = g.ctl.ckt.nslots;
   const int64_t capL = g.lu.capL, capU = g.lu.capU, capR = g.lu.capR;
   auto& a = g.ctl;
   auto& l = g.lu;
-  f(a.par, g.npar * 8, true, true);
-  f(a.st, g.nstate * 8, true, true);
-  f(a.val, ns * 8, true, true);
-  f(a.rhs, n * 8, true, true);
-  f(a.x, n * 8, true, true);
-  f(a.xold, n * 8, true, true);
-  f(a.xsave, n * 8, true, true);
-  f(a.ptprev, n * 8, true, true);
-  f(a.xacc, n * 8, true, true);
-  f(a.absbuf, n * 8, false, false);
-  f(a.cd, (int64_t)NCD * 8, true, true);
-  if (a.lane_spec) f(a.lane_spec, 4 * 8, true, false);
-  if (a.nodeset) f(a.nodeset, (int64_t)(a.ckt.numNodes - 1) * 8, true, false);
+  f(a.par, g.npar * 8, true, true, 7);
+  f(a.st, g.nstate * 8, true, true, 6);
+  f(a.val, ns * 8, true, true, 3);
+  f(a.rhs, n * 8, true, true, 1);
+  f(a.x, n * 8, true, true, 1);
+  f(a.xold, n * 8, true, true, 1);
+  f(a.xsave, n * 8, true, true, 8);
+  f(a.ptprev, n * 8, true, true, 8);
+  f(a.xacc, n * 8, true, true, 8);
+  f(a.absbuf, n * 8, false, false, 8);
+  f(a.cd, (int64_t)NCD * 8, true, true, 0);
+  if (a.lane_spec) f(a.lane_spec, 4 * 8, true, false, 8);
+  if (a.nodeset) f(a.nodeset, (int64_t)(a.ckt.numNodes - 1) * 8, true, false, 8);
   if (a.ckt.dense) {  // lu_dense's workspace: matrix, column maxima, y, pivot rows
-    f(l.lx, n * n * 8, false, false);
-    f(l.udiag, n * 8, false, false);
-    f(l.w, n * 8, false, false);
-    f(a.ci, (int64_t)NCI * 4, true, true);
-    f(a.lu_status, 4, true, true);
-    f(l.prow, n * 4, false, false);
-    f(l.err_step, 4, true, true);
-    f(a.present, ns, true, true);
+    f(l.lx, n * n * 8, false, false, 4);
+    f(l.udiag, n * 8, false, false, 2);
+    f(l.w, n * 8, false, false, 2);
+    f(a.ci, (int64_t)NCI * 4, true, true, 0);
+    f(a.lu_status, 4, true, true, 0);
+    f(l.prow, n * 4, false, false, 2);
+    f(l.err_step, 4, true, true, 0);
+    f(a.present, ns, true, true, 3);
     return;
   }
-  f(l.xw, n * 8, false, false);
-  f(l.w, n * 8, false, false);
-  f(l.udiag, n * 8, false, false);
-  f(l.lx, capL * 8, false, false);
-  f(l.ux, capU * 8, false, false);
-  f(a.ci, (int64_t)NCI * 4, true, true);
-  f(a.lu_status, 4, true, true);
-  f(l.pid, 4, true, false);
-  f(l.li, capL * 4, false, false);
-  f(l.ui, capU * 4, false, false);
-  f(l.lp, (n + 1) * 4, false, false);
-  f(l.up, (n + 1) * 4, false, false);
-  f(l.pinv, n * 4, false, false);
-  f(l.prow, n * 4, true, true);
-  f(l.xi, n * 4, false, false);
-  f(l.pstack, n * 4, false, false);
-  f(l.mark, n * 4, false, false);
-  f(l.err_step, 4, true, true);
-  f(l.rch, capR * 4, true, true);
-  f(l.rp, (n + 1) * 4, true, true);
-  f(l.dk, n * 4, true, true);
-  f(l.cpid, 4, true, true);
-  f(a.present, ns, true, true);
+  f(l.xw, n * 8, false, false, 2);
+  f(l.w, n * 8, false, false, 2);
+  f(l.udiag, n * 8, false, false, 2);
+  f(l.lx, capL * 8, false, false, 4);
+  f(l.ux, capU * 8, false, false, 4);
+  f(a.ci, (int64_t)NCI * 4, true, true, 0);
+  f(a.lu_status, 4, true, true, 0);
+  f(l.pid, 4, true, false, 0);
+  f(l.li, capL * 4, false, false, 4);
+  f(l.ui, capU * 4, false, false, 4);
+  f(l.lp, (n + 1) * 4, false, false, 2);
+  f(l.up, (n + 1) * 4, false, false, 2);
+  f(l.pinv, n * 4, false, false, 2);
+  f(l.prow, n * 4, true, true, 2);
+  f(l.xi, n * 4, false, false, 9);
+  f(l.pstack, n * 4, false, false, 9);
+  f(l.mark, n * 4, false, false, 9);
+  f(l.err_step, 4, true, true, 0);
+  f(l.rch, capR * 4, true, true, 5);
+  f(l.rp, (n + 1) * 4, true, true, 2);
+  f(l.dk, n * 4, true, true, 2);
+  f(l.cpid, 4, true, true, 0);
+  f(a.present, ns, true, true, 3);
 }
 
 KH_HD inline int64_t round8(int64_t b) { return (b + 7) & ~(int64_t)7; }
 
-// Bytes of LDS one circuit of the group needs (each array 8-byte aligned).
+// Bytes of LDS one circuit of the group needs to stage every array (each
+// array 8-byte aligned).
 KH_HD inline int64_t lds_need(const Group& g) {
   int64_t total = 0;
-  for_each_array(g, [&](const auto&, int64_t bytes, bool, bool) { total += round8(bytes); });
+  for_each_array(g, [&](const auto&, int64_t bytes, bool, bool, int) { total += round8(bytes); });
   return total;
+}
+
+// Chooses the arrays a circuit stages within `budget` bytes of LDS: all of
+// them when they fit, otherwise the allocated arrays rank by rank (within a
+// rank in for_each_array order), each taken when it still fits. Sets
+// stage_mask, stage_all and lds_bytes (0: nothing worth staging; the count of
+// 8-byte words is made odd so that the circuits of one wavefront start in
+// different LDS banks).
+inline void plan_staging(Group& g, int64_t budget) {
+  g.stage_mask = 0;
+  g.stage_all = 0;
+  g.lds_bytes = 0;
+  int nar = 0;
+  for_each_array(g, [&](const auto&, int64_t, bool, bool, int) { nar++; });
+  if (nar > 64) return;
+  budget -= 8;  // room for the odd-word pad
+  int64_t used = 0;
+  const int64_t full = lds_need(g);
+  if (full <= budget) {
+    g.stage_mask = nar == 64 ? ~0ull : ((1ull << nar) - 1);
+    g.stage_all = 1;
+    used = full;
+  } else {
+    for (int rank = 0; rank <= 9; rank++) {
+      int i = 0;
+      for_each_array(g, [&](const auto& p, int64_t bytes, bool, bool, int r) {
+        if (r == rank && p && used + round8(bytes) <= budget) {
+          g.stage_mask |= 1ull << i;
+          used += round8(bytes);
+        }
+        i++;
+      });
+    }
+  }
+  if (used < 64) {  // nothing but a few control words: not worth the copies
+    g.stage_mask = 0;
+    g.stage_all = 0;
+    return;
+  }
+  if (((used / 8) & 1) == 0) used += 8;
+  g.lds_bytes = (int32_t)used;
+}
+
+// Visits the arrays of `mask`: f(field, bytes of one circuit's slice, copy
+// in, copy out, offset in the circuit's LDS slice).
+template <typename G, typename F>
+KH_HD inline void for_each_staged(G& g, uint64_t mask, F&& f) {
+  int64_t off = 0;
+  int i = 0;
+  for_each_array(g, [&](auto& p, int64_t bytes, bool in, bool out, int) {
+    if ((mask >> i) & 1) {
+      f(p, bytes, in, out, off);
+      off += round8(bytes);
+    }
+    i++;
+  });
 }
 
 KH_HD inline int64_t sweep_points(const CtlParams& p) {
   return p.analysis == AN_DC ? (int64_t)p.swCount[0] * (p.swN > 1 ? p.swCount[1] : 1) : 0;
 }
 
-// The group as circuit cc sees it once its working set is copied to `base`
-// (arrays in for_each_array order, each 8-byte aligned): run_circuit(view, 0,
-// ...) then works on the copies.
+// The group as circuit cc sees it once its staged arrays are copied to `base`
+// (the staged arrays in for_each_array order, each 8-byte aligned); the
+// arrays left in device memory are circuit cc's slices there: run_circuit(view,
+// 0, ...) then works on the copies and the slices.
 KH_HD inline Group lds_view(const Group& g, char* base, int64_t cc) {
   Group v = g;
   const int64_t pts = sweep_points(g.ctl.p);
   if (v.ctl.sw_x) v.ctl.sw_x += cc * pts * g.ctl.ckt.n;
   if (v.ctl.sw_it) v.ctl.sw_it += cc * pts;
   int64_t off = 0;
-  for_each_array(v, [&](auto& p, int64_t bytes, bool, bool) {
-    p = (std::remove_reference_t<decltype(p)>)(base + off);
-    off += round8(bytes);
+  int i = 0;
+  for_each_array(v, [&](auto& p, int64_t bytes, bool, bool, int) {
+    using Ptr = std::remove_reference_t<decltype(p)>;
+    if ((g.stage_mask >> i) & 1) {
+      p = (Ptr)(base + off);
+      off += round8(bytes);
+    } else if (p) {
+      p = (Ptr)((const char*)p + cc * bytes);
+    }
+    i++;
   });
   v.lu.val = v.ctl.val;
   v.lu.rhs = v.ctl.rhs;

@@ -5,11 +5,12 @@
 // workgroup, one per lane. Each circuit loops: factor + solve its stamped
 // system, then advance its analysis to the next solve (run_circuit).
 //
-// Groups with lds_bytes > 0 run staged: the wavefront copies the working set
-// of each of its circuits (every per-circuit array, kh_group.h) into LDS,
-// the circuits run on the LDS copies, and the wavefront copies back what
-// outlives the launch. Every dependent access of the serial factorization and
-// state machine is then an LDS access instead of a trip to L2/HBM.
+// Groups with lds_bytes > 0 run staged: the wavefront copies the staged
+// arrays of each of its circuits (all per-circuit arrays when they fit, else
+// the most used ones, plan_staging in kh_group.h) into LDS, the circuits run
+// on the LDS copies and their slices of the rest, and the wavefront copies
+// back what outlives the launch. The dependent accesses to staged arrays are
+// then LDS accesses instead of trips to L2/HBM.
 //
 // The kernel is compiled once per register budget W (resident wavefronts per
 // SIMD the allocation must allow): 1 leaves the allocator free; 2 and 4 trade

@@ -34,24 +34,25 @@ __global__ void __launch_bounds__(kBlock, KH_PERSIST_WAVES) k_persist(const Grou
     const int nc = (int)min((int64_t)g.cpb, g.ctl.B - c0);  // uniform over the workgroup
     for (int t = 0; t < nc; t++) {
       char* base = lds + (int64_t)t * g.lds_bytes;
-      int64_t off = 0;
-      for_each_array(g, [&](const auto& p, int64_t bytes, bool in, bool) {
+      for_each_staged(g, g.stage_mask, [&](const auto& p, int64_t bytes, bool in, bool, int64_t off) {
         if (in && p) copy_wave(base + off, (const char*)p + (c0 + t) * bytes, bytes);
-        off += round8(bytes);
       });
     }
     __syncthreads();
     if ((int)threadIdx.x < nc) {
       Group v = lds_view(g, lds + (int64_t)threadIdx.x * g.lds_bytes, c0 + threadIdx.x);
-      run_circuit<true>(v, 0, max_solves, &running, &waiting);
+      // every array in LDS: the LU addresses it as LDS; otherwise generic
+      // pointers reach the staged arrays and the rest alike
+      if (g.stage_all)
+        run_circuit<true>(v, 0, max_solves, &running, &waiting);
+      else
+        run_circuit<false>(v, 0, max_solves, &running, &waiting);
     }
     __syncthreads();
     for (int t = 0; t < nc; t++) {
       char* base = lds + (int64_t)t * g.lds_bytes;
-      int64_t off = 0;
-      for_each_array(g, [&](const auto& p, int64_t bytes, bool, bool out) {
+      for_each_staged(g, g.stage_mask, [&](const auto& p, int64_t bytes, bool, bool out, int64_t off) {
         if (out && p) copy_wave((char*)p + (c0 + t) * bytes, base + off, bytes);
-        off += round8(bytes);
       });
     }
   } else {

@@ -1335,12 +1335,15 @@ struct Runner {
   int32_t max_solves = 4;
   static constexpr float kLaunchTargetMs = 25.0f;
   int32_t cpb = 1;
-  // LDS staging (k_persist): a group is staged when cpb circuits' working
-  // sets fit in lds_limit bytes (the per-workgroup LDS available without
-  // opting in to more).
+  // LDS staging (k_persist): each circuit stages what fits its share of the
+  // LDS (plan_staging): the share keeps every workgroup of the launch resident
+  // at once (a compute unit's LDS split over the workgroups it must hold,
+  // at most lds_limit per workgroup).
   bool lds_staging = false;
   int32_t waves = 1;  // k_persist variant: resident wavefronts per SIMD (1, 2 or 4)
   int64_t lds_limit = 64 * 1024;  // raised to the device's opt-in maximum (160 KiB on gfx950)
+  int64_t lds_per_cu = 0;         // LDS of one compute unit (0: unknown, lds_limit assumed)
+  int32_t ncu = 0;                // compute units (0: unknown)
   size_t lds_launch = 0;
   // k_wave (kh_options.circuit_wave): one wavefront per circuit, circuits
   // taken from a work list, largest first.
@@ -1424,20 +1427,27 @@ struct Runner {
     for (;;) {
       if (dirty) {  // group descriptors: LU workspace pointers change when capacity grows
         if (!wave) lds_launch = 0;
+        // workgroups each compute unit holds when the whole launch is resident
+        int64_t wg_cu = ncu > 0 ? ((int64_t)bg.size() + ncu - 1) / ncu : 1;
+        wg_cu = std::min<int64_t>(std::max<int64_t>(wg_cu, 1), 4 * (int64_t)waves);
+        const int64_t wg_lds = std::min<int64_t>(lds_limit, (lds_per_cu > 0 ? lds_per_cu : lds_limit) / wg_cu);
         for (size_t g = 0; g < ng; g++) {
           hg[g].ctl = groups[g]->ctl_args(p);
           hg[g].lu = groups[g]->lu_args();
           hg[g].lds_bytes = 0;
+          hg[g].stage_all = 0;
+          hg[g].stage_mask = 0;
           if (lds_staging && !wave) {
-            // An odd number of 8-byte words per circuit puts the circuits of
-            // one wavefront in different LDS banks.
-            int64_t b = round8(lds_need(hg[g]));
-            if (((b / 8) & 1) == 0) b += 8;
-            if (b * hg[g].cpb <= lds_limit) {
-              hg[g].lds_bytes = (int32_t)b;
-              lds_launch = std::max(lds_launch, (size_t)(b * hg[g].cpb));
-            }
+            plan_staging(hg[g], wg_lds / hg[g].cpb);
+            if (hg[g].lds_bytes > 0)
+              lds_launch = std::max(lds_launch, (size_t)hg[g].lds_bytes * (size_t)hg[g].cpb);
           }
+        }
+        stats.staged_bytes = 0;
+        stats.staged_partial = 0;
+        for (size_t g = 0; g < ng; g++) {
+          stats.staged_bytes = std::max<int64_t>(stats.staged_bytes, hg[g].lds_bytes);
+          if (hg[g].lds_bytes > 0 && !hg[g].stage_all) stats.staged_partial++;
         }
         KH_HIP(hipMemcpy(dgroups.p, hg.data(), ng * sizeof(Group), hipMemcpyHostToDevice));
         dirty = false;
@@ -1575,10 +1585,14 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
   R.cpb = cpb;
   R.lds_staging = opt && opt->staging != 0;
   if (R.lds_staging) {
-    int dev = 0, optin = 0;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
-      R.lds_limit = std::max<int64_t>(R.lds_limit, optin);
+    int dev = 0, optin = 0, percu = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess) {
+      if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess && optin > 0)
+        R.lds_limit = std::max<int64_t>(R.lds_limit, optin);
+      if (hipDeviceGetAttribute(&percu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
+        R.lds_per_cu = percu;
+      if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) R.ncu = ncu;
+    }
   }
   R.waves = (opt && (opt->waves_per_simd == 2 || opt->waves_per_simd == 4)) ? opt->waves_per_simd : 1;
   R.wave = opt && opt->circuit_wave != 0;

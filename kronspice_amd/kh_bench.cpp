@@ -216,11 +216,11 @@ int main(int argc, char** argv) {
          "\"solves_per_step\": %.1f, \"launches_per_step\": %.1f, \"kernel_seconds_per_step\": %.6f, "
          "\"algorithmic_GBps\": %.4f, \"cycles_lu\": %.6g, \"cycles_control\": %.6g, \"cycles_stamp\": %.6g, "
          "\"cycles_converge\": %.6g, \"cycles_lte\": %.6g, \"cycles_accept\": %.6g, \"cycles_ledger\": %.6g, "
-         "\"status_histogram\": %s}\n",
+         "\"staged_bytes\": %lld, \"staged_partial\": %lld, \"status_histogram\": %s}\n",
          (long long)circuits, ngroups, staging, cpw, waves, precision, circuit_wave, backend, (double)passes / steps, (double)redos / steps, steps, dt / steps, circuits * steps / dt, (double)solves / steps,
          (double)launches / steps, lu_s / steps, lu_s > 0 ? bytes / lu_s / 1e9 : 0.0, cyc_lu / steps, cyc_ctl / steps,
          cyc_part[0] / steps, cyc_part[1] / steps, cyc_part[2] / steps, cyc_part[3] / steps, cyc_part[4] / steps,
-         h.c_str());
+         (long long)st.staged_bytes, (long long)st.staged_partial, h.c_str());
   for (void* p : owned) (void)hipFree(p);
   for (auto t : topo) kh_topology_destroy(t);
   return 0;
