@@ -25,7 +25,7 @@ struct Group {
   int32_t cpb;     // circuits per workgroup
   int32_t npar, nstate;
   int32_t lds_bytes;  // > 0: each circuit's staged arrays take this much LDS
-  int32_t stage_all;  // every array is staged (the LU then addresses LDS directly)
+  int32_t stage_all;  // every array the factorization touches is staged: it then addresses LDS directly
   uint64_t stage_mask;  // bit i: the i-th array of for_each_array is staged
 };
 
@@ -42,9 +42,9 @@ struct Group {
 // orders the arrays for partial staging by how often the state machine and
 // the factorization touch them: 0 control words, 1 the unknowns and
 // right-hand side, 2 the factorization's vectors and column pointers, 3 the
-// stamped values, 4 the factors, 5 the reach lists, 6 device state, 7
-// parameters, 8 vectors one analysis step uses once, 9 the depth-first
-// search's stacks (used only when a reach list is rebuilt).
+// stamped values, 4 the factors, 5 the reach lists and the depth-first
+// search's stacks (the rest of the factorization's working set), 6 device
+// state, 7 parameters, 8 vectors one analysis step uses once.
 template <typename G, typename F>
 KH_HD inline void for_each_array(G& g, F&& f) {
   const int64_t n = g.ctl.ckt.n, ns = g.ctl.ckt.nslots;
@@ -89,9 +89,9 @@ KH_HD inline void for_each_array(G& g, F&& f) {
   f(l.up, (n + 1) * 4, false, false, 2);
   f(l.pinv, n * 4, false, false, 2);
   f(l.prow, n * 4, true, true, 2);
-  f(l.xi, n * 4, false, false, 9);
-  f(l.pstack, n * 4, false, false, 9);
-  f(l.mark, n * 4, false, false, 9);
+  f(l.xi, n * 4, false, false, 5);
+  f(l.pstack, n * 4, false, false, 5);
+  f(l.mark, n * 4, false, false, 5);
   f(l.err_step, 4, true, true, 0);
   f(l.rch, capR * 4, true, true, 5);
   f(l.rp, (n + 1) * 4, true, true, 2);
@@ -113,9 +113,10 @@ KH_HD inline int64_t lds_need(const Group& g) {
 // Chooses the arrays a circuit stages within `budget` bytes of LDS: all of
 // them when they fit, otherwise the allocated arrays rank by rank (within a
 // rank in for_each_array order), each taken when it still fits. Sets
-// stage_mask, stage_all and lds_bytes (0: nothing worth staging; the count of
-// 8-byte words is made odd so that the circuits of one wavefront start in
-// different LDS banks).
+// stage_mask, lds_bytes (0: nothing worth staging; the count of 8-byte words
+// is made odd so that the circuits of one wavefront start in different LDS
+// banks) and stage_all: 1 when every array of the sparse factorization's
+// arguments (LuArgs but the pattern pool) is staged.
 inline void plan_staging(Group& g, int64_t budget) {
   g.stage_mask = 0;
   g.stage_all = 0;
@@ -149,6 +150,23 @@ inline void plan_staging(Group& g, int64_t budget) {
   }
   if (((used / 8) & 1) == 0) used += 8;
   g.lds_bytes = (int32_t)used;
+  if (!g.stage_all && !g.ctl.ckt.dense) {
+    // the factorization's arrays are for_each_array fields, or alias one (val, rhs, x)
+    const void* lu[] = {g.lu.pid, g.lu.val, g.lu.rhs, g.lu.x, g.lu.xw, g.lu.w, g.lu.udiag, g.lu.lx, g.lu.ux,
+                        g.lu.li, g.lu.ui, g.lu.lp, g.lu.up, g.lu.pinv, g.lu.prow, g.lu.xi, g.lu.pstack,
+                        g.lu.mark, g.lu.err_step, g.lu.rch, g.lu.rp, g.lu.dk, g.lu.cpid};
+    bool all = true;
+    for (const void* q : lu) {
+      bool found = false;
+      int i = 0;
+      for_each_array(g, [&](const auto& p, int64_t, bool, bool, int) {
+        if (p && (const void*)p == q && ((g.stage_mask >> i) & 1)) found = true;
+        i++;
+      });
+      if (!q || !found) all = false;
+    }
+    g.stage_all = all ? 1 : 0;
+  }
 }
 
 // Visits the arrays of `mask`: f(field, bytes of one circuit's slice, copy

@@ -123,6 +123,16 @@ int check(const Group& g0, int64_t budget, int64_t B, int* cases) {
       if (spans[i].first < spans[i - 1].second) return fail("staged arrays overlap", (int)i, spans[i].first, 0);
     if (v.lu.val != v.ctl.val || v.lu.rhs != v.ctl.rhs || v.lu.x != v.ctl.x)
       return fail("LU aliases of val/rhs/x differ", 0, 0, 0);
+    if (g.stage_all && !g.ctl.ckt.dense) {
+      // the factorization then addresses every one of its arrays as LDS
+      const void* lu[] = {v.lu.pid, v.lu.val, v.lu.rhs, v.lu.x, v.lu.xw, v.lu.w, v.lu.udiag, v.lu.lx, v.lu.ux,
+                          v.lu.li, v.lu.ui, v.lu.lp, v.lu.up, v.lu.pinv, v.lu.prow, v.lu.xi, v.lu.pstack,
+                          v.lu.mark, v.lu.err_step, v.lu.rch, v.lu.rp, v.lu.dk, v.lu.cpid};
+      for (int j = 0; j < (int)(sizeof(lu) / sizeof(lu[0])); j++) {
+        const int64_t off = (int64_t)((uintptr_t)lu[j] - (uintptr_t)base);
+        if (off < 0 || off >= g.lds_bytes) return fail("LU array outside LDS under stage_all", j, off, g.lds_bytes);
+      }
+    }
     (*cases)++;
   }
   return 0;
