@@ -292,7 +292,24 @@ typedef struct kh_group {
                                  per circuit, node k at row k-1 (Options.NodeSet,
                                  analysis/op.go:39-43, applied as analysis/newton.go:221-243:
                                  the other unknowns start at 0, InitJct off) */
+  double* d_report;           /* optional [B][n_elems][KH_REPORT_FIELDS]: each element's entry of
+                                 output.Result at the circuit's final point (output/result.go:15-35,
+                                 built as analysis/op.go:361-404), fields below */
+  double* d_report_ledger;    /* optional [B][KH_LEDGER_FIELDS]: the circuit's output.Ledger totals
+                                 (em.BuildLedger, em/energy.go:89-119; output/result.go:58-109) */
 } kh_group;
+
+/* Fields of one element's report (d_report) and of one circuit's ledger
+ * (d_report_ledger), in this order. Channel: 0 electrical, 2 thermal, -1 the
+ * element has no modeled exterior port that can carry power. */
+enum {
+  KH_RP_VOLTAGE = 0, KH_RP_CURRENT, KH_RP_POWER, KH_RP_STORAGE, KH_RP_ENERGY, KH_RP_SUPPLIED,
+  KH_RP_EXTERIOR, KH_RP_CHANNEL, KH_REPORT_FIELDS
+};
+enum {
+  KH_LG_INFLOW = 0, KH_LG_OUTFLOW, KH_LG_DISSIPATION, KH_LG_STORAGE_RATE, KH_LG_MODEL_SUPPLIED,
+  KH_LG_HYPOTHETICAL, KH_LG_RESIDUAL, KH_LG_SCALE, KH_LG_ROUNDOFF, KH_LG_RELATIVE, KH_LEDGER_FIELDS
+};
 
 /* Runs analysis.OP (spec == NULL) or analysis.Tran (spec != NULL) on every
  * circuit of every group. */

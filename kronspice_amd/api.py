@@ -320,6 +320,7 @@ class Topology:
         self.elements = np.ascontiguousarray(elements)
         p0 = np.ascontiguousarray(params_c0, np.float64)
         self.nparams = len(p0)
+        self.n_elems = len(self.elements)
         self.h = C.c_void_p()
         _check(lib().kh_topology_create(num_nodes, _p(self.elements), len(self.elements), self.nparams, _p(p0),
                                         C.byref(self.h)))
@@ -341,6 +342,8 @@ class Result:
     ledger: object = None
     stats: dict = field(default_factory=dict)
     points: object = None  # DC sweep: [B] points solved
+    report: object = None         # [B, n_elems, len(REPORT_FIELDS)] device tensor (op(report=True))
+    report_ledger: object = None  # [B, len(LEDGER_FIELDS)] device tensor
 
 
 def to_device(a: np.ndarray, dtype=None):
@@ -349,10 +352,13 @@ def to_device(a: np.ndarray, dtype=None):
     return t.to(device="cuda", dtype=dtype) if dtype is not None else t.to("cuda")
 
 
-def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = False, nodeset=None) -> Result:
+def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = False, nodeset=None,
+       report: bool = False) -> Result:
     """analysis.OP for every circuit; params [P, B] (device tensor or numpy).
     nodeset [numNodes-1, B] (optional): Newton's starting node voltages per
-    circuit (Options.NodeSet, analysis/op.go:39-43)."""
+    circuit (Options.NodeSet, analysis/op.go:39-43). report: also the
+    output.Result entries, Result.report [B, n_elems, len(REPORT_FIELDS)] and
+    Result.report_ledger [B, len(LEDGER_FIELDS)]."""
     torch = _torch()
     if isinstance(params, np.ndarray):
         params = to_device(params)
@@ -366,15 +372,24 @@ def op(topo: Topology, params, opts: Optional[Options] = None, polish: bool = Fa
     o.polish = 1 if polish else o.polish
     s = Stats()
     torch.cuda.synchronize()
-    if nodeset is None:
+    rep = led = None
+    if nodeset is None and not report:
         _check(lib().kh_batch_op(topo.h, B, _p(params), C.byref(o), _p(x), _p(st), _p(it), C.byref(s)))
     else:
-        if tuple(nodeset.shape) != (topo.num_nodes - 1, B):
-            raise ValueError(f"nodeset shape {tuple(nodeset.shape)}, want {(topo.num_nodes - 1, B)}")
         g = Group(topology=topo.h.value, B=B, d_params=params.data_ptr(), d_x=x.data_ptr(), d_status=st.data_ptr(),
-                  d_iters=it.data_ptr(), d_nodeset=nodeset.data_ptr())
+                  d_iters=it.data_ptr())
+        if nodeset is not None:
+            if tuple(nodeset.shape) != (topo.num_nodes - 1, B):
+                raise ValueError(f"nodeset shape {tuple(nodeset.shape)}, want {(topo.num_nodes - 1, B)}")
+            g.d_nodeset = nodeset.data_ptr()
+        if report:
+            rep = torch.zeros((B, topo.n_elems, len(REPORT_FIELDS)), dtype=torch.float64, device="cuda")
+            led = torch.zeros((B, len(LEDGER_FIELDS)), dtype=torch.float64, device="cuda")
+            g.d_report, g.d_report_ledger = rep.data_ptr(), led.data_ptr()
         _check(lib().kh_batch_run(1, C.byref(g), C.byref(o), None, C.byref(s)))
-    return Result(x, st, iters=it, stats=s.as_dict())
+    r = Result(x, st, iters=it, stats=s.as_dict())
+    r.report, r.report_ledger = rep, led
+    return r
 
 
 class DcSpec(C.Structure):
@@ -452,7 +467,12 @@ class Group(C.Structure):
     _fields_ = [("topology", C.c_void_p), ("B", C.c_int32), ("reserved", C.c_int32), ("d_params", C.c_void_p),
                 ("d_lane_spec", C.c_void_p), ("d_x", C.c_void_p), ("d_status", C.c_void_p), ("d_iters", C.c_void_p),
                 ("d_accepted", C.c_void_p), ("d_rejected", C.c_void_p), ("d_ledger", C.c_void_p),
-                ("d_nodeset", C.c_void_p)]
+                ("d_nodeset", C.c_void_p), ("d_report", C.c_void_p), ("d_report_ledger", C.c_void_p)]
+
+
+REPORT_FIELDS = ("voltage", "current", "power", "storage", "energy", "supplied", "exterior", "channel")
+LEDGER_FIELDS = ("inflow", "outflow", "dissipation", "storage_rate", "model_supplied", "hypothetical", "residual",
+                 "scale", "roundoff", "relative_residual")
 
 
 def run_groups(topos, params_list, spec: Optional[TranSpec] = None, opts: Optional[Options] = None,

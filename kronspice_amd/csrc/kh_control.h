@@ -224,27 +224,7 @@ struct CtlT {
     }
     L = lane;
   }
-  KH_HD OpPt operating(const Lane& L, const int32_t* e, const double* x) const {
-    switch (e[ET_KIND]) {
-      case K_R: {
-        OpPt o;
-        o.voltage = L.volt(e, x);
-        o.current = o.voltage * L.P(e, 1);
-        o.power = o.voltage * o.current;
-        o.storage = 0;
-        return o;
-      }
-      case K_C: return op_C(L, e, x);
-      case K_L: return op_L(L, e, x);
-      case K_V: return op_V(L, e, x);
-      case K_I: return op_I(L, e, x);
-      case K_D: if (KH_HAS(K_D)) return op_D(L, e, x); break;
-      case K_M: if (KH_HAS(K_M)) return op_M(L, e, x); break;
-      case K_Q: if (KH_HAS(K_Q)) return op_Q(L, e, x); break;
-    }
-    OpPt o = {0, 0, 0, 0};
-    return o;
-  }
+  KH_HD OpPt operating(const Lane& L, const int32_t* e, const double* x) const { return operating_point(L, e, x); }
   KH_HD void accept(double t, int mode) {  // Sim.accept, analysis/op.go:324-328
     const Lane L = this->L;  // register copy of the device view (see load_all)
     const int nel_ = A.ckt.nel;

@@ -1364,4 +1364,28 @@ KH_HD inline double trunc_Q(const Lane& L, const int32_t* e, const double* x, co
   return h;
 }
 
+// device.Operating of any element (the resistor's, device/passive.go:88-97,
+// without spin-connection terms).
+KH_HD inline OpPt operating_point(const Lane& L, const int32_t* e, const double* x) {
+  switch (e[ET_KIND]) {
+    case K_R: {
+      OpPt o;
+      o.voltage = L.volt(e, x);
+      o.current = o.voltage * L.P(e, 1);
+      o.power = o.voltage * o.current;
+      o.storage = 0;
+      return o;
+    }
+    case K_C: return op_C(L, e, x);
+    case K_L: return op_L(L, e, x);
+    case K_V: return op_V(L, e, x);
+    case K_I: return op_I(L, e, x);
+    case K_D: if (KH_HAS(K_D)) return op_D(L, e, x); break;
+    case K_M: if (KH_HAS(K_M)) return op_M(L, e, x); break;
+    case K_Q: if (KH_HAS(K_Q)) return op_Q(L, e, x); break;
+  }
+  OpPt o = {0, 0, 0, 0};
+  return o;
+}
+
 }  // namespace kh

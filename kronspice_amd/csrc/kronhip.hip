@@ -29,6 +29,7 @@
 #include "kh_ac.h"
 #include "kh_group.h"
 #include "kh_persist.h"
+#include "kh_report.h"
 #include "kh_symbolic.h"
 #include "kh_wave.h"
 
@@ -138,6 +139,45 @@ __global__ void __launch_bounds__(256) k_outputs(const int32_t* ci, const double
   if (ledger) ledger[c] = cd[c * NCD + CD_MAXREL];
   if (solves) solves[c] = w[CI_SOLVES];
   if (points) points[c] = w[CI_SW_DONE];
+}
+
+// ---- element reports (kh_report.h) ----------------------------------------------------
+static_assert(KH_REPORT_FIELDS == RP_FIELDS && KH_RP_CHANNEL == RP_CHANNEL && KH_RP_ENERGY == RP_ENERGY,
+              "report fields of kronhip.h and kh_report.h");
+static_assert(KH_LEDGER_FIELDS == LG_FIELDS && KH_LG_RELATIVE == LG_RELATIVE && KH_LG_ROUNDOFF == LG_ROUNDOFF,
+              "ledger fields of kronhip.h and kh_report.h");
+// One element of one circuit per thread: its output.Result entry at the
+// circuit's final solution and device state, rep[c][k][RP_FIELDS].
+__global__ void __launch_bounds__(256) k_report(const Group g, double* rep) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nel = g.ctl.ckt.nel;
+  if (t >= g.ctl.B * nel) return;
+  const int64_t c = t / nel;
+  const int k = (int)(t % nel);
+  CtlArgs a;
+  LuArgs l;
+  circuit_view(g, c, a, l);
+  Lane L = {};
+  L.n = a.ckt.n;
+  L.et = a.ckt.et;
+  L.par = a.par;
+  L.st = a.st;
+  report_element(L, a.ckt.et + (int64_t)k * ET_W, a.x, rep + t * RP_FIELDS);
+}
+
+// One circuit per thread: the ledger (em.BuildLedger) over its element
+// reports, with the round-off floor of its last stamped system,
+// led[c][LG_FIELDS].
+__global__ void __launch_bounds__(64) k_report_ledger(const Group g, const double* rep, double* led) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.ctl.B) return;
+  CtlArgs a;
+  LuArgs l;
+  circuit_view(g, c, a, l);
+  Ctl ctl(a);
+  const double ro = ctl.roundoff();
+  const int nel = a.ckt.nel;
+  fold_ledger(a.ckt.et, nel, rep + c * nel * RP_FIELDS, ro, led + c * LG_FIELDS);
 }
 
 // ---- AC systems (kh_ac.h) ----------------------------------------------------------
@@ -1256,6 +1296,37 @@ struct GroupRun {
     return KH_OK;
   }
 
+  // Element reports rep [B][nel][RP_FIELDS] and ledgers led [B][LG_FIELDS]
+  // (both optional; rep is computed into scratch when only led is asked for)
+  // at each circuit's final solution and state.
+  kh_status report(const CtlParams& p, double* d_rep, double* d_led) {
+    if (!d_rep && !d_led) return KH_OK;
+    const Topology& T = topo->t;
+    Group g = {};
+    g.ctl = ctl_args(p);
+    g.lu = lu_args();
+    g.npar = T.nparams;
+    g.nstate = std::max(T.nstate, 1);
+    g.cpb = 1;
+    DBuf tmp;
+    double* rep = d_rep;
+    if (!rep) {
+      KH_HIP(tmp.alloc((size_t)B * T.nel * RP_FIELDS * 8));
+      rep = tmp.as<double>();
+    }
+    const int64_t tot = B * (int64_t)T.nel;
+    if (tot > 0) {
+      hipLaunchKernelGGL(k_report, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, g, rep);
+      KH_HIP(hipGetLastError());
+    }
+    if (d_led) {
+      hipLaunchKernelGGL(k_report_ledger, dim3((unsigned)((B + 63) / 64)), dim3(64), 0, 0, g, rep, d_led);
+      KH_HIP(hipGetLastError());
+    }
+    KH_HIP(hipDeviceSynchronize());
+    return KH_OK;
+  }
+
   // Per-circuit results (caller's SoA layout) and batch counters.
   kh_status outputs(double* d_x, int32_t* d_status, int32_t* d_iters, int32_t* d_acc, int32_t* d_rej,
                     double* d_ledger, double* d_sw_x = nullptr, int32_t* d_sw_it = nullptr,
@@ -1447,7 +1518,10 @@ struct Runner {
         stats.staged_partial = 0;
         for (size_t g = 0; g < ng; g++) {
           stats.staged_bytes = std::max<int64_t>(stats.staged_bytes, hg[g].lds_bytes);
-          if (hg[g].lds_bytes > 0 && !hg[g].stage_all) stats.staged_partial++;
+          int nar = 0;
+          for_each_array(hg[g], [&](const auto&, int64_t, bool, bool, int) { nar++; });
+          const uint64_t full = nar >= 64 ? ~0ull : ((1ull << nar) - 1);
+          if (hg[g].lds_bytes > 0 && hg[g].stage_mask != full) stats.staged_partial++;
         }
         KH_HIP(hipMemcpy(dgroups.p, hg.data(), ng * sizeof(Group), hipMemcpyHostToDevice));
         dirty = false;
@@ -1642,6 +1716,8 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
     const kh_group& G = groups[g];
     s = runs[g]->outputs(G.d_x, G.d_status, G.d_iters, G.d_accepted, G.d_rejected, G.d_ledger,
                          sw ? sw->d_x : nullptr, sw ? sw->d_iters : nullptr, sw ? sw->d_points : nullptr);
+    if (s != KH_OK) return s;
+    s = runs[g]->report(p, G.d_report, G.d_report_ledger);
     if (s != KH_OK) return s;
   }
   KH_HIP(hipDeviceSynchronize());
