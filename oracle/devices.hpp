@@ -117,7 +117,11 @@ struct OperatingPoint {  // device/device.go:186-197
 struct Accumulator {
   double inflow = 0, outflow = 0, dissipation = 0, storageRate = 0, modelSupplied = 0, scale = 0;
   // channel: 0 electrical, 2 thermal (exchange/channel.go:28-31)
+  int lastCh = -1;     // the channel and power of the latest addPort (one element's probe)
+  double lastPw = 0;
   void addPort(int ch, double pw) {
+    lastCh = ch;
+    lastPw = pw;
     if (pw == 0) return;
     scale += std::fabs(pw);
     if (ch == 2 && pw < 0) dissipation += -pw;
@@ -212,6 +216,22 @@ struct Wave {
   }
 };
 
+// integrate, device/bjt.go:603-622: 16-point Gauss-Legendre (Abramowitz &
+// Stegun Table 25.4, the 16 digits the reference carries) on n equal panels.
+static const double kGLx[8] = {0.0950125098376374, 0.2816035507792589, 0.4580167776572274, 0.6178762444026438,
+                               0.7554044083550030, 0.8656312023878318, 0.9445750230732326, 0.9894009349916499};
+static const double kGLw[8] = {0.1894506104550685, 0.1826034150449236, 0.1691565193950025, 0.1495959888165767,
+                               0.1246289712555339, 0.0951585116824928, 0.0622535239386479, 0.0271524594117541};
+template <class F>
+static double glIntegrate(const F& f, double a, double b, int n) {
+  double h = (b - a) / n, s = 0.0;
+  for (int p = 0; p < n; p++) {
+    double mid = a + (p + 0.5) * h, half = h / 2;
+    for (int k = 0; k < 8; k++) s += kGLw[k] * (f(mid - half * kGLx[k]) + f(mid + half * kGLx[k]));
+  }
+  return s * h / 2;
+}
+
 struct Device {
   int kind = 0;
   int nodes[4] = {0, 0, 0, 0};
@@ -245,6 +265,8 @@ struct Device {
   virtual void ports(const OperatingPoint& op, Accumulator& acc) = 0;
   virtual bool modelSupplier() const { return false; }
   virtual double modelSupplied(const OperatingPoint&) const { return 0; }
+  // device.Reactive.StoredEnergy at solution x (0 for elements that store nothing)
+  virtual double storedEnergy(const double*, int) { return 0; }
   virtual void loadAC(ACContext& ac) = 0;
   virtual void initialCondition() {}
 };
@@ -349,6 +371,10 @@ struct Capacitor : Device {
     OperatingPoint op = operating(x, nx);
     return lteStep(hist, C * op.voltage, op.current, iPrev, tc.absTol, tc);
   }
+  double storedEnergy(const double* x, int nx) override {  // StoredEnergy, :171-175
+    double v = at(x, nx, u[0]) - at(x, nx, u[1]);
+    return 0.5 * C * v * v;
+  }
   void ports(const OperatingPoint&, Accumulator&) override {}
   void loadAC(ACContext& ac) override {
     omega = ac.omega;
@@ -420,6 +446,10 @@ struct Inductor : Device {
   double truncate(const double* x, int nx, const TruncContext& tc) override {
     OperatingPoint op = operating(x, nx);
     return lteStep(hist, L * op.current, op.voltage, vPrev, tc.vnTol, tc);
+  }
+  double storedEnergy(const double* x, int) override {  // StoredEnergy, :339-342
+    double i = x[branch];
+    return 0.5 * L * i * i;
   }
   void ports(const OperatingPoint&, Accumulator&) override {}
   void loadAC(ACContext& ac) override {
@@ -689,6 +719,40 @@ struct Diode : Device {
     charge(v, id, g0, &q, &c);
     double ic = iq + geq * (v - vd);
     return lteStep(hist, q, ic, iqPrev, tc.absTol, tc);
+  }
+  // StoredEnergy, device/diode.go:325-383: vd·q(vd) − ∫₀^vd qj − TT·∫₀^vd id (closed forms)
+  double intDepletion(double v) const {
+    if (CJO == 0) return 0;
+    double fcv = FC * VJ;
+    auto below = [&](double w) { return CJO * VJ / (1 - M) * (w + VJ / (2 - M) * (std::pow(1 - w / VJ, 2 - M) - 1)); };
+    if (v < fcv) return below(v);
+    double f1 = VJ * (1 - std::pow(1 - FC, 1 - M)) / (1 - M);
+    double f2 = std::pow(1 - FC, 1 + M);
+    double f3 = 1 - FC * (1 + M);
+    double dv = v - fcv;
+    return below(fcv) + CJO * f1 * dv + CJO / f2 * (f3 * dv * dv / 2 + M / (2 * VJ) * ((v * v * v - fcv * fcv * fcv) / 3 - fcv * fcv * dv));
+  }
+  double intCurrent(double v) const {
+    auto fwd = [&](double w) { return IS * (vte * std::expm1(std::min(w / vte, 700.0)) - w); };
+    double v1 = -3 * vte;
+    if (v >= v1) return fwd(v);
+    double total = fwd(v1);
+    double a3 = std::pow(3 * vte / kE, 3.0);
+    auto rev = [&](double w) { return -IS * (w - a3 / (2 * w * w)); };
+    double v2 = v;
+    if (hasBV && v < -xbv) v2 = -xbv;
+    total += rev(v2) - rev(v1);
+    if (v2 == v) return total;
+    auto brk = [&](double w) { return IS * vte * std::exp(std::min(-(xbv + w) / vte, 700.0)); };
+    return total + brk(v) - brk(v2);
+  }
+  double storedEnergy(const double* x, int nx) override {
+    if (!hasCharge) return 0;
+    double v = at(x, nx, ai) - at(x, nx, u[1]);
+    double id, g0, q, c;
+    current(v, &id, &g0);
+    charge(v, id, g0, &q, &c);
+    return v * q - intDepletion(v) - TT * intCurrent(v);
   }
   void ports(const OperatingPoint& op, Accumulator& acc) override { acc.addPort(2, -(op.power - op.storage)); }
   void loadAC(ACContext& ac) override {  // :409-420
@@ -1497,6 +1561,33 @@ struct Mosfet : Device {
     }
     return h;
   }
+  // StoredEnergy, device/mosfet.go:707-728: the junction charges' co-energy
+  // by quadrature on 8 panels, ½·(C_meyer + C_overlap)·v² of the gate charges
+  double storedEnergy(const double* x, int nx) override {
+    if (!hasGcap && !hasJcap) return 0;
+    double a_vgs, a_vds, a_vbs, a_vgb;
+    voltages(x, nx, &a_vgs, &a_vds, &a_vbs, &a_vgb);
+    double v[5] = {a_vgs, a_vgs - a_vds, a_vgb, a_vbs, a_vbs - a_vds};
+    double w = 0.0;
+    double cz[2][2] = {{czbs, czbssw}, {czbd, czbdsw}};
+    for (int k = 0; k < 2; k++) {
+      double vv = v[3 + k];
+      if (vv == 0 || cz[k][0] + cz[k][1] == 0) continue;
+      auto qf = [&](double y) {
+        double qv, cc;
+        bulkCharge(y, cz[k][0], cz[k][1], &qv, &cc);
+        return qv;
+      };
+      w += vv * qf(vv) - glIntegrate(qf, 0.0, vv, 8);
+    }
+    double md, cd, t1, t2, t3, v_on, v_dsat;
+    channel(v[0], v[0] - v[1], v[3], &md, &cd, &t1, &t2, &t3, &v_on, &v_dsat);
+    double mc[3];
+    meyerCaps(v[0], v[1], v_on, v_dsat, md, mc);
+    double ov[3] = {cgso, cgdo, cgbo};
+    for (int k = 0; k < 3; k++) w += 0.5 * (mc[k] + ov[k]) * v[k] * v[k];
+    return w;
+  }
   void ports(const OperatingPoint& op, Accumulator& accu) override { accu.addPort(2, -(op.power - op.storage)); }
   void loadAC(ACContext& ctx) override;
 };
@@ -1814,6 +1905,25 @@ struct Bjt : Device {
       h = std::min(h, lteStep(hist[k], c.q[k], i, iqPrev[k], tc.absTol, tc));
     }
     return h;
+  }
+  // StoredEnergy, device/bjt.go:624-648: Σ_k v_k·q_k(v_k) − ∫₀^{v_k} q_k, each
+  // charge along its own voltage with the others held, quadrature on 8 panels
+  double storedEnergy(const double* x, int nx) override {
+    if (!hasCharge) return 0;
+    double v[4];
+    voltages(x, nx, v);
+    double w = 0.0;
+    for (int k = 0; k < 4; k++) {
+      if (v[k] == 0) continue;
+      auto qk = [&](double y) {
+        double vv[4] = {v[0], v[1], v[2], v[3]};
+        vv[k] = y;
+        DC d = evalDC(vv[0], vv[1]);
+        return charges(vv[0], vv[1], vv[2], vv[3], d).q[k];
+      };
+      w += v[k] * qk(v[k]) - glIntegrate(qk, 0.0, v[k], 8);
+    }
+    return w;
   }
   void ports(const OperatingPoint& op, Accumulator& accu) override { accu.addPort(2, -(op.power - op.storage)); }
   // acBranch, :718-731

@@ -81,6 +81,7 @@ def lib():
         _lib.oracle_size.argtypes = [C.c_int32, P, C.c_int32, P]
         _lib.oracle_op.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P]
         _lib.oracle_op_polished.argtypes = [C.c_int32, P, C.c_int32, P, P, P, P, P]
+        _lib.oracle_op_report.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P]
         _lib.oracle_dc_method.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, C.c_int32, P]
         _lib.oracle_plant_stepping.argtypes = [C.c_int32, C.c_double]
         _lib.oracle_lte_step.restype = C.c_double
@@ -253,6 +254,26 @@ def op_polished(num_nodes, els, params, opts=None, guess=None):
     g = None if guess is None else np.ascontiguousarray(guess, np.float64)
     e = lib().oracle_op_polished(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(g), _p(x), C.byref(it))
     return e, x, it.value
+
+
+REPORT_FIELDS = ("voltage", "current", "power", "storage", "energy", "supplied", "exterior", "channel")
+LEDGER_FIELDS = ("inflow", "outflow", "dissipation", "storage_rate", "model_supplied", "hypothetical", "residual",
+                 "scale", "roundoff", "relative_residual")
+
+
+def op_report(num_nodes, els, params, opts=None, polish=False):
+    """Sim.result after analysis.OP (analysis/op.go:361-404): returns (status,
+    x, rep [nel, 8] in REPORT_FIELDS order, led [10] in LEDGER_FIELDS order);
+    rep and led are filled when the operating point exists (status 0 or the
+    ledger error)."""
+    n = size(num_nodes, els, params)
+    x = np.zeros(n)
+    rep = np.zeros((len(els), len(REPORT_FIELDS)))
+    led = np.zeros(len(LEDGER_FIELDS))
+    o = opts if opts is not None else KhOptions()
+    e = lib().oracle_op_report(num_nodes, _p(els), len(els), _p(params), C.byref(o), 1 if polish else 0, _p(x),
+                               _p(rep), _p(led))
+    return e, x, rep, led
 
 
 def tran(num_nodes: int, els: np.ndarray, params: np.ndarray, spec: KhTranSpec, opts: Optional[KhOptions] = None,

@@ -475,6 +475,42 @@ struct Sim {
     return std::fabs(res / acc.scale);
   }
 
+  // Sim.result's element entries and em.BuildLedger (analysis/op.go:361-404,
+  // em/energy.go:89-119): rep[nel][8] = voltage, current, absorbed power,
+  // storage power, stored energy, model-supplied power, power through the
+  // modeled exterior port and its channel (-1 none); led[10] = inflow,
+  // outflow, dissipation, storage rate, model-supplied, hypothetical (0),
+  // residual, scale, round-off floor, relative residual.
+  void report(double* rep, double* led) {
+    for (size_t k = 0; k < devs.size(); k++) {
+      Device* d = devs[k].get();
+      OperatingPoint op = d->operating(x.data(), n);
+      Accumulator probe;
+      d->ports(op, probe);
+      double* r = rep + k * 8;
+      r[0] = op.voltage;
+      r[1] = op.current;
+      r[2] = op.power;
+      r[3] = op.storage;
+      r[4] = d->storedEnergy(x.data(), n);
+      r[5] = d->modelSupplier() ? d->modelSupplied(op) : 0.0;
+      r[6] = probe.lastCh >= 0 ? probe.lastPw : 0.0;
+      r[7] = probe.lastCh;
+    }
+    Accumulator acc;
+    double rel = ledgerOP(&acc);
+    led[0] = acc.inflow;
+    led[1] = acc.outflow;
+    led[2] = acc.dissipation;
+    led[3] = acc.storageRate;
+    led[4] = acc.modelSupplied;
+    led[5] = 0;
+    led[6] = acc.residual();
+    led[7] = acc.scale;
+    led[8] = roundoff();
+    led[9] = rel;
+  }
+
   // Sim.OP, analysis/op.go:340-357
   int op(const Options& o, int* iters, double* ledgerRel, const double* guess) {
     lc.mode = ModeOP;
@@ -890,6 +926,25 @@ int oracle_op_polished(int32_t num_nodes, const kh_element* el, int32_t nel, con
   if (e == E_OK) e = s.polish(&iters);
   if (x_out) std::copy(s.x.begin(), s.x.end(), x_out);
   if (iters_out) *iters_out = iters;  // Newton iterations and polish solves: every solve of the circuit
+  return e;
+}
+
+// OP (polished when polish != 0), then output.Result's element entries and
+// ledger (Sim::report): rep [nel*8], led [10]; x_out [n]. The entries are
+// filled whenever the operating point exists (status OK or ledger).
+int oracle_op_report(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params,
+                     const kh_options* ko, int32_t polish, double* x_out, double* rep, double* led) {
+  Sim s;
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
+  if (e != E_OK) return e;
+  Options o;
+  o.from(ko);
+  int iters = 0;
+  double lr;
+  e = s.op(o, &iters, &lr, nullptr);
+  if (e == E_OK && polish) e = s.polish(&iters);
+  if (x_out) std::copy(s.x.begin(), s.x.end(), x_out);
+  if (e == E_OK || e == E_LEDGER) s.report(rep, led);
   return e;
 }
 
