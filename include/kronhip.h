@@ -316,6 +316,33 @@ enum {
 kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options* opt,
                        const kh_tran_spec* spec, kh_stats* stats);
 
+/* ---- result formats (output package) ----------------------------------------
+ * The reference's outputs written from what the batched analyses return, byte
+ * for byte as its tools write them: the JSON document of `kronspice -json`
+ * (output.Document, output/json.go:18-316, encoding/json with two-space
+ * indentation) and output.WriteCSV's "name,value" rows (output/csv.go:15-48).
+ * Host arrays in, bytes out; the circuit's names come from its netlist. */
+typedef struct kh_names {
+  const char* title;                /* deck title ("" or NULL: none) */
+  int32_t num_nodes;                /* node names [num_nodes], index 0 = ground ("0") */
+  int32_t n_elems;                  /* element names [n_elems], as written in the netlist */
+  const char* const* node_names;
+  const char* const* elem_names;
+  const kh_element* elems;          /* the element table (kinds, nodes) of the topology */
+} kh_names;
+enum { KH_FORMAT_JSON = 0, KH_FORMAT_CSV = 1 };
+/* One circuit's operating point: the document with one "op" analysis (OPDocument,
+ * output/json.go:164-200) or output.WriteCSV's rows. status is the circuit's;
+ * any other than KH_OK writes the failed run's document (ok false, the error's
+ * class as cmd/kronspice/main.go:284-305 names it and the status text, no
+ * analyses) or no CSV. x [n], report [n_elems * KH_REPORT_FIELDS] and ledger
+ * [KH_LEDGER_FIELDS] are one circuit's slices of kh_batch_run's d_x (a column),
+ * d_report and d_report_ledger, copied to the host. version NULL: "dev". Writes
+ * at most cap bytes to buf and returns the document's length in bytes (size
+ * buf with a first call of cap 0), or -1 on invalid arguments. */
+int64_t kh_format_op(int32_t format, const kh_names* names, const char* version, kh_status status, const double* x,
+                     const double* report, const double* ledger, char* buf, int64_t cap);
+
 /* ---- a whole generation (the population evaluation's batched caller) ------
  * The GA evaluates every candidate of a generation with analysis.OP or Tran on
  * a pool of goroutines, results stored by candidate index (evo/engine.go:519-612,

@@ -524,6 +524,42 @@ def run_groups(topos, params_list, spec: Optional[TranSpec] = None, opts: Option
     return results, stats
 
 
+class Names(C.Structure):
+    _fields_ = [("title", C.c_char_p), ("num_nodes", C.c_int32), ("n_elems", C.c_int32),
+                ("node_names", C.POINTER(C.c_char_p)), ("elem_names", C.POINTER(C.c_char_p)),
+                ("elems", C.c_void_p)]
+
+
+FORMAT_JSON, FORMAT_CSV = 0, 1
+
+
+def format_op(node_names, elem_names, elems: np.ndarray, status: int, x, report, ledger, fmt: int = FORMAT_JSON,
+              title: str = "", version: Optional[str] = None, library=None) -> bytes:
+    """One circuit's operating point as the reference writes it: the
+    `kronspice -json` document (fmt FORMAT_JSON) or output.WriteCSV's rows
+    (FORMAT_CSV). node_names [num_nodes] with "0" first, elem_names and elems
+    (the element table) [n_elems]; x [n], report [n_elems, 8], ledger [10] host
+    arrays (one circuit's slices of op(report=True)'s outputs)."""
+    L = library if library is not None else lib()
+    L.kh_format_op.restype = C.c_int64
+    L.kh_format_op.argtypes = [C.c_int32, C.c_void_p, C.c_char_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_int64]
+    nn = (C.c_char_p * len(node_names))(*[n.encode() for n in node_names])
+    en = (C.c_char_p * max(len(elem_names), 1))(*[n.encode() for n in elem_names])
+    els = np.ascontiguousarray(elems)
+    nm = Names(title=title.encode(), num_nodes=len(node_names), n_elems=len(elem_names), node_names=nn,
+               elem_names=en, elems=els.ctypes.data)
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, report, ledger)]
+    ptrs = [None if a is None else a.ctypes.data for a in arrs]
+    ver = None if version is None else version.encode()
+    size = L.kh_format_op(fmt, C.byref(nm), ver, status, *ptrs, None, 0)
+    if size < 0:
+        raise KhError(BAD_ARGUMENT, "kh_format_op: invalid arguments")
+    buf = C.create_string_buffer(int(size) + 1)
+    L.kh_format_op(fmt, C.byref(nm), ver, status, *ptrs, buf, size)
+    return buf.raw[:size]
+
+
 class Candidate(C.Structure):
     _fields_ = [("num_nodes", C.c_int32), ("n_elems", C.c_int32), ("nparams", C.c_int32), ("reserved", C.c_int32),
                 ("elems", C.c_void_p), ("params", C.c_void_p), ("lane_spec", C.c_void_p)]

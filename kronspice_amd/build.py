@@ -27,6 +27,7 @@ PERSIST_KINDS = (126, 190, 318, 1022)  # diodes, level-1 MOSFETs, BJTs, everythi
 UNITS = [
     ("kronhip.hip", [], "kronhip.o"),
     ("kh_symbolic.cpp", [], "kh_symbolic.o"),
+    ("kh_output.cpp", [], "kh_output.o"),
     ("kh_mlp.hip", [], "kh_mlp.o"),
 ] + [("kh_persist.hip", [f"-DKH_PERSIST_WAVES={w}", f"-DKH_KINDS={k}"], f"kh_persist_w{w}_k{k}.o")
      for w in PERSIST_WAVES for k in PERSIST_KINDS] + [
