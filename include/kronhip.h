@@ -322,6 +322,7 @@ kh_status kh_batch_run(int32_t ngroups, const kh_group* groups, const kh_options
  * (output.Document, output/json.go:18-316, encoding/json with two-space
  * indentation) and output.WriteCSV's "name,value" rows (output/csv.go:15-48).
  * Host arrays in, bytes out; the circuit's names come from its netlist. */
+struct kh_dc_spec;  /* below, with the DC sweep */
 typedef struct kh_names {
   const char* title;                /* deck title ("" or NULL: none) */
   int32_t num_nodes;                /* node names [num_nodes], index 0 = ground ("0") */
@@ -342,6 +343,16 @@ enum { KH_FORMAT_JSON = 0, KH_FORMAT_CSV = 1 };
  * buf with a first call of cap 0), or -1 on invalid arguments. */
 int64_t kh_format_op(int32_t format, const kh_names* names, const char* version, kh_status status, const double* x,
                      const double* report, const double* ledger, char* buf, int64_t cap);
+/* One circuit's DC sweep: the document with one "dc" analysis (DCDocument,
+ * output/json.go:202-221) or output.WriteDCCSV's table (output/ac.go:125-165).
+ * n: the circuit's unknowns; sources [nsweeps]: the swept sources' names as the
+ * .dc line gives them; points: the points solved; sweep_x [points*n] and
+ * sweep_i [points*n_elems]: one circuit's slices of kh_batch_dc_ex's d_sweep_x
+ * and d_sweep_i, point-major; max_rel: its d_ledger value. Return value and
+ * buffer as kh_format_op. */
+int64_t kh_format_dc(int32_t format, const kh_names* names, const char* version, kh_status status, int32_t n,
+                     const struct kh_dc_spec* spec, const char* const* sources, int32_t points, const double* sweep_x,
+                     const double* sweep_i, double max_rel, char* buf, int64_t cap);
 
 /* ---- a whole generation (the population evaluation's batched caller) ------
  * The GA evaluates every candidate of a generation with analysis.OP or Tran on
@@ -395,6 +406,22 @@ typedef struct kh_dc_spec {
 kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
                       const kh_dc_spec* spec, const double* d_nodeset, double* d_sweep_x, int32_t* d_status,
                       int32_t* d_points, int32_t* d_iters, double* d_ledger, kh_stats* stats);
+
+/* kh_batch_dc with the sweep's element currents too (DCSweepResult.I,
+ * analysis/dc.go:87-128): d_sweep_i (optional) [points*n_elems*B], point-major,
+ * then element, then circuit: each element's current (n+ -> n-) at each
+ * point, after that point's Accept. The other outputs are kh_batch_dc's. */
+typedef struct kh_dc_outputs {
+  double* d_sweep_x;   /* [points*n*B] */
+  double* d_sweep_i;   /* optional [points*n_elems*B] */
+  int32_t* d_status;   /* [B] */
+  int32_t* d_points;   /* optional [B] */
+  int32_t* d_iters;    /* optional [points*B] */
+  double* d_ledger;    /* optional [B] */
+} kh_dc_outputs;
+kh_status kh_batch_dc_ex(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                         const kh_dc_spec* spec, const double* d_nodeset, const kh_dc_outputs* out,
+                         kh_stats* stats);
 
 /* analysis.AC for B circuits of one topology (analysis/ac.go:76-193): the
  * operating point and polish, then at every frequency the devices' LoadAC

@@ -412,10 +412,17 @@ def sweep_count(start: float, stop: float, step: float) -> int:
     return int(np.floor((stop - start) / step + 1e-9)) + 1
 
 
-def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=None) -> Result:
+class DcOutputs(C.Structure):
+    _fields_ = [("d_sweep_x", C.c_void_p), ("d_sweep_i", C.c_void_p), ("d_status", C.c_void_p),
+                ("d_points", C.c_void_p), ("d_iters", C.c_void_p), ("d_ledger", C.c_void_p)]
+
+
+def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=None, currents: bool = False) -> Result:
     """analysis.DC for every circuit; sweeps [(element index, start, stop, step)],
     inner first. Result.x is [points, n, B]; iters [points, B] Newton iterations per
-    point; points [B] points solved; ledger [B] largest ledger residual."""
+    point; points [B] points solved; ledger [B] largest ledger residual; with
+    currents, Result.currents [points, n_elems, B] each element's current at
+    each point (DCSweepResult.I)."""
     torch = _torch()
     if isinstance(params, np.ndarray):
         params = to_device(params)
@@ -434,9 +441,18 @@ def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=N
     spec = dc_spec(sweeps)
     s = Stats()
     torch.cuda.synchronize()
-    _check(lib().kh_batch_dc(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(nodeset), _p(x), _p(st), _p(done),
-                             _p(it), _p(led), C.byref(s)))
-    return Result(x, st, iters=it, ledger=led, stats=s.as_dict(), points=done)
+    if not currents:
+        _check(lib().kh_batch_dc(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(nodeset), _p(x), _p(st),
+                                 _p(done), _p(it), _p(led), C.byref(s)))
+        return Result(x, st, iters=it, ledger=led, stats=s.as_dict(), points=done)
+    cur = torch.zeros((pts, topo.n_elems, B), dtype=torch.float64, device="cuda")
+    out = DcOutputs(d_sweep_x=x.data_ptr(), d_sweep_i=cur.data_ptr(), d_status=st.data_ptr(),
+                    d_points=done.data_ptr(), d_iters=it.data_ptr(), d_ledger=led.data_ptr())
+    _check(lib().kh_batch_dc_ex(topo.h, B, _p(params), C.byref(o), C.byref(spec), _p(nodeset), C.byref(out),
+                                C.byref(s)))
+    r = Result(x, st, iters=it, ledger=led, stats=s.as_dict(), points=done)
+    r.currents = cur
+    return r
 
 
 def ac(topo: Topology, params, freqs, opts: Optional[Options] = None, nodeset=None) -> Result:
@@ -557,6 +573,43 @@ def format_op(node_names, elem_names, elems: np.ndarray, status: int, x, report,
         raise KhError(BAD_ARGUMENT, "kh_format_op: invalid arguments")
     buf = C.create_string_buffer(int(size) + 1)
     L.kh_format_op(fmt, C.byref(nm), ver, status, *ptrs, buf, size)
+    return buf.raw[:size]
+
+
+def _names(node_names, elem_names, elems, title):
+    nn = (C.c_char_p * len(node_names))(*[n.encode() for n in node_names])
+    en = (C.c_char_p * max(len(elem_names), 1))(*[n.encode() for n in elem_names])
+    els = np.ascontiguousarray(elems)
+    nm = Names(title=title.encode(), num_nodes=len(node_names), n_elems=len(elem_names), node_names=nn,
+               elem_names=en, elems=els.ctypes.data)
+    return nm, (nn, en, els)
+
+
+def format_dc(node_names, elem_names, elems: np.ndarray, status: int, n: int, sweeps, sources, points: int,
+              sweep_x, sweep_i, max_rel: float, fmt: int = FORMAT_JSON, title: str = "",
+              version: Optional[str] = None, library=None) -> bytes:
+    """One circuit's DC sweep as the reference writes it: the document with a
+    "dc" analysis (FORMAT_JSON) or output.WriteDCCSV's table (FORMAT_CSV).
+    sweeps [(element index, start, stop, step)] inner first, sources their
+    names; sweep_x [points, n] and sweep_i [points, n_elems] host arrays (one
+    circuit's slices of dc(currents=True))."""
+    L = library if library is not None else lib()
+    L.kh_format_dc.restype = C.c_int64
+    L.kh_format_dc.argtypes = [C.c_int32, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                               C.c_int32, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int64]
+    nm, keep = _names(node_names, elem_names, elems, title)
+    spec = dc_spec(sweeps)
+    src = (C.c_char_p * len(sources))(*[s_.encode() for s_ in sources])
+    xs = None if sweep_x is None else np.ascontiguousarray(sweep_x, np.float64)
+    cs = None if sweep_i is None else np.ascontiguousarray(sweep_i, np.float64)
+    ver = None if version is None else version.encode()
+    args = [fmt, C.byref(nm), ver, status, n, C.byref(spec), src, points, None if xs is None else xs.ctypes.data,
+            None if cs is None else cs.ctypes.data, max_rel]
+    size = L.kh_format_dc(*args, None, 0)
+    if size < 0:
+        raise KhError(BAD_ARGUMENT, "kh_format_dc: invalid arguments")
+    buf = C.create_string_buffer(int(size) + 1)
+    L.kh_format_dc(*args, buf, size)
     return buf.raw[:size]
 
 

@@ -113,6 +113,7 @@ struct CtlArgs {
   const double* nodeset;     // optional [numNodes-1][B]: per-circuit Newton start (Options.NodeSet)
   double* sw_x;              // DC sweep: per circuit [points][n] solutions
   int32_t* sw_it;            // DC sweep: per circuit [points] Newton iterations
+  double* sw_i;              // DC sweep (optional): per circuit [points][nel] element currents
   const int32_t* lu_status;  // [B] result of the last factor+solve
   int32_t* counters;         // [0] lanes still running, [1] lanes waiting for a new pattern
 };
@@ -539,6 +540,11 @@ struct CtlT {
   }
   KH_HD void sweep_store(int point) {
     for (int i = 0; i < n; i++) A.sw_x[(int64_t)point * n + i] = X(A.x, i);
+    if (A.sw_i) {  // DCSweepResult.I: each element's Operating current after Accept (analysis/dc.go:121-123)
+      const Lane L = this->L;
+      const int nel_ = A.ckt.nel;
+      for (int k = 0; k < nel_; k++) A.sw_i[(int64_t)point * nel_ + k] = operating(L, el(k), A.x).current;
+    }
   }
   // Slot of the diagonal entry (i, i): the device slot when a device stamps
   // it, else a slot reserved after the device slots.

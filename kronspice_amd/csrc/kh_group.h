@@ -197,6 +197,7 @@ KH_HD inline Group lds_view(const Group& g, char* base, int64_t cc) {
   const int64_t pts = sweep_points(g.ctl.p);
   if (v.ctl.sw_x) v.ctl.sw_x += cc * pts * g.ctl.ckt.n;
   if (v.ctl.sw_it) v.ctl.sw_it += cc * pts;
+  if (v.ctl.sw_i) v.ctl.sw_i += cc * pts * g.ctl.ckt.nel;
   int64_t off = 0;
   int i = 0;
   for_each_array(v, [&](auto& p, int64_t bytes, bool, bool, int) {
@@ -239,6 +240,7 @@ KH_HD inline void circuit_view(const Group& g, int64_t c, CtlArgs& a, LuArgs& l)
   const int64_t pts = sweep_points(a.p);
   if (a.sw_x) a.sw_x += c * pts * n;
   if (a.sw_it) a.sw_it += c * pts;
+  if (a.sw_i) a.sw_i += c * pts * a.ckt.nel;
   l.B = 1;
   l.pid += c;
   l.val = a.val; l.rhs = a.rhs; l.x = a.x;

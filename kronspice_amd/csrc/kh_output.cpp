@@ -410,4 +410,88 @@ int64_t kh_format_op(int32_t format, const kh_names* nm, const char* version, kh
   return emit(out, buf, cap);
 }
 
+int64_t kh_format_dc(int32_t format, const kh_names* nm, const char* version, kh_status status, int32_t n,
+                     const kh_dc_spec* spec, const char* const* sources, int32_t points, const double* sweep_x,
+                     const double* sweep_i, double max_rel, char* buf, int64_t cap) {
+  if (!names_ok(nm) || (format != KH_FORMAT_JSON && format != KH_FORMAT_CSV)) return -1;
+  if (!spec || spec->nsweeps < 1 || spec->nsweeps > 2 || !sources || n < nm->num_nodes - 1) return -1;
+  for (int s = 0; s < spec->nsweeps; s++)
+    if (!sources[s]) return -1;
+  const bool have = status == KH_OK;
+  if (have && (points < 0 || (points > 0 && (!sweep_x || (nm->n_elems > 0 && !sweep_i))))) return -1;
+  const int nel = nm->n_elems, ns = spec->nsweeps;
+  // DCSweep.values (analysis/dc.go:31-46): point k is inner index k % count0, outer k / count0
+  const double cnt0 = (spec->stop[0] - spec->start[0]) / spec->step[0];
+  const int64_t count0 = (int64_t)std::floor(cnt0 + 1e-9) + 1;
+  auto value = [&](int s, int k) {
+    const int64_t idx = s == 0 ? k % count0 : k / count0;
+    return spec->start[s] + (double)idx * spec->step[s];
+  };
+  auto xv = [&](int k, int node) { return sweep_x[(int64_t)k * n + node - 1]; };
+  auto iv = [&](int k, int e) { return sweep_i[(int64_t)k * nel + e]; };
+  if (format == KH_FORMAT_CSV) {  // output.WriteDCCSV (output/ac.go:125-165)
+    if (!have) return emit("", buf, cap);
+    std::string out;
+    std::vector<std::string> head;
+    for (int s = 0; s < ns; s++) head.push_back(lower(sources[s]));
+    for (int i = 1; i < nm->num_nodes; i++) head.push_back(std::string("v(") + nm->node_names[i] + ")");
+    for (int e = 0; e < nel; e++) head.push_back("i(" + lower(nm->elem_names[e]) + ")");
+    auto row = [&](const std::vector<std::string>& r) {
+      for (size_t k = 0; k < r.size(); k++) {
+        if (k) out += ',';
+        csv_field(out, r[k]);
+      }
+      out += '\n';
+    };
+    row(head);
+    for (int k = 0; k < points; k++) {
+      std::vector<std::string> r;
+      for (int s = 0; s < ns; s++) r.push_back(go_g(value(s, k), 10));
+      for (int i = 1; i < nm->num_nodes; i++) r.push_back(go_g(xv(k, i), 10));
+      for (int e = 0; e < nel; e++) r.push_back(go_g(iv(k, e), 10));
+      row(r);
+    }
+    return emit(out, buf, cap);
+  }
+  J dcdoc;
+  if (have) {  // DCDocument (output/json.go:202-221)
+    dcdoc = J::obj();
+    J src = J::arr();
+    for (int s = 0; s < ns; s++) src.push(J::str(sources[s]));
+    dcdoc.add("sources", std::move(src));
+    J vals = J::arr();
+    for (int s = 0; s < ns; s++) {
+      J a = points > 0 ? J::arr() : J::null();
+      for (int k = 0; k < points; k++) a.push(J::num(value(s, k)));
+      vals.push(std::move(a));
+    }
+    dcdoc.add("values", std::move(vals));
+    auto series = [&](auto f) {
+      J a = points > 0 ? J::arr() : J::null();
+      for (int k = 0; k < points; k++) a.push(J::num(f(k)));
+      return a;
+    };
+    dcdoc.add("v", node_map(nm, [&](int i) { return series([&](int k) { return xv(k, i); }); }));
+    std::vector<std::pair<std::string, int>> keys;
+    for (int e = 0; e < nel; e++) keys.emplace_back(lower(nm->elem_names[e]), e);
+    std::stable_sort(keys.begin(), keys.end(), [](const auto& a, const auto& b) {
+      const int c = std::memcmp(a.first.data(), b.first.data(), std::min(a.first.size(), b.first.size()));
+      return c != 0 ? c < 0 : a.first.size() < b.first.size();
+    });
+    J im = J::obj();
+    for (size_t q = 0; q < keys.size(); q++) {
+      if (q + 1 < keys.size() && keys[q + 1].first == keys[q].first) continue;  // a Go map keeps the last
+      const int e = keys[q].second;
+      im.add(keys[q].first, series([&](int k) { return iv(k, e); }));
+    }
+    dcdoc.add("i", std::move(im));
+    dcdoc.add("max_relative_residual", J::num(max_rel));
+  }
+  J d = document(nm, version, status, have ? &dcdoc : nullptr, "dc");
+  std::string out;
+  put(out, d, 0);
+  out += '\n';
+  return emit(out, buf, cap);
+}
+
 }  // extern "C"

@@ -895,6 +895,12 @@ struct WCtl : CtlT<WCtl> {
   }
   __device__ void sweep_store(int point) {
     for (int i = wlane(); i < n; i += kWaveLanes) A.sw_x[(int64_t)point * n + i] = A.x[i];
+    if (A.sw_i) {
+      const Lane Lc = this->L;
+      const int nel_ = A.ckt.nel;
+      for (int k = wlane(); k < nel_; k += kWaveLanes)
+        A.sw_i[(int64_t)point * nel_ + k] = operating_point(Lc, el(k), A.x).current;
+    }
     wsync();
   }
 };

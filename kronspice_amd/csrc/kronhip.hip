@@ -1130,6 +1130,8 @@ struct GroupRun {
   int32_t gid = 0;
   DBuf par, lane, nodeset, st, val, present, rhs, x, xold, xsave, ptprev, xacc, absbuf, ci, cd, lustatus, pid, hash, solves;
   DBuf swx, swit;           // DC sweep: per circuit [points][n] solutions, [points] Newton iterations
+  DBuf swi;                 // DC sweep (when asked for): per circuit [points][nel] element currents
+  bool want_sw_i = false;
   int64_t sw_points = 0;
   DevPatterns pats;
   LuWork work;
@@ -1169,6 +1171,7 @@ struct GroupRun {
     if (sw_points > 0) {
       KH_HIP(swx.alloc((size_t)B * sw_points * T.n * 8));
       KH_HIP(swit.alloc((size_t)B * sw_points * 4));
+      if (want_sw_i) KH_HIP(swi.alloc((size_t)B * sw_points * T.nel * 8));
     }
     has_nodeset = d_nodeset != nullptr && T.num_nodes > 1;
     if (has_nodeset) {
@@ -1221,6 +1224,7 @@ struct GroupRun {
     a.nodeset = has_nodeset ? nodeset.as<double>() : nullptr;
     a.sw_x = sw_points ? swx.as<double>() : nullptr;
     a.sw_it = sw_points ? swit.as<int32_t>() : nullptr;
+    a.sw_i = (sw_points && want_sw_i) ? swi.as<double>() : nullptr;
     return a;
   }
 
@@ -1330,7 +1334,14 @@ struct GroupRun {
   // Per-circuit results (caller's SoA layout) and batch counters.
   kh_status outputs(double* d_x, int32_t* d_status, int32_t* d_iters, int32_t* d_acc, int32_t* d_rej,
                     double* d_ledger, double* d_sw_x = nullptr, int32_t* d_sw_it = nullptr,
-                    int32_t* d_points = nullptr) {
+                    int32_t* d_points = nullptr, double* d_sw_i = nullptr) {
+    if (sw_points > 0 && d_sw_i && want_sw_i) {  // [B][points*nel] -> [points*nel][B]
+      const int64_t cols = (int64_t)sw_points * topo->t.nel;
+      const int64_t tot = cols * B;
+      hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, swi.as<double>(),
+                         d_sw_i, B, cols);
+      KH_HIP(hipGetLastError());
+    }
     if (sw_points > 0 && d_sw_x) {  // [B][points*n] -> [points*n][B]
       int64_t tot = (int64_t)sw_points * n * B;
       hipLaunchKernelGGL(k_transpose<double>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, swx.as<double>(),
@@ -1632,6 +1643,7 @@ std::mutex g_run_mu;  // batch entry points run one at a time per process
 // Outputs of a DC sweep (one group).
 struct SweepOut {
   double* d_x = nullptr;
+  double* d_i = nullptr;  // optional element currents per point
   int32_t* d_iters = nullptr;
   int32_t* d_points = nullptr;
 };
@@ -1682,6 +1694,7 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
     r->B = G.B;
     r->gid = g;
     r->sw_points = sweep_points(p);
+    r->want_sw_i = sw && sw->d_i;
     {  // MOSFET levels and internal nodes: every circuit must have circuit 0's
        // (the compiled currents and the unknown layout follow it)
       const Topology& T = G.topology->t;
@@ -1715,7 +1728,8 @@ kh_status batch_run(int32_t ngroups, const kh_group* groups, const kh_options* o
   for (int32_t g = 0; g < ngroups; g++) {
     const kh_group& G = groups[g];
     s = runs[g]->outputs(G.d_x, G.d_status, G.d_iters, G.d_accepted, G.d_rejected, G.d_ledger,
-                         sw ? sw->d_x : nullptr, sw ? sw->d_iters : nullptr, sw ? sw->d_points : nullptr);
+                         sw ? sw->d_x : nullptr, sw ? sw->d_iters : nullptr, sw ? sw->d_points : nullptr,
+                         sw ? sw->d_i : nullptr);
     if (s != KH_OK) return s;
     s = runs[g]->report(p, G.d_report, G.d_report_ledger);
     if (s != KH_OK) return s;
@@ -1875,7 +1889,19 @@ kh_status kh_generation_run(int32_t n, const kh_candidate* cands, const kh_optio
 kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
                       const kh_dc_spec* spec, const double* d_nodeset, double* d_sweep_x, int32_t* d_status,
                       int32_t* d_points, int32_t* d_iters, double* d_ledger, kh_stats* stats) {
-  if (!t || !spec) return fail(KH_BAD_ARGUMENT, "no topology or sweep");
+  kh_dc_outputs out = {};
+  out.d_sweep_x = d_sweep_x;
+  out.d_status = d_status;
+  out.d_points = d_points;
+  out.d_iters = d_iters;
+  out.d_ledger = d_ledger;
+  return kh_batch_dc_ex(t, B, d_params, opt, spec, d_nodeset, &out, stats);
+}
+
+kh_status kh_batch_dc_ex(kh_topology t, int32_t B, const double* d_params, const kh_options* opt,
+                         const kh_dc_spec* spec, const double* d_nodeset, const kh_dc_outputs* out,
+                         kh_stats* stats) {
+  if (!t || !spec || !out) return fail(KH_BAD_ARGUMENT, "no topology, sweep or outputs");
   if (spec->nsweeps < 1 || spec->nsweeps > 2) return fail(KH_BAD_ARGUMENT, "need 1 or 2 sweeps");
   CtlParams p = resolve(opt, AN_DC);
   p.swN = spec->nsweeps;
@@ -1898,12 +1924,13 @@ kh_status kh_batch_dc(kh_topology t, int32_t B, const double* d_params, const kh
   }
   if (total > 1000000) return fail(KH_BAD_ARGUMENT, "invalid DC sweep: too many points");
   kh_group g = {};
-  g.topology = t; g.B = B; g.d_params = d_params; g.d_nodeset = d_nodeset; g.d_status = d_status;
-  g.d_ledger = d_ledger;
+  g.topology = t; g.B = B; g.d_params = d_params; g.d_nodeset = d_nodeset; g.d_status = out->d_status;
+  g.d_ledger = out->d_ledger;
   SweepOut sw;
-  sw.d_x = d_sweep_x;
-  sw.d_iters = d_iters;
-  sw.d_points = d_points;
+  sw.d_x = out->d_sweep_x;
+  sw.d_i = out->d_sweep_i;
+  sw.d_iters = out->d_iters;
+  sw.d_points = out->d_points;
   return batch_run(1, &g, opt, p, nullptr, &sw, stats);
 }
 

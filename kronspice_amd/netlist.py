@@ -156,7 +156,9 @@ def parse(text: str) -> Deck:
             analyses.append(Analysis("ac", dict(variation=tok[1].lower(), points=int(parse_value(tok[2])),
                                                 fstart=parse_value(tok[3]), fstop=parse_value(tok[4]))))
         elif head == ".dc":
-            analyses.append(Analysis("dc", dict(source=tok[1].lower(), start=parse_value(tok[2]),
+            # source: the lower-cased name for lookups; source_name: as written, as
+            # DCSweep.Source keeps it (netlist/parser.go:367-368)
+            analyses.append(Analysis("dc", dict(source=tok[1].lower(), source_name=tok[1], start=parse_value(tok[2]),
                                                 stop=parse_value(tok[3]), step=parse_value(tok[4]))))
         elif head.startswith("."):
             raise NotImplementedError(f"unsupported control card {head}")

@@ -88,7 +88,7 @@ def lib():
         _lib.oracle_lte_step.argtypes = [C.c_int32, C.c_double, C.c_double, P, P, C.c_int32] + [C.c_double] * 7
         _lib.oracle_tran.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
         _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
-        _lib.oracle_dc.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P, P, P]
+        _lib.oracle_dc.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P, P, P, P]
         _lib.oracle_batch_op.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int32,
                                          C.c_int32, P, P, P]
         _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
@@ -301,10 +301,11 @@ def sweep_values(start: float, stop: float, step: float) -> np.ndarray:
     return start + np.arange(count, dtype=np.float64) * step
 
 
-def dc(num_nodes: int, els: np.ndarray, params: np.ndarray, sweeps, opts: Optional[KhOptions] = None):
+def dc(num_nodes: int, els: np.ndarray, params: np.ndarray, sweeps, opts: Optional[KhOptions] = None,
+       currents: bool = False):
     """analysis.DC restatement. sweeps: [(element index, start, stop, step)], inner
     first (one or two). Returns (status, x [points, n], iters [points], points done,
-    max ledger residual)."""
+    max ledger residual[, element currents [points, nel]])."""
     n = size(num_nodes, els, params)
     total = 1
     for _, a, b, s in sweeps:
@@ -318,8 +319,12 @@ def dc(num_nodes: int, els: np.ndarray, params: np.ndarray, sweeps, opts: Option
     done = C.c_int32()
     mr = C.c_double()
     o = opts if opts is not None else KhOptions()
+    cur = np.zeros((total, len(els))) if currents else None
     e = lib().oracle_dc(num_nodes, _p(els), len(els), _p(np.ascontiguousarray(params, np.float64)), C.byref(o),
-                        len(sweeps), _p(src), _p(st), _p(sp), _p(sd), _p(x), _p(it), C.byref(done), C.byref(mr))
+                        len(sweeps), _p(src), _p(st), _p(sp), _p(sd), _p(x), _p(it), C.byref(done), C.byref(mr),
+                        _p(cur))
+    if currents:
+        return e, x, it, done.value, mr.value, cur
     return e, x, it, done.value, mr.value
 
 

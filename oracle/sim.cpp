@@ -715,7 +715,7 @@ struct Sim {
   // x of each point goes to xout[point][n], its Newton iterations to
   // iters[point]; *done counts the points that succeeded.
   int dcSweep(int nsw, const int* elem, const double* start, const double* stop, const double* step,
-              const Options& o, double* xout, int* iters, int* done, double* maxRel) {
+              const Options& o, double* xout, int* iters, int* done, double* maxRel, double* iout = nullptr) {
     if (nsw < 1 || nsw > 2) return E_BADSPEC;
     std::vector<double> vals[2];
     SourceValue* srcs[2] = {nullptr, nullptr};
@@ -742,6 +742,9 @@ struct Sim {
         if (iters) iters[point] = it;
         if (e != E_OK) { ret = e; goto out; }
         if (xout) std::copy(x.begin(), x.end(), xout + (size_t)point * n);
+        if (iout)  // DCSweepResult.I: the elements' currents after the point's Accept (analysis/dc.go:121-123)
+          for (size_t k = 0; k < devs.size(); k++)
+            iout[(size_t)point * devs.size() + k] = devs[k]->operating(x.data(), n).current;
         if (lr > *maxRel) *maxRel = lr;
         ++*done;
         ++point;
@@ -1054,7 +1057,7 @@ int oracle_tran(int32_t num_nodes, const kh_element* el, int32_t nel, const doub
 // point of the sweep), *done = points solved.
 int oracle_dc(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params, const kh_options* ko,
               int32_t nsweeps, const int32_t* src, const double* start, const double* stop, const double* step,
-              double* xout, int32_t* iters, int32_t* done, double* max_rel) {
+              double* xout, int32_t* iters, int32_t* done, double* max_rel, double* iout) {
   Sim s;
   int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
   if (e != E_OK) return e;
@@ -1062,7 +1065,7 @@ int oracle_dc(int32_t num_nodes, const kh_element* el, int32_t nel, const double
   o.from(ko);
   int el2[2] = {src[0], nsweeps > 1 ? src[1] : -1};
   int d = 0;
-  e = s.dcSweep(nsweeps, el2, start, stop, step, o, xout, iters, &d, max_rel);
+  e = s.dcSweep(nsweeps, el2, start, stop, step, o, xout, iters, &d, max_rel, iout);
   *done = d;
   return e;
 }

@@ -363,3 +363,104 @@ def test_gpu_report_batch_against_oracle():
         assert np.array_equal(rep[c][:, 7], rep0[:, 7])
         for f in range(6):
             assert abs(led[c][f] - led0[f]) <= 1e-9 * max(abs(led0[f]), 1e-15), (c, f)
+
+
+# ---- DC sweeps --------------------------------------------------------------------------------
+SWEEP_NODES = ["0", "in", "out"]
+SWEEP_NAMES = ["V1", "R1", "R2"]
+SWEEP_ELEMS = elems([(V_, 1, 0), (R, 1, 2), (R, 2, 0)])
+
+
+def sweep_arrays(values):
+    """The reference's dcSweep deck (cmd/kronspice/main_test.go:44-50):
+    in = V, out = V/2, every current V/2000 in magnitude."""
+    x = np.array([[v, v / 2, -v / 2000] for v in values])
+    i = np.array([[-v / 2000, v / 2000, v / 2000] for v in values])
+    return x, i
+
+
+def series(name, vals, ind="          "):
+    body = ",\n".join(f"{ind}  {v}" for v in vals)
+    return f'{ind}"{name}": [\n{body}\n{ind}]'
+
+
+def dc_fmt(fmt_, status=0, sweeps=((0, 0.0, 2.0, 1.0),), sources=("V1",), values=(0.0, 1.0, 2.0), max_rel=0.0):
+    x, i = sweep_arrays(values)
+    return api.format_dc(SWEEP_NODES, SWEEP_NAMES, SWEEP_ELEMS, status, 3, list(sweeps), list(sources), len(values),
+                         x, i, max_rel, fmt=fmt_, title="Sweep", library=hostlib()).decode()
+
+
+def test_dc_document_bytes():
+    want = ('{\n  "tool": "kronspice",\n  "version": "dev",\n  "ok": true,\n  "title": "Sweep",\n'
+            '  "analyses": [\n    {\n      "kind": "dc",\n      "dc": {\n'
+            '        "sources": [\n          "V1"\n        ],\n'
+            '        "values": [\n          [\n            0,\n            1,\n            2\n          ]\n        ],\n'
+            '        "v": {\n' + series("in", [0, 1, 2]) + ',\n' + series("out", [0, 0.5, 1]) + '\n        },\n'
+            '        "i": {\n' + series("r1", [0, 0.0005, 0.001]) + ',\n' + series("r2", [0, 0.0005, 0.001]) + ',\n'
+            + series("v1", ["-0", -0.0005, -0.001]) + '\n        },\n'
+            '        "max_relative_residual": 0\n      }\n    }\n  ]\n}\n')
+    got = dc_fmt(api.FORMAT_JSON)
+    assert got == want
+    doc = json.loads(got)["analyses"][0]["dc"]
+    assert doc["v"]["out"] == [0, 0.5, 1]  # the reference's expectation (main_test.go:151-157)
+
+
+def test_dc_csv_bytes():
+    assert dc_fmt(api.FORMAT_CSV) == ("v1,v(in),v(out),i(v1),i(r1),i(r2)\n"
+                                      "0,0,0,-0,0,0\n"
+                                      "1,1,0.5,-0.0005,0.0005,0.0005\n"
+                                      "2,2,1,-0.001,0.001,0.001\n")
+
+
+def test_dc_two_sweeps_values():
+    """Two sources: the inner sweep's values repeat for every outer value,
+    values[1] holds the outer value of each point (analysis/dc.go:104-119)."""
+    vals = [0.0, 0.5, 1.0] * 2
+    out = dc_fmt(api.FORMAT_JSON, sweeps=((0, 0.0, 1.0, 0.5), (0, 3.0, 4.0, 1.0)), sources=("V1", "V2"),
+                 values=vals)
+    doc = json.loads(out)["analyses"][0]["dc"]
+    assert doc["sources"] == ["V1", "V2"]
+    assert doc["values"] == [[0, 0.5, 1, 0, 0.5, 1], [3, 3, 3, 4, 4, 4]]
+    csv = dc_fmt(api.FORMAT_CSV, sweeps=((0, 0.0, 1.0, 0.5), (0, 3.0, 4.0, 1.0)), sources=("V1", "V2"), values=vals)
+    assert csv.splitlines()[0] == "v1,v2,v(in),v(out),i(v1),i(r1),i(r2)" and csv.splitlines()[4].startswith("0,4,")
+
+
+def test_dc_failed_sweep_document():
+    out = dc_fmt(api.FORMAT_JSON, status=api.NOCONV)
+    doc = json.loads(out)
+    assert doc["ok"] is False and doc["error"]["class"] == "no-convergence" and doc["analyses"] == []
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["diode_clipper_dc", "bjt_output_dc", "mos1_idvd_dc", "zener_dc"])
+def test_gpu_dc_currents_and_document(name):
+    """dc(currents=True) on golden sweeps: each point's element currents
+    against the oracle's sweep (the same warm-started points) at 1e-9, and
+    the document's series equal to the arrays it was written from."""
+    from oracle import oracle
+    from tests import golden_runner as gr
+    deck = gr.load_deck(name)
+    c = deck.circuit
+    a = [a for a in deck.analyses if a.kind == "dc"][0]
+    els, p = c.flatten()
+    sw = [(c.element_index(a.args["source"]), a.args["start"], a.args["stop"], a.args["step"])]
+    if a.args.get("source2"):
+        sw.append((c.element_index(a.args["source2"]), a.args["start2"], a.args["stop2"], a.args["step2"]))
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.dc(t, p[:, None], sw, currents=True)
+    st, pts = int(r.status.cpu()[0]), int(r.points.cpu()[0])
+    e0, x0, it0, d0, mr0, i0 = oracle.dc(c.num_nodes, els, p, sw, currents=True)
+    assert st == e0 == 0 and pts == d0
+    cur = r.currents.cpu().numpy()[:, :, 0]
+    x = r.x.cpu().numpy()[:, :, 0]
+    scale = np.maximum(np.abs(i0).max(axis=0), 1e-15)
+    assert (np.abs(cur[:pts] - i0[:pts]) <= 1e-9 * scale).all()
+    names = [e.name for e in c.elements]
+    srcs = [a.args["source_name"]]
+    doc = json.loads(api.format_dc(c.node_names, names, els, st, t.n, sw, srcs, pts, x[:pts], cur[:pts],
+                                   float(r.ledger.cpu()[0]), title=c.title))
+    dcd = doc["analyses"][0]["dc"]
+    for k, e in enumerate(c.elements):
+        assert dcd["i"][e.name.lower()] == cur[:pts, k].tolist()
+    for i, nn in enumerate(c.node_names[1:]):
+        assert dcd["v"][nn] == x[:pts, i].tolist()
