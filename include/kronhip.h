@@ -350,6 +350,16 @@ int64_t kh_format_op(int32_t format, const kh_names* names, const char* version,
  * sweep_i [points*n_elems]: one circuit's slices of kh_batch_dc_ex's d_sweep_x
  * and d_sweep_i, point-major; max_rel: its d_ledger value. Return value and
  * buffer as kh_format_op. */
+/* One circuit's AC sweep: the document with one "ac" analysis (ACDocument,
+ * output/json.go:223-253) or output.WriteACCSV's table (output/ac.go:67-100).
+ * freqs [nf]; xac [nf*n*2], iac [nf*n_elems*2] and ledger_f [nf*6] are one
+ * circuit's slices of kh_batch_ac_ex's d_xac, d_iac and d_ledger_f
+ * (frequency-major); max_rel its d_ledger value. Phases are atan2 of the C
+ * library, within an ulp of Go's math.Atan2. Return value and buffer as
+ * kh_format_op. */
+int64_t kh_format_ac(int32_t format, const kh_names* names, const char* version, kh_status status, int32_t n, int32_t nf,
+                     const double* freqs, const double* xac, const double* iac, const double* ledger_f,
+                     double max_rel, char* buf, int64_t cap);
 int64_t kh_format_dc(int32_t format, const kh_names* names, const char* version, kh_status status, int32_t n,
                      const struct kh_dc_spec* spec, const char* const* sources, int32_t points, const double* sweep_x,
                      const double* sweep_i, double max_rel, char* buf, int64_t cap);
@@ -435,6 +445,23 @@ kh_status kh_batch_dc_ex(kh_topology t, int32_t B, const double* d_params, const
 kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
                       int32_t nf, const double* d_nodeset, double* d_xac, int32_t* d_status, int32_t* d_points,
                       double* d_ledger, kh_stats* stats);
+
+/* kh_batch_ac with ACResult's element currents and per-frequency ledger too
+ * (analysis/ac.go:118-173): d_iac (optional) [nf*n_elems*B*2], frequency-major,
+ * then element, then circuit, (re, im): each element's current phasor (n+ ->
+ * n-); d_ledger_f (optional) [nf*6*B]: per frequency and circuit the average
+ * power inflow, outflow, dissipation, reactive power, model-supplied power
+ * and residual (output.ACLedger). The other outputs are kh_batch_ac's. */
+typedef struct kh_ac_outputs {
+  double* d_xac;       /* [nf*n*B*2] */
+  double* d_iac;       /* optional [nf*n_elems*B*2] */
+  double* d_ledger_f;  /* optional [nf*6*B] */
+  int32_t* d_status;   /* [B] */
+  int32_t* d_points;   /* optional [B] */
+  double* d_ledger;    /* optional [B]: MaxRelativeResidual */
+} kh_ac_outputs;
+kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
+                         int32_t nf, const double* d_nodeset, const kh_ac_outputs* out, kh_stats* stats);
 
 /* analysis.OP for B circuits (analysis/op.go:331-357): dcSolve with the
  * gmin / source / pseudo-transient fallbacks (analysis/newton.go:95-311),

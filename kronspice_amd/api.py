@@ -455,10 +455,17 @@ def dc(topo: Topology, params, sweeps, opts: Optional[Options] = None, nodeset=N
     return r
 
 
-def ac(topo: Topology, params, freqs, opts: Optional[Options] = None, nodeset=None) -> Result:
+class AcOutputs(C.Structure):
+    _fields_ = [("d_xac", C.c_void_p), ("d_iac", C.c_void_p), ("d_ledger_f", C.c_void_p), ("d_status", C.c_void_p),
+                ("d_points", C.c_void_p), ("d_ledger", C.c_void_p)]
+
+
+def ac(topo: Topology, params, freqs, opts: Optional[Options] = None, nodeset=None, currents: bool = False) -> Result:
     """analysis.AC for every circuit at the host frequencies freqs [nf].
     Result.x is a complex128 tensor [nf, n, B]; points [B] frequencies solved;
-    ledger [B] largest ledger residual."""
+    ledger [B] largest ledger residual; with currents, Result.currents
+    [nf, n_elems, B] complex element currents and Result.ledgers [nf, 6, B]
+    (inflow, outflow, dissipation, reactive, model-supplied, residual)."""
     torch = _torch()
     if isinstance(params, np.ndarray):
         params = to_device(params)
@@ -474,9 +481,20 @@ def ac(topo: Topology, params, freqs, opts: Optional[Options] = None, nodeset=No
     o = Options() if opts is None else opts
     s = Stats()
     torch.cuda.synchronize()
-    _check(lib().kh_batch_ac(topo.h, B, _p(params), C.byref(o), _p(f), nf, _p(nodeset), _p(x), _p(st), _p(done),
-                             _p(led), C.byref(s)))
-    return Result(torch.view_as_complex(x), st, ledger=led, stats=s.as_dict(), points=done)
+    if not currents:
+        _check(lib().kh_batch_ac(topo.h, B, _p(params), C.byref(o), _p(f), nf, _p(nodeset), _p(x), _p(st),
+                                 _p(done), _p(led), C.byref(s)))
+        return Result(torch.view_as_complex(x), st, ledger=led, stats=s.as_dict(), points=done)
+    cur = torch.zeros((nf, topo.n_elems, B, 2), dtype=torch.float64, device="cuda")
+    lf = torch.zeros((nf, 6, B), dtype=torch.float64, device="cuda")
+    out = AcOutputs(d_xac=x.data_ptr(), d_iac=cur.data_ptr(), d_ledger_f=lf.data_ptr(), d_status=st.data_ptr(),
+                    d_points=done.data_ptr(), d_ledger=led.data_ptr())
+    _check(lib().kh_batch_ac_ex(topo.h, B, _p(params), C.byref(o), _p(f), nf, _p(nodeset), C.byref(out),
+                                C.byref(s)))
+    r = Result(torch.view_as_complex(x), st, ledger=led, stats=s.as_dict(), points=done)
+    r.currents = torch.view_as_complex(cur)
+    r.ledgers = lf
+    return r
 
 
 class Group(C.Structure):
@@ -610,6 +628,34 @@ def format_dc(node_names, elem_names, elems: np.ndarray, status: int, n: int, sw
         raise KhError(BAD_ARGUMENT, "kh_format_dc: invalid arguments")
     buf = C.create_string_buffer(int(size) + 1)
     L.kh_format_dc(*args, buf, size)
+    return buf.raw[:size]
+
+
+def format_ac(node_names, elem_names, elems: np.ndarray, status: int, n: int, freqs, xac, iac, ledger_f,
+              max_rel: float, fmt: int = FORMAT_JSON, title: str = "", version: Optional[str] = None,
+              library=None) -> bytes:
+    """One circuit's AC sweep as the reference writes it: the document with an
+    "ac" analysis (FORMAT_JSON) or output.WriteACCSV's table (FORMAT_CSV).
+    xac [nf, n] and iac [nf, n_elems] complex, ledger_f [nf, 6] host arrays (one
+    circuit's slices of ac(currents=True))."""
+    L = library if library is not None else lib()
+    L.kh_format_ac.restype = C.c_int64
+    L.kh_format_ac.argtypes = [C.c_int32, C.c_void_p, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int64]
+    nm, keep = _names(node_names, elem_names, elems, title)
+    fr = np.ascontiguousarray(freqs, np.float64)
+    arrs = [None if a is None else np.ascontiguousarray(a) for a in (xac, iac, ledger_f)]
+    for a, t in zip(arrs, (np.complex128, np.complex128, np.float64)):
+        if a is not None and a.dtype != t:
+            raise ValueError(f"format_ac: want {t}, got {a.dtype}")
+    ver = None if version is None else version.encode()
+    args = [fmt, C.byref(nm), ver, status, n, len(fr), fr.ctypes.data] + \
+        [None if a is None else a.ctypes.data for a in arrs] + [max_rel]
+    size = L.kh_format_ac(*args, None, 0)
+    if size < 0:
+        raise KhError(BAD_ARGUMENT, "kh_format_ac: invalid arguments")
+    buf = C.create_string_buffer(int(size) + 1)
+    L.kh_format_ac(*args, buf, size)
     return buf.raw[:size]
 
 

@@ -193,12 +193,22 @@ KH_HD inline void ac_load(ACLane& A, int nel) {
   }
 }
 
+// Fields of one frequency's average-power ledger (output.ACLedger), in this order.
+enum { AL_INFLOW = 0, AL_OUTFLOW, AL_DISSIPATION, AL_REACTIVE, AL_MODEL_SUPPLIED, AL_RESIDUAL, AL_FIELDS };
+
 // Complex power absorbed by every element, summed (sumS) and in magnitude
-// (scaleS): OperatingAC + ACPower (analysis/ac.go:152-169).
-KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS, double* scaleS) {
+// (scaleS): OperatingAC + ACPower (analysis/ac.go:152-169). Optionally each
+// element's current phasor (icur[k * istride]) and the frequency's ledger
+// (led[AL_FIELDS], analysis/ac.go:160-173): the average power Re S drives each
+// element's modeled exterior port (heat out of resistors and junction devices,
+// the sources' power in from the exterior), Im S of the energy-storing
+// elements sums into the reactive power, negative resistances supply Re S.
+KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS, double* scaleS,
+                           kcplx* icur = nullptr, int64_t istride = 1, double* led = nullptr) {
   const Lane& L = A.dev;
   kcplx sum = {0, 0};
   double scale = 0;
+  double inflow = 0, outflow = 0, diss = 0, reactive = 0, sup = 0;
   for (int k = 0; k < nel; k++) {
     const int32_t* e = L.et + (int64_t)k * ET_W;
     kcplx s;
@@ -220,6 +230,7 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
       kcplx ib = ibs + ibd - iqs[2] + iqs[3] + iqs[4];
       s = A.xat(x, L.T(e, 0)) * conj(id) + gg * conj(ig) + A.xat(x, L.T(e, 1)) * conj(is) + bb * conj(ib);
       s = {s.re / 2.0, s.im / 2.0};
+      if (icur) icur[(int64_t)k * istride] = id;
     } else if (e[ET_KIND] == K_Q) {  // acCurrents + ACPower, device/bjt.go:734-765
       BjtAC a = bjt_ac(L, e, A.omega);
       kcplx xb = A.xat(x, L.T(e, 2)), xcp = A.xat(x, L.T(e, 4)), xbp = A.xat(x, L.T(e, 5)), xep = A.xat(x, L.T(e, 6));
@@ -238,6 +249,7 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
       s = A.xat(x, L.T(e, 0)) * conj(ic) + xb * conj(ib) + A.xat(x, L.T(e, 1)) * conj(ie) +
           A.xat(x, L.T(e, 3)) * conj(is);
       s = {s.re / 2.0, s.im / 2.0};
+      if (icur) icur[(int64_t)k * istride] = ic;
     } else {
       kcplx v = A.xat(x, L.T(e, 0)) - A.xat(x, L.T(e, 1));
       kcplx i = {0, 0};
@@ -257,12 +269,41 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
         }
       }
       s = kcplx{0.5 * v.re, 0.5 * v.im} * conj(i);  // 0.5*V times conj(I), in that order
+      if (icur) icur[(int64_t)k * istride] = i;
     }
     sum = sum + s;
     scale += go_hypot(s.re, s.im);
+    if (led) {
+      const int kind = e[ET_KIND];
+      const double pw = s.re;
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) reactive += s.im;
+      double port = 0;
+      int ch = -1;
+      if ((kind == K_R && L.P(e, 0) > 0) || kind == K_D || kind == K_M || kind == K_Q) {
+        port = -pw;
+        ch = 2;
+      } else if (kind == K_V || kind == K_I) {
+        port = -pw;
+        ch = 0;
+      }
+      if (ch >= 0 && port != 0) {  // Accumulator.AddPort
+        if (ch == 2 && port < 0) diss += -port;
+        else if (port > 0) inflow += port;
+        else outflow += -port;
+      }
+      if (kind == K_R && L.P(e, 0) < 0 && pw != 0) sup += -pw;  // AddModelSupplied
+    }
   }
   *sumS = sum;
   *scaleS = scale;
+  if (led) {
+    led[AL_INFLOW] = inflow;
+    led[AL_OUTFLOW] = outflow;
+    led[AL_DISSIPATION] = diss;
+    led[AL_REACTIVE] = reactive;
+    led[AL_MODEL_SUPPLIED] = sup;
+    led[AL_RESIDUAL] = inflow + sup - outflow - diss;
+  }
 }
 
 }  // namespace kh

@@ -494,4 +494,107 @@ int64_t kh_format_dc(int32_t format, const kh_names* nm, const char* version, kh
   return emit(out, buf, cap);
 }
 
+int64_t kh_format_ac(int32_t format, const kh_names* nm, const char* version, kh_status status, int32_t n, int32_t nf,
+                     const double* freqs, const double* xac, const double* iac, const double* ledger_f,
+                     double max_rel, char* buf, int64_t cap) {
+  if (!names_ok(nm) || (format != KH_FORMAT_JSON && format != KH_FORMAT_CSV)) return -1;
+  if (n < nm->num_nodes - 1 || nf < 0 || (nf > 0 && !freqs)) return -1;
+  const bool have = status == KH_OK;
+  if (have && nf > 0 && (!xac || !ledger_f || (nm->n_elems > 0 && !iac))) return -1;
+  const int nel = nm->n_elems;
+  // cmplx.Abs (math.Hypot) and PhaseDeg (cmplx.Phase·180/π, output/ac.go:66)
+  auto mag = [](double re, double im) -> double {
+    double p = std::fabs(re), q = std::fabs(im);
+    if (std::isinf(p) || std::isinf(q)) return HUGE_VAL;
+    if (std::isnan(p) || std::isnan(q)) return NAN;
+    if (p < q) std::swap(p, q);
+    if (p == 0) return 0.0;
+    q = q / p;
+    return p * std::sqrt(1 + q * q);
+  };
+  auto phase = [](double re, double im) { return std::atan2(im, re) * 180 / 3.141592653589793; };
+  auto vre = [&](int k, int node) { return xac[((int64_t)k * n + node - 1) * 2]; };
+  auto vim = [&](int k, int node) { return xac[((int64_t)k * n + node - 1) * 2 + 1]; };
+  auto ire = [&](int k, int e) { return iac[((int64_t)k * nel + e) * 2]; };
+  auto iim = [&](int k, int e) { return iac[((int64_t)k * nel + e) * 2 + 1]; };
+  auto led = [&](int k, int q) { return ledger_f[(int64_t)k * 6 + q]; };
+  if (format == KH_FORMAT_CSV) {  // output.WriteACCSV (output/ac.go:67-100)
+    if (!have) return emit("", buf, cap);
+    std::string out;
+    auto row = [&](const std::vector<std::string>& r) {
+      for (size_t k = 0; k < r.size(); k++) {
+        if (k) out += ',';
+        csv_field(out, r[k]);
+      }
+      out += '\n';
+    };
+    std::vector<std::string> head = {"freq"};
+    for (int i = 1; i < nm->num_nodes; i++) {
+      head.push_back(std::string("vm(") + nm->node_names[i] + ")");
+      head.push_back(std::string("vp(") + nm->node_names[i] + ")");
+    }
+    for (const char* h : {"ledger.inflow", "ledger.dissipation", "ledger.reactive"}) head.push_back(h);
+    row(head);
+    for (int k = 0; k < nf; k++) {
+      std::vector<std::string> r = {go_g(freqs[k], 10)};
+      for (int i = 1; i < nm->num_nodes; i++) {
+        r.push_back(go_g(mag(vre(k, i), vim(k, i)), 10));
+        r.push_back(go_g(phase(vre(k, i), vim(k, i)), 10));
+      }
+      r.push_back(go_g(led(k, 0), 10));
+      r.push_back(go_g(led(k, 2), 10));
+      r.push_back(go_g(led(k, 3), 10));
+      row(r);
+    }
+    return emit(out, buf, cap);
+  }
+  J acdoc;
+  if (have) {  // ACDocument (output/json.go:223-253)
+    acdoc = J::obj();
+    J fr = nf > 0 ? J::arr() : J::null();
+    for (int k = 0; k < nf; k++) fr.push(J::num(freqs[k]));
+    acdoc.add("freq", std::move(fr));
+    auto phasors = [&](auto re, auto im) {
+      J p = J::obj(), m = J::arr(), ph = J::arr();
+      for (int k = 0; k < nf; k++) {
+        m.push(J::num(mag(re(k), im(k))));
+        ph.push(J::num(phase(re(k), im(k))));
+      }
+      p.add("mag", std::move(m));
+      p.add("phase_deg", std::move(ph));
+      return p;
+    };
+    acdoc.add("v", node_map(nm, [&](int i) {
+      return phasors([&](int k) { return vre(k, i); }, [&](int k) { return vim(k, i); });
+    }));
+    std::vector<std::pair<std::string, int>> keys;
+    for (int e = 0; e < nel; e++) keys.emplace_back(lower(nm->elem_names[e]), e);
+    std::stable_sort(keys.begin(), keys.end(), [](const auto& a, const auto& b) {
+      const int c = std::memcmp(a.first.data(), b.first.data(), std::min(a.first.size(), b.first.size()));
+      return c != 0 ? c < 0 : a.first.size() < b.first.size();
+    });
+    J im = J::obj();
+    for (size_t q = 0; q < keys.size(); q++) {
+      if (q + 1 < keys.size() && keys[q + 1].first == keys[q].first) continue;  // a Go map keeps the last
+      const int e = keys[q].second;
+      im.add(keys[q].first, phasors([&](int k) { return ire(k, e); }, [&](int k) { return iim(k, e); }));
+    }
+    acdoc.add("i", std::move(im));
+    J l = J::obj();
+    const char* names[6] = {"inflow", "outflow", "dissipation", "reactive", "model_supplied", "residual"};
+    for (int q = 0; q < 6; q++) {
+      J a = J::arr();
+      for (int k = 0; k < nf; k++) a.push(J::num(led(k, q)));
+      l.add(names[q], std::move(a));
+    }
+    l.add("max_relative_residual", J::num(max_rel));
+    acdoc.add("ledger", std::move(l));
+  }
+  J d = document(nm, version, status, have ? &acdoc : nullptr, "ac");
+  std::string out;
+  put(out, d, 0);
+  out += '\n';
+  return emit(out, buf, cap);
+}
+
 }  // extern "C"

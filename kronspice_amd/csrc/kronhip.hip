@@ -197,6 +197,9 @@ struct AcArgs {
   uint8_t* present;      // [B][nslots] stamped-slot marks (first chunk), or null
   int32_t* lstatus;      // [BL]: -1 skipped (no operating point), else status
   double* lrr;           // [BL] ledger relative residual
+  int32_t nf;            // frequencies of the sweep
+  double* iout;          // optional [nf][nel][B] complex element currents
+  double* lout;          // optional [nf][AL_FIELDS][B] per-frequency ledgers
 };
 
 KH_HD inline ACLane ac_lane(const AcArgs& a, int64_t l) {
@@ -270,7 +273,12 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
   ACLane A = ac_lane(a, l);
   kcplx sumS;
   double scaleS;
-  ac_power(A, a.ckt.nel, lu.x, &sumS, &scaleS);
+  const int64_t cc = l % a.B, ff = a.f0 + l / a.B;
+  kcplx* icur = a.iout ? (kcplx*)a.iout + (ff * a.ckt.nel) * a.B + cc : nullptr;
+  double led[AL_FIELDS];
+  ac_power(A, a.ckt.nel, lu.x, &sumS, &scaleS, icur, a.B, a.lout ? led : nullptr);
+  if (a.lout)
+    for (int q = 0; q < AL_FIELDS; q++) a.lout[(ff * AL_FIELDS + q) * a.B + cc] = led[q];
   // Round-off floor (output.Ledger.Roundoff): 64 eps sum |x_i| (sum_j |A_ij||x_j| + |b_i|)
   kcplx* y = lu.w;  // free after the solve: y[i].re accumulates AbsProduct
   for (int i = 0; i < n; i++) y[(int64_t)i * BL + l] = kcplx{0, 0};
@@ -1937,6 +1945,21 @@ kh_status kh_batch_dc_ex(kh_topology t, int32_t B, const double* d_params, const
 kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
                       int32_t nf, const double* d_nodeset, double* d_xac, int32_t* d_status, int32_t* d_points,
                       double* d_ledger, kh_stats* stats) {
+  kh_ac_outputs out = {};
+  out.d_xac = d_xac;
+  out.d_status = d_status;
+  out.d_points = d_points;
+  out.d_ledger = d_ledger;
+  return kh_batch_ac_ex(t, B, d_params, opt, freqs, nf, d_nodeset, &out, stats);
+}
+
+kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const kh_options* opt, const double* freqs,
+                         int32_t nf, const double* d_nodeset, const kh_ac_outputs* out, kh_stats* stats) {
+  if (!out) return fail(KH_BAD_ARGUMENT, "bad AC arguments");
+  double* d_xac = out->d_xac;
+  int32_t* d_status = out->d_status;
+  int32_t* d_points = out->d_points;
+  double* d_ledger = out->d_ledger;
   if (!t || B <= 0 || !d_params || !freqs || nf <= 0 || !d_xac || !d_status)
     return fail(KH_BAD_ARGUMENT, "bad AC arguments");
   for (int32_t k = 0; k < nf; k++)
@@ -1981,6 +2004,9 @@ kh_status kh_batch_ac(kh_topology t, int32_t B, const double* d_params, const kh
   a.val = val.as<kcplx>();
   a.rhs = rhs.as<kcplx>();
   a.lstatus = lst.as<int32_t>();
+  a.nf = nf;
+  a.iout = out->d_iac;
+  a.lout = out->d_ledger_f;
   hipEvent_t ev[2] = {nullptr, nullptr};
   for (auto& e : ev) KH_HIP(hipEventCreate(&e));
   struct EvGuard {

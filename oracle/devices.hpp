@@ -1959,7 +1959,7 @@ struct Bjt : Device {
     if (ep != u[1]) stampY(m, u[1], ep, cplx(gepr, 0));
   }
   // ACPower, :759-765 (acCurrents :734-751): half the sum of V_k conj(I_k).
-  cplx acPower(const cplx* x, int nx) const {
+  cplx acPower(const cplx* x, int nx, cplx* ic_out = nullptr) const {
     auto cat = [&](int i) { return (i < 0 || i >= nx) ? cplx(0) : x[i]; };
     cplx vbe_ = cat(bp) - cat(ep), vbc_ = cat(bp) - cat(cp), vbx_ = cat(b) - cat(cp), vcs_ = cat(sa) - cat(sb);
     cplx it = cplx(ac.itBE, 0) * vbe_ + cplx(ac.itBC, 0) * vbc_;
@@ -1973,6 +1973,7 @@ struct Bjt : Device {
       ic = it - ibc - ix; ib = ibe + ibc + ix + isb; is = -isb;
     }
     cplx sum = cat(u[0]) * std::conj(ic) + cat(b) * std::conj(ib) + cat(u[1]) * std::conj(ie) + cat(s) * std::conj(is);
+    if (ic_out) *ic_out = ic;  // OperatingAC's current, :767-771
     return cplx(sum.real() / 2, sum.imag() / 2);
   }
 };

@@ -88,6 +88,7 @@ def lib():
         _lib.oracle_lte_step.argtypes = [C.c_int32, C.c_double, C.c_double, P, P, C.c_int32] + [C.c_double] * 7
         _lib.oracle_tran.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P, C.c_int32, P]
         _lib.oracle_ac.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P]
+        _lib.oracle_ac_ex.argtypes = [C.c_int32, P, C.c_int32, P, P, P, C.c_int32, P, P, P, P]
         _lib.oracle_dc.argtypes = [C.c_int32, P, C.c_int32, P, P, C.c_int32, P, P, P, P, P, P, P, P, P]
         _lib.oracle_batch_op.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, C.c_int32,
                                          C.c_int32, P, P, P]
@@ -328,15 +329,25 @@ def dc(num_nodes: int, els: np.ndarray, params: np.ndarray, sweeps, opts: Option
     return e, x, it, done.value, mr.value
 
 
-def ac(num_nodes: int, els: np.ndarray, params: np.ndarray, freqs, opts: Optional[KhOptions] = None):
+def ac(num_nodes: int, els: np.ndarray, params: np.ndarray, freqs, opts: Optional[KhOptions] = None,
+       currents: bool = False):
+    """analysis.AC restatement: (status, X [nf, n] complex, max ledger residual[,
+    element currents [nf, nel] complex, ledgers [nf, 6]: inflow, outflow,
+    dissipation, reactive, model-supplied, residual])."""
     n = size(num_nodes, els, params)
     freqs = np.ascontiguousarray(freqs, np.float64)
     out = np.zeros((len(freqs), n), np.complex128)
     mr = C.c_double()
     o = opts if opts is not None else KhOptions()
-    e = lib().oracle_ac(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(freqs), len(freqs), _p(out),
-                        C.byref(mr))
-    return e, out, mr.value
+    if not currents:
+        e = lib().oracle_ac(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(freqs), len(freqs), _p(out),
+                            C.byref(mr))
+        return e, out, mr.value
+    cur = np.zeros((len(freqs), len(els)), np.complex128)
+    led = np.zeros((len(freqs), 6))
+    e = lib().oracle_ac_ex(num_nodes, _p(els), len(els), _p(params), C.byref(o), _p(freqs), len(freqs), _p(out),
+                           C.byref(mr), _p(cur), _p(led))
+    return e, out, mr.value, cur, led
 
 
 def batch_op(num_nodes: int, els: np.ndarray, params_soa: np.ndarray, n: int, opts=None, threads: int = 1,

@@ -757,7 +757,11 @@ struct Sim {
 
   // AC sweep, analysis/ac.go:76-193: OP + polish, then per frequency LoadAC,
   // complex GP solve and complex-Tellegen check.
-  int acSweep(const double* freqs, int nf, const Options& o, cplx* xout, double* maxRel, int* iters) {
+  // Optionally the element currents iout [nf][nel] and the per-frequency
+  // ledgers lout [nf][6]: inflow, outflow, dissipation, reactive, model-
+  // supplied, residual (analysis/ac.go:118-173).
+  int acSweep(const double* freqs, int nf, const Options& o, cplx* xout, double* maxRel, int* iters,
+              cplx* iout = nullptr, double* lout = nullptr) {
     double lr;
     int e = op(o, iters, &lr, nullptr);
     if (e != E_OK) return e;
@@ -781,7 +785,21 @@ struct Sim {
       std::copy(cx.begin(), cx.end(), xout + (size_t)k * n);
       cplx sumS = 0;
       double scaleS = 0;
-      for (auto& d : devs) {
+      Accumulator acc;
+      double reactive = 0;
+      // the average-power ledger of one element's absorbed complex power s
+      auto book = [&](size_t kd, Device* d, cplx s, cplx cur) {
+        if (iout) iout[(size_t)k * devs.size() + kd] = cur;
+        const int kind = d->kind;
+        if (kind == KH_ELEM_C || kind == KH_ELEM_L || kind == KH_ELEM_D || kind == KH_ELEM_M || kind == KH_ELEM_Q)
+          reactive += s.imag();
+        OperatingPoint op;
+        op.power = s.real();
+        d->ports(op, acc);
+        if (d->modelSupplier()) acc.addModelSupplied(d->modelSupplied(op));
+      };
+      for (size_t kd = 0; kd < devs.size(); kd++) {
+        auto& d = devs[kd];
         cplx v, i;
         if (d->kind == KH_ELEM_M) {
           // ACPower, device/mosfet.go:841-846 via acCurrents :817-832
@@ -804,12 +822,15 @@ struct Sim {
           s /= 2.0;
           sumS += s;
           scaleS += absval(s);
+          book(kd, d.get(), s, id);
           continue;
         }
         if (d->kind == KH_ELEM_Q) {
-          cplx s = static_cast<Bjt*>(d.get())->acPower(cx.data(), n);
+          cplx ic;
+          cplx s = static_cast<Bjt*>(d.get())->acPower(cx.data(), n, &ic);
           sumS += s;
           scaleS += absval(s);
+          book(kd, d.get(), s, ic);
           continue;
         }
         v = d->voltageAC(cx.data());
@@ -835,6 +856,16 @@ struct Sim {
         cplx s = 0.5 * v * std::conj(i);
         sumS += s;
         scaleS += absval(s);
+        book(kd, d.get(), s, i);
+      }
+      if (lout) {
+        double* l = lout + (size_t)k * 6;
+        l[0] = acc.inflow;
+        l[1] = acc.outflow;
+        l[2] = acc.dissipation;
+        l[3] = reactive;
+        l[4] = acc.modelSupplied;
+        l[5] = acc.residual();
       }
       cm.absProduct(cx.data(), ab.data());
       double ro = 0;
@@ -1080,6 +1111,19 @@ int oracle_ac(int32_t num_nodes, const kh_element* el, int32_t nel, const double
   o.from(ko);
   int iters = 0;
   return s.acSweep(freqs, nf, o, reinterpret_cast<cplx*>(xout), max_rel, &iters);
+}
+
+// oracle_ac with ACResult's element currents iout [nf][nel] (re, im) and the
+// per-frequency ledgers lout [nf][6].
+int oracle_ac_ex(int32_t num_nodes, const kh_element* el, int32_t nel, const double* params, const kh_options* ko,
+                 const double* freqs, int32_t nf, double* xout, double* max_rel, double* iout, double* lout) {
+  Sim s;
+  int e = s.build(num_nodes, el, nel, params, ko ? ko->backend : 0, 0, 0);
+  if (e != E_OK) return e;
+  Options o;
+  o.from(ko);
+  int iters = 0;
+  return s.acSweep(freqs, nf, o, reinterpret_cast<cplx*>(xout), max_rel, &iters, reinterpret_cast<cplx*>(iout), lout);
 }
 
 // ---- raw sparse-LU access (solver.Sparse seam) for golden-vector tests -------

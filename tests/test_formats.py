@@ -464,3 +464,109 @@ def test_gpu_dc_currents_and_document(name):
         assert dcd["i"][e.name.lower()] == cur[:pts, k].tolist()
     for i, nn in enumerate(c.node_names[1:]):
         assert dcd["v"][nn] == x[:pts, i].tolist()
+
+
+# ---- AC sweeps --------------------------------------------------------------------------------
+AC_NODES = ["0", "out"]
+AC_NAMES = ["V1", "R1"]
+AC_ELEMS = elems([(V_, 1, 0), (R, 1, 0)])
+AC_FREQS = np.array([10.0, 100.0, 1000.0, 1e4])
+# phasors with exact magnitudes and phases: 0°, 90°, 180°, −90°
+AC_X = np.array([[1 + 0j, -0.001 + 0j], [2j, -0.002j], [-0.5 + 0j, 0.0005 + 0j], [-4j, 0.004j]])  # out, V1 branch
+AC_I = np.array([[-0.001 + 0j, 0.001 + 0j], [-0.002j, 0.002j], [0.0005 + 0j, -0.0005 + 0j], [0.004j, -0.004j]])
+AC_LED = np.array([[0.0005, 0, 0.0005, 0, 0, 0], [0.002, 0, 0.002, 0, 0, 0],
+                   [0.000125, 0, 0.000125, 0, 0, 0], [0.008, 0, 0.008, 0, 0, 0]])
+
+
+def ac_fmt(fmt_, status=0):
+    return api.format_ac(AC_NODES, AC_NAMES, AC_ELEMS, status, 2, AC_FREQS, AC_X, AC_I, AC_LED, 0.0, fmt=fmt_,
+                         title="ac", library=hostlib()).decode()
+
+
+def phasor_json(name, mags, phases, ind="          "):
+    m = ",\n".join(f"{ind}    {v}" for v in mags)
+    p = ",\n".join(f"{ind}    {v}" for v in phases)
+    return (f'{ind}"{name}": {{\n{ind}  "mag": [\n{m}\n{ind}  ],\n'
+            f'{ind}  "phase_deg": [\n{p}\n{ind}  ]\n{ind}}}')
+
+
+def test_ac_document_bytes():
+    ind = "          "
+    led = lambda name, vals: f'{ind}"{name}": [\n' + ",\n".join(f"{ind}  {v}" for v in vals) + f"\n{ind}]"
+    want = ('{\n  "tool": "kronspice",\n  "version": "dev",\n  "ok": true,\n  "title": "ac",\n'
+            '  "analyses": [\n    {\n      "kind": "ac",\n      "ac": {\n'
+            '        "freq": [\n          10,\n          100,\n          1000,\n          10000\n        ],\n'
+            '        "v": {\n' + phasor_json("out", [1, 2, 0.5, 4], [0, 90, 180, -90]) + '\n        },\n'
+            '        "i": {\n' + phasor_json("r1", [0.001, 0.002, 0.0005, 0.004], [0, 90, 180, -90]) + ',\n'
+            + phasor_json("v1", [0.001, 0.002, 0.0005, 0.004], [180, -90, 0, 90]) + '\n        },\n'
+            '        "ledger": {\n'
+            + led("inflow", [0.0005, 0.002, 0.000125, 0.008]) + ',\n'
+            + led("outflow", [0, 0, 0, 0]) + ',\n'
+            + led("dissipation", [0.0005, 0.002, 0.000125, 0.008]) + ',\n'
+            + led("reactive", [0, 0, 0, 0]) + ',\n'
+            + led("model_supplied", [0, 0, 0, 0]) + ',\n'
+            + led("residual", [0, 0, 0, 0]) + ',\n'
+            '          "max_relative_residual": 0\n        }\n      }\n    }\n  ]\n}\n')
+    got = ac_fmt(api.FORMAT_JSON)
+    assert got == want
+
+
+def test_ac_csv_bytes():
+    assert ac_fmt(api.FORMAT_CSV) == ("freq,vm(out),vp(out),ledger.inflow,ledger.dissipation,ledger.reactive\n"
+                                      "10,1,0,0.0005,0.0005,0\n"
+                                      "100,2,90,0.002,0.002,0\n"
+                                      "1000,0.5,180,0.000125,0.000125,0\n"
+                                      "10000,4,-90,0.008,0.008,0\n")
+
+
+def test_ac_reference_expectation_shape():
+    """cmd/kronspice/main_test.go:125-145: nine frequencies of .ac dec 2 10 100k,
+    the corner at index 4 (1 kHz), a dissipation point per frequency."""
+    from kronspice_amd import netlist
+    freqs = netlist.ac_frequencies("dec", 2, 10, 1e5)
+    assert len(freqs) == 9 and abs(freqs[4] - 1000) <= 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["rc_lowpass_ac", "rlc_bandpass_ac", "bjt_ce_ac", "mos1_cs_ac", "mos3_cs_ac"])
+def test_gpu_ac_currents_ledgers_and_document(name):
+    """ac(currents=True) against the oracle's AC sweep: element currents and
+    per-frequency ledgers at 1e-9 (the solutions bit-identical where the
+    operating point involves no transcendental of the device library, as the
+    AC parity tests have it: not for the BJT and level-3 MOSFET decks); the
+    document written from them parses back to the same arrays."""
+    from kronspice_amd import netlist
+    from oracle import oracle
+    from tests import golden_runner as gr
+    deck = gr.load_deck(name)
+    c = deck.circuit
+    a = [a for a in deck.analyses if a.kind == "ac"][0]
+    freqs = np.asarray(netlist.ac_frequencies(a.args["variation"], a.args["points"], a.args["fstart"],
+                                              a.args["fstop"]), np.float64)
+    els, p = c.flatten()
+    t = api.Topology(c.num_nodes, els, p)
+    r = api.ac(t, p[:, None], freqs, currents=True)
+    st = int(r.status.cpu()[0])
+    e0, X0, mr0, cur0, led0 = oracle.ac(c.num_nodes, els, p, freqs, currents=True)
+    assert st == e0 == 0
+    X = r.x.cpu().numpy()[:, :, 0]
+    cur = r.currents.cpu().numpy()[:, :, 0]
+    led = r.ledgers.cpu().numpy()[:, :, 0]
+    if name in ("rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac"):
+        assert np.array_equal(X, X0)
+    else:
+        assert (np.abs(X - X0) <= 1e-9 * np.maximum(np.abs(X0).max(axis=0), 1e-300)).all()
+    sc = np.maximum(np.abs(cur0).max(axis=0), 1e-18)
+    assert (np.abs(cur - cur0) <= 1e-9 * sc).all()
+    # every ledger term against the sweep's largest power (the residual and
+    # tiny reactive terms are round-off themselves)
+    assert (np.abs(led - led0) <= 1e-9 * max(np.abs(led0).max(), 1e-300)).all()
+    names = [e.name for e in c.elements]
+    doc = json.loads(api.format_ac(c.node_names, names, els, st, t.n, freqs, X, cur, np.ascontiguousarray(led),
+                                   float(r.ledger.cpu()[0]), title=c.title))
+    acd = doc["analyses"][0]["ac"]
+    assert acd["freq"] == freqs.tolist()
+    for k, e in enumerate(c.elements):
+        got = np.array(acd["i"][e.name.lower()]["mag"])
+        assert np.allclose(got, np.abs(cur[:, k]), rtol=1e-15, atol=0)
+    assert acd["ledger"]["dissipation"] == led[:, 2].tolist()

@@ -259,3 +259,39 @@ def test_golden_ledgers_balance(name):
     for k, el in enumerate(els):
         if el["kind"] in (6, 7, 8):
             assert rep[k][CH] == 2 and rep[k][EXT] == -(rep[k][P] - rep[k][S])
+
+
+# ---- AC: element currents and the average-power ledger per frequency -------------------------------
+def test_ac_rc_lowpass_currents_and_ledger_closed_forms():
+    """RC low-pass driven by a 1 V AC source (cmd/kronspice/main_test.go:
+    33-39): V_out = 1/(1 + jωRC); the resistor carries (1 − V_out)/R, the
+    capacitor jωC·V_out, the source −(1 − V_out)/R; the source's average power
+    ½·Re(I_R) enters, ½·|I_R|²·R is dissipated, the capacitor's reactive power
+    is −½·ωC·|V_out|², and the residual is round-off (analysis/ac.go:148-173)."""
+    text = """RC low-pass
+V1 in 0 AC 1
+R1 in out 1k
+C1 out 0 159.15n
+.ac dec 2 10 100k
+.end
+"""
+    c = netlist.parse(text).circuit
+    els, p = c.flatten()
+    freqs = np.array([10.0, 1e3, 31622.776601683792, 1e5])
+    e, X, mr, cur, led = oracle.ac(c.num_nodes, els, p, freqs, currents=True)
+    assert e == 0
+    R, Cc = 1e3, 159.15e-9
+    for k, f in enumerate(freqs):
+        w = 2 * math.pi * f
+        vout = 1 / (1 + 1j * w * R * Cc)
+        ir = (1 - vout) / R
+        assert abs(cur[k][1] - ir) <= 1e-12 * abs(ir)
+        assert abs(cur[k][2] - 1j * w * Cc * vout) <= 1e-12 * abs(ir)
+        assert abs(cur[k][0] + ir) <= 1e-12 * abs(ir)
+        inflow, outflow, diss, reactive, sup, res = led[k]
+        # 1 − V_out cancels at low frequency: the solution's round-off is ~1e-12 relative there
+        assert inflow == pytest.approx(0.5 * ir.real, rel=1e-9, abs=0)
+        assert diss == pytest.approx(0.5 * abs(ir) ** 2 * R, rel=1e-9, abs=0)
+        assert reactive == pytest.approx(-0.5 * w * Cc * abs(vout) ** 2, rel=1e-9, abs=0)
+        assert outflow == 0 and sup == 0
+        assert res == inflow + sup - outflow - diss and abs(res) <= 1e-10 * inflow
