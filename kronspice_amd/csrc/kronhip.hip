@@ -87,19 +87,6 @@ __global__ void __launch_bounds__(kBlock) k_lu_mixed(LuArgs a, int32_t* status, 
 
 // FNV-1a over the present slot ids of each waiting circuit (contiguous
 // per-circuit storage).
-// One system on one wavefront (the seams: kh_sys_solve, kh_csys_solve,
-// kh_sys_solve_static): the wave-cooperative factor + solve of kh_wave.h
-// with the working set in LDS; status[0] receives the result.
-template <typename T>
-__global__ void __launch_bounds__(kWaveLanes) k_sys_wave(LuArgsT<T> a, int32_t* status) {
-  extern __shared__ __align__(16) char lds[];
-  WLuT<T> W;
-  wlu_carve(lds, a.n, W);
-  int st = wlu_factor(a, W);
-  if (st == LU_OK) st = wlu_solve(a, W);
-  if (threadIdx.x == 0) status[0] = st;
-}
-
 __global__ void __launch_bounds__(kBlock) k_hash(const uint8_t* present, int32_t nslots, int64_t B,
                                                  const int32_t* ci, uint64_t* hash) {
   int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -591,21 +578,8 @@ kh_status sys_prepare(SysT<T>* s) {
   return KH_OK;
 }
 
-// Largest dynamic LDS a workgroup may use on the current device.
-int64_t lds_optin() {
-  static int64_t v = -1;
-  if (v < 0) {
-    int dev = 0, o = 0;
-    v = (hipGetDevice(&dev) == hipSuccess &&
-         hipDeviceGetAttribute(&o, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess)
-            ? o
-            : 64 * 1024;
-  }
-  return v;
-}
-
-// Launches the factor + solve of one system (B = 1): one lane (k_lu_t), or
-// with wave = true the wavefront form when its working set fits in LDS.
+// The factor + solve of one system (B = 1) on one lane: the seams
+// (kh_sys_solve, kh_csys_solve, kh_sys_solve_static).
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_lu_t(LuArgsT<T> a, int32_t* status) {
   if (threadIdx.x != 0) return;
@@ -614,20 +588,8 @@ __global__ void __launch_bounds__(kBlock) k_lu_t(LuArgsT<T> a, int32_t* status) 
   status[0] = st;
 }
 template <typename T>
-hipError_t launch_sys(const LuArgsT<T>& a, int32_t* status, hipStream_t st, bool wave = false) {
-  const int64_t lds = wlu_lds_bytes(a.n, sizeof(T));
-  if (wave && lds <= lds_optin()) {
-    static int64_t opted = 64 * 1024;
-    if (lds > opted) {
-      const hipError_t e =
-          hipFuncSetAttribute((const void*)k_sys_wave<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-      if (e != hipSuccess) return e;
-      opted = lds;
-    }
-    hipLaunchKernelGGL(k_sys_wave<T>, dim3(1), dim3(kWaveLanes), (size_t)lds, st, a, status);
-  } else {
-    hipLaunchKernelGGL(k_lu_t<T>, dim3(1), dim3(kBlock), 0, st, a, status);
-  }
+hipError_t launch_sys(const LuArgsT<T>& a, int32_t* status, hipStream_t st) {
+  hipLaunchKernelGGL(k_lu_t<T>, dim3(1), dim3(kBlock), 0, st, a, status);
   return hipGetLastError();
 }
 
