@@ -150,13 +150,15 @@ def bench_ac(args, log):
     torch.cuda.synchronize()
     systems = b.B * len(freqs)
 
-    def timed(prec):
-        opts = api.Options(precision=prec)
+    def timed(prec, sched):
+        opts = api.Options(precision=prec, no_schedule=0 if sched else 1)
         r = None
         for _ in range(args.warmup):
             r = api.ac(t, d_params, freqs, opts=opts)
         torch.cuda.synchronize()
-        tot = {"lu_seconds": 0.0, "bytes_algorithmic": 0.0, "lu_launches": 0.0, "refine_passes": 0.0, "fp64_redos": 0.0}
+        tot = {"lu_seconds": 0.0, "bytes_algorithmic": 0.0, "lu_launches": 0.0, "refine_passes": 0.0, "fp64_redos": 0.0,
+               "sched_solves": 0.0, "sched_redos": 0.0, "sched_probes": 0.0, "sched_rounds": 0.0,
+               "sched_host_seconds": 0.0}
         t0 = time.perf_counter()
         for _ in range(args.steps):
             r = api.ac(t, d_params, freqs, opts=opts)
@@ -165,14 +167,16 @@ def bench_ac(args, log):
         torch.cuda.synchronize()
         return r, (time.perf_counter() - t0) / args.steps, tot
 
-    # Plain form: complex128 LU; the other form: complex64 factors with
-    # complex128 refinement. The faster one is the chosen form.
-    forms = {p: timed(p) for p in (0, 1)}
-    prec = min(forms, key=lambda p: forms[p][1])
-    r, dt, tot = forms[prec]
+    # Plain form: complex128 LU, every system alone on its lane. The others:
+    # complex128 LU in lockstep along a compiled schedule (kh_sched.h), and
+    # complex64 factors with complex128 refinement. The fastest is chosen.
+    forms = {"plain": timed(0, False), "sched": timed(0, True), "c64": timed(1, False)}
+    name = min(forms, key=lambda k: forms[k][1])
+    prec = 1 if name == "c64" else 0
+    r, dt, tot = forms[name]
     lu_s, bytes_alg, launches = tot["lu_seconds"], tot["bytes_algorithmic"], tot["lu_launches"]
-    log(f"rlc grid {G}x{G}: n={b.n}, {systems} systems: complex128 {forms[0][1]:.3f} s, "
-        f"complex64+refinement {forms[1][1]:.3f} s")
+    log(f"rlc grid {G}x{G}: n={b.n}, {systems} systems: complex128 alone {forms['plain'][1]:.3f} s, "
+        f"complex128 scheduled {forms['sched'][1]:.3f} s, complex64+refinement {forms['c64'][1]:.3f} s")
     x = r.x.cpu().numpy()
     st = r.status.cpu().numpy()
     # Parity and CPU baseline: the oracle's sweep of every draw at 16 of the frequencies.
@@ -204,11 +208,14 @@ def bench_ac(args, log):
         "config": {"workload": f"rlc_grid_ac: {G}x{G} RLC mesh (series R-L per edge, C per node), unit AC current "
                                f"at one corner, 16 draws x 1024 frequencies 1 kHz-10 GHz",
                    "unknowns": b.n, "systems": systems,
-                   "form": f"chosen: {'complex128 GP LU' if prec == 0 else 'complex64 GP LU + complex128 refinement'}, "
-                           "one lane per system",
+                   "form": "chosen: " + {"plain": "complex128 GP LU, each system alone on its lane",
+                                         "sched": "complex128 GP LU in lockstep along a compiled schedule, "
+                                                  "lane per system",
+                                         "c64": "complex64 GP LU + complex128 refinement, lane per system"}[name],
                    "parallelism": "lane per (draw, frequency) system, 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_ac_solve",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_ac_sched + k_ac_solve" if name == "sched" else "k_ac_solve",
                      "avg_launch_us": lu_s / max(launches, 1) * 1e6,
                      "bytes_per_launch": bytes_alg / max(launches, 1)},
         "cpu_baseline": {"value": len(ok) / tcpu, "unit": "complex systems/s", "cores": threads, "kind": "port",
@@ -216,11 +223,18 @@ def bench_ac(args, log):
                                    f"operating point included) ({tcpu:.2f} s)"},
         "parity": {"sample": len(ok), "solved_fraction": solved, "bit_identical_fraction": float(np.mean(exact)),
                    "max_rel_err": float(max(rels))},
-        "forms": {"plain_c128_gpu": {"systems_per_s": systems / forms[0][1], "seconds": forms[0][1]},
-                  "c64_refined": {"systems_per_s": systems / forms[1][1], "seconds": forms[1][1],
-                                  "refine_passes": forms[1][2]["refine_passes"] / args.steps,
-                                  "c128_redos": forms[1][2]["fp64_redos"] / args.steps},
-                  "chosen_vs_plain": forms[0][1] / dt},
+        "forms": {"plain_c128_gpu": {"systems_per_s": systems / forms["plain"][1], "seconds": forms["plain"][1]},
+                  "c128_scheduled": {"systems_per_s": systems / forms["sched"][1], "seconds": forms["sched"][1],
+                                     "lockstep_systems": forms["sched"][2]["sched_solves"] / args.steps,
+                                     "redone_alone": forms["sched"][2]["sched_redos"] / args.steps,
+                                     "probes": forms["sched"][2]["sched_probes"] / args.steps,
+                                     "rounds": forms["sched"][2]["sched_rounds"] / args.steps,
+                                     "probe_host_seconds": forms["sched"][2]["sched_host_seconds"] / args.steps,
+                                     "kernel_seconds": forms["sched"][2]["lu_seconds"] / args.steps},
+                  "c64_refined": {"systems_per_s": systems / forms["c64"][1], "seconds": forms["c64"][1],
+                                  "refine_passes": forms["c64"][2]["refine_passes"] / args.steps,
+                                  "c128_redos": forms["c64"][2]["fp64_redos"] / args.steps},
+                  "chosen": name, "chosen_vs_plain": forms["plain"][1] / dt},
     }
     print(json.dumps(out))
 

@@ -147,6 +147,11 @@ typedef struct kh_options {
                                (k_wave; FP64 LU: precision is not applied). Switched off in this
                                build: 1 returns KH_UNSUPPORTED (its first device run did not
                                finish); the CPU emulation of its device code checks it */
+  int32_t no_schedule;      /* AC sweeps: 0 -> the systems of a pattern that follow one probe system's
+                               pivot rows are factored in lockstep along that factorization's
+                               compiled structure (bit-identical; any system whose own pivot rule
+                               departs from it is refactored alone); 1 -> every system alone.
+                               precision 1 and the dense backend do not use schedules */
 } kh_options;
 
 /* Transient request: analysis.TranSpec + TranOptions (analysis/tran.go:26-59). */
@@ -185,6 +190,13 @@ typedef struct kh_stats {
   /* staging 1: LDS bytes per circuit of the staged arrays (largest over groups, last launch), and the
      groups of the last launch that stage only part of their arrays (the rest stays in device memory) */
   int64_t staged_bytes, staged_partial;
+  /* AC sweeps: systems solved along a compiled schedule, and systems that tried it and were
+     refactored alone because their pivot rule departed from it (or a check failed) */
+  int64_t sched_solves, sched_redos;
+  /* ... and the probe systems factored on the host into schedules, the lockstep rounds run, and
+     the host time spent on the probes (gathering their values and factoring them) */
+  int64_t sched_probes, sched_rounds;
+  double  sched_host_seconds;
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/

@@ -47,7 +47,8 @@ class Options(C.Structure):
                 # Options.Backend: 0 auto (dense below 30 unknowns), 1 dense, 2 sparse
                 ("backend", C.c_int32),
                 # sparse LU arithmetic: 0 FP64, 1 FP32 factors + FP64 refinement
-                ("precision", C.c_int32), ("circuit_wave", C.c_int32)]
+                ("precision", C.c_int32), ("circuit_wave", C.c_int32),
+                ("no_schedule", C.c_int32)]
 
 
 class TranSpec(C.Structure):
@@ -66,7 +67,9 @@ class Stats(C.Structure):
                 ("refine_passes", C.c_int64), ("fp64_redos", C.c_int64),
                 ("cycles_stamp", C.c_double), ("cycles_converge", C.c_double), ("cycles_lte", C.c_double),
                 ("cycles_accept", C.c_double), ("cycles_ledger", C.c_double),
-                ("staged_bytes", C.c_int64), ("staged_partial", C.c_int64)]
+                ("staged_bytes", C.c_int64), ("staged_partial", C.c_int64),
+                ("sched_solves", C.c_int64), ("sched_redos", C.c_int64), ("sched_probes", C.c_int64),
+                ("sched_rounds", C.c_int64), ("sched_host_seconds", C.c_double)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -17,7 +17,11 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
+#include <atomic>
 #include <map>
+#include <thread>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -30,6 +34,7 @@
 #include "kh_group.h"
 #include "kh_persist.h"
 #include "kh_report.h"
+#include "kh_sched.h"
 #include "kh_symbolic.h"
 #include "kh_wave.h"
 
@@ -168,8 +173,10 @@ __global__ void __launch_bounds__(64) k_report_ledger(const Group g, const doubl
 }
 
 // ---- AC systems (kh_ac.h) ----------------------------------------------------------
-// Lane l of a chunk of AC systems is frequency f0 + l / B of circuit l % B:
-// consecutive lanes are consecutive circuits at one frequency.
+// Lane l of a chunk of AC systems is frequency f0 + l % nfc of circuit
+// l / nfc: consecutive lanes are one circuit's consecutive frequencies (whose
+// systems tend to share pivot rows: the compiled schedules, kh_sched.h, run a
+// workgroup of them in lockstep).
 struct AcArgs {
   Circ ckt;
   const double* par;     // [B][npar] per-circuit parameters
@@ -178,6 +185,7 @@ struct AcArgs {
   int32_t npar, nstate;
   int64_t B, BL;         // circuits; lanes of this chunk
   int32_t f0;            // first frequency of the chunk
+  int32_t nfc;           // frequencies in the chunk (BL = B * nfc)
   const double* freqs;   // [nf]
   kcplx* val;            // [nslots][BL]
   kcplx* rhs;            // [n][BL]
@@ -190,8 +198,8 @@ struct AcArgs {
 };
 
 KH_HD inline ACLane ac_lane(const AcArgs& a, int64_t l) {
-  const int64_t c = l % a.B;
-  const int32_t f = a.f0 + (int32_t)(l / a.B);
+  const int64_t c = l / a.nfc;
+  const int32_t f = a.f0 + (int32_t)(l % a.nfc);
   ACLane A;
   A.dev = Lane{};
   A.dev.n = a.ckt.n;
@@ -213,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
   if (l >= a.BL) return;
   for (int s = 0; s < a.ckt.nslots; s++) a.val[(int64_t)s * a.BL + l] = kcplx{0, 0};
   for (int i = 0; i < a.ckt.n; i++) a.rhs[(int64_t)i * a.BL + l] = kcplx{0, 0};
-  const int64_t c = l % a.B;
+  const int64_t c = l / a.nfc;
   if (a.ci[c * NCI + CI_STATUS] != ST_OK) {
     a.lstatus[l] = -1;
     return;
@@ -230,13 +238,18 @@ __global__ void __launch_bounds__(kBlock) k_ac_stamp(AcArgs a) {
 // precision 1: complex64 factors with complex128 refinement (kh_lu.h), the
 // complex128 path for every system it cannot certify; mixl[l] receives the
 // lane's complex64 solves, -1 where it was redone (summed on the host).
+// lock (or null): the lanes' solutions are already in lu.x — 1 where the
+// compiled schedules solved the lane, else lustat[l] is the per-lane LU's
+// status.
 __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu, double ledger_tol,
                                                      int32_t skip_energy, double* xout, int32_t precision,
-                                                     int32_t* mixl) {
+                                                     int32_t* mixl, const int32_t* lock, const int32_t* lustat) {
   const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (l >= a.BL || a.lstatus[l] < 0) return;
   int st;
-  if (a.ckt.dense) {
+  if (lock) {
+    st = lock[l] ? (int)LU_OK : lustat[l];
+  } else if (a.ckt.dense) {
     st = lu_dense(lu, l, a.ckt.slot_i, a.ckt.slot_j);
   } else {
     st = LU_UNSURE;
@@ -260,29 +273,32 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
   ACLane A = ac_lane(a, l);
   kcplx sumS;
   double scaleS;
-  const int64_t cc = l % a.B, ff = a.f0 + l / a.B;
+  const int64_t cc = l / a.nfc, ff = a.f0 + l % a.nfc;
   kcplx* icur = a.iout ? (kcplx*)a.iout + (ff * a.ckt.nel) * a.B + cc : nullptr;
   double led[AL_FIELDS];
   ac_power(A, a.ckt.nel, lu.x, &sumS, &scaleS, icur, a.B, a.lout ? led : nullptr);
   if (a.lout)
     for (int q = 0; q < AL_FIELDS; q++) a.lout[(ff * AL_FIELDS + q) * a.B + cc] = led[q];
-  // Round-off floor (output.Ledger.Roundoff): 64 eps sum |x_i| (sum_j |A_ij||x_j| + |b_i|)
-  kcplx* y = lu.w;  // free after the solve: y[i].re accumulates AbsProduct
-  for (int i = 0; i < n; i++) y[(int64_t)i * BL + l] = kcplx{0, 0};
-  for (int k = 0; k < a.ckt.nslots; k++) {  // row by row for the dense backend (solver/cdense.go:57-67)
-    const int s = a.ckt.order ? a.ckt.order[k] : k;
-    const int i = a.ckt.slot_i[s], j = a.ckt.slot_j[s];
-    y[(int64_t)i * BL + l].re += lu_mag(a.val[(int64_t)s * BL + l]) * lu_mag(lu.x[(int64_t)j * BL + l]);
-  }
+  // Round-off floor (output.Ledger.Roundoff): 64 eps sum |x_i| (sum_j |A_ij||x_j| + |b_i|).
+  // AbsProduct row by row (ckt.ro_ptr / ro_slot: a row's slots in slot order
+  // for CSparse, solver/sparse.go:519-527, in column order for CDense,
+  // solver/cdense.go:57-67), each row's sum kept in a register: the same
+  // additions in the same order as accumulating over all slots.
   double ro = 0;
-  for (int i = 0; i < n; i++)
-    ro += lu_mag(lu.x[(int64_t)i * BL + l]) * (y[(int64_t)i * BL + l].re + lu_mag(a.rhs[(int64_t)i * BL + l]));
+  for (int i = 0; i < n; i++) {
+    double acc = 0;
+    for (int p = a.ckt.ro_ptr[i]; p < a.ckt.ro_ptr[i + 1]; p++) {
+      const int s = a.ckt.ro_slot[p];
+      acc += lu_mag(a.val[(int64_t)s * BL + l]) * lu_mag(lu.x[(int64_t)a.ckt.slot_j[s] * BL + l]);
+    }
+    ro += lu_mag(lu.x[(int64_t)i * BL + l]) * (acc + lu_mag(a.rhs[(int64_t)i * BL + l]));
+  }
   ro *= 64 * 0x1p-52;
   const double mag = go_hypot(sumS.re, sumS.im);
   const double rr = (scaleS > 0 && mag > ro) ? mag / scaleS : 0.0;
   a.lrr[l] = rr;
   a.lstatus[l] = (!skip_energy && rr > ledger_tol) ? ST_LEDGER : ST_OK;
-  const int64_t c = l % a.B, f = a.f0 + l / a.B;
+  const int64_t c = l / a.nfc, f = a.f0 + l % a.nfc;
   for (int i = 0; i < n; i++) {
     const kcplx v = lu.x[(int64_t)i * BL + l];
     const int64_t o = ((f * n + i) * a.B + c) * 2;
@@ -291,13 +307,73 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
   }
 }
 
+// Lockstep factor + solve along compiled schedules (kh_sched.h): workgroup w
+// runs its lanes along views[wg_view[w]] (-1: none this round); lock[l] = 1
+// where a lane succeeded, else dep[l] is the column at which its pivot rule
+// departed from the schedule (-2: a check the per-lane LU must report). A lane
+// already locked, without an operating point or of another pattern is skipped.
+// compact: the working columns live in LDS, mmax slots per lane
+// (sched_lu_lds); else in device memory (sched_lu).
+__global__ void __launch_bounds__(kBlock) k_ac_sched(AcArgs a, LuArgsT<kcplx> lu, const SchedView* views,
+                                                     const int32_t* wg_view, int32_t* lock, int32_t* dep,
+                                                     int32_t* dpiv, int32_t compact) {
+  extern __shared__ __align__(16) char lds[];
+  const int32_t v = __builtin_amdgcn_readfirstlane(wg_view[blockIdx.x]);  // uniform: one per workgroup
+  if (v < 0) return;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= a.BL || a.lstatus[l] < 0 || lock[l]) return;
+  const SchedView s = ((SchedPtr<SchedView>)views)[v];
+  if (lu.pid[l] != s.pid) return;
+  int d = -2, dp = -1, st;
+  const SchedLane<kcplx> P = sched_lane_blocked(lu, l, (int)blockDim.x);
+  if (compact) {
+    using LdsD = __attribute__((address_space(3))) double*;
+    st = sched_lu_lds(s, lu, l, P, (LdsD)(double*)lds, (int)blockDim.x, (int)threadIdx.x, &d, &dp);
+  } else {
+    st = sched_lu(s, lu, l, P, &d, &dp);
+  }
+  if (st == LU_OK) {
+    lock[l] = 1;
+  } else {
+    dep[l] = d;
+    dpiv[l] = dp;
+  }
+}
+
+// dst[r * m + i] = src[r * stride + lanes[i]] for r < rows, i < m: rows of
+// the listed lanes of a structure-of-arrays batch, packed.
+__global__ void __launch_bounds__(256) k_gather_soa(const kcplx* src, int64_t stride, int64_t rows,
+                                                    const int64_t* lanes, int64_t m, kcplx* dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= rows * m) return;
+  const int64_t r = t / m, i = t % m;
+  dst[t] = src[r * stride + lanes[i]];
+}
+
+// The per-lane LU (lu_factor + lu_solve) of m packed lanes (lc, structure of
+// arrays over i < m), lanes[i] being each one's lane in the chunk: its status
+// into lustat[lanes[i]] and, on success, its solution into x[:, lanes[i]].
+__global__ void __launch_bounds__(kBlock) k_ac_lane_lu(LuArgsT<kcplx> lc, const int64_t* lanes, int64_t m,
+                                                       kcplx* x, int64_t BL, int32_t* lustat) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int st = lu_factor(lc, i);
+  if (st == LU_OK) st = lu_solve(lc, i);
+  const int64_t l = lanes[i];
+  lustat[l] = st;
+  if (st == LU_OK)
+    for (int k = 0; k < lc.n; k++) x[(int64_t)k * BL + l] = lc.x[(int64_t)k * m + i];
+}
+
+constexpr int kAcLaneBlock = 16;  // lanes per workgroup of k_ac_stamp and k_ac_solve on a chunk
+
 // Per circuit, walks the chunk's frequencies in order as the sweep does: the
 // first failing frequency sets the circuit's status and stops it.
 __global__ void __launch_bounds__(256) k_ac_reduce(AcArgs a, int32_t* done, int32_t* status, int32_t* points,
                                                    double* maxrr) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= a.B || done[c]) return;
-  for (int64_t l = c; l < a.BL; l += a.B) {
+  for (int64_t l = c * a.nfc; l < (c + 1) * a.nfc; l++) {
     const int s = a.lstatus[l];
     if (s < 0) { done[c] = 1; return; }  // the operating point failed: its status stands
     if (s == ST_OK || s == ST_LEDGER) maxrr[c] = fmax(maxrr[c], a.lrr[l]);
@@ -426,6 +502,416 @@ struct DevPatterns {
     return hipSuccess;
   }
 };
+
+// Compiled schedules (kh_sched.h) of the patterns of a pool, several per
+// pattern (one per pivot sequence met), each uploaded to its own device
+// buffer, with the device array of their views that k_ac_sched indexes.
+struct DevScheds {
+  struct One {
+    DBuf buf;
+    SchedView v;
+    int64_t circuit = 0;  // the probe's circuit and frequency (where the schedule is likely to fit)
+    int32_t freq = 0;
+    uint64_t hash = 0;
+    int64_t nl = 0, nu = 0;  // entries of L and U
+  };
+  DBuf lx, ux;  // the lanes' L and U values, [entries][lanes], grown to what a round's schedules need
+  int64_t cap_lu = 0, cap_lanes = 0;
+  std::vector<std::unique_ptr<One>> all;
+  DBuf dviews;
+  size_t uploaded = 0;
+  static uint64_t hash_of(const SchedHost& h) {  // FNV-1a over the structure
+    uint64_t x = 1469598103934665603ull ^ (uint64_t)h.pid;
+    auto mix = [&](const std::vector<int32_t>& v) {
+      for (int32_t e : v) x = (x ^ (uint32_t)e) * 1099511628211ull;
+      x = (x ^ v.size()) * 1099511628211ull;
+    };
+    mix(h.prow); mix(h.lp); mix(h.li); mix(h.up);
+    return x;
+  }
+  // Index of the schedule with h's structure, adding it (uploaded) if new.
+  hipError_t add(const SchedHost& h, const PatDesc& pd, const int32_t* dpool, int64_t circuit, int32_t freq,
+                 int32_t* index) {
+    const uint64_t hh = hash_of(h);
+    for (size_t i = 0; i < all.size(); i++)
+      if (all[i]->hash == hh && all[i]->v.pid == h.pid) {
+        *index = (int32_t)i;
+        return hipSuccess;
+      }
+    auto o = std::make_unique<One>();
+    std::vector<int32_t> pk;
+    auto put = [&](const std::vector<int32_t>& v) {
+      const size_t at = pk.size();
+      pk.insert(pk.end(), v.begin(), v.end());
+      while (pk.size() % 4) pk.push_back(0);  // 16-byte alignment of every array
+      return at;
+    };
+    const size_t o_prow = put(h.prow), o_lp = put(h.lp), o_li = put(h.li), o_up = put(h.up), o_dc = put(h.dc),
+                 o_tb = put(h.tbase), o_tp = put(h.tpos), o_ap = put(h.apos), o_ds = put(h.dslot);
+    const size_t o_ue = pk.size();
+    pk.resize(o_ue + 4 * h.ue.size());
+    if (!h.ue.empty()) std::memcpy(pk.data() + o_ue, h.ue.data(), h.ue.size() * sizeof(SchedU));
+    hipError_t e = o->buf.alloc(pk.size() * 4);
+    if (e != hipSuccess) return e;
+    e = hipMemcpy(o->buf.p, pk.data(), pk.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    const int32_t* d = o->buf.as<int32_t>();
+    o->v.n = h.n;
+    o->v.pid = h.pid;
+    o->v.prow = (SchedPtr<int32_t>)(d + o_prow);
+    o->v.lp = (SchedPtr<int32_t>)(d + o_lp);
+    o->v.li = (SchedPtr<int32_t>)(d + o_li);
+    o->v.up = (SchedPtr<int32_t>)(d + o_up);
+    o->v.dc = (SchedPtr<int32_t>)(d + o_dc);
+    o->v.mmax = h.mmax;
+    o->v.tbase = (SchedPtr<int32_t>)(d + o_tb);
+    o->v.tpos = (SchedPtr<int32_t>)(d + o_tp);
+    o->v.apos = (SchedPtr<int32_t>)(d + o_ap);
+    o->v.dslot = (SchedPtr<int32_t>)(d + o_ds);
+    o->v.ue = (SchedPtr<SchedU>)(d + o_ue);
+    o->v.ap = (SchedPtr<int32_t>)(dpool + pd.ap);
+    o->v.arow = (SchedPtr<int32_t>)(dpool + pd.arow);
+    o->v.aslot = (SchedPtr<int32_t>)(dpool + pd.aslot);
+    o->v.q = (SchedPtr<int32_t>)(dpool + pd.q);
+    o->v.qinv = (SchedPtr<int32_t>)(dpool + pd.qinv);
+    o->circuit = circuit;
+    o->freq = freq;
+    o->hash = hh;
+    o->nl = (int64_t)h.li.size();
+    o->nu = (int64_t)h.ue.size();
+    *index = (int32_t)all.size();
+    all.push_back(std::move(o));
+    return hipSuccess;
+  }
+  hipError_t sync() {  // the views array k_ac_sched reads
+    if (uploaded == all.size()) return hipSuccess;
+    std::vector<SchedView> v;
+    for (const auto& o : all) v.push_back(o->v);
+    hipError_t e = dviews.alloc(v.size() * sizeof(SchedView));
+    if (e == hipSuccess) e = hipMemcpy(dviews.p, v.data(), v.size() * sizeof(SchedView), hipMemcpyHostToDevice);
+    if (e == hipSuccess) uploaded = all.size();
+    return e;
+  }
+};
+
+// Lanes per workgroup of k_ac_sched: 16 (a complex128 access of 16 lanes is
+// two whole cache lines), fewer when the chunk would otherwise not give every
+// SIMD a wavefront (the lockstep chain is latency-bound: wavefronts in flight
+// beat fuller ones) or when the compact working columns of mmax slots per
+// lane would not fit kSchedLds, at least 4; KH_SCHED_BLOCK overrides it.
+// LDS a workgroup of k_ac_sched may hold for its compact working columns.
+constexpr int64_t kSchedLds = 40 * 1024;
+
+int sched_block(int64_t lanes, int64_t mmax) {
+  if (const char* e = std::getenv("KH_SCHED_BLOCK")) {
+    const int b = std::atoi(e);
+    if (b >= 1 && b <= 64 && (b & (b - 1)) == 0) return b;
+  }
+  int dev = 0, ncu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu <= 0)
+    ncu = 256;
+  // the widest block whose compact working columns fit the LDS budget
+  int fit = 16;
+  while (fit > 4 && mmax * fit * (int64_t)sizeof(kcplx) > kSchedLds) fit /= 2;
+  for (int b : {16, 8})
+    if (b <= fit && lanes >= (int64_t)b * 4 * ncu) return b;
+  return std::min(4, fit);
+}
+
+// Gathers lanes[k]'s nslots values (structure-of-arrays [nslots][stride]) into
+// dst[k][nslots].
+__global__ void __launch_bounds__(256) k_gather_lanes(const kcplx* src, int64_t stride, int64_t nslots,
+                                                      const int64_t* lanes, int32_t nl, kcplx* dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslots * nl) return;
+  const int64_t k = i / nslots, s = i % nslots;
+  dst[i] = src[s * stride + lanes[k]];
+}
+
+// One chunk of AC lanes through compiled schedules.
+//  * Probes: systems factored on the host (in parallel, kh_sched.h
+//    build_schedule) into schedules; identical structures are kept once.
+//  * Rounds: every workgroup with lanes left is given the untried schedule
+//    whose probe is nearest (same circuit first, then frequency) and runs them
+//    in lockstep. Before round 0 each circuit's frequencies are bisected with
+//    probes (pivot sequences change in bands of frequency); a later round's
+//    probes are one member of each run of lanes that departed together (same
+//    circuit, schedule, column and pivot row, consecutive frequencies).
+//  * Solo round: once at most kSolo lanes are left, each is its own probe and
+//    runs alone (one lane per workgroup), all in one pass.
+// lock[l] = 1 for the lanes solved; `left` receives the lanes with an
+// operating point that were not (the per-lane LU's).
+struct SchedChunk {
+  int64_t ok = 0, left = 0, probes = 0, rounds = 0;
+  double kernel_ms = 0, host_s = 0;
+};
+
+kh_status sched_chunk(const AcArgs& a, const LuArgsT<kcplx>& lu, DevScheds& S, const DevPatterns& pats,
+                      const std::vector<int32_t>& lp, const std::vector<int32_t>& wst, int n, int nslots, int64_t cap,
+                      int32_t* d_lock, int32_t* d_dep, int32_t* d_dpiv, hipEvent_t ev[2], SchedChunk& out,
+                      std::vector<int64_t>& left) {
+  constexpr int kRounds = 4, kFirstProbes = 9, kMaxProbes = 1024, kSolo = 4096;
+  constexpr int64_t kBisect = 8;
+  int64_t lds_total = 0;  // LDS of the whole device
+  {
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) == hipSuccess)
+      lds_total = (int64_t)ncu * per_cu;
+  }
+  const int64_t BL = a.BL, nfc = a.nfc;
+  std::vector<int32_t> hl(BL, 0), hd(BL, -1), hp(BL, -1);
+  std::vector<int8_t> refused(BL, 0);  // a check other than the pivot rule failed: per-lane LU
+  KH_HIP(hipMemsetAsync(d_lock, 0, (size_t)BL * 4, 0));
+  auto open = [&](int64_t l) { return wst[l / nfc] == ST_OK && !hl[l] && !refused[l]; };
+  // Factors the probe lanes on the host; idx[i] = schedule of probes[i] (-1: did not factor).
+  auto factor_probes = [&](const std::vector<int64_t>& probes, std::vector<int32_t>& idx) -> kh_status {
+    idx.assign(probes.size(), -1);
+    if (probes.empty()) return KH_OK;
+    const auto th0 = std::chrono::steady_clock::now();
+    const int np = (int)probes.size();
+    DBuf dl, dv;
+    KH_HIP(dl.alloc((size_t)np * 8));
+    KH_HIP(dv.alloc((size_t)np * nslots * sizeof(kcplx)));
+    KH_HIP(hipMemcpy(dl.p, probes.data(), (size_t)np * 8, hipMemcpyHostToDevice));
+    const int64_t tot = (int64_t)np * nslots;
+    hipLaunchKernelGGL(k_gather_lanes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, a.val, BL,
+                       (int64_t)nslots, dl.as<int64_t>(), np, dv.as<kcplx>());
+    KH_HIP(hipGetLastError());
+    std::vector<kcplx> pv((size_t)tot);
+    KH_HIP(hipMemcpy(pv.data(), dv.p, (size_t)tot * sizeof(kcplx), hipMemcpyDeviceToHost));
+    std::vector<SchedHost> hs(np);
+    std::vector<char> built(np, 0);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int i; (i = next++) < np;)
+        built[i] = build_schedule<kcplx>(pats.host.desc, pats.host.pool, lp[probes[i]], n, nslots,
+                                         pv.data() + (size_t)i * nslots, cap, hs[i]);
+    };
+    const int nt = std::max(1, std::min<int>(np, (int)std::min(16u, std::max(1u, std::thread::hardware_concurrency()))));
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; t++) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+    for (int i = 0; i < np; i++) {
+      if (!built[i]) {
+        refused[probes[i]] = 1;  // the probe itself does not factor: it goes to the per-lane LU
+        continue;
+      }
+      KH_HIP(S.add(hs[i], pats.host.desc[lp[probes[i]]], pats.pool.as<int32_t>(), probes[i] / nfc,
+                   a.f0 + (int32_t)(probes[i] % nfc), &idx[i]));
+    }
+    KH_HIP(S.sync());
+    out.probes += np;
+    out.host_s += std::chrono::duration<double>(std::chrono::steady_clock::now() - th0).count();
+    return KH_OK;
+  };
+  // Runs workgroups of W lanes with schedules wgv (-1: none), then reads back
+  // which lanes were solved and where the others departed.
+  auto run_round = [&](int W, const std::vector<int32_t>& wgv) -> kh_status {
+    // L and U values for the largest schedule of the round (each lane's
+    // factors live only within the launch that factors and solves it).
+    int64_t need = 1;
+    for (int32_t v : wgv)
+      if (v >= 0) need = std::max({need, S.all[v]->nl, S.all[v]->nu});
+    if (need > S.cap_lu || BL > S.cap_lanes) {
+      S.cap_lu = std::max(need, S.cap_lu);
+      S.cap_lanes = std::max((BL + 63) / 64 * 64, S.cap_lanes);  // whole blocks of up to 64 lanes
+      KH_HIP(S.lx.alloc((size_t)S.cap_lu * S.cap_lanes * sizeof(kcplx)));
+      KH_HIP(S.ux.alloc((size_t)S.cap_lu * S.cap_lanes * sizeof(kcplx)));
+    }
+    LuArgsT<kcplx> lr = lu;
+    lr.lx = S.lx.as<kcplx>();
+    lr.ux = S.ux.as<kcplx>();
+    lr.capL = lr.capU = S.cap_lu;
+    // Compact working columns in LDS when the round's largest column fits.
+    int64_t mm = 1;
+    for (int32_t v : wgv)
+      if (v >= 0) mm = std::max<int64_t>(mm, S.all[v]->v.mmax);
+    const int64_t lds = mm * W * (int64_t)sizeof(kcplx);
+    // LDS only when every active workgroup of the round fits at once (LDS
+    // capacity bounds the workgroups resident per compute unit, and a second
+    // pass of the latency-bound chain costs more than LDS saves).
+    int64_t active = 0;
+    for (int32_t v : wgv) active += v >= 0;
+    const int compact = (lds <= kSchedLds && active * lds <= lds_total) ? 1 : 0;
+    if (compact && lds > 64 * 1024)
+      KH_HIP(hipFuncSetAttribute((const void*)k_ac_sched, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    DBuf dwgv;
+    KH_HIP(dwgv.alloc(wgv.size() * 4));
+    KH_HIP(hipMemcpy(dwgv.p, wgv.data(), wgv.size() * 4, hipMemcpyHostToDevice));
+    KH_HIP(hipEventRecord(ev[0], 0));
+    hipLaunchKernelGGL(k_ac_sched, dim3((unsigned)wgv.size()), dim3(W), compact ? (size_t)lds : 0, 0, a, lr,
+                       S.dviews.as<SchedView>(), dwgv.as<int32_t>(), d_lock, d_dep, d_dpiv, compact);
+    KH_HIP(hipGetLastError());
+    KH_HIP(hipEventRecord(ev[1], 0));
+    KH_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    KH_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    out.kernel_ms += ms;
+    out.rounds++;
+    KH_HIP(hipMemcpy(hl.data(), d_lock, (size_t)BL * 4, hipMemcpyDeviceToHost));
+    KH_HIP(hipMemcpy(hd.data(), d_dep, (size_t)BL * 4, hipMemcpyDeviceToHost));
+    KH_HIP(hipMemcpy(hp.data(), d_dpiv, (size_t)BL * 4, hipMemcpyDeviceToHost));
+    if (std::getenv("KH_SCHED_LOG")) {  // KH_SCHED_LOG=1: one line per round on stderr
+      int64_t ok = 0, open_n = 0, ref = 0;
+      for (int64_t l = 0; l < BL; l++) {
+        ok += hl[l] != 0;
+        ref += refused[l] != 0;
+        open_n += open(l);
+      }
+      fprintf(stderr,
+              "kh_sched: round %lld W %d: %.1f ms, solved %lld, open %lld, refused %lld, schedules %zu, L/U %lld, "
+              "slots %lld (%s)\n",
+              (long long)out.rounds, W, ms, (long long)ok, (long long)open_n, (long long)ref, S.all.size(),
+              (long long)need, (long long)mm, compact ? "LDS" : "device memory");
+    }
+    return KH_OK;
+  };
+  int W = 0;  // lanes per workgroup, set once round 0's schedules are known
+  int64_t nwg = 0;
+  std::vector<std::vector<int32_t>> tried;
+  // Every probe of this chunk: (circuit, frequency, schedule). Identical
+  // structures share one schedule, so a schedule may have several probes.
+  struct ProbeAt {
+    int64_t circuit;
+    int32_t freq, sched;
+  };
+  std::vector<ProbeAt> at;
+  // Round 0's probes: pivot sequences change in bands of frequency, so each
+  // circuit's frequencies are bisected — kFirstProbes evenly spaced lanes
+  // (its first and last included), then the middle of every interval whose
+  // two ends got different schedules, until such intervals are at most
+  // kBisect lanes wide. Later rounds catch bands no probe landed in.
+  std::vector<int64_t> probes;
+  {
+    std::map<int64_t, int32_t> got;  // probed lane -> schedule (-1: did not factor)
+    for (int64_t c = 0; c < a.B; c++) {
+      if (wst[c] != ST_OK) continue;
+      const int k0 = (int)std::min<int64_t>(kFirstProbes, nfc);
+      for (int k = 0; k < k0; k++) probes.push_back(c * nfc + (k0 > 1 ? k * (nfc - 1) / (k0 - 1) : 0));
+    }
+    std::sort(probes.begin(), probes.end());
+    probes.erase(std::unique(probes.begin(), probes.end()), probes.end());
+    while (!probes.empty()) {
+      if ((int64_t)probes.size() > kMaxProbes) probes.resize(kMaxProbes);
+      std::vector<int32_t> idx;
+      if (kh_status e = factor_probes(probes, idx); e != KH_OK) return e;
+      for (size_t i = 0; i < probes.size(); i++) {
+        got[probes[i]] = idx[i];
+        if (idx[i] >= 0) at.push_back({probes[i] / nfc, a.f0 + (int32_t)(probes[i] % nfc), idx[i]});
+      }
+      probes.clear();
+      for (auto it = got.begin(); it != got.end(); ++it) {
+        auto nx = std::next(it);
+        if (nx == got.end() || nx->first / nfc != it->first / nfc) continue;
+        if (nx->first - it->first > kBisect && it->second != nx->second)
+          probes.push_back((it->first + nx->first) / 2);
+      }
+    }
+  }
+  for (int round = 0; round < kRounds; round++) {
+    if (round > 0) {
+      // Few lanes left: the solo round below serves them in one pass.
+      int64_t left_n = 0;
+      for (int64_t l = 0; l < BL; l++) left_n += open(l);
+      if (left_n <= kSolo) break;
+    }
+    if ((int64_t)probes.size() > kMaxProbes) probes.resize(kMaxProbes);
+    std::vector<int32_t> idx;
+    if (kh_status e = factor_probes(probes, idx); e != KH_OK) return e;
+    for (size_t i = 0; i < probes.size(); i++)
+      if (idx[i] >= 0) at.push_back({probes[i] / nfc, a.f0 + (int32_t)(probes[i] % nfc), idx[i]});
+    if (W == 0) {
+      int64_t mm = 1;
+      for (const auto& o : S.all) mm = std::max<int64_t>(mm, o->v.mmax);
+      W = sched_block(BL, mm);
+      nwg = (BL + W - 1) / W;
+      tried.assign(nwg, {});
+    }
+    std::vector<int32_t> wgv(nwg, -1);
+    bool any = false;
+    for (int64_t w = 0; w < nwg; w++) {
+      int64_t l0 = -1;
+      for (int64_t l = w * W; l < std::min(BL, (w + 1) * W); l++)
+        if (open(l)) {
+          l0 = l;
+          break;
+        }
+      if (l0 < 0) continue;
+      const int32_t pid = lp[l0];
+      const int64_t c = l0 / nfc;
+      const int32_t f = a.f0 + (int32_t)(l0 % nfc);
+      int32_t best = -1;
+      int64_t bestd = 0;
+      auto consider = [&](int32_t i, int64_t circuit, int32_t freq) {
+        if (S.all[i]->v.pid != pid) return;
+        if (std::find(tried[w].begin(), tried[w].end(), i) != tried[w].end()) return;
+        const int64_t d = (circuit == c ? 0 : (int64_t)1 << 40) + std::llabs((int64_t)freq - f);
+        if (best < 0 || d < bestd) {
+          best = i;
+          bestd = d;
+        }
+      };
+      for (const ProbeAt& q : at) consider(q.sched, q.circuit, q.freq);
+      for (size_t i = 0; i < S.all.size(); i++) consider((int32_t)i, S.all[i]->circuit, S.all[i]->freq);
+      if (best < 0) continue;
+      wgv[w] = best;
+      tried[w].push_back(best);
+      any = true;
+    }
+    if (!any) break;
+    if (kh_status e = run_round(W, wgv); e != KH_OK) return e;
+    // Next probes: runs of consecutive lanes that departed together.
+    probes.clear();
+    std::vector<int64_t> run;
+    auto flush = [&]() {
+      if (!run.empty()) probes.push_back(run[run.size() / 2]);
+      run.clear();
+    };
+    int64_t prev = -2;
+    std::tuple<int32_t, int32_t, int32_t> prev_key{-1, -1, -1};
+    for (int64_t w = 0; w < nwg; w++) {
+      if (wgv[w] < 0) continue;
+      for (int64_t l = w * W; l < std::min(BL, (w + 1) * W); l++) {
+        if (!open(l) || lp[l] != S.all[wgv[w]]->v.pid) continue;
+        if (hd[l] < 0) {
+          refused[l] = 1;
+          continue;
+        }
+        const auto key = std::make_tuple(wgv[w], hd[l], hp[l]);
+        if (l != prev + 1 || l / nfc != prev / nfc || key != prev_key) flush();
+        run.push_back(l);
+        prev = l;
+        prev_key = key;
+      }
+    }
+    flush();
+  }
+  // Solo round: every lane left is its own probe.
+  std::vector<int64_t> rest;
+  for (int64_t l = 0; l < BL; l++)
+    if (open(l)) rest.push_back(l);
+  if (!rest.empty() && (int64_t)rest.size() <= kSolo) {
+    std::vector<int32_t> idx;
+    if (kh_status e = factor_probes(rest, idx); e != KH_OK) return e;
+    std::vector<int32_t> wgv(BL, -1);
+    for (size_t i = 0; i < rest.size(); i++) wgv[rest[i]] = idx[i];
+    if (kh_status e = run_round(1, wgv); e != KH_OK) return e;
+  }
+  for (int64_t l = 0; l < BL; l++) {
+    if (wst[l / nfc] != ST_OK) continue;
+    if (hl[l]) {
+      out.ok++;
+    } else {
+      out.left++;
+      left.push_back(l);
+    }
+  }
+  return KH_OK;
+}
 
 std::string slots_key(const std::vector<int32_t>& slots) {
   return std::string(reinterpret_cast<const char*>(slots.data()), slots.size() * sizeof(int32_t));
@@ -1994,8 +2480,9 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
   std::vector<int32_t> cpid1(B, -1);
   a.BL = B;
   a.present = present.as<uint8_t>();
+  std::vector<int32_t> wst(B);  // each circuit's operating-point status
   {
-    std::vector<int32_t> wst(B), hci((size_t)NCI * B);
+    std::vector<int32_t> hci((size_t)NCI * B);
     KH_HIP(hipMemcpy(hci.data(), G.ci.p, hci.size() * 4, hipMemcpyDeviceToHost));
     for (int64_t c = 0; c < B; c++) wst[c] = hci[(size_t)c * NCI + CI_STATUS];
     bool any_ok = false;
@@ -2004,6 +2491,7 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
     std::vector<uint8_t> pr((size_t)B * T.nslots);
     for (int probe = 0; probe < (fsplit > 0 ? 2 : 1); probe++) {
       a.f0 = probe == 0 ? 0 : fsplit;
+      a.nfc = 1;
       hipLaunchKernelGGL(k_ac_stamp, dim3((unsigned)((B + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, a);
       KH_HIP(hipGetLastError());
       KH_HIP(hipMemcpy(pr.data(), present.p, pr.size(), hipMemcpyDeviceToHost));
@@ -2026,13 +2514,24 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
     KH_HIP(pats.sync());
   }
   const int64_t cap = G.dense ? (int64_t)n * n : lu_capacity(pats.host.max_fill, n);
+  // Compiled schedules: FP64 sparse sweeps unless the caller opts out. Their
+  // lanes hold values only (the structure is the schedule's), L and U sized
+  // per round by the schedules in use (the chunk is sized for the structural
+  // fill plus a margin); the lanes they leave get the per-lane LU's full
+  // workspace, packed.
+  const bool use_sched = !G.dense && p.precision == 0 && !(opt && opt->no_schedule);
+  // L and U each hold about half the structural fill (|L| + |U| + n) under
+  // diagonal pivoting; a margin of n each for pivots off the diagonal.
+  const int64_t capS = pats.host.max_fill / 2 + (int64_t)n + 64;
   // Device bytes per lane: values, right-hand side, solution, status words and
-  // the LU workspace (LuWorkT::ensure / ensure_dense).
+  // the LU workspace (LuWorkT::ensure / ensure_dense, or the schedules' values).
+  const int64_t full_lu_bytes = 3 * (int64_t)n * (int64_t)sizeof(kcplx) + 2 * cap * ((int64_t)sizeof(kcplx) + 4) +
+                                (2 * cap + n) * 4 + 8 * ((int64_t)n + 1) + 24 * (int64_t)n + 8;
   const int64_t lane_bytes =
       ((int64_t)T.nslots + 2 * (int64_t)n) * (int64_t)sizeof(kcplx) + 16 +
       (G.dense ? cap * (int64_t)sizeof(kcplx) + 2 * (int64_t)n * (int64_t)sizeof(kcplx) + 4 * (int64_t)n + 4
-               : 3 * (int64_t)n * (int64_t)sizeof(kcplx) + 2 * cap * ((int64_t)sizeof(kcplx) + 4) +
-                     (2 * cap + n) * 4 + 8 * ((int64_t)n + 1) + 24 * (int64_t)n + 8);
+       : use_sched ? 3 * (int64_t)n * (int64_t)sizeof(kcplx) + 2 * capS * (int64_t)sizeof(kcplx) + 16
+                   : full_lu_bytes);
   size_t free_b = 0, total_b = 0;
   KH_HIP(hipMemGetInfo(&free_b, &total_b));
   const int64_t fit = std::max<int64_t>(1, (int64_t)(0.6 * (double)free_b) / (lane_bytes * B));
@@ -2049,6 +2548,19 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
   KH_HIP(pid.alloc((size_t)BLmax * 4));
   KH_HIP(mixl.alloc((size_t)BLmax * 4));
   std::vector<int32_t> hmix(p.precision ? (size_t)BLmax : 0);
+  DevScheds scheds;
+  DBuf lock, dep, dpiv, lustat, sx, sw, sud;
+  if (use_sched && nf > 0) {
+    KH_HIP(lock.alloc((size_t)BLmax * 4));
+    KH_HIP(dep.alloc((size_t)BLmax * 4));
+    KH_HIP(dpiv.alloc((size_t)BLmax * 4));
+    KH_HIP(lustat.alloc((size_t)BLmax * 4));
+    // blocked per workgroup (sched_lane_blocked): room for whole blocks of up to 64 lanes
+    const size_t padded = (size_t)(BLmax + 63) / 64 * 64;
+    KH_HIP(sx.alloc((size_t)n * padded * sizeof(kcplx)));
+    KH_HIP(sw.alloc((size_t)n * padded * sizeof(kcplx)));
+    KH_HIP(sud.alloc((size_t)n * padded * sizeof(kcplx)));
+  }
   a.val = val.as<kcplx>();
   a.rhs = rhs.as<kcplx>();
   a.lstatus = lst.as<int32_t>();
@@ -2056,20 +2568,23 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
   a.present = nullptr;
   if (nf > 0) {
     if (G.dense) KH_HIP(work.ensure_dense(n, BLmax));
-    else KH_HIP(work.ensure(n, BLmax, cap));
+    else if (!use_sched) KH_HIP(work.ensure(n, BLmax, cap));
   }
   std::vector<int32_t> lp((size_t)BLmax), lp_dev;  // pattern id per lane (lp_dev: what the device holds)
   for (int32_t f0 = 0; f0 < nf; f0 += fchunk) {
     const int32_t nfc = std::min(fchunk, nf - f0);
     a.f0 = f0;
+    a.nfc = nfc;
     a.BL = (int64_t)nfc * B;
-    for (int64_t l = 0; l < a.BL; l++) lp[l] = std::max((f0 + l / B < fsplit ? cpid : cpid1)[l % B], 0);
+    for (int64_t l = 0; l < a.BL; l++) lp[l] = std::max((f0 + l % nfc < fsplit ? cpid : cpid1)[l / nfc], 0);
     if (lp != lp_dev) {
       KH_HIP(hipMemcpy(pid.p, lp.data(), (size_t)a.BL * 4, hipMemcpyHostToDevice));
       lp_dev = lp;
     }
-    const unsigned grid = (unsigned)((a.BL + kBlock - 1) / kBlock);
-    hipLaunchKernelGGL(k_ac_stamp, dim3(grid), dim3(kBlock), 0, 0, a);
+    // Per-lane kernels of the chunk in workgroups of kAcLaneBlock lanes: more
+    // wavefronts in flight for their long per-lane loops.
+    const unsigned grid = (unsigned)((a.BL + kAcLaneBlock - 1) / kAcLaneBlock);
+    hipLaunchKernelGGL(k_ac_stamp, dim3(grid), dim3(kAcLaneBlock), 0, 0, a);
     KH_HIP(hipGetLastError());
     LuArgsT<kcplx> lu = {};
     lu.n = n;
@@ -2081,11 +2596,73 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
     lu.val = a.val;
     lu.rhs = a.rhs;
     lu.x = x.as<kcplx>();
-    work.fill(lu);
-    if (!G.dense) KH_HIP(work.invalidate());
+    SchedChunk sc;
+    double fallback_ms = 0;
+    if (use_sched) {
+      // Compiled schedules (sched_chunk) on value-only lanes; the lanes they
+      // leave run the per-lane LU packed (k_ac_lane_lu) in pieces that fit.
+      lu.xw = sx.as<kcplx>();
+      lu.w = sw.as<kcplx>();
+      lu.udiag = sud.as<kcplx>();
+      std::vector<int64_t> left;
+      if (kh_status ss = sched_chunk(a, lu, scheds, pats, lp, wst, n, T.nslots, cap, lock.as<int32_t>(),
+                                     dep.as<int32_t>(), dpiv.as<int32_t>(), ev, sc, left);
+          ss != KH_OK)
+        return ss;
+      size_t fb_free = 0, fb_total = 0;
+      KH_HIP(hipMemGetInfo(&fb_free, &fb_total));
+      const int64_t per = (int64_t)full_lu_bytes + ((int64_t)T.nslots + 2 * (int64_t)n) * (int64_t)sizeof(kcplx) + 12;
+      const int64_t piece = std::max<int64_t>(1, (int64_t)(0.5 * (double)fb_free) / per);
+      for (size_t at = 0; at < left.size(); at += (size_t)piece) {
+        const int64_t m = std::min<int64_t>(piece, (int64_t)(left.size() - at));
+        LuWorkT<kcplx> fw;
+        DBuf dl, cval, crhs, cx, cpidb;
+        KH_HIP(fw.ensure(n, m, cap));
+        KH_HIP(dl.alloc((size_t)m * 8));
+        KH_HIP(cval.alloc((size_t)T.nslots * m * sizeof(kcplx)));
+        KH_HIP(crhs.alloc((size_t)n * m * sizeof(kcplx)));
+        KH_HIP(cx.alloc((size_t)n * m * sizeof(kcplx)));
+        KH_HIP(cpidb.alloc((size_t)m * 4));
+        KH_HIP(hipMemcpy(dl.p, left.data() + at, (size_t)m * 8, hipMemcpyHostToDevice));
+        std::vector<int32_t> cp((size_t)m);
+        for (int64_t i = 0; i < m; i++) cp[i] = lp[left[at + i]];
+        KH_HIP(hipMemcpy(cpidb.p, cp.data(), (size_t)m * 4, hipMemcpyHostToDevice));
+        const int64_t tv = (int64_t)T.nslots * m, tr = (int64_t)n * m;
+        hipLaunchKernelGGL(k_gather_soa, dim3((unsigned)((tv + 255) / 256)), dim3(256), 0, 0, a.val, a.BL,
+                           (int64_t)T.nslots, dl.as<int64_t>(), m, cval.as<kcplx>());
+        KH_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_gather_soa, dim3((unsigned)((tr + 255) / 256)), dim3(256), 0, 0, a.rhs, a.BL, (int64_t)n,
+                           dl.as<int64_t>(), m, crhs.as<kcplx>());
+        KH_HIP(hipGetLastError());
+        LuArgsT<kcplx> lc = {};
+        lc.n = n;
+        lc.nslots = T.nslots;
+        lc.B = m;
+        lc.pats = pats.desc.as<PatDesc>();
+        lc.pool = pats.pool.as<int32_t>();
+        lc.pid = cpidb.as<int32_t>();
+        lc.val = cval.as<kcplx>();
+        lc.rhs = crhs.as<kcplx>();
+        lc.x = cx.as<kcplx>();
+        fw.fill(lc);
+        KH_HIP(hipEventRecord(ev[0], 0));
+        hipLaunchKernelGGL(k_ac_lane_lu, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, 0, lc,
+                           dl.as<int64_t>(), m, x.as<kcplx>(), a.BL, lustat.as<int32_t>());
+        KH_HIP(hipGetLastError());
+        KH_HIP(hipEventRecord(ev[1], 0));
+        KH_HIP(hipEventSynchronize(ev[1]));
+        float fms = 0;
+        KH_HIP(hipEventElapsedTime(&fms, ev[0], ev[1]));
+        fallback_ms += fms;
+      }
+    } else {
+      work.fill(lu);
+      if (!G.dense) KH_HIP(work.invalidate());
+    }
     KH_HIP(hipEventRecord(ev[0], 0));
-    hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac,
-                       p.precision, mixl.as<int32_t>());
+    hipLaunchKernelGGL(k_ac_solve, dim3(grid), dim3(kAcLaneBlock), 0, 0, a, lu, p.ledgerTol, p.skipEnergy, d_xac,
+                       p.precision, mixl.as<int32_t>(), use_sched ? lock.as<int32_t>() : nullptr,
+                       use_sched ? lustat.as<int32_t>() : nullptr);
     KH_HIP(hipGetLastError());
     KH_HIP(hipEventRecord(ev[1], 0));
     hipLaunchKernelGGL(k_ac_reduce, dim3((unsigned)((B + 255) / 256)), dim3(256), 0, 0, a, done.as<int32_t>(),
@@ -2103,6 +2680,12 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
       stats->lu_seconds += ms * 1e-3;
       stats->lu_launches++;
       stats->bytes_algorithmic += a.BL * 16 * (nnz + 2 * (int64_t)n);
+      stats->sched_solves += sc.ok;
+      stats->sched_redos += sc.left;
+      stats->lu_seconds += (sc.kernel_ms + fallback_ms) * 1e-3;
+      stats->sched_probes += sc.probes;
+      stats->sched_rounds += sc.rounds;
+      stats->sched_host_seconds += sc.host_s;
       if (p.precision) {
         KH_HIP(hipMemcpy(hmix.data(), mixl.p, (size_t)a.BL * 4, hipMemcpyDeviceToHost));
         for (int64_t l = 0; l < a.BL; l++) {

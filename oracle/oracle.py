@@ -30,7 +30,8 @@ class KhOptions(C.Structure):
                 ("skip_energy_check", C.c_int32), ("ledger_tol", C.c_double),
                 ("polish", C.c_int32), ("circuits_per_wave", C.c_int32),
                 ("staging", C.c_int32), ("waves_per_simd", C.c_int32), ("backend", C.c_int32),
-                ("precision", C.c_int32), ("circuit_wave", C.c_int32)]  # read by the GPU library only
+                ("precision", C.c_int32), ("circuit_wave", C.c_int32),
+                ("no_schedule", C.c_int32)]  # read by the GPU library only
 
 
 class KhTranSpec(C.Structure):
