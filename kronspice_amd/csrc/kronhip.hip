@@ -312,11 +312,12 @@ __global__ void __launch_bounds__(kBlock) k_ac_solve(AcArgs a, LuArgsT<kcplx> lu
 // where a lane succeeded, else dep[l] is the column at which its pivot rule
 // departed from the schedule (-2: a check the per-lane LU must report). A lane
 // already locked, without an operating point or of another pattern is skipped.
-// compact: the working columns live in LDS, mmax slots per lane
-// (sched_lu_lds); else in device memory (sched_lu).
+// compact: 1, the working columns are compact (sched_lu_lds) in LDS; 2,
+// compact in device memory, workgroup w's block at xg + w * xg_block doubles
+// (small enough to stay in cache); 0, dense in device memory (sched_lu).
 __global__ void __launch_bounds__(kBlock) k_ac_sched(AcArgs a, LuArgsT<kcplx> lu, const SchedView* views,
                                                      const int32_t* wg_view, int32_t* lock, int32_t* dep,
-                                                     int32_t* dpiv, int32_t compact) {
+                                                     int32_t* dpiv, int32_t compact, double* xg, int64_t xg_block) {
   extern __shared__ __align__(16) char lds[];
   const int32_t v = __builtin_amdgcn_readfirstlane(wg_view[blockIdx.x]);  // uniform: one per workgroup
   if (v < 0) return;
@@ -326,9 +327,11 @@ __global__ void __launch_bounds__(kBlock) k_ac_sched(AcArgs a, LuArgsT<kcplx> lu
   if (lu.pid[l] != s.pid) return;
   int d = -2, dp = -1, st;
   const SchedLane<kcplx> P = sched_lane_blocked(lu, l, (int)blockDim.x);
-  if (compact) {
+  if (compact == 1) {
     using LdsD = __attribute__((address_space(3))) double*;
     st = sched_lu_lds(s, lu, l, P, (LdsD)(double*)lds, (int)blockDim.x, (int)threadIdx.x, &d, &dp);
+  } else if (compact == 2) {
+    st = sched_lu_lds(s, lu, l, P, xg + (int64_t)blockIdx.x * xg_block, (int)blockDim.x, (int)threadIdx.x, &d, &dp);
   } else {
     st = sched_lu(s, lu, l, P, &d, &dp);
   }
@@ -517,6 +520,8 @@ struct DevScheds {
   };
   DBuf lx, ux;  // the lanes' L and U values, [entries][lanes], grown to what a round's schedules need
   int64_t cap_lu = 0, cap_lanes = 0;
+  DBuf xg;  // compact working columns in device memory (k_ac_sched compact 2)
+  int64_t xg_doubles = 0;
   std::vector<std::unique_ptr<One>> all;
   DBuf dviews;
   size_t uploaded = 0;
@@ -736,15 +741,22 @@ kh_status sched_chunk(const AcArgs& a, const LuArgsT<kcplx>& lu, DevScheds& S, c
     // pass of the latency-bound chain costs more than LDS saves).
     int64_t active = 0;
     for (int32_t v : wgv) active += v >= 0;
-    const int compact = (lds <= kSchedLds && active * lds <= lds_total) ? 1 : 0;
-    if (compact && lds > 64 * 1024)
-      KH_HIP(hipFuncSetAttribute((const void*)k_ac_sched, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    // Otherwise compact in device memory: a workgroup's block of mm slots is
+    // small enough to stay in the caches, where the dense working columns
+    // (n slots per lane) spill to HBM.
+    const int compact = (lds <= kSchedLds && active * lds <= lds_total) ? 1 : 2;
+    const int64_t xg_block = mm * W * (int64_t)(sizeof(kcplx) / sizeof(double));
+    if (compact == 2 && (int64_t)wgv.size() * xg_block > S.xg_doubles) {
+      S.xg_doubles = (int64_t)wgv.size() * xg_block;
+      KH_HIP(S.xg.alloc((size_t)S.xg_doubles * sizeof(double)));
+    }
     DBuf dwgv;
     KH_HIP(dwgv.alloc(wgv.size() * 4));
     KH_HIP(hipMemcpy(dwgv.p, wgv.data(), wgv.size() * 4, hipMemcpyHostToDevice));
     KH_HIP(hipEventRecord(ev[0], 0));
-    hipLaunchKernelGGL(k_ac_sched, dim3((unsigned)wgv.size()), dim3(W), compact ? (size_t)lds : 0, 0, a, lr,
-                       S.dviews.as<SchedView>(), dwgv.as<int32_t>(), d_lock, d_dep, d_dpiv, compact);
+    hipLaunchKernelGGL(k_ac_sched, dim3((unsigned)wgv.size()), dim3(W), compact == 1 ? (size_t)lds : 0, 0, a, lr,
+                       S.dviews.as<SchedView>(), dwgv.as<int32_t>(), d_lock, d_dep, d_dpiv, compact,
+                       compact == 2 ? S.xg.as<double>() : nullptr, xg_block);
     KH_HIP(hipGetLastError());
     KH_HIP(hipEventRecord(ev[1], 0));
     KH_HIP(hipEventSynchronize(ev[1]));
@@ -766,7 +778,7 @@ kh_status sched_chunk(const AcArgs& a, const LuArgsT<kcplx>& lu, DevScheds& S, c
               "kh_sched: round %lld W %d: %.1f ms, solved %lld, open %lld, refused %lld, schedules %zu, L/U %lld, "
               "slots %lld (%s)\n",
               (long long)out.rounds, W, ms, (long long)ok, (long long)open_n, (long long)ref, S.all.size(),
-              (long long)need, (long long)mm, compact ? "LDS" : "device memory");
+              (long long)need, (long long)mm, compact == 1 ? "LDS" : "compact in device memory");
     }
     return KH_OK;
   };
