@@ -74,6 +74,20 @@ def measured_traffic_per_solve(workload: str, staging: int, cpw: int, waves: int
     return None
 
 
+def measured_ac_traffic(grid: int):
+    """HBM bytes per k_ac_sched launch on the G x G mesh sweep, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (None if absent)."""
+    key = f"traffic_bytes_per_launch[grid={grid}]:"
+    try:
+        with open(os.path.join(ROOT, "profiles", f"k_ac_sched_rlc_grid{grid}_hbm.md")) as f:
+            for line in f:
+                if line.startswith(key):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 class Work:
     """One batched workload: groups of circuits sharing a topology, run in
     one uneven batch through kh_batch_run."""
@@ -136,9 +150,15 @@ def make_work(name, batch):
 def bench_ac(args, log):
     """The batched complex AC sweep of an RLC grid mesh: `--grid` G x G nodes,
     16 value draws x 1024 log-spaced frequencies (1 kHz - 10 GHz), each
-    (draw, frequency) one complex system of one pattern (complex128 GP LU,
-    lane per system). Prints the same JSON fields as the transient workloads,
-    in complex systems per second."""
+    (draw, frequency) one complex system of one pattern. Forms: complex128 GP
+    LU with every system alone on its lane (plain); complex128 GP LU in
+    lockstep along compiled schedules (kh_sched.h: systems that share a pivot
+    sequence share one structure; bit-identical); complex64 factors with
+    complex128 refinement. Prints the same JSON fields as the transient
+    workloads, in complex systems per second; with the scheduled form chosen,
+    `roofline` is that of k_ac_sched (algorithmic bytes of the systems it
+    solved over its summed launch time) and `traffic` its HBM bytes per launch
+    from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary."""
     import torch
     from kronspice_amd import api, workloads as W
     from oracle import oracle
@@ -158,7 +178,7 @@ def bench_ac(args, log):
         torch.cuda.synchronize()
         tot = {"lu_seconds": 0.0, "bytes_algorithmic": 0.0, "lu_launches": 0.0, "refine_passes": 0.0, "fp64_redos": 0.0,
                "sched_solves": 0.0, "sched_redos": 0.0, "sched_probes": 0.0, "sched_rounds": 0.0,
-               "sched_host_seconds": 0.0}
+               "sched_host_seconds": 0.0, "sched_seconds": 0.0}
         t0 = time.perf_counter()
         for _ in range(args.steps):
             r = api.ac(t, d_params, freqs, opts=opts)
@@ -200,6 +220,15 @@ def bench_ac(args, log):
             ok.append(bool(e == 0 and st[d] == 0 and rel <= 1e-9))
     solved = float(np.mean(ok))
     achieved = bytes_alg / lu_s / 1e9 if lu_s > 0 else 0.0
+    kernel, launches_k, secs_k, bytes_k = "k_ac_solve", launches, lu_s, bytes_alg
+    traffic = None
+    if name == "sched" and tot["sched_seconds"] > 0:
+        # k_ac_sched: the systems it solved, 16*(nnzA + 2n) bytes each
+        per_sys = bytes_alg / (systems * args.steps)
+        kernel, launches_k, secs_k = "k_ac_sched", tot["sched_rounds"], tot["sched_seconds"]
+        bytes_k = tot["sched_solves"] * per_sys
+        achieved = bytes_k / secs_k / 1e9
+        traffic = measured_ac_traffic(G)
     out = {
         "metric": METRIC, "value": systems * solved / dt, "unit": "complex systems/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3, "higher_is_better": True,
@@ -214,10 +243,9 @@ def bench_ac(args, log):
                                          "c64": "complex64 GP LU + complex128 refinement, lane per system"}[name],
                    "parallelism": "lane per (draw, frequency) system, 1 GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_ac_sched + k_ac_solve" if name == "sched" else "k_ac_solve",
-                     "avg_launch_us": lu_s / max(launches, 1) * 1e6,
-                     "bytes_per_launch": bytes_alg / max(launches, 1)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "kernel": kernel,
+                     "avg_launch_us": secs_k / max(launches_k, 1) * 1e6,
+                     "bytes_per_launch": bytes_k / max(launches_k, 1)},
         "cpu_baseline": {"value": len(ok) / tcpu, "unit": "complex systems/s", "cores": threads, "kind": "port",
                          "sample": f"{b.B} draws x 16 frequencies, FP64 C++ restatement (one task per draw, its "
                                    f"operating point included) ({tcpu:.2f} s)"},

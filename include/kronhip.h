@@ -197,6 +197,7 @@ typedef struct kh_stats {
      the host time spent on the probes (gathering their values and factoring them) */
   int64_t sched_probes, sched_rounds;
   double  sched_host_seconds;
+  double  sched_seconds;     /* summed duration of the lockstep launches (HIP events; part of lu_seconds) */
 } kh_stats;
 
 /* ---- library ---------------------------------------------------------------*/

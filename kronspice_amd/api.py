@@ -69,7 +69,7 @@ class Stats(C.Structure):
                 ("cycles_accept", C.c_double), ("cycles_ledger", C.c_double),
                 ("staged_bytes", C.c_int64), ("staged_partial", C.c_int64),
                 ("sched_solves", C.c_int64), ("sched_redos", C.c_int64), ("sched_probes", C.c_int64),
-                ("sched_rounds", C.c_int64), ("sched_host_seconds", C.c_double)]
+                ("sched_rounds", C.c_int64), ("sched_host_seconds", C.c_double), ("sched_seconds", C.c_double)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_}

@@ -2698,6 +2698,7 @@ kh_status kh_batch_ac_ex(kh_topology t, int32_t B, const double* d_params, const
       stats->sched_probes += sc.probes;
       stats->sched_rounds += sc.rounds;
       stats->sched_host_seconds += sc.host_s;
+      stats->sched_seconds += sc.kernel_ms * 1e-3;
       if (p.precision) {
         KH_HIP(hipMemcpy(hmix.data(), mixl.p, (size_t)a.BL * 4, hipMemcpyDeviceToHost));
         for (int64_t l = 0; l < a.BL; l++) {
