@@ -3,7 +3,8 @@
 Follows kronspice's netlist package for the cards this path runs
 (netlist/parser.go, netlist/units.go): R, C, L, V, I, D, M and Q elements,
 .model, .op, .tran, .ac, .dc and .end; comments (* lines and ; tails) and +
-continuations.  Elements are handed to the circuit builder in file order, with
+continuations; .param, .global, .subckt/.ends with X instances and
+{expressions} through netlist_pre (netlist/preprocess.go, netlist/expr.go).  Elements are handed to the circuit builder in file order, with
 multi-terminal node order as circuit.FromDeck (circuit/builder.go:307-470):
 MOSFET drain, source, gate, bulk; BJT collector, emitter, base[, substrate].
 """
@@ -126,7 +127,11 @@ def _source_spec(f: List[str]):
 
 
 def parse(text: str) -> Deck:
+    from .netlist_pre import expand
     title, lines = _logical_lines(text)
+    # .param / .global / .subckt and X instances, {expressions}: flat lines
+    # (netlist/parser.go:156-182, netlist/preprocess.go)
+    lines, _params = expand(lines)
     b = Builder(title)
     analyses: List[Analysis] = []
     elements: List[List[str]] = []

@@ -86,7 +86,8 @@ def perturbed(c, els, p, B, seed):
     return P
 
 
-@pytest.mark.parametrize("name", ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac", "mos3_cs_ac"])
+@pytest.mark.parametrize("name", ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac", "mos3_cs_ac",
+                                  "param_rc_ac"])
 def test_sched_forced_sparse_decks_identical(name):
     """The AC goldens under forced sparse (their size is below the dense
     threshold), 64 value draws each: whatever the draws' pivots do, the
