@@ -275,6 +275,63 @@ kh_status kh_lu_batch_info(kh_lu_batch h, int32_t* q, int64_t* nnzL, int64_t* nn
                            int64_t* capL, int64_t* capU);
 void kh_lu_batch_destroy(kh_lu_batch h);
 
+/* ---- JacobianTearing: torn solves on the GPU (kron/jacobian.go:66-712) -------
+ * kh_tear is a kron.JacobianTearing whose cut is resolved to node-unknown pairs:
+ * cuts [2*ncuts], -1 = ground, as NewJacobianTearing builds them from the torn
+ * elements' terminals (kron/jacobian.go:95-170; the name resolution stays with
+ * the caller). reuse_tol is JacobianTearing.ReuseTol (:71-73). Its matrices
+ * (kh_tmat: JacobianTearing.Matrix / CMatrix, :180-191, the SolverFactory seam
+ * of analysis.Options.Solver, analysis/op.go:50-70) learn their pattern from
+ * the stamps in first-Add order, as tornMat does (:199-253), and solve on the
+ * GPU by tearing (:393-589): the pattern's pieces (components once the cut
+ * pairs' couplings are dropped), each factored by an equilibrated dense LU and
+ * kept while its block is unchanged (or moved by at most reuse_tol of its
+ * largest entry), the interconnection system of the coupling rows, and
+ * refinement against the whole matrix to a componentwise backward error of
+ * 1e-14, accepted at 1e-12 (kron/diakoptics.go:83-85); a system refused on the
+ * way (a singular piece, no convergence, a piece over solver.MaxDense) is solved
+ * by the untorn LU of solver.Auto (dense below 30 unknowns, else sparse).
+ * Complex values are (re, im) double pairs. kh_tear_batch solves B systems of
+ * one pattern (entries ri, ci in first-Add order) in one call, one system per
+ * lane, each keeping its own piece factors between calls: d_vals [nslots*B],
+ * d_b and d_x [n*B] device arrays, SoA (entry k of system c at [k*B + c]);
+ * d_status [B] receives KH_OK, KH_SINGULAR or KH_NONFINITE per system. The
+ * counters of every matrix and batch add up in their kh_tear, as JacobianStats do (
+ * :80-92). A kh_tear, its matrices and batches are used by one thread at a time. */
+typedef struct kh_tear_s* kh_tear;
+typedef struct kh_tear_stats {
+  int64_t solves;        /* systems solved */
+  int64_t factors;       /* pieces factored */
+  int64_t reuses;        /* pieces whose unchanged factors were reused */
+  int64_t stale_reuses;  /* pieces reused within reuse_tol */
+  int64_t fallbacks;     /* systems solved by the untorn LU */
+  int64_t refinements;   /* refinement steps */
+  int64_t max_pieces;    /* the most pieces of one system's plan */
+} kh_tear_stats;
+kh_status kh_tear_create(const int32_t* cuts, int32_t ncuts, double reuse_tol, kh_tear* out);  /* NewJacobianTearing */
+kh_status kh_tear_get_stats(kh_tear t, kh_tear_stats* out);                                  /* JacobianTearing.Stats */
+void kh_tear_destroy(kh_tear t);
+
+typedef struct kh_tmat_s* kh_tmat;
+kh_status kh_tmat_create(kh_tear t, int32_t n, int32_t complex_values, kh_tmat* out);  /* Matrix(n) / CMatrix(n) */
+kh_status kh_tmat_clear(kh_tmat m);                                                 /* tornMat.Clear */
+kh_status kh_tmat_add_batch(kh_tmat m, const int32_t* i, const int32_t* j,
+                            const double* v, int64_t count);                       /* tornMat.Add x count */
+kh_status kh_tmat_solve(kh_tmat m, const double* b, double* x);                     /* tornMat.Solve */
+kh_status kh_tmat_absproduct(kh_tmat m, const double* x, double* y);                /* tornMat.AbsProduct */
+/* Pieces of the plan the last solve used (*npieces = -1 before the first), and
+ * the piece of each unknown (piece_of [n], optional). */
+kh_status kh_tmat_pieces(kh_tmat m, int32_t* npieces, int32_t* piece_of);
+void kh_tmat_destroy(kh_tmat m);
+
+typedef struct kh_tear_batch_s* kh_tear_batch;
+kh_status kh_tear_batch_create(kh_tear t, int32_t n, int64_t nslots, const int32_t* ri, const int32_t* ci,
+                               int32_t B, int32_t complex_values, kh_tear_batch* out);
+kh_status kh_tear_batch_solve(kh_tear_batch h, const double* d_vals, const double* d_b, double* d_x,
+                              int32_t* d_status, void* stream);
+kh_status kh_tear_batch_pieces(kh_tear_batch h, int32_t* npieces, int32_t* piece_of);
+void kh_tear_batch_destroy(kh_tear_batch h);
+
 /* ---- batched analyses of circuits sharing one topology ---------------------
  * A topology is the element list (kinds, nodes, parameter offsets) with nparams
  * doubles per circuit. Parameters for B circuits are SoA on the device:

@@ -118,6 +118,23 @@ def lib():
         L.kh_lu_batch_solve_mixed.argtypes = [P, P, P, P, P, P, P, P]
         L.kh_lu_batch_info.argtypes = [P, P, P, P, P, P]
         L.kh_lu_batch_destroy.argtypes = [P]
+        L.kh_tear_create.argtypes = [P, i32, C.c_double, P]
+        L.kh_tear_get_stats.argtypes = [P, P]
+        L.kh_tear_destroy.argtypes = [P]
+        L.kh_tear_destroy.restype = None
+        L.kh_tmat_create.argtypes = [P, i32, i32, P]
+        L.kh_tmat_clear.argtypes = [P]
+        L.kh_tmat_add_batch.argtypes = [P, P, P, P, i64]
+        L.kh_tmat_solve.argtypes = [P, P, P]
+        L.kh_tmat_absproduct.argtypes = [P, P, P]
+        L.kh_tmat_pieces.argtypes = [P, P, P]
+        L.kh_tmat_destroy.argtypes = [P]
+        L.kh_tmat_destroy.restype = None
+        L.kh_tear_batch_create.argtypes = [P, i32, i64, P, P, i32, i32, P]
+        L.kh_tear_batch_solve.argtypes = [P, P, P, P, P, P]
+        L.kh_tear_batch_pieces.argtypes = [P, P, P]
+        L.kh_tear_batch_destroy.argtypes = [P]
+        L.kh_tear_batch_destroy.restype = None
         L.kh_lu_batch_destroy.restype = None
         L.kh_topology_create.argtypes = [i32, P, i32, i32, P, P]
         L.kh_topology_size.argtypes = [P]
@@ -278,6 +295,136 @@ class CSparse:
         q = np.zeros(self.n, np.int32); prow = np.zeros(self.n, np.int32)
         _check(lib().kh_csys_pivots(self.h, _p(q), _p(prow)))
         return q, prow
+
+
+# ------------------------------------------------------------------------------ JacobianTearing
+class TearStats(C.Structure):
+    _fields_ = [("solves", C.c_int64), ("factors", C.c_int64), ("reuses", C.c_int64), ("stale_reuses", C.c_int64),
+                ("fallbacks", C.c_int64), ("refinements", C.c_int64), ("max_pieces", C.c_int64)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+class JacobianTearing:
+    """kron.JacobianTearing on the GPU (kron/jacobian.go:66-712): a
+    SolverFactory whose matrices solve by tearing along `cuts` (node-unknown
+    pairs, -1 ground, as kronspice_amd.tear.cut_pairs resolves them from
+    element names), reusing unchanged piece factors (or those within
+    reuse_tol) between solves."""
+
+    def __init__(self, cuts, reuse_tol: float = 0.0):
+        c = np.ascontiguousarray(np.asarray(cuts, np.int32).reshape(-1, 2))
+        self.h = C.c_void_p()
+        _check(lib().kh_tear_create(_p(c) if len(c) else None, len(c), float(reuse_tol), C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_tear_destroy(self.h)
+            self.h = None
+
+    def stats(self) -> dict:
+        s = TearStats()
+        _check(lib().kh_tear_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def matrix(self, n: int) -> "TornMatrix":
+        return TornMatrix(self, n, False)
+
+    def cmatrix(self, n: int) -> "TornMatrix":
+        return TornMatrix(self, n, True)
+
+    def batch(self, n: int, ri, ci, B: int, complex_: bool = False) -> "TearBatch":
+        return TearBatch(self, n, ri, ci, B, complex_)
+
+
+class TornMatrix:
+    """JacobianTearing.Matrix / CMatrix: stamps staged on the host, one
+    kh_tmat_add_batch per Solve; the torn solve runs on the GPU."""
+
+    def __init__(self, jt: JacobianTearing, n: int, complex_: bool):
+        self.jt = jt  # the tearing outlives its matrices
+        self.n = n
+        self.cplx = complex_
+        self.dt = np.complex128 if complex_ else np.float64
+        self.h = C.c_void_p()
+        _check(lib().kh_tmat_create(jt.h, n, 1 if complex_ else 0, C.byref(self.h)))
+        self._i, self._j, self._v = [], [], []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_tmat_destroy(self.h)
+            self.h = None
+
+    def clear(self):
+        self._flush()
+        _check(lib().kh_tmat_clear(self.h))
+
+    def add(self, i: int, j: int, v):
+        self._i.append(i); self._j.append(j); self._v.append(v)
+
+    def add_batch(self, i, j, v):
+        self._flush()
+        i = np.ascontiguousarray(i, np.int32); j = np.ascontiguousarray(j, np.int32)
+        v = np.ascontiguousarray(v, self.dt)
+        _check(lib().kh_tmat_add_batch(self.h, _p(i), _p(j), _p(v), len(i)))
+
+    def _flush(self):
+        if self._i:
+            i, j, v = self._i, self._j, self._v
+            self._i, self._j, self._v = [], [], []
+            self.add_batch(i, j, v)
+
+    def solve(self, b) -> np.ndarray:
+        self._flush()
+        b = np.ascontiguousarray(b, self.dt)
+        x = np.zeros(self.n, self.dt)
+        _check(lib().kh_tmat_solve(self.h, _p(b), _p(x)))
+        return x
+
+    def abs_product(self, x) -> np.ndarray:
+        self._flush()
+        x = np.ascontiguousarray(x, self.dt)
+        y = np.zeros(self.n)
+        _check(lib().kh_tmat_absproduct(self.h, _p(x), _p(y)))
+        return y
+
+    def pieces(self):
+        k = C.c_int32()
+        po = np.zeros(self.n, np.int32)
+        _check(lib().kh_tmat_pieces(self.h, C.byref(k), _p(po)))
+        return k.value, po
+
+
+class TearBatch:
+    """B torn systems of one pattern (entries ri, ci in first-Add order), one
+    per lane, each keeping its piece factors between solves."""
+
+    def __init__(self, jt: JacobianTearing, n: int, ri, ci, B: int, complex_: bool = False):
+        self.jt = jt
+        self.n, self.B, self.cplx = n, B, complex_
+        ri = np.ascontiguousarray(ri, np.int32); ci = np.ascontiguousarray(ci, np.int32)
+        self.nslots = len(ri)
+        self.h = C.c_void_p()
+        _check(lib().kh_tear_batch_create(jt.h, n, self.nslots, _p(ri), _p(ci), B, 1 if complex_ else 0,
+                                          C.byref(self.h)))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().kh_tear_batch_destroy(self.h)
+            self.h = None
+
+    def solve(self, vals, b, x, status, stream=None):
+        """vals [nslots, B], b and x [n, B] device tensors (float64, or
+        complex128 when complex); status [B] int32."""
+        _check(lib().kh_tear_batch_solve(self.h, _p(vals), _p(b), _p(x), _p(status),
+                                         None if stream is None else C.c_void_p(stream)))
+
+    def pieces(self):
+        k = C.c_int32()
+        po = np.zeros(self.n, np.int32)
+        _check(lib().kh_tear_batch_pieces(self.h, C.byref(k), _p(po)))
+        return k.value, po
 
 
 # ------------------------------------------------------------------------------ batched LU

@@ -9,6 +9,8 @@
 //               fastest (SoA): the per-system seam and the batched LU
 //   k_hash    — fingerprints the sparsity pattern of lanes whose stamps
 //               created new entries, for the host's symbolic phase
+//   k_tear    — torn solves (kh_tear.h), one system per lane; k_tear_untorn
+//               solves the systems they refuse with the untorn LU
 // The host relaunches k_persist until every circuit is done and runs the
 // symbolic phase (kh_symbolic.cpp) once per distinct pattern in between.
 #include <hip/hip_runtime.h>
@@ -36,6 +38,7 @@
 #include "kh_report.h"
 #include "kh_sched.h"
 #include "kh_symbolic.h"
+#include "kh_tear.h"
 #include "kh_wave.h"
 
 using namespace kh;
@@ -2733,5 +2736,373 @@ kh_status kh_batch_tran(kh_topology t, int32_t B, const double* d_params, const 
   g.d_accepted = d_accepted; g.d_rejected = d_rejected; g.d_ledger = d_ledger;
   return kh_batch_run(1, &g, opt, spec, stats);
 }
+
+}  // extern "C"
+
+// ---- JacobianTearing (kron/jacobian.go:66-712) ----------------------------------------------
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_tear(TearPlanView P, int64_t B, int32_t refused, double reuse_tol,
+                                                 const T* val, const T* rhs, T* x, T* tv, double* dv, int32_t* iv,
+                                                 int64_t* cnt, int32_t* status, int32_t* any_fallback) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= B) return;
+  const TearLane<T> L{P, B, c, val, rhs, x, tv, dv, iv, cnt, reuse_tol};
+  const int st = L.solve(refused != 0);
+  status[c] = st;
+  if (st == TEAR_FALLBACK) atomicOr(any_fallback, 1);
+}
+
+// untorn (kron/jacobian.go:418-432): the refused systems solved by
+// solver.NewMatrix(n, solver.Auto) from their stamps in slot order.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_tear_untorn(LuArgsT<T> a, int32_t dense, const int32_t* slot_i,
+                                                        const int32_t* slot_j, int32_t* status) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= a.B || status[c] != TEAR_FALLBACK) return;
+  int st;
+  if (dense) {
+    st = lu_dense<T>(a, c, slot_i, slot_j);
+  } else {
+    st = lu_factor<T>(a, c);
+    if (st == LU_OK) st = lu_solve<T>(a, c);
+  }
+  status[c] = st;
+}
+
+template <typename T>
+struct TearBatchT {
+  int n = 0;
+  int64_t B = 0;
+  TearHost host;
+  TearPlanView dview = {};
+  DBuf plan_i, plan_p, tv, dv, iv, cnt, flag, status;
+  // the untorn LU
+  bool fb_dense = false, fb_ready = false;
+  DBuf si, sj, pid;
+  DevPatterns pats;
+  Pattern pat;
+  LuWorkT<T> work;
+};
+
+}  // namespace
+
+struct kh_tear_s {
+  std::vector<std::pair<int32_t, int32_t>> cuts;
+  double reuse_tol = 0;
+  kh_tear_stats st = {};
+};
+
+struct kh_tear_batch_s {
+  kh_tear_s* jt = nullptr;
+  int cplx = 0;
+  std::unique_ptr<TearBatchT<double>> r;
+  std::unique_ptr<TearBatchT<kcplx>> z;
+};
+
+struct kh_tmat_s {
+  kh_tear_s* jt = nullptr;
+  int n = 0;
+  int cplx = 0;
+  std::unordered_map<int64_t, int32_t> slot;
+  std::vector<int32_t> ri, ci;
+  std::vector<double> val;  // (re, im) pairs when complex
+  kh_tear_batch_s* batch = nullptr;  // the plan and device state of the current pattern (null: pattern grew)
+  DBuf dval, db, dx, dst;
+  ~kh_tmat_s();
+};
+
+namespace {
+
+template <typename T>
+kh_status tear_batch_init(TearBatchT<T>& b, kh_tear_s* jt, int n, const std::vector<int32_t>& ri,
+                          const std::vector<int32_t>& ci, int64_t B) {
+  b.n = n;
+  b.B = B;
+  b.host = tear_plan(n, ri, ci, jt->cuts);
+  const TearHost& h = b.host;
+  if (!h.refused) jt->st.max_pieces = std::max<int64_t>(jt->st.max_pieces, (int64_t)h.pcs.size());
+  // one int32 pool of every index array, and the pieces
+  std::vector<int32_t> pool;
+  std::vector<size_t> at;
+  for (const std::vector<int32_t>* a : {&h.ri, &h.ci, &h.pieceOf, &h.local, &h.rowIdx, &h.off, &h.rows, &h.idx,
+                                        &h.pslot, &h.pli, &h.plj, &h.ports}) {
+    at.push_back(pool.size());
+    pool.insert(pool.end(), a->begin(), a->end());
+  }
+  KH_HIP(b.plan_i.alloc(std::max<size_t>(1, pool.size()) * 4));
+  if (!pool.empty()) KH_HIP(hipMemcpy(b.plan_i.p, pool.data(), pool.size() * 4, hipMemcpyHostToDevice));
+  KH_HIP(b.plan_p.alloc(std::max<size_t>(1, h.pcs.size()) * sizeof(TearPiece)));
+  if (!h.pcs.empty())
+    KH_HIP(hipMemcpy(b.plan_p.p, h.pcs.data(), h.pcs.size() * sizeof(TearPiece), hipMemcpyHostToDevice));
+  TearPlanView v = h.v;
+  const int32_t* base = b.plan_i.template as<int32_t>();
+  v.ri = base + at[0]; v.ci = base + at[1]; v.pieceOf = base + at[2]; v.local = base + at[3];
+  v.rowIdx = base + at[4]; v.off = base + at[5]; v.rows = base + at[6]; v.idx = base + at[7];
+  v.pslot = base + at[8]; v.pli = base + at[9]; v.plj = base + at[10]; v.ports = base + at[11];
+  v.pc = b.plan_p.template as<TearPiece>();
+  b.dview = v;
+  const int64_t nT = h.refused ? 0 : v.nT, nD = h.refused ? 0 : v.nD, nI = h.refused ? 0 : v.nI;
+  KH_HIP(b.tv.alloc((size_t)nT * B * sizeof(T)));
+  KH_HIP(b.dv.alloc((size_t)nD * B * 8));
+  KH_HIP(b.iv.alloc((size_t)nI * B * 4));  // zero: no piece has factors yet
+  KH_HIP(b.cnt.alloc((size_t)kTearCounters * B * 8));
+  KH_HIP(b.flag.alloc(4));
+  KH_HIP(b.status.alloc((size_t)B * 4));
+  return KH_OK;
+}
+
+// The untorn LU's pattern and workspace, made when a system first needs it.
+template <typename T>
+kh_status tear_fallback_ready(TearBatchT<T>& b) {
+  if (b.fb_ready) return KH_OK;
+  const TearHost& h = b.host;
+  b.fb_dense = h.n < 30;  // Backend.Resolve(Auto), solver/matrix.go:77-85
+  if (b.fb_dense) {
+    KH_HIP(b.si.alloc((size_t)std::max<int64_t>(1, h.nslots) * 4));
+    KH_HIP(b.sj.alloc((size_t)std::max<int64_t>(1, h.nslots) * 4));
+    if (h.nslots) {
+      KH_HIP(hipMemcpy(b.si.p, h.ri.data(), (size_t)h.nslots * 4, hipMemcpyHostToDevice));
+      KH_HIP(hipMemcpy(b.sj.p, h.ci.data(), (size_t)h.nslots * 4, hipMemcpyHostToDevice));
+    }
+    KH_HIP(b.work.ensure_dense(h.n, b.B));
+  } else {
+    std::vector<int32_t> slots(h.nslots);
+    for (int64_t k = 0; k < h.nslots; k++) slots[k] = (int32_t)k;
+    b.pat = analyze_pattern(h.n, h.ri, h.ci, slots);
+    b.pats.host.add(b.pat, slots_key(slots));
+    KH_HIP(b.pats.sync());
+    KH_HIP(b.work.ensure(h.n, b.B, lu_capacity(b.pat.fill_estimate, h.n)));
+    KH_HIP(b.pid.alloc((size_t)b.B * 4));
+  }
+  b.fb_ready = true;
+  return KH_OK;
+}
+
+template <typename T>
+kh_status tear_batch_solve(TearBatchT<T>& b, kh_tear_s* jt, const T* d_vals, const T* d_b, T* d_x, int32_t* d_status,
+                           hipStream_t st) {
+  const int64_t B = b.B;
+  int32_t* status = d_status ? d_status : b.status.template as<int32_t>();
+  KH_HIP(hipMemsetAsync(b.cnt.p, 0, (size_t)kTearCounters * B * 8, st));
+  KH_HIP(hipMemsetAsync(b.flag.p, 0, 4, st));
+  hipLaunchKernelGGL(k_tear<T>, dim3(grid_for(B)), dim3(kBlock), 0, st, b.dview, B, b.host.refused ? 1 : 0,
+                     jt->reuse_tol, d_vals, d_b, d_x, b.tv.template as<T>(), b.dv.template as<double>(), b.iv.template as<int32_t>(),
+                     b.cnt.template as<int64_t>(), status, b.flag.template as<int32_t>());
+  KH_HIP(hipGetLastError());
+  int32_t any = 0;
+  KH_HIP(hipMemcpyAsync(&any, b.flag.p, 4, hipMemcpyDeviceToHost, st));
+  KH_HIP(hipStreamSynchronize(st));
+  if (any) {
+    kh_status e = tear_fallback_ready(b);
+    if (e != KH_OK) return e;
+    LuArgsT<T> a = {};
+    a.n = b.n;
+    a.nslots = (int32_t)b.host.nslots;
+    a.B = B;
+    a.val = d_vals;
+    a.rhs = d_b;
+    a.x = d_x;
+    if (!b.fb_dense) {
+      a.pats = b.pats.desc.template as<PatDesc>();
+      a.pool = b.pats.pool.template as<int32_t>();
+      a.pid = b.pid.template as<int32_t>();
+      b.work.fill(a);
+    } else {
+      a.lx = b.work.lx.template as<T>();
+      a.udiag = b.work.udiag.template as<T>();
+      a.w = b.work.w.template as<T>();
+      a.prow = b.work.prow.template as<int32_t>();
+      a.err_step = b.work.err.template as<int32_t>();
+    }
+    hipLaunchKernelGGL(k_tear_untorn<T>, dim3(grid_for(B)), dim3(kBlock), 0, st, a, b.fb_dense ? 1 : 0,
+                       b.si.template as<int32_t>(), b.sj.template as<int32_t>(), status);
+    KH_HIP(hipGetLastError());
+  }
+  std::vector<int64_t> cnt((size_t)kTearCounters * B);
+  KH_HIP(hipMemcpyAsync(cnt.data(), b.cnt.p, cnt.size() * 8, hipMemcpyDeviceToHost, st));
+  KH_HIP(hipStreamSynchronize(st));
+  int64_t sum[kTearCounters] = {};
+  for (int k = 0; k < kTearCounters; k++)
+    for (int64_t c = 0; c < B; c++) sum[k] += cnt[(size_t)k * B + c];
+  jt->st.solves += sum[TC_SOLVES];
+  jt->st.factors += sum[TC_FACTORS];
+  jt->st.reuses += sum[TC_REUSES];
+  jt->st.stale_reuses += sum[TC_STALE];
+  jt->st.fallbacks += sum[TC_FALLBACKS];
+  jt->st.refinements += sum[TC_REFINES];
+  return KH_OK;
+}
+
+kh_status tear_batch_make(kh_tear_s* jt, int n, const std::vector<int32_t>& ri, const std::vector<int32_t>& ci,
+                          int64_t B, int cplx, kh_tear_batch_s** out) {
+  auto h = std::make_unique<kh_tear_batch_s>();
+  h->jt = jt;
+  h->cplx = cplx;
+  kh_status e;
+  if (cplx) {
+    h->z = std::make_unique<TearBatchT<kcplx>>();
+    e = tear_batch_init(*h->z, jt, n, ri, ci, B);
+  } else {
+    h->r = std::make_unique<TearBatchT<double>>();
+    e = tear_batch_init(*h->r, jt, n, ri, ci, B);
+  }
+  if (e != KH_OK) return e;
+  *out = h.release();
+  return KH_OK;
+}
+
+kh_status tear_pieces(const TearHost& h, int32_t* npieces, int32_t* piece_of) {
+  if (npieces) *npieces = h.refused ? -1 : (int32_t)h.pcs.size();
+  if (piece_of) std::memcpy(piece_of, h.pieceOf.data(), (size_t)h.n * 4);
+  return KH_OK;
+}
+
+}  // namespace
+
+kh_tmat_s::~kh_tmat_s() { delete batch; }
+
+extern "C" {
+
+kh_status kh_tear_create(const int32_t* cuts, int32_t ncuts, double reuse_tol, kh_tear* out) {
+  if (!out || ncuts < 0 || (ncuts > 0 && !cuts) || !(reuse_tol >= 0)) return fail(KH_BAD_ARGUMENT, "bad argument");
+  auto t = std::make_unique<kh_tear_s>();
+  for (int32_t k = 0; k < ncuts; k++) t->cuts.push_back({cuts[2 * k], cuts[2 * k + 1]});
+  t->reuse_tol = reuse_tol;
+  *out = t.release();
+  return KH_OK;
+}
+
+kh_status kh_tear_get_stats(kh_tear t, kh_tear_stats* out) {
+  if (!t || !out) return fail(KH_BAD_ARGUMENT, "null argument");
+  *out = t->st;
+  return KH_OK;
+}
+
+void kh_tear_destroy(kh_tear t) { delete t; }
+
+kh_status kh_tmat_create(kh_tear t, int32_t n, int32_t complex_values, kh_tmat* out) {
+  if (!t || !out) return fail(KH_BAD_ARGUMENT, "null argument");
+  if (n <= 0) return fail(KH_BADSIZE, "size " + std::to_string(n));  // solver.ErrBadSize, kron/jacobian.go:224-226
+  auto m = std::make_unique<kh_tmat_s>();
+  m->jt = t;
+  m->n = n;
+  m->cplx = complex_values ? 1 : 0;
+  *out = m.release();
+  return KH_OK;
+}
+
+kh_status kh_tmat_clear(kh_tmat m) {
+  if (!m) return fail(KH_BAD_ARGUMENT, "null handle");
+  std::fill(m->val.begin(), m->val.end(), 0.0);
+  return KH_OK;
+}
+
+kh_status kh_tmat_add_batch(kh_tmat m, const int32_t* i, const int32_t* j, const double* v, int64_t count) {
+  if (!m || (count > 0 && (!i || !j || !v))) return fail(KH_BAD_ARGUMENT, "null argument");
+  const int w = m->cplx ? 2 : 1;
+  for (int64_t k = 0; k < count; k++) {  // tornMat.Add, kron/jacobian.go:238-253
+    const int ii = i[k], jj = j[k];
+    if (ii < 0 || jj < 0 || ii >= m->n || jj >= m->n) continue;
+    const int64_t key = (int64_t)ii * m->n + jj;
+    auto it = m->slot.find(key);
+    int32_t s;
+    if (it == m->slot.end()) {
+      s = (int32_t)m->ri.size();
+      m->slot[key] = s;
+      m->ri.push_back(ii);
+      m->ci.push_back(jj);
+      for (int q = 0; q < w; q++) m->val.push_back(0.0);
+      delete m->batch;  // a new entry: a new plan, pieces without factors
+      m->batch = nullptr;
+    } else {
+      s = it->second;
+    }
+    for (int q = 0; q < w; q++) m->val[(size_t)s * w + q] = m->val[(size_t)s * w + q] + v[k * w + q];
+  }
+  return KH_OK;
+}
+
+kh_status kh_tmat_solve(kh_tmat m, const double* b, double* x) {
+  if (!m || !b || !x) return fail(KH_BAD_ARGUMENT, "null argument");
+  const int w = m->cplx ? 2 : 1;
+  if (!m->batch) {
+    kh_status e = tear_batch_make(m->jt, m->n, m->ri, m->ci, 1, m->cplx, &m->batch);
+    if (e != KH_OK) return e;
+    KH_HIP(m->dval.alloc(std::max<size_t>(1, m->val.size()) * 8));
+    KH_HIP(m->db.alloc((size_t)m->n * w * 8));
+    KH_HIP(m->dx.alloc((size_t)m->n * w * 8));
+    KH_HIP(m->dst.alloc(4));
+  }
+  if (!m->val.empty()) KH_HIP(hipMemcpy(m->dval.p, m->val.data(), m->val.size() * 8, hipMemcpyHostToDevice));
+  KH_HIP(hipMemcpy(m->db.p, b, (size_t)m->n * w * 8, hipMemcpyHostToDevice));
+  kh_status e = kh_tear_batch_solve(m->batch, m->dval.as<double>(), m->db.as<double>(), m->dx.as<double>(),
+                                    m->dst.as<int32_t>(), nullptr);
+  if (e != KH_OK) return e;
+  int32_t hs = 0;
+  KH_HIP(hipMemcpy(&hs, m->dst.p, 4, hipMemcpyDeviceToHost));
+  if (hs == LU_OK || hs == LU_NONFINITE) KH_HIP(hipMemcpy(x, m->dx.p, (size_t)m->n * w * 8, hipMemcpyDeviceToHost));
+  if (hs == LU_OK) return KH_OK;
+  if (hs == LU_SINGULAR) return fail(KH_SINGULAR, "solver: singular matrix");
+  if (hs == LU_NONFINITE) return fail(KH_NONFINITE, "solver: non-finite value in system");
+  return lu_status_error(hs);
+}
+
+kh_status kh_tmat_absproduct(kh_tmat m, const double* x, double* y) {
+  if (!m || !x || !y) return fail(KH_BAD_ARGUMENT, "null argument");
+  for (int i = 0; i < m->n; i++) y[i] = 0;  // tornMat.AbsProduct, kron/jacobian.go:255-262
+  for (size_t s = 0; s < m->ri.size(); s++) {
+    if (m->cplx) {
+      const kcplx v = {m->val[2 * s], m->val[2 * s + 1]}, xv = {x[2 * m->ci[s]], x[2 * m->ci[s] + 1]};
+      y[m->ri[s]] += lu_mag(v) * lu_mag(xv);
+    } else {
+      y[m->ri[s]] += std::fabs(m->val[s]) * std::fabs(x[m->ci[s]]);
+    }
+  }
+  return KH_OK;
+}
+
+kh_status kh_tmat_pieces(kh_tmat m, int32_t* npieces, int32_t* piece_of) {
+  if (!m) return fail(KH_BAD_ARGUMENT, "null handle");
+  if (!m->batch) {
+    if (npieces) *npieces = -1;
+    return KH_OK;
+  }
+  return kh_tear_batch_pieces(m->batch, npieces, piece_of);
+}
+
+void kh_tmat_destroy(kh_tmat m) { delete m; }
+
+kh_status kh_tear_batch_create(kh_tear t, int32_t n, int64_t nslots, const int32_t* ri, const int32_t* ci,
+                               int32_t B, int32_t complex_values, kh_tear_batch* out) {
+  if (!t || !out || B <= 0 || nslots < 0 || (nslots > 0 && (!ri || !ci))) return fail(KH_BAD_ARGUMENT, "bad argument");
+  if (n <= 0) return fail(KH_BADSIZE, "size " + std::to_string(n));
+  std::vector<int32_t> r(ri, ri + nslots), c(ci, ci + nslots);
+  std::unordered_map<int64_t, int> seen;
+  for (int64_t k = 0; k < nslots; k++) {
+    if (r[k] < 0 || c[k] < 0 || r[k] >= n || c[k] >= n) return fail(KH_BAD_ARGUMENT, "entry out of range");
+    if (!seen.emplace((int64_t)r[k] * n + c[k], 1).second) return fail(KH_BAD_ARGUMENT, "duplicate entry");
+  }
+  return tear_batch_make(t, n, r, c, B, complex_values ? 1 : 0, out);
+}
+
+kh_status kh_tear_batch_solve(kh_tear_batch h, const double* d_vals, const double* d_b, double* d_x,
+                              int32_t* d_status, void* stream) {
+  if (!h || !d_b || !d_x || (!d_vals && (h->cplx ? h->z->host.nslots : h->r->host.nslots) > 0))
+    return fail(KH_BAD_ARGUMENT, "null argument");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (h->cplx)
+    return tear_batch_solve(*h->z, h->jt, reinterpret_cast<const kcplx*>(d_vals), reinterpret_cast<const kcplx*>(d_b),
+                            reinterpret_cast<kcplx*>(d_x), d_status, st);
+  return tear_batch_solve(*h->r, h->jt, d_vals, d_b, d_x, d_status, st);
+}
+
+kh_status kh_tear_batch_pieces(kh_tear_batch h, int32_t* npieces, int32_t* piece_of) {
+  if (!h) return fail(KH_BAD_ARGUMENT, "null handle");
+  return tear_pieces(h->cplx ? h->z->host : h->r->host, npieces, piece_of);
+}
+
+void kh_tear_batch_destroy(kh_tear_batch h) { delete h; }
 
 }  // extern "C"

@@ -95,6 +95,23 @@ def lib():
                                          C.c_int32, P, P, P]
         _lib.oracle_batch_tran.argtypes = [C.c_int32, P, C.c_int32, C.c_int32, C.c_int32, P, P, P, P,
                                            C.c_int32, P, P, P, P, P]
+        _lib.oracle_tear_new.restype = P
+        _lib.oracle_tear_new.argtypes = [P, C.c_int32, C.c_double]
+        _lib.oracle_tear_free.argtypes = [P]
+        _lib.oracle_tear_use.argtypes = [P]
+        _lib.oracle_tear_stats.argtypes = [P, P]
+        _lib.oracle_tmat_new.restype = P
+        _lib.oracle_tmat_new.argtypes = [P, C.c_int32, C.c_int32]
+        _lib.oracle_tmat_free.argtypes = [P, C.c_int32]
+        _lib.oracle_tmat_clear.argtypes = [P, C.c_int32]
+        _lib.oracle_tmat_add_batch.argtypes = [P, C.c_int32, P, P, P, C.c_int64]
+        _lib.oracle_tmat_solve.argtypes = [P, C.c_int32, P, P]
+        _lib.oracle_tmat_absproduct.argtypes = [P, C.c_int32, P, P]
+        _lib.oracle_tmat_pieces.argtypes = [P, C.c_int32, P]
+        _lib.oracle_tear_record.argtypes = [C.c_int32]
+        _lib.oracle_tear_rec_count.restype = C.c_int64
+        _lib.oracle_tear_rec_info.argtypes = [C.c_int64, P, P, P, P, P]
+        _lib.oracle_tear_rec_get.argtypes = [C.c_int64, P, P, P, P, P]
     return _lib
 
 
@@ -159,6 +176,111 @@ class Sparse:
             x = np.ascontiguousarray(x, dtype=np.float64)
             lib().oracle_sparse_absproduct(self.h, _p(x), _p(y))
         return y
+
+
+TEAR_STATS = ("solves", "factors", "reuses", "stale", "fallbacks", "refines", "max_pieces")
+
+
+class Tearing:
+    """kron.JacobianTearing restatement (kron/jacobian.go:66-712): the cut
+    node pairs (unknown indices, -1 ground) and the counters its matrices add
+    to. `with tearing:` makes every matrix the oracle's analyses create inside
+    the block a torn one of this tearing (Options.Solver)."""
+
+    def __init__(self, cuts, reuse_tol: float = 0.0):
+        c = np.ascontiguousarray(np.asarray(cuts, np.int32).reshape(-1, 2))
+        self.cuts = c
+        self.h = lib().oracle_tear_new(_p(c) if len(c) else None, len(c), reuse_tol)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_tear_free(self.h)
+            self.h = None
+
+    def __enter__(self):
+        lib().oracle_tear_use(self.h)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_tear_use(None)
+
+    def stats(self) -> dict:
+        out = np.zeros(7, np.int64)
+        lib().oracle_tear_stats(self.h, _p(out))
+        return dict(zip(TEAR_STATS, (int(v) for v in out)))
+
+    def matrix(self, n: int, complex_: bool = False) -> "TornMatrix":
+        return TornMatrix(self, n, complex_)
+
+
+class TornMatrix:
+    """One torn system (tornMat, kron/jacobian.go:199-589)."""
+
+    def __init__(self, tear: Tearing, n: int, complex_: bool = False):
+        self.tear = tear  # keeps the tearing alive
+        self.n = n
+        self.cplx = 1 if complex_ else 0
+        self.h = lib().oracle_tmat_new(tear.h, n, self.cplx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_tmat_free(self.h, self.cplx)
+            self.h = None
+
+    def clear(self):
+        lib().oracle_tmat_clear(self.h, self.cplx)
+
+    def add_batch(self, i, j, v):
+        i = np.ascontiguousarray(i, np.int32)
+        j = np.ascontiguousarray(j, np.int32)
+        v = np.ascontiguousarray(v, np.complex128 if self.cplx else np.float64)
+        lib().oracle_tmat_add_batch(self.h, self.cplx, _p(i), _p(j), _p(v), len(i))
+
+    def solve(self, b) -> Tuple[int, np.ndarray]:
+        dt = np.complex128 if self.cplx else np.float64
+        b = np.ascontiguousarray(b, dtype=dt)
+        x = np.zeros(self.n, dtype=dt)
+        e = lib().oracle_tmat_solve(self.h, self.cplx, _p(b), _p(x))
+        return e, x
+
+    def absproduct(self, x) -> np.ndarray:
+        y = np.zeros(self.n)
+        x = np.ascontiguousarray(x, np.complex128 if self.cplx else np.float64)
+        lib().oracle_tmat_absproduct(self.h, self.cplx, _p(x), _p(y))
+        return y
+
+    def pieces(self):
+        po = np.zeros(self.n, np.int32)
+        k = lib().oracle_tmat_pieces(self.h, self.cplx, _p(po))
+        return k, po
+
+
+class TearRecording:
+    """`with TearRecording() as rec:` records every torn solve the oracle
+    makes inside the block; rec.records then lists, per solve in call order,
+    dict(mat, cplx, n, status, ri, ci, val, b, x) (complex arrays when cplx)."""
+
+    def __enter__(self):
+        lib().oracle_tear_record(1)
+        self.records = []
+        return self
+
+    def __exit__(self, *exc):
+        L = lib()
+        for k in range(L.oracle_tear_rec_count()):
+            mat, cp, n, st = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+            ns = C.c_int64()
+            L.oracle_tear_rec_info(k, C.byref(mat), C.byref(cp), C.byref(n), C.byref(ns), C.byref(st))
+            dt = np.complex128 if cp.value else np.float64
+            ri = np.zeros(ns.value, np.int32)
+            ci = np.zeros(ns.value, np.int32)
+            val = np.zeros(ns.value, dt)
+            b = np.zeros(n.value, dt)
+            x = np.zeros(n.value, dt)
+            L.oracle_tear_rec_get(k, _p(ri), _p(ci), _p(val), _p(b), _p(x))
+            self.records.append(dict(mat=mat.value, cplx=cp.value, n=n.value, status=st.value, ri=ri, ci=ci,
+                                     val=val, b=b, x=x))
+        L.oracle_tear_record(0)
 
 
 def min_degree(n: int, ri, ci) -> np.ndarray:

@@ -20,6 +20,7 @@
 
 #include "devices.hpp"
 #include "sparse.hpp"
+#include "tear.hpp"
 
 namespace oracle {
 
@@ -59,36 +60,63 @@ void Mosfet::loadAC(ACContext& ctx) {
   if (sp != u[1]) stampY(ctx.M, u[1], sp, cplx(gspr, 0));
 }
 
-// Real system: Sparse or Dense behind one stamper (solver/matrix.go:23-36, :77-93).
+// Options.Solver for the analyses the tests run through the C entry points:
+// when set, every real and complex system an analysis makes is a torn one of
+// this JacobianTearing (analysis/op.go:228-241 useSolver, :73-78 cmatrix).
+static Tearing* g_tear = nullptr;
+
+// Real system: Sparse or Dense behind one stamper (solver/matrix.go:23-36,
+// :77-93), or the factory's torn matrix.
 struct RealMatrix : Stamper {
   bool sparse = true;
   SparseLU<double> s;
   Dense d;
+  std::unique_ptr<TornMat<double>> t;
   int n = 0;
   int init(int n_, bool sp) {
     n = n_;
     sparse = sp;
+    if (g_tear) {
+      t = std::make_unique<TornMat<double>>();
+      t->init(n_, g_tear);
+      return OK;
+    }
     return sp ? s.init(n_) : d.init(n_);
   }
-  void add(int i, int j, double v) override { if (sparse) s.add(i, j, v); else d.add(i, j, v); }
-  void clear() { if (sparse) s.clear(); else d.clear(); }
-  int solve(const double* b, double* x) { return sparse ? s.solve(b, x) : d.solve(b, x); }
-  void absProduct(const double* x, double* y) const { if (sparse) s.absProduct(x, y); else d.absProduct(x, y); }
+  void add(int i, int j, double v) override { if (t) t->add(i, j, v); else if (sparse) s.add(i, j, v); else d.add(i, j, v); }
+  void clear() { if (t) t->clear(); else if (sparse) s.clear(); else d.clear(); }
+  int solve(const double* b, double* x) { return t ? t->solve(b, x) : sparse ? s.solve(b, x) : d.solve(b, x); }
+  void absProduct(const double* x, double* y) const {
+    if (t) t->absProduct(x, y);
+    else if (sparse) s.absProduct(x, y);
+    else d.absProduct(x, y);
+  }
 };
 
-// Complex system: CSparse or CDense (solver/matrix.go:95-101).
+// Complex system: CSparse or CDense (solver/matrix.go:95-101), or the
+// factory's torn complex matrix.
 struct ComplexMatrix : CStamper {
   bool sparse = true;
   SparseLU<cplx> s;
   CDense d;
+  std::unique_ptr<TornMat<cplx>> t;
   int init(int n_, bool sp) {
     sparse = sp;
+    if (g_tear) {
+      t = std::make_unique<TornMat<cplx>>();
+      t->init(n_, g_tear);
+      return OK;
+    }
     return sp ? s.init(n_) : d.init(n_);
   }
-  void add(int i, int j, cplx v) override { if (sparse) s.add(i, j, v); else d.add(i, j, v); }
-  void clear() { if (sparse) s.clear(); else d.clear(); }
-  int solve(const cplx* b, cplx* x) { return sparse ? s.solve(b, x) : d.solve(b, x); }
-  void absProduct(const cplx* x, double* y) const { if (sparse) s.absProduct(x, y); else d.absProduct(x, y); }
+  void add(int i, int j, cplx v) override { if (t) t->add(i, j, v); else if (sparse) s.add(i, j, v); else d.add(i, j, v); }
+  void clear() { if (t) t->clear(); else if (sparse) s.clear(); else d.clear(); }
+  int solve(const cplx* b, cplx* x) { return t ? t->solve(b, x) : sparse ? s.solve(b, x) : d.solve(b, x); }
+  void absProduct(const cplx* x, double* y) const {
+    if (t) t->absProduct(x, y);
+    else if (sparse) s.absProduct(x, y);
+    else d.absProduct(x, y);
+  }
 };
 
 // Backend.Resolve, solver/matrix.go:77-85: 0 auto, 1 dense, 2 sparse.
@@ -1127,6 +1155,92 @@ int oracle_ac_ex(int32_t num_nodes, const kh_element* el, int32_t nel, const dou
 }
 
 // ---- raw sparse-LU access (solver.Sparse seam) for golden-vector tests -------
+// ---- JacobianTearing (kron/jacobian.go:66-712) --------------------------------
+// A Tearing of the given cut pairs (node unknowns, -1 ground; [2*ncuts]), its
+// torn matrices, and the switch that makes the analyses' matrices torn ones
+// (Options.Solver). Single-threaded: the counters are plain integers.
+void* oracle_tear_new(const int32_t* cuts, int32_t ncuts, double reuse_tol) {
+  auto* t = new Tearing();
+  for (int k = 0; k < ncuts; k++) t->cuts.push_back({cuts[2 * k], cuts[2 * k + 1]});
+  t->reuseTol = reuse_tol;
+  return t;
+}
+void oracle_tear_free(void* t) {
+  if (g_tear == t) g_tear = nullptr;
+  delete static_cast<Tearing*>(t);
+}
+void oracle_tear_use(void* t) { g_tear = static_cast<Tearing*>(t); }
+// solves, factors, reuses, stale reuses, fallbacks, refinements, max pieces
+void oracle_tear_stats(void* h, int64_t* out) {
+  const Tearing* t = static_cast<const Tearing*>(h);
+  out[0] = t->solves; out[1] = t->factors; out[2] = t->reuses; out[3] = t->stale;
+  out[4] = t->fallbacks; out[5] = t->refines; out[6] = t->maxPieces;
+}
+void* oracle_tmat_new(void* tear, int32_t n, int32_t cplx_) {
+  if (cplx_) {
+    auto* m = new TornMat<cplx>();
+    m->init(n, static_cast<Tearing*>(tear));
+    return m;
+  }
+  auto* m = new TornMat<double>();
+  m->init(n, static_cast<Tearing*>(tear));
+  return m;
+}
+void oracle_tmat_free(void* h, int32_t cplx_) {
+  if (cplx_) delete static_cast<TornMat<cplx>*>(h);
+  else delete static_cast<TornMat<double>*>(h);
+}
+void oracle_tmat_clear(void* h, int32_t cplx_) {
+  if (cplx_) static_cast<TornMat<cplx>*>(h)->clear();
+  else static_cast<TornMat<double>*>(h)->clear();
+}
+// v holds (re, im) pairs when complex
+void oracle_tmat_add_batch(void* h, int32_t cplx_, const int32_t* i, const int32_t* j, const double* v, int64_t count) {
+  for (int64_t k = 0; k < count; k++) {
+    if (cplx_) static_cast<TornMat<cplx>*>(h)->add(i[k], j[k], cplx(v[2 * k], v[2 * k + 1]));
+    else static_cast<TornMat<double>*>(h)->add(i[k], j[k], v[k]);
+  }
+}
+int oracle_tmat_solve(void* h, int32_t cplx_, const double* b, double* x) {
+  if (cplx_)
+    return static_cast<TornMat<cplx>*>(h)->solve(reinterpret_cast<const cplx*>(b), reinterpret_cast<cplx*>(x));
+  return static_cast<TornMat<double>*>(h)->solve(b, x);
+}
+void oracle_tmat_absproduct(void* h, int32_t cplx_, const double* x, double* y) {
+  if (cplx_) static_cast<TornMat<cplx>*>(h)->absProduct(reinterpret_cast<const cplx*>(x), y);
+  else static_cast<TornMat<double>*>(h)->absProduct(x, y);
+}
+// Recording of torn solves (TearRecord): on / off (off drops the records),
+// their count, and one record's sizes and contents.
+static std::vector<TearRecord> g_records;
+void oracle_tear_record(int32_t on) {
+  g_records.clear();
+  g_tear_rec = on ? &g_records : nullptr;
+}
+int64_t oracle_tear_rec_count() { return (int64_t)g_records.size(); }
+void oracle_tear_rec_info(int64_t k, int32_t* mat, int32_t* cplx_, int32_t* n, int64_t* nslots, int32_t* status) {
+  const TearRecord& r = g_records[k];
+  *mat = r.mat; *cplx_ = r.cplx; *n = r.n; *nslots = (int64_t)r.ri.size(); *status = r.status;
+}
+void oracle_tear_rec_get(int64_t k, int32_t* ri, int32_t* ci, double* val, double* b, double* x) {
+  const TearRecord& r = g_records[k];
+  std::copy(r.ri.begin(), r.ri.end(), ri);
+  std::copy(r.ci.begin(), r.ci.end(), ci);
+  std::copy(r.val.begin(), r.val.end(), val);
+  std::copy(r.b.begin(), r.b.end(), b);
+  std::copy(r.x.begin(), r.x.end(), x);
+}
+// Pieces of the held plan (-1: none), and the piece of each unknown [n].
+int oracle_tmat_pieces(void* h, int32_t cplx_, int32_t* piece_of) {
+  auto get = [&](auto* m) -> int {
+    if (!m->plan) return -1;
+    if (piece_of) for (int i = 0; i < m->n; i++) piece_of[i] = m->plan->pieceOf[i];
+    return (int)m->plan->pieces.size();
+  };
+  if (cplx_) return get(static_cast<TornMat<cplx>*>(h));
+  return get(static_cast<TornMat<double>*>(h));
+}
+
 void* oracle_sparse_new(int32_t n, int32_t cplx_) {
   if (cplx_) {
     auto* s = new SparseLU<cplx>();
