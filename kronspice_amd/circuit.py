@@ -24,8 +24,9 @@ import numpy as np
 
 # Element kinds (include/kronhip.h)
 R, C, L, V, I, D, M, Q = 1, 2, 3, 4, 5, 6, 7, 8
-KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q"}
-NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 58, Q: 52}
+E, F, G, H = 10, 11, 12, 13  # VCVS, CCCS, VCCS, CCVS
+KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q", E: "E", F: "F", G: "G", H: "H"}
+NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 58, Q: 52, E: 1, F: 1, G: 1, H: 1}
 
 ELEMENT_DTYPE = np.dtype([("kind", "<i4"), ("node", "<i4", (4,)), ("poff", "<i4")])
 
@@ -87,6 +88,8 @@ class Element:
     model: str = ""
     area: float = 0.0
     params: Dict[str, float] = field(default_factory=dict)
+    ctrl_nodes: Tuple[int, ...] = ()  # E, G: nc+, nc- (node numbers)
+    ctrl_source: str = ""             # F, H: the controlling element's name, lower-cased
 
 
 class Circuit:
@@ -130,7 +133,7 @@ class Circuit:
             return i
 
         for k, e in enumerate(self.elements):
-            if e.kind in (V, L):
+            if e.kind in (V, L, E, H):
                 branch_of[k] = alloc()
             elif e.kind == D:
                 p = diode_params(self, e)
@@ -173,6 +176,14 @@ class Circuit:
         for k, e in enumerate(self.elements):
             els[k]["kind"] = e.kind
             nodes = list(e.nodes) + [0] * (4 - len(e.nodes))  # a BJT without substrate: ground
+            if e.kind in (E, G):
+                nodes = [e.nodes[0], e.nodes[1], e.ctrl_nodes[0], e.ctrl_nodes[1]]
+            elif e.kind in (F, H):  # node[2]: the controlling element's index (Link, device/controlled.go:67-77)
+                try:
+                    ctl = self.element_index(e.ctrl_source)
+                except KeyError:
+                    raise ValueError(f"{e.name}: controlling source {e.ctrl_source} not found") from None
+                nodes = [e.nodes[0], e.nodes[1], ctl, 0]
             els[k]["node"] = nodes[:4]
             els[k]["poff"] = len(params)
             params.extend(element_params(self, e, step, stop))
@@ -241,6 +252,27 @@ class Builder:
         # AddMulti(e, drain, source, gate, bulk): circuit/builder.go:113-118
         e = Element(name, M, (), model=model.lower(), params={k.lower(): float(v) for k, v in (params or {}).items()})
         return self.add(e, drain, source, gate, bulk)
+
+    def controlled(self, name, kind, p, n, gain, ctrl_nodes=None, ctrl_source: str = ""):
+        """E and G (ctrl_nodes = (nc+, nc-)), F and H (ctrl_source): the control
+        nodes are numbered before the output nodes, as circuit.FromDeck and
+        Builder.VCVS / VCCS register them (circuit/builder.go:119-127, :409-412)."""
+        e = Element(name, kind, (), value=float(gain), ctrl_source=ctrl_source.lower())
+        if kind in (E, G):
+            e.ctrl_nodes = (self.node(ctrl_nodes[0]), self.node(ctrl_nodes[1]))
+        return self.add(e, p, n)
+
+    def vcvs(self, name, p, n, cp, cn, gain):
+        return self.controlled(name, E, p, n, gain, ctrl_nodes=(cp, cn))
+
+    def vccs(self, name, p, n, cp, cn, gm):
+        return self.controlled(name, G, p, n, gm, ctrl_nodes=(cp, cn))
+
+    def cccs(self, name, p, n, vname, gain):
+        return self.controlled(name, F, p, n, gain, ctrl_source=vname)
+
+    def ccvs(self, name, p, n, vname, r):
+        return self.controlled(name, H, p, n, r, ctrl_source=vname)
 
     def bjt(self, name, collector, base, emitter, model, substrate: Optional[str] = None, area: float = 0.0):
         # AddMulti(e, collector, emitter, base[, substrate]): circuit/builder.go:99-103, netlist
@@ -555,6 +587,8 @@ def element_params(c: Circuit, e: Element, step: float, stop: float) -> List[flo
         return mosfet_params(c, e)
     if e.kind == Q:
         return bjt_params(c, e)
+    if e.kind in (E, F, G, H):
+        return [e.value]  # the gain (device/controlled.go:45-51)
     raise NotImplementedError(f"element kind {e.kind} ({e.name}) is not on this path")
 
 

@@ -189,6 +189,30 @@ KH_HD inline void ac_load(ACLane& A, int nel) {
         if (L.T(e, 6) != L.T(e, 1)) A.stampY(e, 1, 6, {L.P(e, QP_GEPR), 0});
         break;
       }
+      case K_E:
+      case K_F:
+      case K_G:
+      case K_H: {  // the DC stamps with a real gain (device/controlled.go:138-143)
+        const double g = L.P(e, 0);
+        const int kind = e[ET_KIND];
+        if (kind == K_E || kind == K_H) {
+          A.add(e, 0, 2, {1, 0});
+          A.add(e, 1, 2, {-1, 0});
+          A.add(e, 2, 0, {1, 0});
+          A.add(e, 2, 1, {-1, 0});
+          A.add(e, 2, 3, {-g, 0});
+          if (kind == K_E) A.add(e, 2, 4, {g, 0});
+        } else if (kind == K_G) {
+          A.add(e, 0, 3, {g, 0});
+          A.add(e, 0, 4, {-g, 0});
+          A.add(e, 1, 3, {-g, 0});
+          A.add(e, 1, 4, {g, 0});
+        } else {
+          A.add(e, 0, 3, {g, 0});
+          A.add(e, 1, 3, {-g, 0});
+        }
+        break;
+      }
     }
   }
 }
@@ -257,8 +281,12 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
         case K_R: i = v * kcplx{L.P(e, 1), 0}; break;
         case K_C: i = v * kcplx{0, A.omega * L.P(e, 0)}; break;
         case K_L:
-        case K_V: i = A.xat(x, L.T(e, 2)); break;
+        case K_V:
+        case K_E:
+        case K_H: i = A.xat(x, L.T(e, 2)); break;
         case K_I: i = source_ac(L, e); break;
+        case K_G: i = kcplx{L.P(e, 0), 0} * (A.xat(x, L.T(e, 3)) - A.xat(x, L.T(e, 4))); break;  // OperatingAC
+        case K_F: i = kcplx{L.P(e, 0), 0} * A.xat(x, L.T(e, 3)); break;
         case K_D: {
           kcplx vj = A.xat(x, L.T(e, 2)) - A.xat(x, L.T(e, 1));
           double vd = L.S(e, 10), id, g0, q, cc;
@@ -291,7 +319,7 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
         else if (port > 0) inflow += port;
         else outflow += -port;
       }
-      if (kind == K_R && L.P(e, 0) < 0 && pw != 0) sup += -pw;  // AddModelSupplied
+      if (((kind == K_R && L.P(e, 0) < 0) || is_controlled(kind)) && pw != 0) sup += -pw;  // AddModelSupplied
     }
   }
   *sumS = sum;

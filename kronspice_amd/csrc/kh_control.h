@@ -221,6 +221,10 @@ struct CtlT {
         case K_D: if (KH_HAS(K_D)) load_D(lane, e); break;
         case K_M: if (KH_HAS(K_M)) load_M(lane, e); break;
         case K_Q: if (KH_HAS(K_Q)) load_Q(lane, e); break;
+        case K_E: if (KH_HAS(K_E)) load_ctl(lane, e); break;
+        case K_F: if (KH_HAS(K_F)) load_ctl(lane, e); break;
+        case K_G: if (KH_HAS(K_G)) load_ctl(lane, e); break;
+        case K_H: if (KH_HAS(K_H)) load_ctl(lane, e); break;
       }
     }
     L = lane;
@@ -414,7 +418,7 @@ struct CtlT {
         stor += o.storage;
         scale += fabs(o.storage);
       }
-      if (kind == K_R && L.P(e, 0) < 0) {
+      if ((kind == K_R && L.P(e, 0) < 0) || is_controlled(kind)) {  // model-supplied power
         double ms = -o.power;
         if (ms != 0) { sup += ms; scale += fabs(ms); }
       }
@@ -450,9 +454,9 @@ struct CtlT {
         else outflow += -pw;
       }
     }
-    for (int k = 0; k < A.ckt.nel; k++) {  // model-supplied (negative resistors)
+    for (int k = 0; k < A.ckt.nel; k++) {  // model-supplied (negative resistors, controlled sources)
       const int32_t* e = el(k);
-      if (e[ET_KIND] == K_R && L.P(e, 0) < 0) {
+      if ((e[ET_KIND] == K_R && L.P(e, 0) < 0) || is_controlled(e[ET_KIND])) {
         OpPt o = operating(L, e, x_);
         double ms = -o.power;
         if (ms != 0) { sup += ms; scale += fabs(ms); }

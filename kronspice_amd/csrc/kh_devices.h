@@ -32,11 +32,14 @@ enum {
 enum { K_R = 1, K_C = 2, K_L = 3, K_V = 4, K_I = 5, K_D = 6, K_M = 7, K_Q = 8 };
 // K_M23: not an element kind, the bit of the MOSFET level 2 and 3 currents.
 enum { K_M23 = 9 };
+// Linear controlled sources: E (VCVS), F (CCCS), G (VCCS), H (CCVS).
+enum { K_E = 10, K_F = 11, K_G = 12, K_H = 13 };
+KH_HD inline bool is_controlled(int kind) { return kind >= K_E && kind <= K_H; }
 
 // Element kinds a translation unit's circuit code handles (bit k: kind k).
 // The persistent kernel is compiled once per set, so a batch runs code for
 // the devices it has and nothing else (smaller code, fewer registers).
-#define KH_KINDS_ALL 1022
+#define KH_KINDS_ALL 16382
 #ifndef KH_KINDS
 #define KH_KINDS KH_KINDS_ALL
 #endif
@@ -47,7 +50,7 @@ enum { MODE_OP = 0, MODE_TRANOP = 1, MODE_UICINIT = 2, MODE_TRAN = 3 };
 enum { BE = 1, TRAP = 2 };
 
 // State sizes (doubles per element).
-enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96, NS_Q = 66 };
+enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96, NS_Q = 66, NS_CTL = 0 };
 
 // history (device/lte.go:23-37) lives in 7 state words: t0 t1 t2 q0 q1 q2 n.
 constexpr double kInf = __builtin_huge_val();
@@ -132,6 +135,10 @@ KH_HD inline int record_capacity(int kind) {
     case K_D: return 10;
     case K_M: return 64;
     case K_Q: return 68;
+    case K_E: return 6;
+    case K_F: return 2;
+    case K_G: return 4;
+    case K_H: return 5;
   }
   return 0;
 }
@@ -264,6 +271,53 @@ KH_HD inline void load_R(Lane& L, const int32_t* e) {
   L.add(e, 1, 1, g);
   L.add(e, 0, 1, -g);
   L.add(e, 1, 0, -g);
+}
+
+// ---- E, F, G, H (device/controlled.go:76-161) --------------------------------
+// terminals: 0 n+, 1 n-, 2 the element's own branch (E, H), 3 nc+ (E, G) or
+// the controlling element's branch (F, H), 4 nc- (E, G); parameter 0 the gain.
+// Stamps in the reference's order; no state, no right-hand side.
+KH_HD inline void load_ctl(Lane& L, const int32_t* e) {
+  const double g = L.P(e, 0);
+  const int kind = e[ET_KIND];
+  if (kind == K_E || kind == K_H) {
+    L.add(e, 0, 2, 1);
+    L.add(e, 1, 2, -1);
+    L.add(e, 2, 0, 1);
+    L.add(e, 2, 1, -1);
+    if (kind == K_E) {
+      L.add(e, 2, 3, -g);
+      L.add(e, 2, 4, g);
+    } else {
+      L.add(e, 2, 3, -g);
+    }
+    return;
+  }
+  if (kind == K_G) {
+    L.add(e, 0, 3, g);
+    L.add(e, 0, 4, -g);
+    L.add(e, 1, 3, -g);
+    L.add(e, 1, 4, g);
+  } else {
+    L.add(e, 0, 3, g);
+    L.add(e, 1, 3, -g);
+  }
+}
+// Controlled.Operating: the output port's voltage and current (n+ -> n-
+// through the element), absorbed power v·i.
+KH_HD inline OpPt op_ctl(const Lane& L, const int32_t* e, const double* x) {
+  OpPt o;
+  const int kind = e[ET_KIND];
+  o.voltage = L.volt(e, x);
+  if (kind == K_E || kind == K_H) {
+    o.current = L.xat(x, L.T(e, 2));
+  } else {
+    const double ctl = kind == K_G ? L.xat(x, L.T(e, 3)) - L.xat(x, L.T(e, 4)) : L.xat(x, L.T(e, 3));
+    o.current = L.P(e, 0) * ctl;
+  }
+  o.power = o.voltage * o.current;
+  o.storage = 0;
+  return o;
 }
 
 // ---- C (device/reactive.go:60-181) ------------------------------------------
@@ -1383,6 +1437,7 @@ KH_HD inline OpPt operating_point(const Lane& L, const int32_t* e, const double*
     case K_D: if (KH_HAS(K_D)) return op_D(L, e, x); break;
     case K_M: if (KH_HAS(K_M)) return op_M(L, e, x); break;
     case K_Q: if (KH_HAS(K_Q)) return op_Q(L, e, x); break;
+    case K_E: case K_F: case K_G: case K_H: return op_ctl(L, e, x);
   }
   OpPt o = {0, 0, 0, 0};
   return o;

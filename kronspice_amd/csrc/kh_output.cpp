@@ -206,6 +206,10 @@ const char* kind_name(int k) {
     case KH_ELEM_D: return "diode";
     case KH_ELEM_M: return "MOSFET";
     case KH_ELEM_Q: return "BJT";
+    case KH_ELEM_E: return "VCVS";
+    case KH_ELEM_F: return "CCCS";
+    case KH_ELEM_G: return "VCCS";
+    case KH_ELEM_H: return "CCVS";
   }
   return "unknown";
 }
@@ -213,7 +217,7 @@ const char* kind_name(int k) {
 // The element's modeled ports in channel order, with their notes (the
 // SetIntrinsic calls of device/passive.go:51-59, device/reactive.go:45, :216,
 // device/source.go:221-234, device/diode.go:125-126, device/mosfet.go:211-212,
-// device/bjt.go:238-239): (channel, note).
+// device/bjt.go:238-239, device/controlled.go:49-50): (channel, note).
 std::vector<std::pair<int, const char*>> ports_of(int kind, bool negative) {
   const char* inside = "terminals connect inside the network";
   switch (kind) {
@@ -233,6 +237,10 @@ std::vector<std::pair<int, const char*>> ports_of(int kind, bool negative) {
     case KH_ELEM_Q:
       return {{0, "terminals connect inside the network; stores junction charge"},
               {2, "junction, transport and series-resistance heating to surroundings"}};
+    case KH_ELEM_E:
+    case KH_ELEM_F:
+    case KH_ELEM_G:
+    case KH_ELEM_H: return {{0, "idealized controlled source: its power, of either sign, is model-supplied"}};
   }
   return {};
 }

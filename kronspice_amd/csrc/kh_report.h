@@ -16,7 +16,7 @@ enum {
   RP_POWER,         // absorbed
   RP_STORAGE,       // part of the absorbed power going into storage
   RP_ENERGY,        // stored energy [J]
-  RP_SUPPLIED,      // model-supplied power (negative resistances)
+  RP_SUPPLIED,      // model-supplied power (negative resistances, controlled sources)
   RP_EXTERIOR,      // power through the modeled exterior port: + in from the exterior
   RP_CHANNEL,       // that port's channel: 0 electrical, 2 thermal, -1 none
   RP_FIELDS
@@ -167,6 +167,12 @@ KH_HD inline void report_element(const Lane& L, const int32_t* e, const double* 
       if (KH_HAS(K_Q)) energy = stored_Q(L, e, x);
       exterior = -(o.power - o.storage);
       channel = 2;
+      break;
+    case K_E:
+    case K_F:
+    case K_G:
+    case K_H:  // device/controlled.go:132-136: all of its power is model-supplied
+      supplied = -o.power;
       break;
   }
   out[RP_VOLTAGE] = o.voltage;

@@ -201,6 +201,13 @@ static const int kPairsQ[][2] = {
     {0, 0}, {4, 4}, {0, 4}, {4, 0},                                  // RC
     {1, 1}, {6, 6}, {1, 6}, {6, 1}};                                 // RE
 
+// Controlled sources (device/controlled.go:76-107): terminals 0 n+, 1 n-, 2
+// own branch, 3 nc+ or the controlling branch, 4 nc-; pairs in stamp order.
+static const int kPairsE[6][2] = {{0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 3}, {2, 4}};
+static const int kPairsH[5][2] = {{0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 3}};
+static const int kPairsG[4][2] = {{0, 3}, {0, 4}, {1, 3}, {1, 4}};
+static const int kPairsF[2][2] = {{0, 3}, {1, 3}};
+
 bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0) {
   num_nodes = numNodes;
   nel = n_elems;
@@ -286,6 +293,20 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         nonlin = true;
         break;
       }
+      case K_E:
+      case K_F:
+      case K_G:
+      case K_H:
+        t[ET_TERM] = unk(E.node[0]);
+        t[ET_TERM + 1] = unk(E.node[1]);
+        if (E.kind == K_E || E.kind == K_H) t[ET_TERM + 2] = alloc();  // Setup: the branch of a voltage output
+        if (E.kind == K_E || E.kind == K_G) {
+          t[ET_TERM + 3] = unk(E.node[2]);
+          t[ET_TERM + 4] = unk(E.node[3]);
+        }
+        ns = NS_CTL;
+        npar = KH_NPAR_CTL;
+        break;
       default:
         error = "element " + std::to_string(e) + ": kind " + std::to_string(E.kind) + " is not supported";
         return false;
@@ -307,6 +328,19 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     }
   }
   nstate = soff;
+  // Link (device/controlled.go:67-77): F and H sense the branch current of the
+  // element node[2] names, which must own a registered branch (V, L, E, H).
+  for (int e = 0; e < nel; e++) {
+    const kh_element& E = elems[e];
+    if (E.kind != K_F && E.kind != K_H) continue;
+    const int c = E.node[2];
+    const int ck = (c >= 0 && c < nel) ? elems[c].kind : -1;
+    if (ck != K_V && ck != K_L && ck != K_E && ck != K_H) {
+      error = "element " + std::to_string(e) + ": controlling source not found";
+      return false;
+    }
+    et[(size_t)e * ET_W + ET_TERM + 3] = et[(size_t)c * ET_W + ET_TERM + 2];
+  }
   n = numNodes - 1 + branches;
   if (n <= 0) {
     error = "circuit has no unknowns";
@@ -340,6 +374,10 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
       case K_D: pairs = kPairsD; np2 = 7; break;
       case K_M: pairs = kPairsM; np2 = (int)(sizeof(kPairsM) / sizeof(kPairsM[0])); break;
       case K_Q: pairs = kPairsQ; np2 = (int)(sizeof(kPairsQ) / sizeof(kPairsQ[0])); break;
+      case K_E: pairs = kPairsE; np2 = 6; break;
+      case K_F: pairs = kPairsF; np2 = 2; break;
+      case K_G: pairs = kPairsG; np2 = 4; break;
+      case K_H: pairs = kPairsH; np2 = 5; break;
       default: break;
     }
     for (int k = 0; k < KH_NT * KH_NT; k++) t[ET_SLOT + k] = -1;

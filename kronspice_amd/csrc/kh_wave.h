@@ -544,6 +544,10 @@ struct WCtl : CtlT<WCtl> {
           case K_D: if (KH_HAS(K_D)) load_D(l, e); break;
           case K_M: if (KH_HAS(K_M)) load_M(l, e); break;
           case K_Q: if (KH_HAS(K_Q)) load_Q(l, e); break;
+          case K_E: if (KH_HAS(K_E)) load_ctl(l, e); break;
+          case K_F: if (KH_HAS(K_F)) load_ctl(l, e); break;
+          case K_G: if (KH_HAS(K_G)) load_ctl(l, e); break;
+          case K_H: if (KH_HAS(K_H)) load_ctl(l, e); break;
         }
         if (l.rn > l.rcap) over = true;
         R.rcnt[lane] = min(l.rn, l.rcap);
@@ -758,7 +762,7 @@ struct WCtl : CtlT<WCtl> {
         Tpw[lane] = pw;
         Tch[lane] = ch + (rx ? 16 : 0);  // the channel, and whether the element stores energy
         Tst[lane] = rx ? o.storage : 0.0;
-        Tms[lane] = (kind == K_R && Lc.P(e, 0) < 0) ? -o.power : 0.0;
+        Tms[lane] = ((kind == K_R && Lc.P(e, 0) < 0) || is_controlled(kind)) ? -o.power : 0.0;
       }
       lds_fence();
       const int cnt = min(kWaveLanes, A.ckt.nel - k0);
@@ -814,7 +818,7 @@ struct WCtl : CtlT<WCtl> {
       double ms = 0;
       if (k < A.ckt.nel) {
         const int32_t* e = A.ckt.et + (int64_t)k * ET_W;
-        if (e[ET_KIND] == K_R && Lc.P(e, 0) < 0) ms = -operating(Lc, e, A.x).power;
+        if ((e[ET_KIND] == K_R && Lc.P(e, 0) < 0) || is_controlled(e[ET_KIND])) ms = -operating(Lc, e, A.x).power;
       }
       if (!wany(ms != 0)) continue;
       Tpw[lane] = ms;

@@ -14,6 +14,7 @@ import re
 from dataclasses import dataclass, field
 from typing import Dict, List, Optional, Tuple
 
+from . import circuit as kc
 from .circuit import Builder, Circuit, Wave, WAVE_EXP, WAVE_PULSE, WAVE_SIN
 
 _NUM = re.compile(r"^[+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?")
@@ -210,6 +211,15 @@ def parse(text: str) -> Deck:
             else:
                 sub, model, area = t[3], t[4], parse_value(t[5])
             b.bjt(name, t[0], t[1], t[2], model, substrate=sub, area=area)
+        elif k in ("E", "G"):  # Ename n+ n- nc+ nc- gain, netlist/parser.go:468-478
+            if len(tok) != 6:
+                raise ValueError(f"{name}: {k} element needs: name n+ n- nc+ nc- gain")
+            b.controlled(name, kc.E if k == "E" else kc.G, tok[1], tok[2], parse_value(tok[5]),
+                         ctrl_nodes=(tok[3], tok[4]))
+        elif k in ("F", "H"):  # Fname n+ n- vname gain, netlist/parser.go:620-630
+            if len(tok) != 5:
+                raise ValueError(f"{name}: {k} element needs: name n+ n- vname gain")
+            b.controlled(name, kc.F if k == "F" else kc.H, tok[1], tok[2], parse_value(tok[4]), ctrl_source=tok[3])
         else:
             raise NotImplementedError(f"element {name} is not on this path")
     return Deck(title, b.build(), analyses)

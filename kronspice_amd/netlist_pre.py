@@ -566,11 +566,13 @@ class Expander:
     def rename(self, text: str, s: Scope, inst: _Inst) -> str:
         """rename, netlist/preprocess.go:610-632, for the element kinds this
         path reads: R, C, L, V, I (two nodes), D (two nodes, model), M (four
-        nodes, model), Q (three or four nodes, model)."""
+        nodes, model), Q (three or four nodes, model), E and G (two nodes, two
+        control nodes), F and H (two nodes, the controlling source, named as
+        the instance names it)."""
         f = text.split()
         name = f[0]
         k = name[0].upper()
-        nnodes = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4}.get(k)
+        nnodes = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2, "H": 2}.get(k)
         if k == "Q":
             nnodes = 4 if len(f) >= 6 and "=" not in f[5] and not _is_number(f[5]) else 3
         if nnodes is None:
@@ -580,6 +582,8 @@ class Expander:
             f[q] = self.node(f[q].lower(), inst)
         if k in ("D", "M", "Q"):
             f[1 + nnodes] = s.model(f[1 + nnodes].lower())
+        if k in ("F", "H") and len(f) > 3:
+            f[3] = (f[3][:1] + "." + inst.path + "." + f[3]).lower()
         return " ".join(f)
 
     def instantiate(self, text: str, caller: Scope, parent: Optional[_Inst]):

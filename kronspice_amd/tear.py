@@ -21,11 +21,11 @@ def tear_terminals(e: kc.Element) -> List[int]:
     coupling may be torn, each once, in terminal order; elements whose
     coupling runs through a branch-current unknown are refused
     (kron/jacobian.go:136-170)."""
-    if e.kind in (kc.V, kc.L):
+    if e.kind in (kc.V, kc.L, kc.E, kc.F, kc.H):
         raise TearError(f"tearing {e.name}: its coupling runs through a branch-current unknown; "
                         "tear a neighbouring element")
     out: List[int] = []
-    for n in e.nodes:
+    for n in tuple(e.nodes) + (tuple(e.ctrl_nodes) if e.kind == kc.G else ()):
         if n != 0 and n not in out:
             out.append(n)
     return out

@@ -533,6 +533,76 @@ struct VoltageSource : Device {
   }
 };
 
+// Controlled source, device/controlled.go:13-161: E (VCVS) and H (CCVS) force
+// v(n+) - v(n-) through a branch unknown of their own, G (VCCS) and F (CCCS)
+// drive a current n+ -> n-; E and G sense v(nc+) - v(nc-), F and H the branch
+// current of another element (ctrl: its index, linked after setup). Linear,
+// no state; all of its power is model-supplied.
+struct Controlled : Device {
+  double gain = 0;
+  int cnode[2] = {0, 0};
+  int ctrl = -1;
+  int cu[2] = {-1, -1}, ck = -1, branch = -1;
+  bool vout() const { return kind == KH_ELEM_E || kind == KH_ELEM_H; }
+  bool vctl() const { return kind == KH_ELEM_E || kind == KH_ELEM_G; }
+  void setup(SetupContext& c) override {
+    setupNodes(c);
+    if (vctl()) {
+      cu[0] = c.unknown(cnode[0]);
+      cu[1] = c.unknown(cnode[1]);
+    }
+    if (vout()) branch = c.allocBranch();
+  }
+  template <class M, class T>
+  void stamp(M* m, T one, T g) {  // :80-114
+    const int p = u[0], n = u[1];
+    if (vout()) {
+      m->add(p, branch, one);
+      m->add(n, branch, -one);
+      m->add(branch, p, one);
+      m->add(branch, n, -one);
+      if (kind == KH_ELEM_E) {
+        m->add(branch, cu[0], -g);
+        m->add(branch, cu[1], g);
+      } else {
+        m->add(branch, ck, -g);
+      }
+      return;
+    }
+    if (kind == KH_ELEM_G) {
+      m->add(p, cu[0], g);
+      m->add(p, cu[1], -g);
+      m->add(n, cu[0], -g);
+      m->add(n, cu[1], g);
+    } else {
+      m->add(p, ck, g);
+      m->add(n, ck, -g);
+    }
+  }
+  void load(LoadContext& lc) override { stamp(lc.M, 1.0, gain); }
+  double control(const double* x) const {  // :116-121
+    auto at = [&](int i) { return i < 0 ? 0.0 : x[i]; };
+    return vctl() ? at(cu[0]) - at(cu[1]) : at(ck);
+  }
+  OperatingPoint operating(const double* x, int) override {  // :123-133
+    OperatingPoint op;
+    op.voltage = voltage(x);
+    op.current = vout() ? x[branch] : gain * control(x);
+    op.power = op.voltage * op.current;
+    return op;
+  }
+  void ports(const OperatingPoint&, Accumulator&) override {}  // no exterior port of its own
+  bool modelSupplier() const override { return true; }
+  double modelSupplied(const OperatingPoint& op) const override { return -op.power; }
+  void loadAC(ACContext& ac) override { stamp(ac.M, cplx(1, 0), cplx(gain, 0)); }
+  cplx currentAC(const cplx* x) const {  // OperatingAC, :146-161
+    auto at = [&](int i) { return i < 0 ? cplx(0) : x[i]; };
+    if (vout()) return at(branch);
+    const cplx ctl = kind == KH_ELEM_G ? at(cu[0]) - at(cu[1]) : at(ck);
+    return cplx(gain, 0) * ctl;
+  }
+};
+
 // CurrentSource, device/source.go:133-177, :207-217
 struct CurrentSource : Device {
   SourceValue src;

@@ -228,6 +228,21 @@ struct Sim {
           d.reset(q);
           break;
         }
+        case KH_ELEM_E:
+        case KH_ELEM_F:
+        case KH_ELEM_G:
+        case KH_ELEM_H: {
+          auto c = new Controlled();
+          c->gain = p[0];
+          if (el[e].kind == KH_ELEM_E || el[e].kind == KH_ELEM_G) {
+            c->cnode[0] = el[e].node[2];
+            c->cnode[1] = el[e].node[3];
+          } else {
+            c->ctrl = el[e].node[2];
+          }
+          d.reset(c);
+          break;
+        }
         default:
           return E_UNSUPPORTED;
       }
@@ -241,6 +256,19 @@ struct Sim {
       if (devs[i]->reactive()) reactive.push_back((int)i);
       if (devs[i]->nonlinear()) nonlin.push_back((int)i);
       if (devs[i]->breakpointer()) breakers.push_back((int)i);
+    }
+    for (auto& d : devs) {  // Link, device/controlled.go:67-77: F and H find their controlling branch
+      if (d->kind != KH_ELEM_F && d->kind != KH_ELEM_H) continue;
+      auto* c = static_cast<Controlled*>(d.get());
+      int b = -1;
+      if (c->ctrl >= 0 && c->ctrl < (int)devs.size()) {
+        Device* t = devs[c->ctrl].get();
+        if (auto* v = dynamic_cast<VoltageSource*>(t)) b = v->branch;
+        else if (auto* l = dynamic_cast<Inductor*>(t)) b = l->branch;
+        else if (auto* o = dynamic_cast<Controlled*>(t)) b = o->vout() ? o->branch : -1;
+      }
+      if (b < 0) return E_UNSUPPORTED;
+      c->ck = b;
     }
     n = ctx.size();
     bool sp = resolve_sparse(backend, n);
@@ -879,6 +907,10 @@ struct Sim {
             i = vj * cplx(g0 + dd->gmin, dd->omega * c);
             break;
           }
+          case KH_ELEM_E:
+          case KH_ELEM_F:
+          case KH_ELEM_G:
+          case KH_ELEM_H: i = static_cast<Controlled*>(d.get())->currentAC(cx.data()); break;
           default: i = 0;
         }
         cplx s = 0.5 * v * std::conj(i);

@@ -67,9 +67,9 @@ def test_param_errors(bad):
         netlist.parse("t\n" + bad + "\n")
 
 
-# The reference's subcircuit deck with its E and F elements replaced by
-# resistors and a voltage source (this path's element set); the naming,
-# scoping, defaults, nested definitions and local models are the same.
+# The reference's subcircuit deck (netlist/preprocess_test.go:88-115): naming,
+# scoping, defaults, nested definitions, a controlling source named inside an
+# instance, local models.
 SUBCKT_DECK = """subcircuits
 .param rload=4k
 .subckt divider top mid params: rt=1k rb={rt}
@@ -78,12 +78,12 @@ R2 mid 0 {rb}
 .ends divider
 .subckt stage in out params: g=2
 .subckt buf a b
-RB1 a b 1
+E1 b 0 a 0 1
 .ends
 Vs in sense 0
 X1 sense n divider rt={1k*g}
 X2 n out buf
-RF1 0 out {g}
+F1 0 out Vs {g}
 D1 out 0 dloc
 .model dloc D(IS=1e-14)
 .ends
@@ -115,8 +115,10 @@ def test_subckt_expansion():
     check("V.XS.Vs", 0.0, "a", "xs.sense")
     check("R.XS.X1.R1", 3000.0, "xs.sense", "xs.n")
     check("R.XS.X1.R2", 3000.0, "xs.n", "0")  # rb defaults to rt
-    check("R.XS.X2.RB1", 1.0, "xs.n", "s")
-    check("R.XS.RF1", 3.0, "0", "s")
+    check("E.XS.X2.E1", 1.0, "s", "0")
+    assert tuple(name(k) for k in by["E.XS.X2.E1"].ctrl_nodes) == ("xs.n", "0")
+    check("F.XS.F1", 3.0, "0", "s")
+    assert by["F.XS.F1"].ctrl_source == "v.xs.vs"
     check("D.XS.D1", None, "s", "0")
     assert len(c.elements) == 11
     lines = [ln.strip() for ln in SUBCKT_DECK.splitlines()[1:] if ln.strip()]
