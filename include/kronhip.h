@@ -62,6 +62,8 @@ enum {
  *   E G (VCVS, VCCS): node[0] = n+, node[1] = n-, node[2] = nc+, node[3] = nc-
  *   F H (CCCS, CCVS): node[0] = n+, node[1] = n-, node[2] = index in the element list of the
  *                     controlling element (a V, L, E or H: its branch current, positive into its n+)
+ *   K (mutual)      : node[0], node[1] = indices in the element list of the two coupled
+ *                     inductors (circuit.Element.Couples, device/mutual.go:57-72)
  *
  * Each element reads its parameters at params[poff ...]; the per-kind layout
  * is below (already resolved by the host setup step: defaults, area factors,
@@ -69,7 +71,8 @@ enum {
 enum {
   KH_ELEM_R = 1, KH_ELEM_C = 2, KH_ELEM_L = 3, KH_ELEM_V = 4,
   KH_ELEM_I = 5, KH_ELEM_D = 6, KH_ELEM_M = 7, KH_ELEM_Q = 8,
-  KH_ELEM_E = 10, KH_ELEM_F = 11, KH_ELEM_G = 12, KH_ELEM_H = 13   /* linear controlled sources */
+  KH_ELEM_E = 10, KH_ELEM_F = 11, KH_ELEM_G = 12, KH_ELEM_H = 13,  /* linear controlled sources */
+  KH_ELEM_K = 14                                                   /* mutual inductance */
 };
 
 typedef struct kh_element {
@@ -104,6 +107,7 @@ typedef struct kh_element {
  *    38 vt, 39 vcrit, 40 rbpr, 41 rbpi, 42 gcpr, 43 gepr, 44 ovtf,
  *    45..47 f1e f2e f3e, 48..50 f1c f2c f3c, 51 hasCharge  (device/bjt.go:142-242)
  * E, F, G, H: 0 gain (V/V, A/A, S, ohm)                  (device/controlled.go:45-51)
+ * K: 0 M = k·sqrt(L1·L2) [H]                              (device/mutual.go:44-50, device/device.go:407-416)
  */
 #define KH_NPAR_R 2
 #define KH_NPAR_C 3
@@ -113,6 +117,7 @@ typedef struct kh_element {
 #define KH_NPAR_M 58
 #define KH_NPAR_Q 52
 #define KH_NPAR_CTL 1
+#define KH_NPAR_K 1
 
 /* Analysis options: analysis.Options (analysis/op.go:22-62) with the same
  * zero-value defaults (analysis/op.go:91-110). */

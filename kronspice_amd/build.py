@@ -23,7 +23,7 @@ FLAGS = [
 # wavefront-per-circuit kernel once per element-kind set, each in its own
 # translation unit.
 PERSIST_WAVES = (1, 2, 4)
-PERSIST_KINDS = (126, 190, 318, 16382)  # diodes, level-1 MOSFETs, BJTs, everything (kh_persist.h)
+PERSIST_KINDS = (126, 190, 318, 32766)  # diodes, level-1 MOSFETs, BJTs, everything (kh_persist.h)
 UNITS = [
     ("kronhip.hip", [], "kronhip.o"),
     ("kh_symbolic.cpp", [], "kh_symbolic.o"),

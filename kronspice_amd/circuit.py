@@ -25,8 +25,10 @@ import numpy as np
 # Element kinds (include/kronhip.h)
 R, C, L, V, I, D, M, Q = 1, 2, 3, 4, 5, 6, 7, 8
 E, F, G, H = 10, 11, 12, 13  # VCVS, CCCS, VCCS, CCVS
-KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q", E: "E", F: "F", G: "G", H: "H"}
-NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 58, Q: 52, E: 1, F: 1, G: 1, H: 1}
+K = 14  # mutual inductance
+KIND_NAMES = {R: "R", C: "C", L: "L", V: "V", I: "I", D: "D", M: "M", Q: "Q", E: "E", F: "F", G: "G", H: "H",
+              K: "K"}
+NPAR = {R: 2, C: 3, L: 3, V: 12, I: 12, D: 16, M: 58, Q: 52, E: 1, F: 1, G: 1, H: 1, K: 1}
 
 ELEMENT_DTYPE = np.dtype([("kind", "<i4"), ("node", "<i4", (4,)), ("poff", "<i4")])
 
@@ -90,6 +92,7 @@ class Element:
     params: Dict[str, float] = field(default_factory=dict)
     ctrl_nodes: Tuple[int, ...] = ()  # E, G: nc+, nc- (node numbers)
     ctrl_source: str = ""             # F, H: the controlling element's name, lower-cased
+    couples: Tuple[str, ...] = ()     # K: the two coupled inductors' names, lower-cased
 
 
 class Circuit:
@@ -184,6 +187,8 @@ class Circuit:
                 except KeyError:
                     raise ValueError(f"{e.name}: controlling source {e.ctrl_source} not found") from None
                 nodes = [e.nodes[0], e.nodes[1], ctl, 0]
+            elif e.kind == K:  # node[0], node[1]: the coupled inductors' indices (Link, device/mutual.go:57-72)
+                nodes = [self.element_index(n) for n in e.couples] + [0, 0]
             els[k]["node"] = nodes[:4]
             els[k]["poff"] = len(params)
             params.extend(element_params(self, e, step, stop))
@@ -273,6 +278,12 @@ class Builder:
 
     def ccvs(self, name, p, n, vname, r):
         return self.controlled(name, H, p, n, r, ctrl_source=vname)
+
+    def mutual(self, name, l1, l2, k):
+        """Mutual inductance (SPICE K) between inductors l1 and l2, coupling
+        coefficient k; its nodes are ground (circuit/builder.go:80-84)."""
+        e = Element(name, K, (), value=float(k), couples=(l1.lower(), l2.lower()))
+        return self.add(e, "0", "0")
 
     def bjt(self, name, collector, base, emitter, model, substrate: Optional[str] = None, area: float = 0.0):
         # AddMulti(e, collector, emitter, base[, substrate]): circuit/builder.go:99-103, netlist
@@ -589,7 +600,35 @@ def element_params(c: Circuit, e: Element, step: float, stop: float) -> List[flo
         return bjt_params(c, e)
     if e.kind in (E, F, G, H):
         return [e.value]  # the gain (device/controlled.go:45-51)
+    if e.kind == K:
+        return [mutual_inductance(c, e)]
     raise NotImplementedError(f"element kind {e.kind} ({e.name}) is not on this path")
+
+
+def mutual_inductance(c: Circuit, e: Element) -> float:
+    """M = k·sqrt(L1·L2) of a K element (device/mutual.go:44-50 with the inductances
+    device/device.go:407-416 reads), after circuit.validateMutual's checks
+    (circuit/circuit.go:804-830): 0 < k <= 1, two distinct linear inductors, one K
+    per pair."""
+    if not (e.value > 0 and e.value <= 1):
+        raise ValueError(f"{e.name}: coupling coefficient {e.value:g} must be in (0, 1]")
+    a, b = e.couples
+    if a == b:
+        raise ValueError(f"{e.name} couples {a} with itself")
+    ls = []
+    for n in (a, b):
+        try:
+            el = c.elements[c.element_index(n)]
+        except KeyError:
+            el = None
+        if el is None or el.kind != L:
+            raise ValueError(f"{e.name}: {n!r} is not a linear inductor")
+        ls.append(el.value)
+    pair = tuple(sorted((a, b)))
+    for o in c.elements:
+        if o is not e and o.kind == K and tuple(sorted(o.couples)) == pair and c.elements.index(o) < c.elements.index(e):
+            raise ValueError(f"{e.name}: {pair[0]} and {pair[1]} are already coupled")
+    return e.value * math.sqrt(ls[0] * ls[1])
 
 
 def unknown_of_node(node: int) -> int:

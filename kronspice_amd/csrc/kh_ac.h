@@ -135,6 +135,12 @@ KH_HD inline void ac_load(ACLane& A, int nel) {
         A.add(e, 2, 1, {-1, 0});
         A.add(e, 2, 2, {0, -A.omega * L.P(e, 0)});
         break;
+      case K_K: {  // Mutual.LoadAC, device/mutual.go:141-148: -jwM between the branch rows
+        const double im = -A.omega * L.P(e, 0);
+        A.add(e, 0, 1, {-0.0, im});  // complex(-0, -wM) - complex(0, 0)
+        A.add(e, 1, 0, {0.0, im});   // complex(-0, -wM) + complex(0, 0)
+        break;
+      }
       case K_V:
         A.add(e, 0, 2, {1, 0});
         A.add(e, 1, 2, {-1, 0});
@@ -275,7 +281,8 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
       s = {s.re / 2.0, s.im / 2.0};
       if (icur) icur[(int64_t)k * istride] = ic;
     } else {
-      kcplx v = A.xat(x, L.T(e, 0)) - A.xat(x, L.T(e, 1));
+      // a K element has no phasors of its own (device/mutual.go:150-157)
+      kcplx v = e[ET_KIND] == K_K ? kcplx{0, 0} : A.xat(x, L.T(e, 0)) - A.xat(x, L.T(e, 1));
       kcplx i = {0, 0};
       switch (e[ET_KIND]) {
         case K_R: i = v * kcplx{L.P(e, 1), 0}; break;
@@ -304,7 +311,7 @@ KH_HD inline void ac_power(const ACLane& A, int nel, const kcplx* x, kcplx* sumS
     if (led) {
       const int kind = e[ET_KIND];
       const double pw = s.re;
-      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q) reactive += s.im;
+      if (kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q || kind == K_K) reactive += s.im;
       double port = 0;
       int ch = -1;
       if ((kind == K_R && L.P(e, 0) > 0) || kind == K_D || kind == K_M || kind == K_Q) {

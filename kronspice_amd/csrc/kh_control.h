@@ -225,6 +225,7 @@ struct CtlT {
         case K_F: if (KH_HAS(K_F)) load_ctl(lane, e); break;
         case K_G: if (KH_HAS(K_G)) load_ctl(lane, e); break;
         case K_H: if (KH_HAS(K_H)) load_ctl(lane, e); break;
+        case K_K: if (KH_HAS(K_K)) load_K(lane, e); break;
       }
     }
     L = lane;
@@ -244,6 +245,7 @@ struct CtlT {
         case K_D: if (KH_HAS(K_D)) accept_D(L, e, x_, t, mode); break;
         case K_M: if (KH_HAS(K_M)) accept_M(L, e, x_, t, mode); break;
         case K_Q: if (KH_HAS(K_Q)) accept_Q(L, e, x_, t, mode); break;
+        case K_K: if (KH_HAS(K_K)) accept_K(L, e, x_); break;
       }
     }
   }

@@ -35,11 +35,13 @@ enum { K_M23 = 9 };
 // Linear controlled sources: E (VCVS), F (CCCS), G (VCCS), H (CCVS).
 enum { K_E = 10, K_F = 11, K_G = 12, K_H = 13 };
 KH_HD inline bool is_controlled(int kind) { return kind >= K_E && kind <= K_H; }
+// Mutual inductance between two inductors (SPICE K).
+enum { K_K = 14 };
 
 // Element kinds a translation unit's circuit code handles (bit k: kind k).
 // The persistent kernel is compiled once per set, so a batch runs code for
 // the devices it has and nothing else (smaller code, fewer registers).
-#define KH_KINDS_ALL 16382
+#define KH_KINDS_ALL 32766
 #ifndef KH_KINDS
 #define KH_KINDS KH_KINDS_ALL
 #endif
@@ -50,7 +52,7 @@ enum { MODE_OP = 0, MODE_TRANOP = 1, MODE_UICINIT = 2, MODE_TRAN = 3 };
 enum { BE = 1, TRAP = 2 };
 
 // State sizes (doubles per element).
-enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96, NS_Q = 66, NS_CTL = 0 };
+enum { NS_R = 0, NS_C = 13, NS_L = 9, NS_V = 1, NS_I = 1, NS_D = 19, NS_M = 96, NS_Q = 66, NS_CTL = 0, NS_K = 2 };
 
 // history (device/lte.go:23-37) lives in 7 state words: t0 t1 t2 q0 q1 q2 n.
 constexpr double kInf = __builtin_huge_val();
@@ -139,6 +141,7 @@ KH_HD inline int record_capacity(int kind) {
     case K_F: return 2;
     case K_G: return 4;
     case K_H: return 5;
+    case K_K: return 4;
   }
   return 0;
 }
@@ -318,6 +321,33 @@ KH_HD inline OpPt op_ctl(const Lane& L, const int32_t* e, const double* x) {
   o.power = o.voltage * o.current;
   o.storage = 0;
   return o;
+}
+
+// ---- K (device/mutual.go:74-157) --------------------------------------------
+// terminals: 0 the first inductor's branch, 1 the second's; parameter 0 M;
+// state: 0 i1Prev, 1 i2Prev. Nothing at DC (the inductors are shorts); in a
+// transient step each inductor's integration formula applied to the flux the
+// other's current links: -f off the diagonal and -f·i_other,prev on the
+// right-hand side, f = M/h (backward Euler) or 2M/h (trapezoidal).
+KH_HD inline void load_K(Lane& L, const int32_t* e) {
+  if (L.mode != MODE_TRAN) return;
+  const double m = L.P(e, 0);
+  double f = m / L.H;
+  if (L.method == TRAP) f = 2 * m / L.H;
+  L.add(e, 0, 1, -f);
+  L.add(e, 1, 0, -f);
+  L.addRHS(L.T(e, 0), -f * L.S(e, 1));
+  L.addRHS(L.T(e, 1), -f * L.S(e, 0));
+}
+// Mutual.Operating: no voltage, current or power of its own.
+KH_HD inline OpPt op_K() {
+  OpPt o = {0, 0, 0, 0};
+  return o;
+}
+// Mutual.Accept: the two currents, in every mode.
+KH_HD inline void accept_K(const Lane& L, const int32_t* e, const double* x) {
+  L.S(e, 0) = x[L.T(e, 0)];
+  L.S(e, 1) = x[L.T(e, 1)];
 }
 
 // ---- C (device/reactive.go:60-181) ------------------------------------------
@@ -1438,6 +1468,7 @@ KH_HD inline OpPt operating_point(const Lane& L, const int32_t* e, const double*
     case K_M: if (KH_HAS(K_M)) return op_M(L, e, x); break;
     case K_Q: if (KH_HAS(K_Q)) return op_Q(L, e, x); break;
     case K_E: case K_F: case K_G: case K_H: return op_ctl(L, e, x);
+    case K_K: return op_K();
   }
   OpPt o = {0, 0, 0, 0};
   return o;

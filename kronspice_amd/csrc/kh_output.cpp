@@ -210,6 +210,7 @@ const char* kind_name(int k) {
     case KH_ELEM_F: return "CCCS";
     case KH_ELEM_G: return "VCCS";
     case KH_ELEM_H: return "CCVS";
+    case KH_ELEM_K: return "K";
   }
   return "unknown";
 }
@@ -241,6 +242,7 @@ std::vector<std::pair<int, const char*>> ports_of(int kind, bool negative) {
     case KH_ELEM_F:
     case KH_ELEM_G:
     case KH_ELEM_H: return {{0, "idealized controlled source: its power, of either sign, is model-supplied"}};
+    case KH_ELEM_K: return {{0, "couples two inductors inside the network; stores M·i1·i2"}};  // device/mutual.go:46-47
   }
   return {};
 }
@@ -367,8 +369,10 @@ int64_t kh_format_op(int32_t format, const kh_names* nm, const char* version, kh
       e.add("name", J::str(nm->elem_names[k]));
       e.add("kind", J::str(kind_name(nm->elems[k].kind)));
       J nodes = J::arr();
-      nodes.push(J::str(nm->node_names[nm->elems[k].node[0]]));
-      nodes.push(J::str(nm->node_names[nm->elems[k].node[1]]));
+      // a K element's nodes are ground (circuit/builder.go:80-84); node[] holds the inductors
+      const bool kk = nm->elems[k].kind == KH_ELEM_K;
+      nodes.push(J::str(nm->node_names[kk ? 0 : nm->elems[k].node[0]]));
+      nodes.push(J::str(nm->node_names[kk ? 0 : nm->elems[k].node[1]]));
       e.add("nodes", std::move(nodes));
       e.add("voltage", J::num(rp(k, KH_RP_VOLTAGE)));
       e.add("current", J::num(rp(k, KH_RP_CURRENT)));

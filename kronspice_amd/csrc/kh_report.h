@@ -148,6 +148,7 @@ KH_HD inline void report_element(const Lane& L, const int32_t* e, const double* 
       break;
     case K_C: energy = 0.5 * L.P(e, 0) * o.voltage * o.voltage; break;  // device/reactive.go:171-175
     case K_L: energy = 0.5 * L.P(e, 0) * o.current * o.current; break;  // device/reactive.go:339-342
+    case K_K: energy = L.P(e, 0) * x[L.T(e, 0)] * x[L.T(e, 1)]; break;  // M·i1·i2, device/mutual.go:132-133
     case K_V:
     case K_I:  // device/source.go:123-129, :171-177: the source's power enters from the exterior
       exterior = -o.power;
@@ -203,7 +204,9 @@ enum {
 
 // Elements that store energy (device.Reactive): their storage power enters
 // the ledger.
-KH_HD inline bool kind_stores(int kind) { return kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q; }
+KH_HD inline bool kind_stores(int kind) {
+  return kind == K_C || kind == K_L || kind == K_D || kind == K_M || kind == K_Q || kind == K_K;
+}
 
 // em.BuildLedger (em/energy.go:89-119) over the element reports rep[nel][RP_FIELDS]
 // in element order: each element's ports in channel order (only the one

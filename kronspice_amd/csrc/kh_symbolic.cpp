@@ -207,6 +207,9 @@ static const int kPairsE[6][2] = {{0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 3}, {2, 4}
 static const int kPairsH[5][2] = {{0, 2}, {1, 2}, {2, 0}, {2, 1}, {2, 3}};
 static const int kPairsG[4][2] = {{0, 3}, {0, 4}, {1, 3}, {1, 4}};
 static const int kPairsF[2][2] = {{0, 3}, {1, 3}};
+// Mutual inductance (device/mutual.go:104-108): terminals 0 and 1 are the
+// coupled inductors' branches.
+static const int kPairsK[2][2] = {{0, 1}, {1, 0}};
 
 bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems, int32_t np, const double* params_c0) {
   num_nodes = numNodes;
@@ -307,6 +310,10 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
         ns = NS_CTL;
         npar = KH_NPAR_CTL;
         break;
+      case K_K:  // Setup allocates nothing; Link below finds the inductors' branches
+        ns = NS_K;
+        npar = KH_NPAR_K;
+        break;
       default:
         error = "element " + std::to_string(e) + ": kind " + std::to_string(E.kind) + " is not supported";
         return false;
@@ -319,7 +326,8 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
     soff += ns;
     kinds |= 1u << E.kind;
     if (E.kind == K_D || E.kind == K_M || E.kind == K_Q) nl_idx.push_back(e);
-    if (E.kind == K_C || E.kind == K_L || E.kind == K_D || E.kind == K_M || E.kind == K_Q) rx_idx.push_back(e);
+    if (E.kind == K_C || E.kind == K_L || E.kind == K_D || E.kind == K_M || E.kind == K_Q || E.kind == K_K)
+      rx_idx.push_back(e);
     if (E.kind == K_V || E.kind == K_I) src_idx.push_back(e);
     if (E.kind == K_M) {
       if (params_c0[E.poff + 34] >= 2) kinds |= 1u << K_M23;  // level 2 or 3 currents
@@ -340,6 +348,19 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
       return false;
     }
     et[(size_t)e * ET_W + ET_TERM + 3] = et[(size_t)c * ET_W + ET_TERM + 2];
+  }
+  // Mutual.Link (device/mutual.go:57-72): the two coupled inductors' branches.
+  for (int e = 0; e < nel; e++) {
+    const kh_element& E = elems[e];
+    if (E.kind != K_K) continue;
+    for (int t = 0; t < 2; t++) {
+      const int c = E.node[t];
+      if (c < 0 || c >= nel || elems[c].kind != K_L || (t == 1 && c == E.node[0])) {
+        error = "element " + std::to_string(e) + ": coupled inductor not found";
+        return false;
+      }
+      et[(size_t)e * ET_W + ET_TERM + t] = et[(size_t)c * ET_W + ET_TERM + 2];
+    }
   }
   n = numNodes - 1 + branches;
   if (n <= 0) {
@@ -378,6 +399,7 @@ bool Topology::build(int32_t numNodes, const kh_element* elems, int32_t n_elems,
       case K_F: pairs = kPairsF; np2 = 2; break;
       case K_G: pairs = kPairsG; np2 = 4; break;
       case K_H: pairs = kPairsH; np2 = 5; break;
+      case K_K: pairs = kPairsK; np2 = 2; break;
       default: break;
     }
     for (int k = 0; k < KH_NT * KH_NT; k++) t[ET_SLOT + k] = -1;

@@ -548,6 +548,7 @@ struct WCtl : CtlT<WCtl> {
           case K_F: if (KH_HAS(K_F)) load_ctl(l, e); break;
           case K_G: if (KH_HAS(K_G)) load_ctl(l, e); break;
           case K_H: if (KH_HAS(K_H)) load_ctl(l, e); break;
+          case K_K: if (KH_HAS(K_K)) load_K(l, e); break;
         }
         if (l.rn > l.rcap) over = true;
         R.rcnt[lane] = min(l.rn, l.rcap);
@@ -621,6 +622,7 @@ struct WCtl : CtlT<WCtl> {
         case K_D: if (KH_HAS(K_D)) accept_D(Lc, e, A.x, t, mode); break;
         case K_M: if (KH_HAS(K_M)) accept_M(Lc, e, A.x, t, mode); break;
         case K_Q: if (KH_HAS(K_Q)) accept_Q(Lc, e, A.x, t, mode); break;
+        case K_K: if (KH_HAS(K_K)) accept_K(Lc, e, A.x); break;
       }
     }
     wsync();

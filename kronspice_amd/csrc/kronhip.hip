@@ -1550,7 +1550,7 @@ namespace kh {
 #define KH_PERSIST_VARIANT(W, K)                                                                          \
   hipError_t kh_launch_persist_w##W##_k##K(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
 #define KH_PERSIST_KINDS(W) KH_PERSIST_VARIANT(W, 126) KH_PERSIST_VARIANT(W, 190) KH_PERSIST_VARIANT(W, 318) \
-  KH_PERSIST_VARIANT(W, 16382)
+  KH_PERSIST_VARIANT(W, 32766)
 KH_PERSIST_KINDS(1)
 KH_PERSIST_KINDS(2)
 KH_PERSIST_KINDS(4)
@@ -1563,14 +1563,14 @@ KH_PERSIST_KINDS(4)
 KH_WAVE_VARIANT(126)
 KH_WAVE_VARIANT(190)
 KH_WAVE_VARIANT(318)
-KH_WAVE_VARIANT(16382)
+KH_WAVE_VARIANT(32766)
 #undef KH_WAVE_VARIANT
 
 hipError_t kh_launch_wave(unsigned kinds, unsigned grid, size_t lds_bytes, const Group* G, const int32_t* gw,
                           const int32_t* cw, int32_t nwork, int32_t* next, int32_t max_solves, int32_t* counters) {
   using L = hipError_t (*)(unsigned, size_t, const Group*, const int32_t*, const int32_t*, int32_t, int32_t*, int32_t,
                            int32_t*);
-  static const L table[4] = {kh_launch_wave_k126, kh_launch_wave_k190, kh_launch_wave_k318, kh_launch_wave_k16382};
+  static const L table[4] = {kh_launch_wave_k126, kh_launch_wave_k190, kh_launch_wave_k318, kh_launch_wave_k32766};
   const unsigned v = persist_kinds(kinds);
   const int kv = v == kKindsD ? 0 : v == kKindsM ? 1 : v == kKindsQ ? 2 : 3;
   return table[kv](grid, lds_bytes, G, gw, cw, nwork, next, max_solves, counters);
@@ -1580,9 +1580,9 @@ hipError_t kh_launch_persist(int waves, unsigned kinds, unsigned grid, size_t ld
                              const int32_t* bgroup, int32_t max_solves, int32_t* counters) {
   using L = hipError_t (*)(unsigned, size_t, const Group*, const int32_t*, int32_t, int32_t*);
   static const L table[3][4] = {
-      {kh_launch_persist_w1_k126, kh_launch_persist_w1_k190, kh_launch_persist_w1_k318, kh_launch_persist_w1_k16382},
-      {kh_launch_persist_w2_k126, kh_launch_persist_w2_k190, kh_launch_persist_w2_k318, kh_launch_persist_w2_k16382},
-      {kh_launch_persist_w4_k126, kh_launch_persist_w4_k190, kh_launch_persist_w4_k318, kh_launch_persist_w4_k16382}};
+      {kh_launch_persist_w1_k126, kh_launch_persist_w1_k190, kh_launch_persist_w1_k318, kh_launch_persist_w1_k32766},
+      {kh_launch_persist_w2_k126, kh_launch_persist_w2_k190, kh_launch_persist_w2_k318, kh_launch_persist_w2_k32766},
+      {kh_launch_persist_w4_k126, kh_launch_persist_w4_k190, kh_launch_persist_w4_k318, kh_launch_persist_w4_k32766}};
   const unsigned v = persist_kinds(kinds);
   const int kv = v == kKindsD ? 0 : v == kKindsM ? 1 : v == kKindsQ ? 2 : 3;
   const int wv = waves == 4 ? 2 : waves == 2 ? 1 : 0;

@@ -220,6 +220,10 @@ def parse(text: str) -> Deck:
             if len(tok) != 5:
                 raise ValueError(f"{name}: {k} element needs: name n+ n- vname gain")
             b.controlled(name, kc.F if k == "F" else kc.H, tok[1], tok[2], parse_value(tok[4]), ctrl_source=tok[3])
+        elif k == "K":  # Kname L1 L2 k, netlist/parser.go:589-621 (no spin-connection terms here)
+            if len(tok) != 4:
+                raise ValueError(f"{name}: K element needs: Kname L1 L2 coupling")
+            b.mutual(name, tok[1], tok[2], parse_value(tok[3]))
         else:
             raise NotImplementedError(f"element {name} is not on this path")
     return Deck(title, b.build(), analyses)

@@ -462,6 +462,41 @@ struct Inductor : Device {
   }
 };
 
+// Mutual inductance (SPICE K), device/mutual.go:13-157 without the
+// spin-connection terms: couples the branch rows of two inductors (k1, k2,
+// found by Link); nothing at DC; in a transient step each inductor's
+// integration formula applied to the flux the other's current links; -jwM in
+// AC. Stores M·i1·i2, has no port and no voltage or current of its own.
+struct Mutual : Device {
+  double M = 0;
+  int l[2] = {-1, -1};  // the coupled inductors' element indices
+  int k1 = -1, k2 = -1;
+  double i1Prev = 0, i2Prev = 0;
+  void setup(SetupContext&) override {}  // Setup allocates nothing (:54-55)
+  void load(LoadContext& lc) override {  // :74-113
+    if (lc.mode == ModeOP || lc.mode == ModeTranOP || lc.mode == ModeUICInit) return;
+    double f = M / lc.H;
+    if (lc.method == Trapezoidal) f = 2 * M / lc.H;
+    lc.M->add(k1, k2, -f);
+    lc.M->add(k2, k1, -f);
+    lc.addRHS(k1, -f * i2Prev);
+    lc.addRHS(k2, -f * i1Prev);
+  }
+  OperatingPoint operating(const double*, int) override { return OperatingPoint(); }  // :115-126
+  void accept(const double* x, int, double, int) override {  // :128-131
+    i1Prev = x[k1];
+    i2Prev = x[k2];
+  }
+  bool reactive() const override { return true; }  // Truncate: +Inf (:135-136)
+  double storedEnergy(const double* x, int) override { return M * x[k1] * x[k2]; }  // :132-133
+  void ports(const OperatingPoint&, Accumulator&) override {}
+  void loadAC(ACContext& ac) override {  // :141-148
+    const cplx z(-0.0, -ac.omega * M);
+    ac.M->add(k1, k2, z - cplx(0, 0));
+    ac.M->add(k2, k1, z + cplx(0, 0));
+  }
+};
+
 // sourceValue, device/source.go:21-73
 struct SourceValue {
   double dc = 0;

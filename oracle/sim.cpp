@@ -243,11 +243,19 @@ struct Sim {
           d.reset(c);
           break;
         }
+        case KH_ELEM_K: {
+          auto mu = new Mutual();
+          mu->M = p[0];
+          mu->l[0] = el[e].node[0];
+          mu->l[1] = el[e].node[1];
+          d.reset(mu);
+          break;
+        }
         default:
           return E_UNSUPPORTED;
       }
       d->kind = el[e].kind;
-      for (int k = 0; k < 4; k++) d->nodes[k] = el[e].node[k];
+      for (int k = 0; k < 4; k++) d->nodes[k] = el[e].kind == KH_ELEM_K ? 0 : el[e].node[k];
       devs.push_back(std::move(d));
     }
     (void)step; (void)stop;
@@ -269,6 +277,17 @@ struct Sim {
       }
       if (b < 0) return E_UNSUPPORTED;
       c->ck = b;
+    }
+    for (auto& d : devs) {  // Link, device/mutual.go:57-72: the coupled inductors' branches
+      if (d->kind != KH_ELEM_K) continue;
+      auto* mu = static_cast<Mutual*>(d.get());
+      int b[2] = {-1, -1};
+      for (int t = 0; t < 2; t++)
+        if (mu->l[t] >= 0 && mu->l[t] < (int)devs.size())
+          if (auto* l = dynamic_cast<Inductor*>(devs[mu->l[t]].get())) b[t] = l->branch;
+      if (b[0] < 0 || b[1] < 0 || b[0] == b[1]) return E_UNSUPPORTED;
+      mu->k1 = b[0];
+      mu->k2 = b[1];
     }
     n = ctx.size();
     bool sp = resolve_sparse(backend, n);

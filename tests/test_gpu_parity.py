@@ -338,12 +338,12 @@ def test_dc_sweep_matches_oracle(name):
     assert host(r.ledger).max() <= 1e-9
 
 
-AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac", "mos3_cs_ac", "param_rc_ac"]
+AC_DECKS = ["rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "bjt_ce_ac", "mos3_cs_ac", "param_rc_ac", "mutual_ac"]
 # Decks whose operating point involves no transcendental function of a value
 # that differs between the device library and glibc: bit for bit. The BJT's
 # exponentials and the level-3 MOSFET's exponentials and powers may differ in
 # the last bit: within 1e-9.
-AC_BITEXACT = {"rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "param_rc_ac"}
+AC_BITEXACT = {"rc_lowpass_ac", "rlc_bandpass_ac", "mos1_cs_ac", "param_rc_ac", "mutual_ac"}
 
 
 @pytest.mark.parametrize("name", AC_DECKS)
