@@ -568,11 +568,12 @@ class Expander:
         path reads: R, C, L, V, I (two nodes), D (two nodes, model), M (four
         nodes, model), Q (three or four nodes, model), E and G (two nodes, two
         control nodes), F and H (two nodes, the controlling source, named as
-        the instance names it)."""
+        the instance names it), K (the two coupled inductors, named so)."""
         f = text.split()
         name = f[0]
         k = name[0].upper()
-        nnodes = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2, "H": 2}.get(k)
+        nnodes = {"R": 2, "C": 2, "L": 2, "V": 2, "I": 2, "D": 2, "M": 4, "E": 4, "G": 4, "F": 2, "H": 2,
+                  "K": 0}.get(k)
         if k == "Q":
             nnodes = 4 if len(f) >= 6 and "=" not in f[5] and not _is_number(f[5]) else 3
         if nnodes is None:
@@ -584,6 +585,10 @@ class Expander:
             f[1 + nnodes] = s.model(f[1 + nnodes].lower())
         if k in ("F", "H") and len(f) > 3:
             f[3] = (f[3][:1] + "." + inst.path + "." + f[3]).lower()
+        if k == "K":  # el.Couples, netlist/preprocess.go:649-651
+            for q in (1, 2):
+                if q < len(f):
+                    f[q] = (f[q][:1] + "." + inst.path + "." + f[q]).lower()
         return " ".join(f)
 
     def instantiate(self, text: str, caller: Scope, parent: Optional[_Inst]):

@@ -198,3 +198,31 @@ def test_gpu_tran_bitexact():
         st, xo, info = oracle.tran(c.num_nodes, els, P[:, d].copy(), ospec, cap=200000)
         assert st == 0 and acc[d] == info["accepted"] and rej[d] == info["rejected"], d
         assert np.array_equal(x[:, d], xo), d
+
+
+def test_subckt_coupled_pair():
+    """A K element inside a .subckt couples that instance's inductors
+    (netlist/preprocess.go:649-651): two instances give two independent
+    transformers with the top-level deck's answer."""
+    sub = """two transformers
+.subckt xf in out
+R1 in p 10
+L1 p 0 1m
+L2 out 0 4m
+K1 L1 L2 0.8
+R2 out 0 100
+.ends
+V1 a 0 DC 0 AC 1
+X1 a s1 xf
+X2 a s2 xf
+.end
+"""
+    c = netlist.parse(sub).circuit
+    els, p = c.flatten()
+    st, X, _ = oracle.ac(c.num_nodes, els, p, [1e3])
+    assert st == 0
+    ct, et, pt = flat(TRANSFORMER)
+    st, Xt, _ = oracle.ac(ct.num_nodes, et, pt, [1e3])
+    want = Xt[0, ct.node("s") - 1]
+    for n in ("s1", "s2"):
+        assert abs(X[0, c.node(n) - 1] - want) <= 1e-13 * abs(want), n
